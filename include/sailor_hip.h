@@ -1,0 +1,217 @@
+/*
+ * sailor_hip.h - C ABI of the MI355X (gfx950) implementation of SAILOR's
+ * VoxelEncoding + FastNerf rendering path.
+ *
+ * Conventions
+ *  - Every pointer argument is a device pointer to a dense, row-major buffer
+ *    unless documented otherwise.  Shapes are given as [dim0, dim1, ...].
+ *  - `stream` is a hipStream_t passed as void* (NULL = default stream).  All
+ *    work is enqueued on it; no entry point synchronises the device unless
+ *    stated.
+ *  - Every entry point returns SAILOR_OK (0) or an error code; the message of
+ *    the last error on the calling thread is returned by sailor_last_error().
+ *    Nothing here terminates the process.
+ *  - Outputs are written into caller-provided buffers (the Python layer
+ *    allocates them with torch), so capacity is explicit and every call can be
+ *    captured into a hipGraph.
+ *
+ * Reference citations are paths relative to the zdlarr/SAILOR repository.
+ */
+#ifndef SAILOR_HIP_H
+#define SAILOR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SAILOR_OK 0
+#define SAILOR_ERR_ARG 1      /* invalid shape / argument */
+#define SAILOR_ERR_CAPACITY 2 /* an output buffer is too small */
+#define SAILOR_ERR_HIP 3      /* HIP runtime / launch failure */
+
+const char* sailor_last_error(void);
+int sailor_abi_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* TSDF fusion                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces fusion_cuda_integrate (c_lib/VoxelEncoding/src/fusion_kernel.cu:600-727),
+ * bound as VoxelEncoding.integrate (src/binding.cpp:18).
+ *  occ_vol    int32 [B,X,Y,Z]  out: v_id of occupied voxels, -1 otherwise
+ *  num_occ    int32 [B]        in/out: += number of occupied voxels
+ *  K          f32 [B,V,3,3], RT f32 [B,V,3,4], R f32 [B,V,3,3], T f32 [B,V,3]
+ *  colors     f32 [B,V,H,W] packed b*65536+g*256+r; depths f32 [B,V,H,W] (m)
+ *  masks      int32 [B,V,H,W] (unused by the algorithm, kept for the ABI)
+ *  fixed_bbox f32 [B,3,2] or NULL.  NULL: adaptive cubic bbox from the depth
+ *             maps (reference behaviour); otherwise the given box is used and
+ *             voxel_size = (max-min along x)/(X-1).
+ *  tsdf, weight, color  f32 [B,X,Y,Z] (fully overwritten)
+ *  vol_origin f32 [B,3,2], vol_bbox f32 [B,3,2], voxel_size f32 [B]        */
+int sailor_integrate(int32_t* occ_vol, int32_t* num_occ, const float* K,
+                     const float* RT, const float* R, const float* T,
+                     const float* colors, const float* depths,
+                     const int32_t* masks, int B, int V, int H, int W, int X,
+                     int Y, int Z, float obs_weight, float th_bbox,
+                     float trunc_weight, float tsdf_th_low, float tsdf_th_high,
+                     const float* fixed_bbox, float* tsdf, float* weight,
+                     float* color, float* vol_origin, float* vol_bbox,
+                     float* voxel_size, void* stream);
+
+/* Replaces get_origin_bbox (fusion_kernel.cu:473-534).
+ *  depths f32 [B,V,H,W], K/R f32 [B,V,3,3], T f32 [B,V,3] -> vol_origin [B,3,2] */
+int sailor_get_origin_bbox(const float* depths, const float* K, const float* R,
+                           const float* T, int B, int V, int H, int W,
+                           float th_bbox, float* vol_origin, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Two-layer occupancy tree                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces build_multi_res_occupied_volumes (octree_kernel.cu:367-411), bound as
+ * VoxelEncoding.build_multi_res_volumes (binding.cpp:24).
+ *  vols   host array of `levels` device pointers; vols[l] is int32 [B,d_l,d_l,d_l]
+ *         with d_l = dim / rate^l.  vols[l+1] is rewritten from vols[l].
+ *  counts host array of `levels` device pointers to int32 [B]; counts[l+1] += */
+int sailor_build_multi_res_volumes(int32_t* const* vols, int32_t* const* counts,
+                                   int levels, int B, int dim, int rate,
+                                   void* stream);
+
+/* Node record layout (structure of arrays, capacity = sum of per-level totals):
+ *  node_pos  f32 [cap,4]  (centre x, y, z, voxel size)
+ *  node_info int32 [cap,4] (batch id or -1 when invalid, parent gindex or -1,
+ *                           first child gindex or -1, number of children)
+ * Node order is canonical: fine nodes occupy [fine_start, fine_start+n_fine)
+ * ordered by (batch, parent coarse cell in x,y,z raster order, child in raster
+ * order inside the block); coarse nodes occupy [coarse_start, coarse_start+n_c)
+ * in (batch, raster) order.  Children of a coarse node are contiguous. */
+
+/* Bytes of device workspace sailor_octree_build needs. */
+size_t sailor_octree_workspace_bytes(int B, int dim_coarse);
+
+/* Replaces MultiLayerOctree::init_octree + build_octree for two levels
+ * (octree_kernel.cu:569-697, kernel gen_root_octree_nodes :126-239).
+ *  vol_fine int32 [B,df,df,df] in/out, vol_coarse int32 [B,dc,dc,dc] in/out:
+ *    occupied cells are rewritten with node gindex.  Occupied coarse cells
+ *    with no occupied 26-neighbour are not allocated and are set to -1.
+ *  fine_counts / coarse_counts int32 [B]: occupied cells per batch (the
+ *    num_occ counters of integrate / build_multi_res_volumes).
+ *  coarse_start < 0 means "sum of fine_counts[0..B)" read on the device.
+ *  Entries [0, coarse_start + sum(coarse_counts)) that hold no node are
+ *  marked invalid.
+ *  queue_counts int32 [2] out: number of allocated coarse / fine nodes; both
+ *  are set to -1 (and nothing is written) if the nodes do not fit into
+ *  `capacity` or the fine nodes would overlap the coarse range.            */
+int sailor_octree_build(int32_t* vol_fine, int32_t* vol_coarse,
+                        const int32_t* fine_counts,
+                        const int32_t* coarse_counts, int B, int dim_fine,
+                        int dim_coarse, const float* voxel_size,
+                        const float* vol_bbox, int coarse_start, int capacity,
+                        float* node_pos, int32_t* node_info,
+                        int32_t* queue_counts, void* workspace,
+                        size_t workspace_bytes, void* stream);
+
+/* Replaces MultiLayerOctree::output_all_nodes_tensor (octree_kernel.cu:699-717):
+ *  out f32 [n,6] = (x, y, z, voxel size, valid, batch).                     */
+int sailor_octree_nodes_tensor(const float* node_pos, const int32_t* node_info,
+                               int n, float* out, void* stream);
+
+/* Replaces MultiLayerOctree::voxel_traversal (intersection_kernel.cu:827-895;
+ * kernels ray_voxel_traversal :286-439, RayVoxelIntersectionNodeListNew
+ * :128-284, RayVoxelIntersection :36-71).
+ *  n_nodes: length of the node arrays; volume entries >= n_nodes are
+ *  treated as empty cells.
+ *  ray_ori/ray_dir f32 [B,R,3]; outputs hit_idx int32 [B,R,n_max_hits],
+ *  tmin/tmax f32 [B,R,n_max_hits] (-1 padded), valid_len f32 [B,R].
+ *  status int32 [2] device: [0] = max hits over all rays, [1] = 1 if any ray
+ *  needed more than n_max_hits entries (those rays are truncated).          */
+int sailor_octree_voxel_traversal(
+    const int32_t* vol_coarse, const float* node_pos, const int32_t* node_info,
+    int n_nodes, const float* vol_bbox, const float* voxel_size, int B, int dim_fine,
+    int dim_coarse, int rate, const float* ray_ori, const float* ray_dir, int R,
+    int n_max_hits, int th_early_stop, int is_training,
+    int opt_num_intersected, int32_t* hit_idx, float* tmin, float* tmax,
+    float* valid_len, int32_t* status, void* stream);
+
+/* Replaces MultiLayerOctree::ray_voxels_points_sampling_coarse
+ * (sampling_kernel.cu:1251-1304; kernel :1070-1177).
+ *  noise f32 [B,R,S]; outputs sampled_idx int32 [B,R,S], sampled_t f32 [B,R,S]
+ *  (-1 for unused slots).  start_scale = rate^(levels-start_level-1).      */
+int sailor_octree_sampling_coarse(const int32_t* hit_idx, const float* tmin,
+                                  const float* tmax, const float* noise,
+                                  const float* valid_len, const float* node_pos,
+                                  const float* voxel_size, int B, int R,
+                                  int n_max_hits, int S, int start_scale,
+                                  int32_t* sampled_idx, float* sampled_t,
+                                  void* stream);
+
+/* Replaces MultiLayerOctree::sort_samplings (sampling_kernel.cu:1308-1329):
+ * ascending sort of each row of S values; ties keep their original order.
+ *  t f32 [n_rows,S] -> sorted f32 [n_rows,S], order int32 [n_rows,S]       */
+int sailor_sort_samplings(const float* t, int n_rows, int S, float* sorted,
+                          int32_t* order, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Rays                                                                      */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces rays_calculating (sampling_kernel.cu:674-733) and
+ * rays_calculating_parallel (:609-670, row interleave h*num_gpus+device).
+ *  K,R f32 [B,V,3,3], T f32 [B,V,3] -> out f32 [B,V,H,W,6] (origin, dir),
+ *  image row of output row h = h*row_stride + row_offset.                   */
+int sailor_rays_calculating(const float* K, const float* R, const float* T,
+                            int B, int V, int H, int W, int row_stride,
+                            int row_offset, float* out, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Volume compositing                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces volume_rendering_occ_forward (volume_rendering.cu:749-787; kernel
+ * :666-745), generalised from 3 to F feature channels.
+ *  t f32 [B,R,S] (sorted), occ f32 [B,R,S], feats f32 [B,R,S,F] ->
+ *  out_feats [B,R,F], out_depth/out_ws/out_alpha [B,R].                     */
+int sailor_volume_rendering_occ_forward(const float* t, const float* occ,
+                                        const float* feats, int B, int R, int S,
+                                        int F, float t_thresh, float* out_feats,
+                                        float* out_depth, float* out_ws,
+                                        float* out_alpha, void* stream);
+
+/* Replaces volume_rendering_training_forward (volume_rendering.cu:525-583;
+ * kernels :17-126 density form, :129-240 occupancy form).                   */
+int sailor_volume_rendering_training_forward(
+    const float* t, const int32_t* sort_idx, const float* sigma,
+    const float* feats, int B, int R, int S, int F, float t_thresh,
+    int support_occ, float* out_feats, float* out_depth, float* out_ws,
+    float* out_alpha, void* stream);
+
+/* Replaces volume_rendering_training_backward (volume_rendering.cu:585-663;
+ * kernels :244-381, :384-522).  grad_feats [B,R,S,F], grad_sigma [B,R,S]
+ * are fully overwritten (zero where the reference leaves zero).             */
+int sailor_volume_rendering_training_backward(
+    const float* g_feats, const float* g_depth, const float* g_ws,
+    const float* g_alpha, const float* out_feats, const float* out_depth,
+    const float* out_ws, const float* out_alpha, const float* t,
+    const int32_t* sort_idx, const float* sigma, const float* feats, int B,
+    int R, int S, int F, float t_thresh, int support_occ, float* grad_feats,
+    float* grad_sigma, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Frequency encoding                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces freq_encode_forward/backward (freq_encoding.cu:89-151).
+ *  x f32 [N,D] -> y f32 [N, D + 2*D*deg] = [x, sin(2^f x), sin(2^f x + pi/2) ...] */
+int sailor_freq_encode_forward(const float* x, int N, int D, int deg, float* y,
+                               void* stream);
+int sailor_freq_encode_backward(const float* grad_y, const float* y, int N,
+                                int D, int deg, float* grad_x, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SAILOR_HIP_H */

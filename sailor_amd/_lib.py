@@ -1,0 +1,109 @@
+"""ctypes binding of libsailor_hip.so (the C ABI declared in include/sailor_hip.h).
+
+The library is loaded from this package directory; if it is missing the import
+fails loudly -- there is no CPU or PyTorch fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsailor_hip.so")
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+F = ctypes.c_float
+SZ = ctypes.c_size_t
+
+# name -> argtypes (restype int unless listed in _RESTYPES)
+_SIGS = {
+    "sailor_last_error": [],
+    "sailor_abi_version": [],
+    "sailor_integrate": [P] * 10 + [I] * 7 + [F] * 5 + [P] * 7,
+    "sailor_get_origin_bbox": [P] * 4 + [I] * 4 + [F, P, P],
+    "sailor_build_multi_res_volumes": [P, P, I, I, I, I, P],
+    "sailor_octree_workspace_bytes": [I, I],
+    "sailor_octree_build": [P, P, P, P, I, I, I, P, P, I, I, P, P, P, P, SZ, P],
+    "sailor_octree_nodes_tensor": [P, P, I, P, P],
+    "sailor_octree_voxel_traversal": [P, P, P, I, P, P] + [I] * 4 + [P, P] + [I] * 5 + [P] * 5 + [P],
+    "sailor_octree_sampling_coarse": [P] * 7 + [I] * 5 + [P, P, P],
+    "sailor_sort_samplings": [P, I, I, P, P, P],
+    "sailor_rays_calculating": [P, P, P] + [I] * 6 + [P, P],
+    "sailor_volume_rendering_occ_forward": [P, P, P, I, I, I, I, F, P, P, P, P, P],
+    "sailor_volume_rendering_training_forward": [P, P, P, P, I, I, I, I, F, I, P, P, P, P, P],
+    "sailor_volume_rendering_training_backward": [P] * 12 + [I, I, I, I, F, I, P, P, P],
+    "sailor_freq_encode_forward": [P, I, I, I, P, P],
+    "sailor_freq_encode_backward": [P, P, I, I, I, P, P],
+}
+_RESTYPES = {
+    "sailor_last_error": ctypes.c_char_p,
+    "sailor_octree_workspace_bytes": SZ,
+}
+
+_lib = None
+
+
+class SailorError(RuntimeError):
+    """Raised when a C-ABI entry point returns a non-zero status."""
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m sailor_amd.build` "
+                "(there is no fallback implementation)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPES.get(name, ctypes.c_int)
+        _lib = L
+    return _lib
+
+
+def call(name, *args):
+    """Calls entry point `name`, raising SailorError on a non-zero status."""
+    rc = getattr(lib(), name)(*args)
+    if rc != 0:
+        msg = lib().sailor_last_error().decode(errors="replace")
+        raise SailorError(f"{name} failed (status {rc}): {msg}")
+
+
+def ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def stream_of(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def check_input(t, name, dtype=None, device=None):
+    """TORCH_CHECK-style validation (cuda_helper.h:23-25): CUDA + contiguous."""
+    if not isinstance(t, torch.Tensor):
+        raise RuntimeError(f"{name} input must be a tensor")
+    if not t.is_cuda:
+        raise RuntimeError(f"{name} input must be a cuda tensor")
+    if not t.is_contiguous():
+        raise RuntimeError(f"{name} input must be contiguous.")
+    if dtype is not None and t.dtype != dtype:
+        raise RuntimeError(f"{name} must have dtype {dtype}, got {t.dtype}")
+    if device is not None and t.device != device:
+        raise RuntimeError(f"{name} is on {t.device}, expected {device}")
+    return t
+
+
+def as_input(t, name, dtype, device=None):
+    """Read-only input: cast/contiguous copies are allowed (the reference calls
+    .contiguous().data_ptr() on its inputs)."""
+    if not isinstance(t, torch.Tensor):
+        raise RuntimeError(f"{name} input must be a tensor")
+    if not t.is_cuda:
+        raise RuntimeError(f"{name} input must be a cuda tensor")
+    if device is not None and t.device != device:
+        raise RuntimeError(f"{name} is on {t.device}, expected {device}")
+    return t.to(dtype).contiguous()

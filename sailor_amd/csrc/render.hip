@@ -1,0 +1,272 @@
+// Front-to-back volume compositing (forward) and its reverse-scan gradient.
+//
+// Follows c_lib/VoxelEncoding/src/volume_rendering.cu:
+//  * volume_rendering_occ_kernel (:666-745): inference, alpha = occupancy,
+//    samples already sorted;
+//  * volume_rendering_training_forward_kernel (:17-126, alpha = 1-exp(-s*d))
+//    and volume_rendering_occ_training_forward_kernel (:129-240);
+//  * the two backward kernels (:244-381, :384-522), which re-walk the prefix
+//    and use the stored totals for the suffix sums.
+// The reference hard-codes 3 feature channels (FEATS_DIM, :13); here the
+// channel count F is a runtime argument.  exp uses the IEEE-accurate expf,
+// not __expf.
+//
+// One lane per ray with the samples walked in order keeps the reference's
+// exact floating-point recurrence (T *= 1 - alpha, running sums); the
+// per-ray rows are contiguous so consecutive lanes stream consecutive rows.
+#include "common.h"
+
+namespace {
+
+constexpr int kThreads = 128;
+constexpr int kMaxF = 64;  // channels held in registers; larger F loops twice
+
+// Inference compositing over sorted samples.
+template <int FMAX>
+__global__ void __launch_bounds__(kThreads)
+occ_forward_kernel(const float* __restrict__ t, const float* __restrict__ occ,
+                   const float* __restrict__ feats, long long n_rays, int S, int F,
+                   int f0, float t_thres, float* __restrict__ out_feats,
+                   float* __restrict__ out_depth, float* __restrict__ out_ws,
+                   float* __restrict__ out_alpha) {
+  const long long ray = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ray >= n_rays) return;
+  const float* tr = t + ray * S;
+  const float* orow = occ + ray * S;
+  const float* frow = feats + ray * S * F;
+  const int nf = min(FMAX, F - f0);
+  float acc[FMAX];
+#pragma unroll
+  for (int j = 0; j < FMAX; ++j) acc[j] = 0.0f;
+  float T = 1.0f, ws = 0.0f, d = 0.0f, alphas = 0.0f;
+  if (!(tr[S - 1] < 0)) {
+    for (int i = 0; i < S; ++i) {
+      const float z = tr[i];
+      if (z < 0) continue;
+      const float alpha = orow[i];
+      const float w = alpha * T;
+#pragma unroll
+      for (int j = 0; j < FMAX; ++j)
+        if (j < nf) acc[j] += w * frow[(size_t)i * F + f0 + j];
+      d += w * z;
+      ws += w;
+      alphas += w * alpha;
+      T *= 1.0f - alpha;
+      if (T < t_thres) break;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < FMAX; ++j)
+    if (j < nf) out_feats[ray * F + f0 + j] = acc[j];
+  if (f0 == 0) {
+    out_depth[ray] = d;
+    out_ws[ray] = ws;
+    out_alpha[ray] = alphas;
+  }
+}
+
+// Next valid depth after sorted position i (reference :87-97): the last value
+// probed is used even if it is not positive, as in the reference loop.
+__device__ __forceinline__ float next_depth(const float* tr, const int* idx, int i, int S) {
+  float nxt = 0.0f;
+  for (int m = i + 1; m < S; ++m) {
+    const int k = idx[m];
+    nxt = (k >= 0 && k < S) ? tr[k] : -1.0f;
+    if (nxt > 0) break;
+  }
+  return nxt;
+}
+
+template <int FMAX>
+__global__ void __launch_bounds__(kThreads)
+train_forward_kernel(const float* __restrict__ t, const int* __restrict__ sidx,
+                     const float* __restrict__ sigma, const float* __restrict__ feats,
+                     long long n_rays, int S, int F, int f0, float t_thres, int occ_mode,
+                     float* __restrict__ out_feats, float* __restrict__ out_depth,
+                     float* __restrict__ out_ws, float* __restrict__ out_alpha) {
+  const long long ray = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ray >= n_rays) return;
+  const float* tr = t + ray * S;
+  const int* ir = sidx + ray * S;
+  const float* sr = sigma + ray * S;
+  const float* frow = feats + ray * S * F;
+  const int nf = min(FMAX, F - f0);
+  float acc[FMAX];
+#pragma unroll
+  for (int j = 0; j < FMAX; ++j) acc[j] = 0.0f;
+  float T = 1.0f, ws = 0.0f, d = 0.0f, alphas = 0.0f;
+  const int last = ir[S - 1];
+  if (last >= 0 && last < S && !(tr[last] < 0)) {
+    for (int i = 0; i < S; ++i) {
+      const int si = ir[i];
+      if (si < 0 || si >= S) continue;  // malformed permutation entry
+      const float z = tr[si];
+      if (z < 0) continue;
+      float alpha;
+      if (occ_mode) {
+        alpha = sr[si];
+      } else {
+        const float delta = (i < S - 1) ? next_depth(tr, ir, i, S) - z : 1e10f;
+        alpha = 1.0f - expf(-sr[si] * delta);
+      }
+      const float w = alpha * T;
+#pragma unroll
+      for (int j = 0; j < FMAX; ++j)
+        if (j < nf) acc[j] += w * frow[(size_t)si * F + f0 + j];
+      d += w * z;
+      ws += w;
+      alphas += w * alpha;
+      T *= 1.0f - alpha;
+      if (T < t_thres) break;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < FMAX; ++j)
+    if (j < nf) out_feats[ray * F + f0 + j] = acc[j];
+  if (f0 == 0) {
+    out_depth[ray] = d;
+    out_ws[ray] = ws;
+    out_alpha[ray] = alphas;
+  }
+}
+
+// Gradient of the compositing w.r.t. per-sample features and sigma/occupancy.
+// The grad outputs are zero-initialised by the caller-side fill kernel; only
+// samples before early termination receive a gradient (as in the reference,
+// the sample that ends the walk is skipped, :346).
+__global__ void __launch_bounds__(kThreads)
+train_backward_kernel(const float* __restrict__ g_feats, const float* __restrict__ g_depth,
+                      const float* __restrict__ g_ws, const float* __restrict__ g_alpha,
+                      const float* __restrict__ o_feats, const float* __restrict__ o_depth,
+                      const float* __restrict__ o_ws, const float* __restrict__ o_alpha,
+                      const float* __restrict__ t, const int* __restrict__ sidx,
+                      const float* __restrict__ sigma, const float* __restrict__ feats,
+                      long long n_rays, int S, int F, float t_thres, int occ_mode,
+                      float* __restrict__ grad_feats, float* __restrict__ grad_sigma) {
+  const long long ray = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ray >= n_rays) return;
+  const float* tr = t + ray * S;
+  const int* ir = sidx + ray * S;
+  const int last = ir[S - 1];
+  if (last < 0 || last >= S || tr[last] < 0) return;
+  const float* sr = sigma + ray * S;
+  const float* frow = feats + ray * S * F;
+  const float* gf = g_feats + ray * F;
+  const float* of = o_feats + ray * F;
+  float* gfo = grad_feats + ray * S * F;
+  float* gso = grad_sigma + ray * S;
+  const float gd = g_depth[ray], gw = g_ws[ray], ga = g_alpha[ray];
+  const float d_final = o_depth[ray], ws_final = o_ws[ray], a_final = o_alpha[ray];
+  // running prefix of the features (recomputed alongside the walk)
+  float pf[kMaxF];
+  for (int j = 0; j < kMaxF; ++j) pf[j] = 0.0f;
+  float T = 1.0f, ws = 0.0f, d = 0.0f, alphas = 0.0f;
+  for (int i = 0; i < S; ++i) {
+    const int si = ir[i];
+    if (si < 0 || si >= S) continue;  // malformed permutation entry
+    const float z = tr[si];
+    if (z < 0) continue;
+    float alpha, delta = 0.0f;
+    if (occ_mode) {
+      alpha = sr[si];
+    } else {
+      delta = (i < S - 1) ? next_depth(tr, ir, i, S) - z : 1e10f;
+      alpha = 1.0f - expf(-sr[si] * delta);
+    }
+    const float w = alpha * T;
+    const float* fs = frow + (size_t)si * F;
+    for (int j = 0; j < F && j < kMaxF; ++j) pf[j] += w * fs[j];
+    d += w * z;
+    ws += w;
+    alphas += w * alpha;
+    T *= 1.0f - alpha;
+    if (T < t_thres) break;
+    for (int j = 0; j < F; ++j) gfo[(size_t)si * F + j] = gf[j] * w;
+    float gsf = 0.0f;
+    for (int j = 0; j < F && j < kMaxF; ++j) gsf += gf[j] * (T * fs[j] - (of[j] - pf[j]));
+    float g;
+    if (occ_mode) {
+      // clipped 1/(1-alpha) (:497); the test is evaluated in double there
+      const float tmp = 1.0 / ((double)(1.0f - alpha) + 1e-7) <= 15.0
+                            ? 1.0f / (1.0f - alpha)
+                            : 15.0f;
+      const float g_ws_ = gw * (T * 1.0f - (ws_final - ws)) * tmp;
+      const float g_d = gd * (T * z - (d_final - d)) * tmp;
+      const float g_a = ga * (2.0f * T * alpha - (a_final - alphas)) * tmp;
+      g = g_ws_ + g_d + gsf * tmp + g_a;
+    } else {
+      const float tmp = gw * (1.0f - ws_final);
+      const float g_ws_ = delta * (gw * (T * 1.0f - (ws_final - ws)) + tmp);
+      const float g_d = delta * (gd * (T * z - (d_final - d)) + tmp);
+      const float g_a = delta * (ga * (2.0f * T * alpha - (a_final - alphas)) + tmp);
+      const float g_f = delta * (gsf + tmp);
+      g = g_ws_ + g_d + g_f + g_a;
+    }
+    gso[si] = g;
+  }
+}
+
+__global__ void zero2(float* __restrict__ a, long long na, float* __restrict__ b,
+                      long long nb) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < na + nb;
+       i += (long long)gridDim.x * blockDim.x) {
+    if (i < na) a[i] = 0.0f;
+    else b[i - na] = 0.0f;
+  }
+}
+
+}  // namespace
+
+extern "C" int sailor_volume_rendering_occ_forward(const float* t, const float* occ,
+                                                   const float* feats, int B, int R, int S,
+                                                   int F, float t_thresh, float* out_feats,
+                                                   float* out_depth, float* out_ws,
+                                                   float* out_alpha, void* stream) {
+  SAILOR_REQUIRE(B > 0 && R >= 0 && S > 0 && F > 0, "volume_rendering_occ_forward: bad sizes");
+  const long long n = (long long)B * R;
+  if (n == 0) return SAILOR_OK;
+  hipStream_t st = sailor::as_stream(stream);
+  for (int f0 = 0; f0 < F; f0 += 8) {
+    hipLaunchKernelGGL(occ_forward_kernel<8>, dim3(sailor::ceil_div(n, kThreads)),
+                       dim3(kThreads), 0, st, t, occ, feats, n, S, F, f0, t_thresh,
+                       out_feats, out_depth, out_ws, out_alpha);
+  }
+  return sailor::check_launch("volume_rendering_occ_forward");
+}
+
+extern "C" int sailor_volume_rendering_training_forward(
+    const float* t, const int32_t* sort_idx, const float* sigma, const float* feats,
+    int B, int R, int S, int F, float t_thresh, int support_occ, float* out_feats,
+    float* out_depth, float* out_ws, float* out_alpha, void* stream) {
+  SAILOR_REQUIRE(B > 0 && R >= 0 && S > 0 && F > 0, "volume_rendering_training_forward: bad sizes");
+  const long long n = (long long)B * R;
+  if (n == 0) return SAILOR_OK;
+  hipStream_t st = sailor::as_stream(stream);
+  for (int f0 = 0; f0 < F; f0 += 8) {
+    hipLaunchKernelGGL(train_forward_kernel<8>, dim3(sailor::ceil_div(n, kThreads)),
+                       dim3(kThreads), 0, st, t, sort_idx, sigma, feats, n, S, F, f0,
+                       t_thresh, support_occ, out_feats, out_depth, out_ws, out_alpha);
+  }
+  return sailor::check_launch("volume_rendering_training_forward");
+}
+
+extern "C" int sailor_volume_rendering_training_backward(
+    const float* g_feats, const float* g_depth, const float* g_ws, const float* g_alpha,
+    const float* out_feats, const float* out_depth, const float* out_ws,
+    const float* out_alpha, const float* t, const int32_t* sort_idx, const float* sigma,
+    const float* feats, int B, int R, int S, int F, float t_thresh, int support_occ,
+    float* grad_feats, float* grad_sigma, void* stream) {
+  SAILOR_REQUIRE(B > 0 && R >= 0 && S > 0 && F > 0, "volume_rendering_training_backward: bad sizes");
+  SAILOR_REQUIRE(F <= kMaxF, "volume_rendering_training_backward: at most %d channels", kMaxF);
+  const long long n = (long long)B * R;
+  if (n == 0) return SAILOR_OK;
+  hipStream_t st = sailor::as_stream(stream);
+  const long long na = n * S * F, nb = n * S;
+  int zb = (int)std::min<long long>(4096, (na + nb + 255) / 256);
+  hipLaunchKernelGGL(zero2, dim3(zb), dim3(256), 0, st, grad_feats, na, grad_sigma, nb);
+  hipLaunchKernelGGL(train_backward_kernel, dim3(sailor::ceil_div(n, kThreads)),
+                     dim3(kThreads), 0, st, g_feats, g_depth, g_ws, g_alpha, out_feats,
+                     out_depth, out_ws, out_alpha, t, sort_idx, sigma, feats, n, S, F,
+                     t_thresh, support_occ, grad_feats, grad_sigma);
+  return sailor::check_launch("volume_rendering_training_backward");
+}

@@ -1,0 +1,244 @@
+"""Drop-in for c_lib/VoxelEncoding/voxel_encoding_helper.py: the autograd
+Functions and stateful helpers that SRONet / SRONetUp call.
+
+  integrate_depths   _integrate_depths (:206-294)
+  OcTreeOptions      (:535-662)
+  volume_render_infer  _volume_rendering_inference (:62-80)
+  volume_render        _volume_render_training (:83-127) -- forward and
+                       backward run the HIP compositing kernels
+  volume_render_torch  _volume_render_torch (:130-189) -- the reference's own
+                       PyTorch formulation, kept for training-time parity
+"""
+from __future__ import annotations
+
+import torch
+import torch.nn as nn
+from torch.autograd import Function
+
+from . import VoxelEncoding
+
+MAX_VOL_DIST = 999999
+
+
+def _gpu_id(t):
+    return int(t.device.index or 0)
+
+
+class _volume_rendering_inference(Function):
+    @staticmethod
+    def forward(ctx, depths, sigmas, feats, t_thres=1e-4):
+        return VoxelEncoding.volume_rendering_occ_forward(
+            depths.float(), sigmas.float(), feats.float(), _gpu_id(depths), t_thres)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        return None, None, None, None
+
+
+volume_render_infer = _volume_rendering_inference.apply
+
+
+class _volume_render_training(Function):
+    @staticmethod
+    def forward(ctx, depths, depth_sort_idxs, sigmas, feats, t_thres=1e-4, using_occ=True):
+        depths, sigmas, feats = depths.float(), sigmas.float(), feats.float()
+        ctx.t_thres = t_thres
+        ctx.using_occ = using_occ
+        out = VoxelEncoding.volume_rendering_training_forward(
+            depths, depth_sort_idxs, sigmas, feats, _gpu_id(depths), t_thres, using_occ)
+        ctx.save_for_backward(depths, depth_sort_idxs, sigmas, feats, *out)
+        return out
+
+    @staticmethod
+    def backward(ctx, g_feats, g_depths, g_ws, g_alphas):
+        depths, sort_idx, sigmas, feats, o_feats, o_depths, o_ws, o_alphas = ctx.saved_tensors
+        zeros = lambda x, g: torch.zeros_like(x) if g is None else g
+        grad_feats, grad_sigmas = VoxelEncoding.volume_rendering_training_backward(
+            zeros(o_feats, g_feats), zeros(o_depths, g_depths), zeros(o_ws, g_ws),
+            zeros(o_alphas, g_alphas), o_feats, o_depths, o_ws, o_alphas, depths, sort_idx,
+            sigmas, feats, _gpu_id(depths), ctx.t_thres, ctx.using_occ)
+        grad_sigmas = grad_sigmas.view_as(sigmas) if ctx.needs_input_grad[2] else None
+        grad_feats = grad_feats if ctx.needs_input_grad[3] else None
+        return None, None, grad_sigmas, grad_feats, None, None
+
+
+volume_render = _volume_render_training.apply
+
+
+class _volume_render_torch(nn.Module):
+    """The reference's PyTorch compositing (autograd through torch ops)."""
+
+    def forward(self, sampled_depths, sigma, feats, max_depths, valid_rays=None):
+        batch_size, num_rays, num_sampled = sampled_depths.shape
+        _sd = sampled_depths.view(-1, num_sampled)
+        density = sigma.view(-1, num_sampled)
+        dists = _sd[:, 1:] - _sd[:, :-1]
+        dists = torch.cat([dists, torch.full([dists.shape[0], 1], 0.0, device=dists.device)], -1)
+        free_energy = dists * density
+        shifted = torch.cat([torch.full([dists.shape[0], 1], 0.0, device=dists.device),
+                             free_energy], dim=-1)
+        alpha = 1 - torch.exp(-free_energy)
+        transmittance = torch.exp(-torch.cumsum(shifted, dim=-1))
+        weights = (alpha * transmittance[:, :-1]).view(batch_size, num_rays, num_sampled, 1)
+        depths = torch.sum(weights * sampled_depths[..., None], -2)
+        alphas = torch.sum(weights * alpha.view(batch_size, num_rays, num_sampled, 1), -2)
+        feats = torch.sum(weights * feats, -2)
+        ws = torch.sum(weights, -2)
+        if valid_rays is not None:
+            inv = ~valid_rays[..., None].bool()
+            depths = depths.masked_fill(inv, 0)
+            feats = feats.masked_fill(inv, 0)
+            alphas = alphas.masked_fill(inv, 0)
+            ws = ws.masked_fill(inv, 0)
+        return feats, depths, ws, alphas
+
+    def forward_occ(self, sampled_depths, sigma, feats, max_depths, valid_rays=None):
+        batch_size, num_rays, num_sampled = sampled_depths.shape
+        alpha = sigma.view(-1, num_sampled)
+        eps = 1e-6
+        trans = torch.cumprod(torch.cat([torch.ones((alpha.shape[0], 1), device=alpha.device),
+                                         1. - alpha + eps], -1), -1)[:, :-1]
+        weights = (alpha * trans).view(batch_size, num_rays, num_sampled, 1)
+        depths = torch.sum(weights * sampled_depths[..., None], -2)
+        alphas = torch.sum(weights * alpha.view(batch_size, num_rays, num_sampled, 1), -2)
+        feats = torch.sum(weights * feats, -2)
+        ws = torch.sum(weights, -2)
+        if valid_rays is not None:
+            inv = ~valid_rays[..., None].bool()
+            depths = depths.masked_fill(inv, 0)
+            feats = feats.masked_fill(inv, 0)
+            alphas = alphas.masked_fill(inv, 0)
+            ws = ws.masked_fill(inv, 0)
+        return feats, depths, ws, alphas
+
+
+volume_render_torch = _volume_render_torch()
+
+
+class _integrate_depths(nn.Module):
+    """TSDF integration + multi-resolution volumes (voxel_encoding_helper.py:206-294)."""
+
+    def preprocess_data(self, batch_size, inputs_data):
+        colors, depths, masks, Ks, RTs = inputs_data
+        h, w = depths.shape[-2:]
+        num_views = depths.shape[0] // batch_size
+        colors = torch.floor(colors[:, 2] * 256.0 * 256.0 + colors[:, 1] * 256.0 + colors[:, 0])
+        if len(depths.shape) == 4:
+            depths = depths[:, 0]
+        if len(masks.shape) == 4:
+            masks = masks[:, 0]
+        depths = depths.view(-1, num_views, h, w).contiguous()
+        colors = colors.view(-1, num_views, h, w).contiguous()
+        masks = masks.view(-1, num_views, h, w).contiguous().int()
+        Ks = Ks.view(-1, num_views, 3, 3).contiguous()
+        RTs = RTs.view(-1, num_views, 3, 4).contiguous()
+        Rs = RTs[..., :3, :3].contiguous()
+        Ts = RTs[..., :3, -1].contiguous()
+        return colors, depths, masks, Ks, RTs, Rs, Ts
+
+    def clear_vol_buf(self, volumes, _level_volumes):
+        occ, num = volumes
+        for i in range(_level_volumes):
+            occ[i].fill_(-1)
+            num[i].fill_(0)
+        return occ, num
+
+    @torch.no_grad()
+    def get_origin_bbox(self, inputs_data, batch_size=1, _th_bbox=0.025):
+        _, depths, _, Ks, _, Rs, Ts = self.preprocess_data(batch_size, inputs_data)
+        return VoxelEncoding.get_origin_bbox(depths, Ks, Rs, Ts, _th_bbox, _gpu_id(depths))
+
+    @torch.no_grad()
+    def forward(self, inputs_data, volumes, batch_size=1, _obs_weight=1.0, _th_bbox=0.025,
+                _tsdf_th_low=8.0, _tsdf_th_high=8.0, _level_volumes=6, _volume_res_rate=2,
+                build_octree=True, fixed_bbox=None):
+        colors, depths, masks, Ks, RTs, Rs, Ts = self.preprocess_data(batch_size, inputs_data)
+        occ, num = self.clear_vol_buf(volumes, _level_volumes)
+        gid = _gpu_id(colors)
+        tsdf_vol, _, color_vol, vol_origin, vol_bbox, voxel_size = VoxelEncoding.integrate(
+            occ[0], num[0], Ks, RTs, Rs, Ts, colors, depths, masks, _obs_weight, _th_bbox,
+            3.0, _tsdf_th_low, _tsdf_th_high, gid, fixed_bbox=fixed_bbox)
+        if build_octree:
+            VoxelEncoding.build_multi_res_volumes(occ, num, _level_volumes, _volume_res_rate, gid)
+        return occ, num, tsdf_vol, color_vol, vol_origin, vol_bbox, voxel_size
+
+
+integrate_depths = _integrate_depths()
+
+
+class OcTreeOptions(nn.Module):
+    """voxel_encoding_helper.py:535-662."""
+
+    def __init__(self, levels=6, batch_size=1, rate=2, vol_dim=256, n_max_hits=512,
+                 device="cuda:0"):
+        super().__init__()
+        self._levels, self._batch_size, self._rate = levels, batch_size, rate
+        self._gpu_id = int(str(device).split(":")[-1])
+        self._octree = VoxelEncoding.MultiLayerOctree(levels, batch_size, rate, self._gpu_id)
+        self._vol_dim = (vol_dim, vol_dim, vol_dim)
+        self._n_max_hits = n_max_hits
+        self._n_max_hits_coarse = None
+
+    @torch.no_grad()
+    def _init_octree(self, _occupied_volumes, _num_occ_voxels, _voxel_size, _vol_bbox):
+        self._vol_dim = _occupied_volumes[0].shape[-3:]
+        assert len(_occupied_volumes) == self._levels and len(_num_occ_voxels) == self._levels
+        total, index = [], []
+        for i, nv in enumerate(_num_occ_voxels):
+            total.append(int(torch.sum(nv)))
+            index.append(sum(total[:i]))
+        self._octree.init_octree(_occupied_volumes, total, _num_occ_voxels, index, _voxel_size,
+                                 _vol_bbox)
+        self._octree.build_octree()
+        return total, index
+
+    @torch.no_grad()
+    def _ray_intersection(self, _ray_ori, _ray_dir, _instersect_start_level=1, th_early_stop=-1,
+                          only_voxel_traversal=True, is_training=False,
+                          opt_num_intersected_voxels=-1):
+        if not only_voxel_traversal:
+            raise NotImplementedError("octree_traversal (recursive path) is not provided")
+        self._n_max_hits_coarse = 1024
+        return self._octree.voxel_traversal(_ray_ori, _ray_dir, self._n_max_hits_coarse,
+                                            _instersect_start_level, th_early_stop, is_training,
+                                            opt_num_intersected_voxels)
+
+    @torch.no_grad()
+    def _sort_sampling(self, sampled_depths):
+        return self._octree.sort_samplings(sampled_depths)
+
+    @torch.no_grad()
+    def _ray_voxels_points_sampling_coarse(self, _idx, _min_depths, _max_depths, _uniform_noise,
+                                           _ray_total_valid_len, _instersect_start_level,
+                                           _n_sampled_points):
+        return self._octree.ray_voxels_points_sampling_coarse(
+            _idx, _min_depths, _max_depths, _uniform_noise, _ray_total_valid_len,
+            _instersect_start_level, _n_sampled_points)
+
+    def init_octree(self, occupied_volumes, num_occ_voxels, voxel_size, vol_bbox):
+        return self._init_octree(occupied_volumes, num_occ_voxels, voxel_size, vol_bbox)
+
+    def ray_intersection(self, ray_ori, ray_dir, instersect_start_level=1, th_early_stop=-1,
+                         only_voxel_traversal=True, is_training=False,
+                         opt_num_intersected_voxels=-1):
+        return self._ray_intersection(ray_ori, ray_dir, instersect_start_level, th_early_stop,
+                                      only_voxel_traversal, is_training,
+                                      opt_num_intersected_voxels)
+
+    def ray_voxels_points_sampling_coarse(self, idx, min_depths, max_depths, uniform_noise,
+                                          ray_total_valid_len, instersect_start_level=1,
+                                          n_sampled_points=128):
+        return self._ray_voxels_points_sampling_coarse(idx, min_depths, max_depths,
+                                                       uniform_noise, ray_total_valid_len,
+                                                       instersect_start_level, n_sampled_points)
+
+    def sort_sampling(self, _sampled_depths):
+        return self._sort_sampling(_sampled_depths)
+
+    @torch.no_grad()
+    def get_all_nodes(self):
+        return self._octree.get_nodes_tensor()
+
+    @torch.no_grad()
+    def get_octree(self):
+        return self._octree
