@@ -22,7 +22,7 @@ SZ = ctypes.c_size_t
 _SIGS = {
     "sailor_last_error": [],
     "sailor_abi_version": [],
-    "sailor_integrate": [P] * 10 + [I] * 7 + [F] * 5 + [P] * 7,
+    "sailor_integrate": [P] * 9 + [I] * 7 + [F] * 5 + [P] * 8,
     "sailor_get_origin_bbox": [P] * 4 + [I] * 4 + [F, P, P],
     "sailor_build_multi_res_volumes": [P, P, I, I, I, I, P],
     "sailor_octree_workspace_bytes": [I, I],

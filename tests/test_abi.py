@@ -55,3 +55,43 @@ def test_argument_errors_are_reported_not_fatal():
     with pytest.raises(_lib.SailorError, match="not a multiple"):
         _lib.call("sailor_octree_build", None, None, None, None, 1, 30, 8, None, None, 0, 1,
                   None, None, None, None, 0, None)
+
+
+def _header_params():
+    """name -> list of parameter kinds ('P', 'I', 'F', 'SZ') parsed from include/*.h."""
+    out = {}
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if not fn.endswith(".h"):
+            continue
+        src = open(os.path.join(ROOT, "include", fn)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        for m in re.finditer(r"\b(sailor_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", src):
+            kinds = []
+            params = [p.strip() for p in m.group(2).split(",") if p.strip()]
+            if params == ["void"]:
+                params = []
+            for p in params:
+                if "*" in p:
+                    kinds.append("P")
+                elif p.startswith("size_t"):
+                    kinds.append("SZ")
+                elif p.startswith("float"):
+                    kinds.append("F")
+                elif p.startswith("int"):
+                    kinds.append("I")
+                else:
+                    raise AssertionError(f"unclassified parameter {p!r} of {m.group(1)}")
+            out[m.group(1)] = kinds
+    return out
+
+
+def test_ctypes_signatures_match_header():
+    from sailor_amd import _lib
+    kind = {_lib.P: "P", _lib.I: "I", _lib.F: "F", _lib.SZ: "SZ"}
+    hdr = _header_params()
+    bad = []
+    for name, params in hdr.items():
+        got = [kind[a] for a in _lib._SIGS[name]]
+        if got != params:
+            bad.append(f"{name}: header {params} vs ctypes {got}")
+    assert not bad, "\n".join(bad)

@@ -73,8 +73,10 @@ def test_freq_encoding():
     x = rng.uniform(-1, 1, (4097, 3)).astype(np.float32)
     y = VE.freq_encode_forward(torch.from_numpy(x).cuda(), 10, 0).cpu().numpy()
     y64 = orc.freq_f64(x, 10)
-    # column order of the reference: [x, sin f0, cos f0, sin f1, ...]
-    np.testing.assert_allclose(y, y64, atol=2e-5)
+    # column order of the reference: [x, sin f0, cos f0, sin f1, ...]; the
+    # float32 argument 2^f*x (f <= 9, |x| <= 1) carries up to 2^9 * 2^-24 * 2 of
+    # rounding error, so the float64 closed form agrees to ~6e-5
+    np.testing.assert_allclose(y, y64, atol=1e-4)
     np.testing.assert_allclose(y, orc.freq_fwd(x, 10), atol=2e-6)
     gy = rng.standard_normal(y.shape).astype(np.float32)
     gx = VE.freq_encode_backward(torch.from_numpy(gy).cuda(), torch.from_numpy(x).cuda(),
