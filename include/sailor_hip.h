@@ -154,6 +154,25 @@ int sailor_octree_sampling_coarse(const int32_t* hit_idx, const float* tmin,
 int sailor_sort_samplings(const float* t, int n_rows, int S, float* sorted,
                           int32_t* order, void* stream);
 
+/* Fused form of the chain rays_calculating (sampling_kernel.cu:674-733) ->
+ * MultiLayerOctree::voxel_traversal (intersection_kernel.cu:827-895, inference
+ * mode, no early stop) -> ray_voxels_points_sampling_coarse
+ * (sampling_kernel.cu:1251-1304) for one target view per batch, without the
+ * [B,R,1024] hit lists: the DDA is walked again instead of storing hits.
+ * Samples are bit-identical to the unfused chain.
+ *  K, R f32 [B,3,3], T f32 [B,3]: target camera per batch; image H x W.
+ *  noise f32 [B,H*W,S]; outputs sampled_idx int32 [B,H*W,S] and
+ *  sampled_t f32 [B,H*W,S] (-1 padded, unsorted, as the reference returns
+ *  them); rays_out f32 [B,H*W,6] and valid_len f32 [B,H*W] may be NULL.    */
+int sailor_trace_sample(const float* K, const float* R, const float* T, int B,
+                        int H, int W, const int32_t* vol_coarse,
+                        const float* node_pos, const int32_t* node_info,
+                        int n_nodes, const float* vol_bbox,
+                        const float* voxel_size, int dim_fine, int dim_coarse,
+                        int start_scale, int S, const float* noise,
+                        float* rays_out, int32_t* sampled_idx, float* sampled_t,
+                        float* valid_len, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Rays                                                                      */
 /* ------------------------------------------------------------------------ */
@@ -198,6 +217,45 @@ int sailor_volume_rendering_training_backward(
     const int32_t* sort_idx, const float* sigma, const float* feats, int B,
     int R, int S, int F, float t_thresh, int support_occ, float* grad_feats,
     float* grad_sigma, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Fused radiance evaluation                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Number of floats of the small-weight buffer (occupancy head, Hydra
+ * encoder, RGB head) for feature width F and hidden width H; the layout is
+ * given in sailor_amd/csrc/mlp_layout.h and written by sailor_amd/model.py. */
+size_t sailor_render_small_weights_floats(int F, int H);
+
+/* Fused form of the FastNerf inference path (c_lib/FastNerf/src/infer_mlp.cu:
+ * infer_fast_rendering :638-789 -- density_feat_kernel per (point, view),
+ * mean over views, density_sigma_kernel, per-view feature compositing,
+ * hydra_attention_kernel, RGB head) followed by the occupancy compositing of
+ * volume_rendering_occ_forward (volume_rendering.cu:666-745), with feature
+ * gather (proj_persp + grid_sample, utils_render.py:380-410,
+ * SRONet/BasicRenNet.py:245-259) and frequency encoding fused in.  Widths are
+ * template parameters: F (feature channels) in {32, 64}, H (hidden) in
+ * {128, 256}; Hydra inner width 32, RGB head 64-32-3.
+ *  t        f32 [B,R,S] sorted sample distances (-1 padded rows are dropped)
+ *  rays     f32 [B,R,6] origin and unit direction
+ *  Kv, RTv  f32 [B,V,3,3], [B,V,3,4] source cameras; img_h x img_w images
+ *  fmap     f32 [B,V,Hf,Wf,F] channels-last per-view feature maps
+ *  bbox     f32 [B,3,2] volume box (frame of the positional encoding)
+ *  zdir     f32 [B,3] third row of the target rotation (depth = t * zdir.dir)
+ *  precision 0 = fp32 (MFMA f32), 1 = bf16, 2 = fp16 (MFMA, fp32 accumulate)
+ *  w_dense  packed dense layers (sailor_amd/model.py), b_dense f32 [4,H]
+ *  early_exit  nonzero: stop evaluating a ray once its transmittance is
+ *              below t_thresh (outputs are unchanged)
+ *  out_rgb f32 [B,R,3], out_depth f32 [B,R]; out_aux f32 [B,R,2] (weight sum,
+ *  alpha sum) and out_occ f32 [B,R,S] (per-sample occupancy) may be NULL.   */
+int sailor_render_rays(const float* t, const float* rays, int B, int R, int S,
+                       const float* Kv, const float* RTv, int V, int img_h,
+                       int img_w, const float* fmap, int Hf, int Wf, int F,
+                       const float* bbox, const float* zdir, int H,
+                       int precision, const void* w_dense, const float* b_dense,
+                       const float* w_small, float t_thresh, int early_exit,
+                       float* out_rgb, float* out_depth, float* out_aux,
+                       float* out_occ, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Frequency encoding                                                        */

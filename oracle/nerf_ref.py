@@ -1,0 +1,120 @@
+"""PyTorch restatement of the fused radiance evaluation -- TEST
+INFRASTRUCTURE ONLY (imported by tests/ and bench.py's CPU baseline).
+
+Written from the reference modules, not from the HIP kernel:
+  projection + grid_sample   utils_render/utils_render.py:380-410 (proj_persp),
+                             SRONet/BasicRenNet.py:245-259 (F.grid_sample,
+                             align_corners=True, zero padding)
+  frequency encoding         c_lib/VoxelEncoding/src/freq_encoding.cu:37-45
+  per-view MLP, view mean    SRONet/nerf.py:139-152, FastNerf infer_mlp.cu:687-713
+  occupancy compositing      c_lib/VoxelEncoding/src/volume_rendering.cu:666-745
+  per-view token compositing c_lib/FastNerf/src/infer_mlp.cu:724-747
+  Hydra encoder              SRONet/aggregation.py:85-128,155-171,288-307
+  RGB head                   SRONet/nerf.py:117-124
+Runs in any dtype/device; tests use float64 on the CPU as the accuracy oracle.
+"""
+from __future__ import annotations
+
+import torch
+import torch.nn.functional as Fn
+
+
+def freq_encode(x, deg=10):
+    outs = [x]
+    for f in range(deg):
+        outs.append(torch.sin(x * (2.0 ** f)))
+        outs.append(torch.sin(x * (2.0 ** f) + torch.pi / 2))
+    # reference layout: [x, sin f0 (3), cos f0 (3), sin f1, ...]
+    return torch.cat(outs, -1)
+
+
+def render_rays(weights, t, rays, Kv, RTv, fmaps, bbox, zdir, img_hw, t_thres=1e-4,
+                dtype=torch.float64, device="cpu"):
+    """weights: sailor_amd.model.NerfWeights; t [B,R,S] sorted; rays [B,R,6];
+    Kv [B,V,3,3]; RTv [B,V,3,4]; fmaps [B,V,F,Hf,Wf] (channel-first);
+    bbox [B,3,2]; zdir [B,3].  Returns rgb [B,R,3], depth [B,R], ws [B,R],
+    asum [B,R], occ [B,R,S] (zeros for dropped rays)."""
+    T_ = lambda a: torch.as_tensor(a).to(device=device, dtype=dtype)
+    t, rays, Kv, RTv, fmaps, bbox, zdir = map(T_, (t, rays, Kv, RTv, fmaps, bbox, zdir))
+    B, R, S = t.shape
+    V = Kv.shape[1]
+    img_h, img_w = img_hw
+    W = [T_(w) for w in weights.dense_w]
+    Bs = [T_(b) for b in weights.dense_b]
+    rgb_out = torch.zeros((B, R, 3), dtype=dtype, device=device)
+    dep_out = torch.zeros((B, R), dtype=dtype, device=device)
+    ws_out = torch.zeros((B, R), dtype=dtype, device=device)
+    as_out = torch.zeros((B, R), dtype=dtype, device=device)
+    occ_out = torch.zeros((B, R, S), dtype=dtype, device=device)
+    for b in range(B):
+        valid = t[b, :, 0] >= 0
+        idx = torch.nonzero(valid).flatten()
+        if idx.numel() == 0:
+            continue
+        tb = t[b, idx]                                   # [N, S]
+        o, d = rays[b, idx, :3], rays[b, idx, 3:]
+        x = o[:, None, :] + tb[..., None] * d[:, None, :]  # [N, S, 3]
+        c = 0.5 * (bbox[b, :, 0] + bbox[b, :, 1])
+        ih = 2.0 / (bbox[b, :, 1] - bbox[b, :, 0])
+        pe = freq_encode((x - c) * ih)                   # [N, S, 63]
+        feats = []
+        for v in range(V):
+            Rv, Tv = RTv[b, v, :, :3], RTv[b, v, :, 3]
+            p = x @ Rv.T + Tv                            # [N, S, 3]
+            z = p[..., 2:3]
+            inv = z < 0.01
+            zz = torch.where(inv, torch.ones_like(z), z)
+            q = p / zz
+            scr = q @ Kv[b, v].T
+            res = torch.tensor([img_w // 2, img_h // 2], dtype=dtype, device=device)
+            grid = (scr[..., :2] - res) / res
+            grid = torch.where(inv.expand_as(grid), torch.full_like(grid, -2.0), grid)
+            smp = Fn.grid_sample(fmaps[b, v][None], grid[None], mode="bilinear",
+                                 padding_mode="zeros", align_corners=True)[0]  # [F, N, S]
+            feats.append(smp.permute(1, 2, 0))           # [N, S, F]
+        h_sum = 0
+        for v in range(V):
+            h = torch.cat([pe, feats[v]], -1)
+            for li in range(4):
+                h = torch.relu(h @ W[li].T + Bs[li])
+            h_sum = h_sum + h
+        h_mean = h_sum / V
+        alpha = torch.sigmoid(h_mean @ T_(weights.sigma_w) + weights.sigma_b)  # [N, S]
+        # sequential front-to-back compositing with early termination
+        N = tb.shape[0]
+        Tt = torch.ones(N, dtype=dtype, device=device)
+        alive = torch.ones(N, dtype=torch.bool, device=device)
+        wsum = torch.zeros(N, dtype=dtype, device=device)
+        dsum = torch.zeros(N, dtype=dtype, device=device)
+        asum = torch.zeros(N, dtype=dtype, device=device)
+        toks = torch.zeros((N, V, feats[0].shape[-1]), dtype=dtype, device=device)
+        for s in range(S):
+            a = alpha[:, s]
+            w = torch.where(alive, a * Tt, torch.zeros_like(a))
+            dsum = dsum + w * tb[:, s]
+            wsum = wsum + w
+            asum = asum + w * a
+            for v in range(V):
+                toks[:, v] += w[:, None] * feats[v][:, s]
+            Tt = torch.where(alive, Tt * (1 - a), Tt)
+            alive = alive & ~(Tt < t_thres)
+        x_tok = toks
+        for st in weights.hydra:
+            q = Fn.normalize(x_tok @ T_(st["q"]).T, dim=-1)
+            k = Fn.normalize(x_tok @ T_(st["k"]).T, dim=-1)
+            vv = x_tok @ T_(st["v"]).T
+            kv = (k * vv).sum(dim=-2, keepdim=True)
+            out = (q * kv) @ T_(st["fc"]).T + x_tok
+            x_tok = torch.relu(out @ T_(st["w1"]).T + T_(st["b1"])) @ T_(st["w2"]).T + \
+                T_(st["b2"]) + out
+        h = x_tok[:, 0]
+        (w1, b1), (w2, b2), (w3, b3) = weights.rgb
+        h = torch.relu(h @ T_(w1).T + T_(b1))
+        h = torch.relu(h @ T_(w2).T + T_(b2))
+        rgb = torch.sigmoid(h @ T_(w3).T + T_(b3))
+        rgb_out[b, idx] = rgb
+        dep_out[b, idx] = dsum * (d @ zdir[b])
+        ws_out[b, idx] = wsum
+        as_out[b, idx] = asum
+        occ_out[b, idx] = alpha
+    return rgb_out, dep_out, ws_out, as_out, occ_out

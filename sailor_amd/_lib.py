@@ -31,16 +31,21 @@ _SIGS = {
     "sailor_octree_voxel_traversal": [P, P, P, I, P, P] + [I] * 4 + [P, P] + [I] * 5 + [P] * 5 + [P],
     "sailor_octree_sampling_coarse": [P] * 7 + [I] * 5 + [P, P, P],
     "sailor_sort_samplings": [P, I, I, P, P, P],
+    "sailor_trace_sample": [P, P, P, I, I, I, P, P, P, I, P, P, I, I, I, I, P, P, P, P, P, P],
     "sailor_rays_calculating": [P, P, P] + [I] * 6 + [P, P],
     "sailor_volume_rendering_occ_forward": [P, P, P, I, I, I, I, F, P, P, P, P, P],
     "sailor_volume_rendering_training_forward": [P, P, P, P, I, I, I, I, F, I, P, P, P, P, P],
     "sailor_volume_rendering_training_backward": [P] * 12 + [I, I, I, I, F, I, P, P, P],
+    "sailor_render_small_weights_floats": [I, I],
+    "sailor_render_rays": [P, P, I, I, I, P, P, I, I, I, P, I, I, I, P, P, I, I, P, P, P, F, I,
+                           P, P, P, P, P],
     "sailor_freq_encode_forward": [P, I, I, I, P, P],
     "sailor_freq_encode_backward": [P, P, I, I, I, P, P],
 }
 _RESTYPES = {
     "sailor_last_error": ctypes.c_char_p,
     "sailor_octree_workspace_bytes": SZ,
+    "sailor_render_small_weights_floats": SZ,
 }
 
 _lib = None

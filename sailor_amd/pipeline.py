@@ -1,0 +1,223 @@
+"""One frame of the real-time path as a fixed sequence of HIP launches with
+static, capacity-bounded buffers (no host synchronisation), capturable into
+a hipGraph:
+
+  integrate -> build_multi_res_volumes -> octree build -> fused trace+sample
+  -> per-ray sort -> fused radiance evaluation + compositing
+
+mirroring SRONet's forward_build + forward_query (SRONet/BasicRenNet.py:170-216,
+269-326) / the GUI render loop (gui/gui_render.py:921-934) for one target view.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib, scenes
+from ._lib import call, ptr, stream_of
+from .model import CONFIG_MODELS, NerfWeights
+from .render import PackedModel
+
+_f32, _i32 = torch.float32, torch.int32
+
+
+@dataclass
+class FrameConfig:
+    name: str
+    n_views: int
+    img: int          # source images img x img
+    fx: float
+    dim: int          # TSDF grid dim^3
+    rate: int         # fine voxels per coarse cell edge
+    target: int       # target image target x target
+    target_fx: float
+    S: int            # samples per ray
+    model: str        # key of CONFIG_MODELS
+    cube: float = 2.2
+    trunc_weight: float = 3.0
+    th_low: float = 13.5
+    th_high: float = 13.5
+    node_capacity: int = 4_000_000
+
+
+CONFIGS = {
+    "c1": FrameConfig("c1", 4, 512, 600.0, 128, 8, 256, 300.0, 64, "c1"),
+    "c2": FrameConfig("c2", 4, 1024, 1200.0, 256, 8, 512, 600.0, 48, "c2"),
+    "c3": FrameConfig("c3", 4, 1024, 1200.0, 256, 8, 1024, 1200.0, 48, "c2"),
+}
+
+
+class FramePipeline:
+    def __init__(self, cfg: FrameConfig, prec="fp32", device="cuda", early_exit=False,
+                 weights: NerfWeights = None):
+        self.cfg = cfg
+        self.dev = torch.device(device)
+        self.prec = prec
+        self.early_exit = early_exit
+        mc = CONFIG_MODELS[cfg.model]
+        self.mc = mc
+        self.weights = weights or NerfWeights.init(mc.F, mc.H)
+        self.model = PackedModel(self.weights, prec, self.dev)
+        V, n, d, r = cfg.n_views, cfg.img, cfg.dim, cfg.rate
+        dc = d // r
+        self.dc = dc
+        o = dict(device=self.dev)
+        self.K = torch.empty((1, V, 3, 3), dtype=_f32, **o)
+        self.RT = torch.empty((1, V, 3, 4), dtype=_f32, **o)
+        self.R = torch.empty((1, V, 3, 3), dtype=_f32, **o)
+        self.T = torch.empty((1, V, 3), dtype=_f32, **o)
+        self.colors = torch.empty((1, V, n, n), dtype=_f32, **o)
+        self.depths = torch.empty((1, V, n, n), dtype=_f32, **o)
+        self.masks = torch.empty((1, V, n, n), dtype=_i32, **o)
+        self.fixed_bbox = torch.empty((1, 3, 2), dtype=_f32, **o)
+        self.fmap = torch.empty((1, V, mc.Hf, mc.Wf, mc.F), dtype=_f32, **o)
+        self.occ0 = torch.empty((1, d, d, d), dtype=_i32, **o)
+        self.occ1 = torch.empty((1, dc, dc, dc), dtype=_i32, **o)
+        self.num = torch.zeros((2, 1), dtype=_i32, **o)
+        self.tsdf = torch.empty((1, d, d, d), dtype=_f32, **o)
+        self.weight = torch.empty_like(self.tsdf)
+        self.color = torch.empty_like(self.tsdf)
+        self.origin = torch.empty((1, 3, 2), dtype=_f32, **o)
+        self.bbox = torch.empty((1, 3, 2), dtype=_f32, **o)
+        self.vs = torch.empty((1, 1), dtype=_f32, **o)
+        cap = min(d ** 3 + dc ** 3, cfg.node_capacity)
+        self.cap = cap
+        self.node_pos = torch.zeros((cap, 4), dtype=_f32, **o)
+        self.node_info = torch.full((cap, 4), -1, dtype=_i32, **o)
+        self.queue = torch.zeros((2,), dtype=_i32, **o)
+        wsb = _lib.lib().sailor_octree_workspace_bytes(1, dc)
+        self.tree_ws = torch.empty(((wsb + 3) // 4,), dtype=_i32, **o)
+        nt = cfg.target * cfg.target
+        self.Kt = torch.empty((1, 3, 3), dtype=_f32, **o)
+        self.Rt = torch.empty((1, 3, 3), dtype=_f32, **o)
+        self.Tt = torch.empty((1, 3), dtype=_f32, **o)
+        self.zdir = torch.empty((1, 3), dtype=_f32, **o)
+        self.noise = torch.full((1, nt, cfg.S), 0.5, dtype=_f32, **o)
+        self.rays = torch.empty((1, nt, 6), dtype=_f32, **o)
+        self.sidx = torch.empty((1, nt, cfg.S), dtype=_i32, **o)
+        self.st = torch.empty((1, nt, cfg.S), dtype=_f32, **o)
+        self.vl = torch.empty((1, nt), dtype=_f32, **o)
+        self.sorted = torch.empty((1, nt, cfg.S), dtype=_f32, **o)
+        self.order = torch.empty((1, nt, cfg.S), dtype=_i32, **o)
+        self.rgb = torch.empty((1, nt, 3), dtype=_f32, **o)
+        self.depth = torch.empty((1, nt), dtype=_f32, **o)
+        self.aux = torch.empty((1, nt, 2), dtype=_f32, **o)
+        self.graph = None
+
+    # ------------------------------------------------------------------
+    def load_scene(self, scene: scenes.Scene, fmaps: np.ndarray, target: scenes.Camera):
+        """Uploads one frame's inputs (host -> device, outside any timing)."""
+        cu = lambda a: torch.from_numpy(np.ascontiguousarray(a))
+        RT = scene.RT()[None]
+        self.K.copy_(cu(scene.K()[None]))
+        self.RT.copy_(cu(RT))
+        self.R.copy_(cu(RT[..., :3]))
+        self.T.copy_(cu(RT[..., 3]))
+        self.colors.copy_(cu(scenes.pack_colors(scene.rgb)[None]))
+        self.depths.copy_(cu(scene.depth[None]))
+        self.masks.copy_(cu(scene.mask.astype(np.int32)[None]))
+        self.fixed_bbox.copy_(cu(scenes.fixed_cube(self.cfg.cube)[None]))
+        self.fmap.copy_(cu(np.transpose(fmaps, (0, 2, 3, 1))[None]))
+        self.Kt.copy_(cu(target.K[None]))
+        self.Rt.copy_(cu(target.R[None]))
+        self.Tt.copy_(cu(target.T[None]))
+        self.zdir.copy_(cu(target.R[2][None]))
+
+    @staticmethod
+    def synthetic(cfg: FrameConfig, frame_id=0, **kw):
+        """Pipeline loaded with the seeded capsule scene of `cfg`."""
+        p = FramePipeline(cfg, **kw)
+        sc = scenes.make_scene(frame_id, cfg.n_views, cfg.img, cfg.img, cfg.fx)
+        mc = p.mc
+        fm = scenes.feature_maps(frame_id, cfg.n_views, mc.F, mc.Hf, mc.Wf)
+        tgt = scenes.target_camera(cfg.target, cfg.target, cfg.target_fx)
+        p.load_scene(sc, fm, tgt)
+        p.scene, p.fmaps_np, p.target_cam = sc, fm, tgt
+        return p
+
+    # ------------------------------------------------------------------
+    def build(self):
+        """TSDF fusion + multi-resolution volumes + tree (forward_build)."""
+        cfg, V, n = self.cfg, self.cfg.n_views, self.cfg.img
+        d, dc = cfg.dim, self.dc
+        st = stream_of(self.dev)
+        self.num.zero_()
+        call("sailor_integrate", ptr(self.occ0), ptr(self.num[0]), ptr(self.K), ptr(self.RT),
+             ptr(self.R), ptr(self.T), ptr(self.colors), ptr(self.depths), ptr(self.masks), 1,
+             V, n, n, d, d, d, 1.0, 0.025, cfg.trunc_weight, cfg.th_low, cfg.th_high,
+             ptr(self.fixed_bbox), ptr(self.tsdf), ptr(self.weight), ptr(self.color),
+             ptr(self.origin), ptr(self.bbox), ptr(self.vs), st)
+        vols = _ptr_array([self.occ0, self.occ1])
+        cnts = _ptr_array([self.num[0], self.num[1]])
+        call("sailor_build_multi_res_volumes", vols, cnts, 2, 1, d, cfg.rate, st)
+        call("sailor_octree_build", ptr(self.occ0), ptr(self.occ1), ptr(self.num[0]),
+             ptr(self.num[1]), 1, d, dc, ptr(self.vs), ptr(self.bbox), -1, self.cap,
+             ptr(self.node_pos), ptr(self.node_info), ptr(self.queue), ptr(self.tree_ws),
+             self.tree_ws.numel() * 4, st)
+
+    def trace(self):
+        """Target rays -> DDA -> samples -> per-ray sort (forward_ray_intersect)."""
+        cfg = self.cfg
+        st = stream_of(self.dev)
+        call("sailor_trace_sample", ptr(self.Kt), ptr(self.Rt), ptr(self.Tt), 1, cfg.target,
+             cfg.target, ptr(self.occ1), ptr(self.node_pos), ptr(self.node_info), self.cap,
+             ptr(self.bbox), ptr(self.vs), cfg.dim, self.dc, cfg.rate, cfg.S, ptr(self.noise),
+             ptr(self.rays), ptr(self.sidx), ptr(self.st), ptr(self.vl), st)
+        call("sailor_sort_samplings", ptr(self.st), cfg.target * cfg.target, cfg.S,
+             ptr(self.sorted), ptr(self.order), st)
+
+    def shade(self):
+        """Fused radiance evaluation + compositing (forward_ray_nerf / FastNerf)."""
+        cfg, mc, m = self.cfg, self.mc, self.model
+        from .model import PREC_IDS
+        call("sailor_render_rays", ptr(self.sorted), ptr(self.rays), 1,
+             cfg.target * cfg.target, cfg.S, ptr(self.K), ptr(self.RT), cfg.n_views, cfg.img,
+             cfg.img, ptr(self.fmap), mc.Hf, mc.Wf, mc.F, ptr(self.bbox), ptr(self.zdir), mc.H,
+             PREC_IDS[m.prec], ptr(m.w_dense), ptr(m.b_dense), ptr(m.w_small), 1e-4,
+             int(self.early_exit), ptr(self.rgb), ptr(self.depth), ptr(self.aux), None,
+             stream_of(self.dev))
+
+    def run(self):
+        self.build()
+        self.trace()
+        self.shade()
+
+    # ------------------------------------------------------------------
+    def capture(self):
+        """Captures one frame into a hipGraph (all buffers are static)."""
+        s = torch.cuda.Stream(self.dev)
+        s.wait_stream(torch.cuda.current_stream(self.dev))
+        with torch.cuda.stream(s):
+            self.run()  # warm-up outside capture
+        torch.cuda.current_stream(self.dev).wait_stream(s)
+        torch.cuda.synchronize(self.dev)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.run()
+        self.graph = g
+        return g
+
+    def replay(self):
+        if self.graph is None:
+            self.run()
+        else:
+            self.graph.replay()
+
+    def check(self):
+        """Host-side validation after a frame (synchronises)."""
+        q = self.queue.tolist()
+        if q[0] < 0:
+            raise RuntimeError("octree node capacity exceeded")
+        return dict(n_coarse=q[0], n_fine=q[1], occupied=int(self.num[0].item()),
+                    valid_rays=int((self.sorted[0, :, 0] >= 0).sum().item()))
+
+    def image(self):
+        n = self.cfg.target
+        return self.rgb.view(n, n, 3), self.depth.view(n, n)
+
+
+def _ptr_array(tensors):
+    import ctypes
+    return (ctypes.c_void_p * len(tensors))(*[t.data_ptr() for t in tensors])

@@ -1,0 +1,147 @@
+"""GPU parity of the fused render path:
+  * sailor_trace_sample (fused rays + DDA + sampling) vs the oracle chain,
+    bit for bit;
+  * sailor_render_rays (fused gather + PE + MLP on MFMA + compositing + Hydra +
+    RGB head) vs the float64 PyTorch restatement oracle/nerf_ref.py;
+  * bf16 / fp16 forms vs the fp32 form (PSNR);
+  * the whole frame captured in a hipGraph replays to the eager result.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import nerf_ref
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def small_cfg():
+    from sailor_amd.pipeline import FrameConfig
+    # config-1 geometry and model widths with a reduced target image
+    return FrameConfig("c1_small", 4, 512, 600.0, 128, 8, 48, 56.25, 64, "c1")
+
+
+_pipes = {}
+
+
+def pipe(prec="fp32", cfg=None):
+    from sailor_amd.pipeline import FramePipeline
+    cfg = cfg or small_cfg()
+    key = (cfg.name, prec)
+    if key not in _pipes:
+        p = FramePipeline.synthetic(cfg, prec=prec)
+        p.run()
+        torch.cuda.synchronize()
+        _pipes[key] = p
+    return _pipes[key]
+
+
+def test_fused_trace_matches_oracle_chain():
+    p = pipe()
+    cfg = p.cfg
+    info = p.check()
+    assert info["valid_rays"] > 50, info
+    # oracle chain on the GPU-built tree (the tree itself is checked bit-exactly
+    # in test_geometry_gpu.py)
+    vc = p.occ1.cpu().numpy()
+    pos = p.node_pos.cpu().numpy()
+    nfo = p.node_info.cpu().numpy()
+    bbox = p.bbox.cpu().numpy()
+    vs = p.vs.cpu().numpy().reshape(-1)
+    cam = p.target_cam
+    r6 = orc.rays(cam.K[None, None], cam.R[None, None], cam.T[None, None], cfg.target,
+                  cfg.target)[0, 0].reshape(1, -1, 6)
+    np.testing.assert_array_equal(p.rays.cpu().numpy(), r6)
+    trv = orc.traverse(vc, pos, nfo, bbox, vs, cfg.dim, cfg.rate, r6[..., :3], r6[..., 3:], 1024)
+    noise = np.full((1, r6.shape[1], cfg.S), 0.5, np.float32)
+    sidx, st = orc.sampling(trv["idx"], trv["tmin"], trv["tmax"], noise, trv["valid_len"], pos,
+                            vs, cfg.S, cfg.rate, trv["gmax"])
+    np.testing.assert_array_equal(p.sidx.cpu().numpy(), sidx)
+    np.testing.assert_array_equal(p.st.cpu().numpy(), st)
+    np.testing.assert_array_equal(p.vl.cpu().numpy(), trv["valid_len"])
+    srt, order = orc.sort_samplings(st)
+    np.testing.assert_array_equal(p.sorted.cpu().numpy(), srt)
+    np.testing.assert_array_equal(p.order.cpu().numpy(), order)
+
+
+def _reference(p):
+    cfg = p.cfg
+    t = p.sorted.cpu().numpy()
+    rays = p.rays.cpu().numpy()
+    fm = p.fmaps_np[None]  # [1,V,F,Hf,Wf]
+    return nerf_ref.render_rays(p.weights, t, rays, p.K.cpu().numpy(), p.RT.cpu().numpy(), fm,
+                                p.bbox.cpu().numpy(), p.zdir.cpu().numpy(), (cfg.img, cfg.img))
+
+
+_ref_cache = {}
+
+
+def test_render_fp32_matches_float64_reference():
+    from sailor_amd.render import render_rays
+    p = pipe()
+    out = render_rays(p.sorted, p.rays, p.K, p.RT, (p.cfg.img, p.cfg.img), p.fmap, p.bbox,
+                      p.zdir, p.model, want_occ=True)
+    torch.cuda.synchronize()
+    ref = _ref_cache.setdefault("fp32", _reference(p))
+    rgb, dep, ws, asum, occ = [r.numpy() for r in ref]
+    valid = p.sorted[0, :, 0].cpu().numpy() >= 0
+    assert valid.sum() > 50
+    g_occ = out["occ"].cpu().numpy()
+    print("max|occ| err", np.abs(g_occ - occ).max(), "max|rgb| err",
+          np.abs(out["rgb"].cpu().numpy() - rgb).max())
+    np.testing.assert_allclose(g_occ, occ, atol=2e-5)
+    np.testing.assert_allclose(out["aux"].cpu().numpy()[..., 0], ws, atol=1e-4)
+    np.testing.assert_allclose(out["aux"].cpu().numpy()[..., 1], asum, atol=1e-4)
+    np.testing.assert_allclose(out["depth"].cpu().numpy(), dep, atol=5e-4)
+    np.testing.assert_allclose(out["rgb"].cpu().numpy(), rgb, atol=1e-4)
+    # the pipeline's own shade() call wrote the same image
+    np.testing.assert_allclose(p.rgb.cpu().numpy(), out["rgb"].cpu().numpy(), atol=0)
+
+
+def psnr(a, b, mask):
+    """cal_psnr (utils_render/utils_render.py:25-35) on [0,1] images -> 255 scale."""
+    m = mask[..., None].astype(np.float64)
+    mse = ((np.abs(a * 255 * m - b * 255 * m)) ** 2).sum() / max(m.sum(), 1)
+    return 10 * np.log10(255 * 255 / max(mse, 1e-20))
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_low_precision_forms(prec):
+    p32 = pipe("fp32")
+    plo = pipe(prec)
+    valid = p32.sorted[0, :, 0].cpu().numpy() >= 0
+    a = p32.rgb[0].cpu().numpy()
+    b = plo.rgb[0].cpu().numpy()
+    ps = psnr(a, b, valid)
+    print(f"{prec}: PSNR vs fp32 = {ps:.2f} dB")
+    assert ps > 40.0
+    np.testing.assert_allclose(plo.depth.cpu().numpy(), p32.depth.cpu().numpy(), atol=5e-3)
+
+
+def test_graph_replay_matches_eager():
+    from sailor_amd.pipeline import FramePipeline
+    p = FramePipeline.synthetic(small_cfg(), prec="fp32")
+    p.run()
+    torch.cuda.synchronize()
+    eager = (p.rgb.clone(), p.depth.clone(), p.sorted.clone(), p.node_info.clone())
+    p.capture()
+    for _ in range(3):
+        p.rgb.zero_()
+        p.depth.zero_()
+        p.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(p.rgb, eager[0])
+        assert torch.equal(p.depth, eager[1])
+        assert torch.equal(p.sorted, eager[2])
+        assert torch.equal(p.node_info, eager[3])
+
+
+def test_early_exit_gives_same_image():
+    from sailor_amd.pipeline import FramePipeline
+    p = FramePipeline.synthetic(small_cfg(), prec="fp32", early_exit=True)
+    p.run()
+    torch.cuda.synchronize()
+    ref = pipe("fp32")
+    np.testing.assert_allclose(p.rgb.cpu().numpy(), ref.rgb.cpu().numpy(), atol=0)
+    np.testing.assert_allclose(p.depth.cpu().numpy(), ref.depth.cpu().numpy(), atol=0)
