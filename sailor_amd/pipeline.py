@@ -21,6 +21,7 @@ from .model import CONFIG_MODELS, NerfWeights
 from .render import PackedModel
 
 _f32, _i32 = torch.float32, torch.int32
+_SCENE_CACHE = {}
 
 
 @dataclass
@@ -129,7 +130,11 @@ class FramePipeline:
     def synthetic(cfg: FrameConfig, frame_id=0, **kw):
         """Pipeline loaded with the seeded capsule scene of `cfg`."""
         p = FramePipeline(cfg, **kw)
-        sc = scenes.make_scene(frame_id, cfg.n_views, cfg.img, cfg.img, cfg.fx)
+        key = (frame_id, cfg.n_views, cfg.img, cfg.fx)
+        if key not in _SCENE_CACHE:
+            _SCENE_CACHE[key] = scenes.make_scene(frame_id, cfg.n_views, cfg.img, cfg.img,
+                                                  cfg.fx)
+        sc = _SCENE_CACHE[key]
         mc = p.mc
         fm = scenes.feature_maps(frame_id, cfg.n_views, mc.F, mc.Hf, mc.Wf)
         tgt = scenes.target_camera(cfg.target, cfg.target, cfg.target_fx)
