@@ -246,16 +246,22 @@ size_t sailor_render_small_weights_floats(int F, int H);
  *  w_dense  packed dense layers (sailor_amd/model.py), b_dense f32 [4,H]
  *  early_exit  nonzero: stop evaluating a ray once its transmittance is
  *              below t_thresh (outputs are unchanged)
+ *  chunk_samples  samples evaluated together per workgroup step (8, 16, 24
+ *              or 48; 0 = default); S must be a multiple of it; a chunk
+ *              whose activations do not fit the 160 KiB of LDS of a
+ *              workgroup (fp32 with 48 samples) is rejected
  *  out_rgb f32 [B,R,3], out_depth f32 [B,R]; out_aux f32 [B,R,2] (weight sum,
- *  alpha sum) and out_occ f32 [B,R,S] (per-sample occupancy) may be NULL.   */
+ *  alpha sum) and out_occ f32 [B,R,S] (per-sample occupancy) may be NULL.
+ *  tokens   f32 [B,R,4,F] workspace: composited per-view features handed
+ *           from the sample kernel to the Hydra/RGB kernel (required).     */
 int sailor_render_rays(const float* t, const float* rays, int B, int R, int S,
                        const float* Kv, const float* RTv, int V, int img_h,
                        int img_w, const float* fmap, int Hf, int Wf, int F,
                        const float* bbox, const float* zdir, int H,
                        int precision, const void* w_dense, const float* b_dense,
                        const float* w_small, float t_thresh, int early_exit,
-                       float* out_rgb, float* out_depth, float* out_aux,
-                       float* out_occ, void* stream);
+                       int chunk_samples, float* out_rgb, float* out_depth,
+                       float* out_aux, float* out_occ, float* tokens, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Frequency encoding                                                        */

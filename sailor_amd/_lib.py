@@ -38,7 +38,7 @@ _SIGS = {
     "sailor_volume_rendering_training_backward": [P] * 12 + [I, I, I, I, F, I, P, P, P],
     "sailor_render_small_weights_floats": [I, I],
     "sailor_render_rays": [P, P, I, I, I, P, P, I, I, I, P, I, I, I, P, P, I, I, P, P, P, F, I,
-                           P, P, P, P, P],
+                           I, P, P, P, P, P, P],
     "sailor_freq_encode_forward": [P, I, I, I, P, P],
     "sailor_freq_encode_backward": [P, P, I, I, I, P, P],
 }

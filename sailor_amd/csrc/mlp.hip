@@ -1,6 +1,6 @@
 // Fused per-ray radiance kernel: feature gather + positional encoding +
 // per-view density MLP on MFMA + view mean + occupancy head + front-to-back
-// compositing + multi-view Hydra attention + RGB head, one wavefront per ray.
+// compositing + multi-view Hydra attention + RGB head.
 //
 // Semantics follow the reference's FastNerf path (c_lib/FastNerf/src/
 // infer_mlp.cu: density_feat_kernel :120-199 per (point, view), torch::mean
@@ -8,23 +8,31 @@
 // compositing along the ray :724-747 with volume_rendering_occ_kernel
 // :508-581, hydra_attention_kernel :242-445 on the composited per-view
 // tokens, RGB head :427-432) and the PyTorch modules it fuses
-// (SRONet/nerf.py:139-186, SRONet/aggregation.py:85-128,155-171,288-307), with
-// the widths of the benchmark model as template parameters instead of the
-// reference's hard-coded 64/32/30.
+// (SRONet/nerf.py:139-186, SRONet/aggregation.py:85-128,155-171,288-307),
+// with the widths of the benchmark model as template parameters instead of
+// the reference's hard-coded 64/32/30.
 //
-// MI355X design
-//  * columns (MFMA N) are (sample, view) pairs, rows (MFMA M) hidden units:
-//    a layer's fp32 accumulator tiles are the next layer's B fragments after a
-//    fixed permutation of the reduction index that the host applies to the
-//    weights, so activations never leave registers (no LDS transposes);
-//  * inputs are gathered straight into the first layer's B fragments: each
-//    lane of a column fetches a contiguous quarter of the channels of the
-//    four bilinear taps (channels-last feature maps -> coalesced 256 B rows);
-//  * per-sample results are composited in registers in sample order, so the
-//    per-sample density, colour features and the per-view tokens never reach
-//    HBM; the per-ray Hydra attention and RGB head run from LDS.
+// MI355X mapping (layouts in mlp_layout.h)
+//  * one workgroup (4 waves) owns one ray at a time (persistent grid-stride
+//    loop over rays); the ray's samples are processed in chunks of NS samples
+//    x 4 views = NC columns, in sample order;
+//  * the per-(sample, view) MLP runs transposed on MFMA: columns are the NC
+//    (sample, view) pairs, each wave owns a quarter of the output rows, so a
+//    weight fragment read from L2 is reused across all NC/16 column tiles;
+//  * layer inputs / activations of the chunk live in LDS (rows padded by 16
+//    bytes so the 16-byte B-fragment reads of 16 columns hit distinct banks);
+//    the fp32 accumulators of a layer are written back as the next layer's
+//    input after ReLU (bf16/fp16 forms store the activations in 16 bit);
+//  * feature gather (channels-last maps: 4 lanes fetch the 4 quarters of a
+//    tap's channel row) and positional encoding write the first layer's input
+//    directly into LDS; per-sample occupancy and compositing weights never
+//    reach HBM; the composited per-view tokens (4 x F floats per ray) do;
 //  * the view mean is folded into the occupancy head (mean of per-column dot
-//    products) so it costs four cross-lane adds per column.
+//    products); compositing keeps the reference's sequential recurrence;
+//  * Hydra + RGB head run in a second kernel over 16 rays (64 tokens) per
+//    workgroup: lane = token, so the 4 views of a ray are 4 adjacent lanes
+//    (cross-view sums are two shuffles), the weights are wave-uniform reads
+//    shared by all 64 tokens, and the 4 waves split each layer's outputs.
 #include "common.h"
 #include "mlp_layout.h"
 
@@ -33,14 +41,15 @@ namespace {
 using namespace sailor::mlp;
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 
 enum Prec { kF32 = 0, kBF16 = 1, kF16 = 2 };
-
-constexpr int kWaves = 4;   // rays per workgroup
-constexpr int kNT = 2;      // N-tiles (4 samples x 4 views each) per iteration
-constexpr int kVP = 4;      // view slots per sample
+constexpr int kThreads = kWaves * 64;
+constexpr int kVP = 4;  // view slots per sample
 constexpr float kHalfPi = 1.5707963267948966f;
 
 struct RenderArgs {
@@ -50,11 +59,12 @@ struct RenderArgs {
   const float* RTv;        // [B, V, 3, 4] source extrinsics
   const float* fmap;       // [B, V, Hf, Wf, F] channels-last feature maps
   const float* bbox;       // [B, 3, 2] volume box (positional-encoding frame)
-  const float* zdir;       // [B, 3] third row of the target rotation (depth = t * zdir.d)
+  const float* zdir;       // [B, 3] third row of the target rotation
   const void* wdense;      // packed dense layers
   const float* bdense;     // [4, H] biases
-  const float* wsmall;     // occupancy head, Hydra, RGB head (see mlp_layout.h)
+  const float* wsmall;     // occupancy head, Hydra, RGB head (mlp_layout.h)
   float* out_rgb;          // [n_rays, 3]
+  float* out_tok;          // [n_rays, 4, F] composited per-view features (workspace)
   float* out_depth;        // [n_rays]
   float* out_aux;          // [n_rays, 2] (weight sum, alpha sum) or null
   float* out_occ;          // [n_rays, S] per-sample occupancy or null
@@ -65,431 +75,601 @@ struct RenderArgs {
   int early_exit;
 };
 
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+template <int PREC> struct Tr;
+template <> struct Tr<kF32> {
+  typedef float E;
+  static constexpr int EPL = 1, PAD = 4;
+};
+template <> struct Tr<kBF16> {
+  typedef __bf16 E;
+  typedef bf16x8 V8;
+  typedef bf16x4 V4;
+  static constexpr int EPL = 8, PAD = 8;
+};
+template <> struct Tr<kF16> {
+  typedef _Float16 E;
+  typedef f16x8 V8;
+  typedef f16x4 V4;
+  static constexpr int EPL = 8, PAD = 8;
+};
 
 __device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
 
-template <int PREC>
-struct ElemOf { typedef float type; };
-template <>
-struct ElemOf<kBF16> { typedef __bf16 type; };
-template <>
-struct ElemOf<kF16> { typedef _Float16 type; };
-
-// Bilinear sample (grid_sample, align_corners=True, zero padding) of CPL
-// channels starting at c0 for one (point, view); projection as proj_persp
-// (utils_render/utils_render.py:380-410) with res = image size.
-template <int F, int CPL>
-__device__ __forceinline__ void gather_feats(const float* __restrict__ fmap, int Hf, int Wf,
-                                             const float* Kv, const float* RT, int img_w,
-                                             int img_h, bool vvalid, float x, float y, float z,
-                                             int c0, float* out) {
-  float px = RT[0] * x + RT[1] * y + RT[2] * z + RT[3];
-  float py = RT[4] * x + RT[5] * y + RT[6] * z + RT[7];
-  float pz = RT[8] * x + RT[9] * y + RT[10] * z + RT[11];
-  const bool inv = !(pz >= 0.01f) || !vvalid;
-  const float dz = inv ? 1.0f : pz;
-  px /= dz;
-  py /= dz;
-  const float pzn = pz / dz;
-  const float u = Kv[0] * px + Kv[1] * py + Kv[2] * pzn;
-  const float w = Kv[3] * px + Kv[4] * py + Kv[5] * pzn;
-  const float hw = (float)(img_w / 2), hh = (float)(img_h / 2);
-  float gx = (u - hw) / hw, gy = (w - hh) / hh;
-  if (inv) { gx = -2.0f; gy = -2.0f; }
-  const float ix = (gx + 1.0f) * 0.5f * (float)(Wf - 1);
-  const float iy = (gy + 1.0f) * 0.5f * (float)(Hf - 1);
+// acc[mt][nt] = W_wave(layer) x act for KS MFMA steps (see mlp_layout.h).
+template <int H, int PREC, int NTL, int KS>
+__device__ __forceinline__ void mfma_layer(const typename Tr<PREC>::E* __restrict__ W,
+                                           const typename Tr<PREC>::E* act, int lane,
+                                           f32x4 (&acc)[H / 64][NTL]) {
+  constexpr int MTW = H / 64;
+  constexpr int KP = H + Tr<PREC>::PAD;
+  const int n = lane & 15, g = lane >> 4;
 #pragma unroll
-  for (int c = 0; c < CPL; ++c) out[c] = 0.0f;
-  if (!(ix > -1.0f && ix < (float)Wf && iy > -1.0f && iy < (float)Hf)) return;
-  const float fx0 = floorf(ix), fy0 = floorf(iy);
-  const int x0 = (int)fx0, y0 = (int)fy0;
-  const float ax = ix - fx0, ay = iy - fy0;
+  for (int mt = 0; mt < MTW; ++mt)
 #pragma unroll
-  for (int tap = 0; tap < 4; ++tap) {
-    const int xx = x0 + (tap & 1), yy = y0 + (tap >> 1);
-    if (xx < 0 || xx >= Wf || yy < 0 || yy >= Hf) continue;
-    const float wt = ((tap & 1) ? ax : 1.0f - ax) * ((tap >> 1) ? ay : 1.0f - ay);
-    const f32x4* src = reinterpret_cast<const f32x4*>(fmap + ((size_t)yy * Wf + xx) * F + c0);
+    for (int nt = 0; nt < NTL; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  if constexpr (PREC == kF32) {
+    // weights for 4 steps (one 16-byte B read) are fetched one group ahead
+    static_assert(KS % 4 == 0, "fp32 steps come in groups of 4");
+    constexpr int NG = KS / 4;
+    float abuf[2][4][MTW];
+    auto load_group = [&](int q, float (&dst)[4][MTW]) {
 #pragma unroll
-    for (int q = 0; q < CPL / 4; ++q) {
-      f32x4 v4 = src[q];
-      out[4 * q + 0] += wt * v4[0];
-      out[4 * q + 1] += wt * v4[1];
-      out[4 * q + 2] += wt * v4[2];
-      out[4 * q + 3] += wt * v4[3];
-    }
-  }
-}
-
-// 16 positional-encoding dims [16g, 16g+16) of xn: [x, sin(2^f x), sin(2^f x + pi/2), ...]
-// (freq_encoding.cu:37-45 layout, degree 10 -> 63 dims, dim 63 is padding).
-__device__ __forceinline__ void pe16(const float* xn, int g, float* out) {
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const int q = 16 * g + k;
-    float v;
-    if (q < 3) {
-      v = xn[q];
-    } else if (q < 63) {
-      const int d = q % 3, col = q / 3 - 1, f = col >> 1;
-      v = sinf(ldexpf(xn[d], f) + (float)(col & 1) * kHalfPi);
-    } else {
-      v = 0.0f;
-    }
-    out[k] = v;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// dense layers
-template <int H, int PREC>
-struct Dense {
-  static constexpr int MT = H / 16;
-  typedef typename ElemOf<PREC>::type E;
-  static constexpr int EPL = PREC == kF32 ? 1 : 8;  // elements per lane per fragment
-
-  static __device__ __forceinline__ f32x4 mfma(f32x4 acc, const E* a, const E* b) {
-    if constexpr (PREC == kF32) {
-      return __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], b[0], acc, 0, 0, 0);
-    } else if constexpr (PREC == kBF16) {
-      bf16x8 av, bv;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) { av[j] = a[j]; bv[j] = b[j]; }
-      return __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
-    } else {
-      f16x8 av, bv;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) { av[j] = a[j]; bv[j] = b[j]; }
-      return __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, acc, 0, 0, 0);
-    }
-  }
-
-  // acc[nt][mt] += A(step) * Bfrag(step) over KS steps; bfrag(nt, s, E* out)
-  template <int KS, class BFrag>
-  static __device__ __forceinline__ void layer(const E* __restrict__ W, int lane,
-                                               f32x4 (&acc)[kNT][MT], BFrag bfrag) {
-#pragma unroll
-    for (int nt = 0; nt < kNT; ++nt)
-#pragma unroll
-      for (int mt = 0; mt < MT; ++mt) acc[nt][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      E a[MT * EPL];
-      const E* ap = W + ((size_t)s * 64 + lane) * MT * EPL;
-      if constexpr (PREC == kF32) {
-#pragma unroll
-        for (int q = 0; q < MT / 4; ++q) {
-          f32x4 v = reinterpret_cast<const f32x4*>(ap)[q];
-          a[4 * q + 0] = v[0]; a[4 * q + 1] = v[1]; a[4 * q + 2] = v[2]; a[4 * q + 3] = v[3];
-        }
-      } else {
-#pragma unroll
-        for (int q = 0; q < MT; ++q) {
-          f32x4 v = reinterpret_cast<const f32x4*>(ap)[q];  // 8 x 16-bit
-          const E* pv = reinterpret_cast<const E*>(&v);
-#pragma unroll
-          for (int j = 0; j < 8; ++j) a[8 * q + j] = pv[j];
-        }
-      }
-#pragma unroll
-      for (int nt = 0; nt < kNT; ++nt) {
-        E b[EPL];
-        bfrag(nt, s, b);
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) acc[nt][mt] = mfma(acc[nt][mt], a + mt * EPL, b);
-      }
-    }
-  }
-};
-
-template <int F, int H, int PREC>
-__global__ void __launch_bounds__(kWaves * 64)
-render_kernel(RenderArgs a) {
-  constexpr int MT = H / 16;
-  constexpr int CPL = F / 4;
-  typedef Dense<H, PREC> D;
-  typedef typename D::E E;
-  constexpr int EPL = D::EPL;
-  constexpr int KS0 = PREC == kF32 ? (F + kPeSlots) / 4 : (F + kPeSlots) / 32;
-  constexpr int KSH = PREC == kF32 ? H / 4 : H / 32;
-  constexpr int FS = PREC == kF32 ? CPL : F / 32;  // first-layer feature steps
-  constexpr SmallOffsets so = small_offsets(F, H);
-
-  __shared__ float s_x[kWaves][kVP][F];
-  __shared__ float s_o[kWaves][kVP][F];
-  __shared__ float s_q[kWaves][3][kVP][kHydraInner];
-  __shared__ float s_h[kWaves][kVP][kHydraInner];
-  __shared__ float s_r[kWaves][kRgbH1 + kRgbH2];
-  __shared__ float s_nrm[kWaves][2][kVP];
-
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const long long ray = (long long)blockIdx.x * kWaves + wid;
-  if (ray >= a.n_rays) return;
-  const int S = a.S;
-  const float* trow = a.t + ray * S;
-  if (!(trow[0] >= 0.0f)) {  // ray without a full set of samples: dropped
-    if (lane < 3) a.out_rgb[ray * 3 + lane] = 0.0f;
-    if (lane == 3) a.out_depth[ray] = 0.0f;
-    if (a.out_aux && lane < 2) a.out_aux[ray * 2 + lane] = 0.0f;
-    return;
-  }
-  const int b = (int)(ray / a.rays_per_batch);
-  const int n = lane & 15, g = lane >> 4, vv = n & 3, sl = n >> 2;
-  const bool vvalid = vv < a.V;
-  const int vsel = vvalid ? vv : 0;
-  float Kc[9], RTc[12];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) Kc[k] = a.Kv[((size_t)b * a.V + vsel) * 9 + k];
-#pragma unroll
-  for (int k = 0; k < 12; ++k) RTc[k] = a.RTv[((size_t)b * a.V + vsel) * 12 + k];
-  const float* fm = a.fmap + ((size_t)b * a.V + vsel) * a.Hf * a.Wf * F;
-  const float ox = a.rays[ray * 6 + 0], oy = a.rays[ray * 6 + 1], oz = a.rays[ray * 6 + 2];
-  const float dx = a.rays[ray * 6 + 3], dy = a.rays[ray * 6 + 4], dz = a.rays[ray * 6 + 5];
-  const float* bb = a.bbox + b * 6;
-  const float cx = 0.5f * (bb[0] + bb[1]), cy = 0.5f * (bb[2] + bb[3]),
-              cz = 0.5f * (bb[4] + bb[5]);
-  const float ihx = 2.0f / (bb[1] - bb[0]), ihy = 2.0f / (bb[3] - bb[2]),
-              ihz = 2.0f / (bb[5] - bb[4]);
-  const float inv_v = 1.0f / (float)a.V;
-  const E* W0 = reinterpret_cast<const E*>(a.wdense);
-  const size_t l0_elems = (size_t)KS0 * 64 * MT * EPL;
-  const size_t lh_elems = (size_t)KSH * 64 * MT * EPL;
-  const float* ws = a.wsmall;
-
-  float T = 1.0f, dsum = 0.0f, wsum = 0.0f, asum = 0.0f;
-  bool done = false;
-  float tok[CPL];
-#pragma unroll
-  for (int c = 0; c < CPL; ++c) tok[c] = 0.0f;
-
-  const int n_iter = S / (4 * kNT);
-  for (int it = 0; it < n_iter; ++it) {
-    if (a.early_exit && done) break;
-    float feat[kNT][CPL], pe[kNT][16], ts[kNT];
-#pragma unroll
-    for (int nt = 0; nt < kNT; ++nt) {
-      const int s = it * 4 * kNT + nt * 4 + sl;
-      const float t = trow[s];
-      ts[nt] = t;
-      const float x = ox + t * dx, y = oy + t * dy, z = oz + t * dz;
-      gather_feats<F, CPL>(fm, a.Hf, a.Wf, Kc, RTc, a.img_w, a.img_h, vvalid, x, y, z, CPL * g,
-                           feat[nt]);
-      const float xn[3] = {(x - cx) * ihx, (y - cy) * ihy, (z - cz) * ihz};
-      pe16(xn, g, pe[nt]);
-    }
-    f32x4 acc[kNT][MT];
-    // layer 0: [pe | feat] -> H
-    D::template layer<KS0>(W0, lane, acc, [&](int nt, int s, E* out) {
-      if constexpr (PREC == kF32) {
-        out[0] = s < FS ? feat[nt][s] : pe[nt][s - FS];
-      } else {
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-          out[j] = (E)(s < FS ? feat[nt][8 * s + j] : pe[nt][8 * (s - FS) + j]);
-      }
-    });
-    // hidden layers 1..3 (runtime loop keeps one copy of the unrolled body)
-    for (int l = 1; l < 4; ++l) {
-      const float* bias = a.bdense + (l - 1) * H;
-      f32x4 h[kNT][MT];
-#pragma unroll
-      for (int nt = 0; nt < kNT; ++nt)
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-          const f32x4 bv = *reinterpret_cast<const f32x4*>(bias + 16 * mt + 4 * g);
-#pragma unroll
-          for (int i = 0; i < 4; ++i) h[nt][mt][i] = fmaxf(acc[nt][mt][i] + bv[i], 0.0f);
-        }
-      const E* Wl = W0 + l0_elems + (size_t)(l - 1) * lh_elems;
-      D::template layer<KSH>(Wl, lane, acc, [&](int nt, int s, E* out) {
-        if constexpr (PREC == kF32) {
-          out[0] = h[nt][s >> 2][s & 3];
+      for (int i = 0; i < 4; ++i) {
+        const float* ap = W + ((size_t)(4 * q + i) * 64 + lane) * MTW;
+        if constexpr (MTW == 4) {
+          const f32x4 v = *reinterpret_cast<const f32x4*>(ap);
+          dst[i][0] = v[0]; dst[i][1] = v[1]; dst[i][2] = v[2]; dst[i][3] = v[3];
         } else {
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            out[j] = (E)h[nt][2 * s][j];
-            out[4 + j] = (E)h[nt][2 * s + 1][j];
-          }
+          for (int mt = 0; mt < MTW; ++mt) dst[i][mt] = ap[mt];
         }
-      });
+      }
+    };
+    load_group(0, abuf[0]);
+#pragma unroll
+    for (int q = 0; q < NG; ++q) {
+      if (q + 1 < NG) load_group(q + 1, abuf[(q + 1) & 1]);
+      f32x4 b4[NTL];
+#pragma unroll
+      for (int nt = 0; nt < NTL; ++nt)
+        b4[nt] = *reinterpret_cast<const f32x4*>(act + (16 * nt + n) * KP + 16 * q + 4 * g);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int nt = 0; nt < NTL; ++nt)
+#pragma unroll
+          for (int mt = 0; mt < MTW; ++mt)
+            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(abuf[q & 1][i][mt], b4[nt][i],
+                                                               acc[mt][nt], 0, 0, 0);
     }
-    // last ReLU, occupancy head on the view mean, compositing in sample order
-    float alpha[kNT];
-    {
-      const float* bias = a.bdense + 3 * H;
+  } else {
+    // weights are fetched PD steps ahead (PD + 1 rotating buffers); wide
+    // chunks have less register room left beside their accumulators
+    typedef typename Tr<PREC>::V8 V8;
+    constexpr int PD = NTL >= 12 ? 1 : 2, NB = PD + 1;
+    V8 abuf[NB][MTW];
+    auto load_step = [&](int s, V8 (&dst)[MTW]) {
+      const V8* ap = reinterpret_cast<const V8*>(W + ((size_t)s * 64 + lane) * MTW * 8);
 #pragma unroll
-      for (int nt = 0; nt < kNT; ++nt) {
-        float part = 0.0f;
+      for (int mt = 0; mt < MTW; ++mt) dst[mt] = ap[mt];
+    };
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-          const f32x4 bv = *reinterpret_cast<const f32x4*>(bias + 16 * mt + 4 * g);
-          const f32x4 wv = *reinterpret_cast<const f32x4*>(ws + so.sigma_w + 16 * mt + 4 * g);
+    for (int s = 0; s < PD && s < KS; ++s) load_step(s, abuf[s]);
 #pragma unroll
-          for (int i = 0; i < 4; ++i) part += wv[i] * fmaxf(acc[nt][mt][i] + bv[i], 0.0f);
+    for (int s = 0; s < KS; ++s) {
+      if (s + PD < KS) load_step(s + PD, abuf[(s + PD) % NB]);
+#pragma unroll
+      for (int nt = 0; nt < NTL; ++nt) {
+        const V8 b = *reinterpret_cast<const V8*>(act + (16 * nt + n) * KP + 32 * s + 8 * g);
+#pragma unroll
+        for (int mt = 0; mt < MTW; ++mt) {
+          if constexpr (PREC == kBF16)
+            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(abuf[s % NB][mt], b,
+                                                                  acc[mt][nt], 0, 0, 0);
+          else
+            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(abuf[s % NB][mt], b,
+                                                                 acc[mt][nt], 0, 0, 0);
         }
-        part = vvalid ? part : 0.0f;
-        part += __shfl_xor(part, 16, 64);
-        part += __shfl_xor(part, 32, 64);
-        part += __shfl_xor(part, 1, 64);
-        part += __shfl_xor(part, 2, 64);
-        alpha[nt] = sigmoidf(part * inv_v + ws[so.sigma_b]);
       }
-    }
-#pragma unroll
-    for (int nt = 0; nt < kNT; ++nt) {
-      float wmine = 0.0f;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const float al = __shfl(alpha[nt], 4 * k, 64);
-        const float tk = __shfl(ts[nt], 4 * k, 64);
-        float w = 0.0f;
-        if (!done) {
-          w = al * T;
-          dsum += w * tk;
-          wsum += w;
-          asum += w * al;
-          T *= 1.0f - al;
-          if (T < a.t_thres) done = true;
-        }
-        wmine = (sl == k) ? w : wmine;
-      }
-      if (a.out_occ && g == 0 && vv == 0)
-        a.out_occ[ray * S + it * 4 * kNT + nt * 4 + sl] = alpha[nt];
-#pragma unroll
-      for (int c = 0; c < CPL; ++c) tok[c] += wmine * feat[nt][c];
-    }
-  }
-  // per-view tokens: sum the four sample lanes (lane bits 2, 3)
-#pragma unroll
-  for (int c = 0; c < CPL; ++c) {
-    tok[c] += __shfl_xor(tok[c], 4, 64);
-    tok[c] += __shfl_xor(tok[c], 8, 64);
-  }
-  float(*x)[F] = s_x[wid];
-  float(*o)[F] = s_o[wid];
-  if (sl == 0) {
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) x[vv][CPL * g + c] = vvalid ? tok[c] : 0.0f;
-  }
-  wave_lds_sync();
-  const int V = a.V;
-  // ---- Hydra encoder, 2 stacks (aggregation.py:101-128,164-171,299-307)
-  for (int st = 0; st < kHydraStacks; ++st) {
-    // q, k, v projections (no bias): 3 * 4 * 32 outputs
-    for (int oi = lane; oi < 3 * kVP * kHydraInner; oi += 64) {
-      const int which = oi / (kVP * kHydraInner), rem = oi % (kVP * kHydraInner);
-      const int v = rem / kHydraInner, e = rem % kHydraInner;
-      const int off = which == 0 ? so.q[st] : (which == 1 ? so.k[st] : so.v[st]);
-      float accv = 0.0f;
-      if (v < V) {
-        const float* wr = ws + off + e * F;
-#pragma unroll 8
-        for (int c = 0; c < F; ++c) accv += wr[c] * x[v][c];
-      }
-      s_q[wid][which][v][e] = accv;
-    }
-    wave_lds_sync();
-    // L2 norms of q and k (F.normalize, eps 1e-12)
-    if (lane < 2 * kVP) {
-      const int which = lane / kVP, v = lane % kVP;
-      float ss = 0.0f;
-      for (int e = 0; e < kHydraInner; ++e) ss += s_q[wid][which][v][e] * s_q[wid][which][v][e];
-      s_nrm[wid][which][v] = 1.0f / fmaxf(sqrtf(ss), 1e-12f);
-    }
-    wave_lds_sync();
-    // kv = sum_v khat * v ;  attn_v = qhat_v * kv
-    if (lane < kHydraInner) {
-      const int e = lane;
-      float kv = 0.0f;
-      for (int v = 0; v < V; ++v) kv += s_q[wid][1][v][e] * s_nrm[wid][1][v] * s_q[wid][2][v][e];
-      for (int v = 0; v < kVP; ++v) s_h[wid][v][e] = s_q[wid][0][v][e] * s_nrm[wid][0][v] * kv;
-    }
-    wave_lds_sync();
-    // out = fc(attn) + residual (the layer input)
-    for (int oi = lane; oi < kVP * F; oi += 64) {
-      const int v = oi / F, c = oi % F;
-      float accv = 0.0f;
-      const float* wr = ws + so.fc[st] + c * kHydraInner;
-#pragma unroll 8
-      for (int e = 0; e < kHydraInner; ++e) accv += wr[e] * s_h[wid][v][e];
-      o[v][c] = accv + x[v][c];
-    }
-    wave_lds_sync();
-    // FFN: relu(W1 out + b1)
-    for (int oi = lane; oi < kVP * kHydraInner; oi += 64) {
-      const int v = oi / kHydraInner, e = oi % kHydraInner;
-      float accv = ws[so.b1[st] + e];
-      const float* wr = ws + so.w1[st] + e * F;
-#pragma unroll 8
-      for (int c = 0; c < F; ++c) accv += wr[c] * o[v][c];
-      s_h[wid][v][e] = fmaxf(accv, 0.0f);
-    }
-    wave_lds_sync();
-    // W2 h + b2 + out -> next layer input
-    for (int oi = lane; oi < kVP * F; oi += 64) {
-      const int v = oi / F, c = oi % F;
-      float accv = ws[so.b2[st] + c];
-      const float* wr = ws + so.w2[st] + c * kHydraInner;
-#pragma unroll 8
-      for (int e = 0; e < kHydraInner; ++e) accv += wr[e] * s_h[wid][v][e];
-      x[v][c] = accv + o[v][c];
-    }
-    wave_lds_sync();
-  }
-  // ---- RGB head on token 0: F -> 64 -> 32 -> 3 (nerf.py:117-124)
-  float* r = s_r[wid];
-  {
-    const int e = lane;  // 64 outputs
-    float accv = ws[so.r1b + e];
-    const float* wr = ws + so.r1w + e * F;
-#pragma unroll 8
-    for (int c = 0; c < F; ++c) accv += wr[c] * x[0][c];
-    r[e] = fmaxf(accv, 0.0f);
-  }
-  wave_lds_sync();
-  if (lane < kRgbH2) {
-    float accv = ws[so.r2b + lane];
-    const float* wr = ws + so.r2w + lane * kRgbH1;
-    for (int c = 0; c < kRgbH1; ++c) accv += wr[c] * r[c];
-    r[kRgbH1 + lane] = fmaxf(accv, 0.0f);
-  }
-  wave_lds_sync();
-  if (lane < 3) {
-    float accv = ws[so.r3b + lane];
-    const float* wr = ws + so.r3w + lane * kRgbH2;
-    for (int c = 0; c < kRgbH2; ++c) accv += wr[c] * r[kRgbH1 + c];
-    a.out_rgb[ray * 3 + lane] = sigmoidf(accv);
-  }
-  if (lane == 0) {
-    const float* zd = a.zdir + b * 3;
-    a.out_depth[ray] = dsum * (zd[0] * dx + zd[1] * dy + zd[2] * dz);
-    if (a.out_aux) {
-      a.out_aux[ray * 2 + 0] = wsum;
-      a.out_aux[ray * 2 + 1] = asum;
     }
   }
 }
 
-template <int F, int H, int PREC>
-int launch(const RenderArgs& a, hipStream_t st) {
-  const int blocks = (int)((a.n_rays + kWaves - 1) / kWaves);
-  hipLaunchKernelGGL((render_kernel<F, H, PREC>), dim3(blocks), dim3(kWaves * 64), 0, st, a);
-  return sailor::check_launch("render_rays");
+// act[col][row] = relu(acc + bias) for this wave's rows.
+template <int H, int PREC, int NTL>
+__device__ __forceinline__ void write_act(typename Tr<PREC>::E* act, const float* bias,
+                                          int wave, int lane, const f32x4 (&acc)[H / 64][NTL]) {
+  constexpr int MTW = H / 64;
+  constexpr int KP = H + Tr<PREC>::PAD;
+  const int n = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int mt = 0; mt < MTW; ++mt) {
+    const int row0 = (H / 4) * wave + 16 * mt + 4 * g;
+    const f32x4 bv = *reinterpret_cast<const f32x4*>(bias + row0);
+#pragma unroll
+    for (int nt = 0; nt < NTL; ++nt) {
+      f32x4 v;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = fmaxf(acc[mt][nt][i] + bv[i], 0.0f);
+      typename Tr<PREC>::E* dst = act + (16 * nt + n) * KP + row0;
+      if constexpr (PREC == kF32) {
+        *reinterpret_cast<f32x4*>(dst) = v;
+      } else {
+        typename Tr<PREC>::V4 h;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h[i] = (typename Tr<PREC>::E)v[i];
+        *reinterpret_cast<typename Tr<PREC>::V4*>(dst) = h;
+      }
+    }
+  }
+}
+
+template <int F, int H, int PREC, int NS>
+struct Smem {
+  static constexpr int NC = NS * kVP;
+  static constexpr int KP = H + Tr<PREC>::PAD;
+  static constexpr size_t act = 0;
+  static constexpr size_t act_bytes = (size_t)NC * KP * sizeof(typename Tr<PREC>::E);
+  static constexpr size_t keep = (act + act_bytes + 15) / 16 * 16;      // f32 [NS][4][F]
+  static constexpr size_t sig = keep + (size_t)NS * kVP * F * 4;       // f32 [4][NC]
+  static constexpr size_t alpha = sig + (size_t)kWaves * NC * 4;       // f32 [NS]
+  static constexpr size_t tval = alpha + (size_t)NS * 4;               // f32 [NS]
+  static constexpr size_t wts = tval + (size_t)NS * 4;                 // f32 [NS]
+  static constexpr size_t flag = wts + (size_t)NS * 4;                 // int [4]
+  static constexpr size_t total = flag + 16;
+};
+
+constexpr size_t kLdsBytes = 160 * 1024;  // LDS per workgroup on gfx950
+
+template <int F, int H, int PREC, int NS>
+__global__ void __launch_bounds__(kThreads)
+render_kernel(RenderArgs a) {
+  typedef typename Tr<PREC>::E E;
+  typedef Smem<F, H, PREC, NS> L;
+  constexpr int NC = L::NC, KP = L::KP, NTL = NC / 16, MTW = H / 64;
+  constexpr int CPL = F / 4;
+  constexpr int K0 = F + kPeSlots;
+  constexpr int STEP = PREC == kF32 ? 4 : 32;
+  constexpr int KS0 = K0 / STEP, KSH = H / STEP;
+  constexpr int EPL = Tr<PREC>::EPL;
+  constexpr SmallOffsets so = small_offsets(F, H);
+  static_assert(K0 <= KP, "first-layer input must fit an activation row");
+  static_assert(kVP * F <= kThreads, "one thread per (view, channel) token entry");
+
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  E* act = reinterpret_cast<E*>(smem + L::act);
+  float* keep = reinterpret_cast<float*>(smem + L::keep);
+  float* sig = reinterpret_cast<float*>(smem + L::sig);
+  float* s_alpha = reinterpret_cast<float*>(smem + L::alpha);
+  float* s_t = reinterpret_cast<float*>(smem + L::tval);
+  float* s_w = reinterpret_cast<float*>(smem + L::wts);
+  int* s_flag = reinterpret_cast<int*>(smem + L::flag);
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int S = a.S, V = a.V;
+  const float* ws = a.wsmall;
+  const E* W0 = reinterpret_cast<const E*>(a.wdense);
+  const size_t l0 = (size_t)kWaves * KS0 * 64 * MTW * EPL;   // elements of layer 0
+  const size_t lh = (size_t)kWaves * KSH * 64 * MTW * EPL;   // elements of a hidden layer
+  const size_t w0off = (size_t)wave * KS0 * 64 * MTW * EPL;
+  const size_t whoff = (size_t)wave * KSH * 64 * MTW * EPL;
+  const float inv_v = 1.0f / (float)V;
+
+  for (long long ray = blockIdx.x; ray < a.n_rays; ray += gridDim.x) {
+    __syncthreads();  // LDS of the previous ray is no longer read
+    const float* trow = a.t + ray * S;
+    if (!(trow[0] >= 0.0f)) {  // no full set of samples: the ray is dropped
+      if (tid == 0) a.out_depth[ray] = 0.0f;
+      if (a.out_aux && tid >= 1 && tid < 3) a.out_aux[ray * 2 + tid - 1] = 0.0f;
+      continue;
+    }
+    const int b = (int)(ray / a.rays_per_batch);
+    const float ox = a.rays[ray * 6 + 0], oy = a.rays[ray * 6 + 1], oz = a.rays[ray * 6 + 2];
+    const float dx = a.rays[ray * 6 + 3], dy = a.rays[ray * 6 + 4], dz = a.rays[ray * 6 + 5];
+    const float* bb = a.bbox + b * 6;
+    const float cx = 0.5f * (bb[0] + bb[1]), cy = 0.5f * (bb[2] + bb[3]),
+                cz = 0.5f * (bb[4] + bb[5]);
+    const float ihx = 2.0f / (bb[1] - bb[0]), ihy = 2.0f / (bb[3] - bb[2]),
+                ihz = 2.0f / (bb[5] - bb[4]);
+    // compositing state lives in thread 0; tokens: thread (v, c) = (tid / F, tid % F)
+    float T = 1.0f, dsum = 0.0f, wsum = 0.0f, asum = 0.0f;
+    bool done = false;
+    float tok = 0.0f;
+    if (tid == 0) s_flag[0] = 0;
+
+    for (int c0 = 0; c0 < S; c0 += NS) {
+      __syncthreads();
+      if (a.early_exit && s_flag[0]) break;
+      // ---- first-layer input: bilinear features (4 threads per column) ----
+      for (int item = tid; item < NC * 4; item += kThreads) {
+        const int col = item >> 2, quarter = item & 3;
+        const int sl = col >> 2, v = col & 3;
+        const float t = trow[c0 + sl];
+        const float x = ox + t * dx, y = oy + t * dy, z = oz + t * dz;
+        float f[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) f[c] = 0.0f;
+        if (v < V) {
+          const float* RT = a.RTv + ((size_t)b * V + v) * 12;
+          const float* K = a.Kv + ((size_t)b * V + v) * 9;
+          float px = RT[0] * x + RT[1] * y + RT[2] * z + RT[3];
+          float py = RT[4] * x + RT[5] * y + RT[6] * z + RT[7];
+          float pz = RT[8] * x + RT[9] * y + RT[10] * z + RT[11];
+          // proj_persp (utils_render.py:380-410): depth < 0.01 -> outside
+          if (pz >= 0.01f) {
+            px /= pz;
+            py /= pz;
+            const float u = K[0] * px + K[1] * py + K[2];
+            const float w = K[3] * px + K[4] * py + K[5];
+            const float hw = (float)(a.img_w / 2), hh2 = (float)(a.img_h / 2);
+            const float gx = (u - hw) / hw, gy = (w - hh2) / hh2;
+            // grid_sample, align_corners=True, zero padding
+            const float ix = (gx + 1.0f) * 0.5f * (float)(a.Wf - 1);
+            const float iy = (gy + 1.0f) * 0.5f * (float)(a.Hf - 1);
+            if (ix > -1.0f && ix < (float)a.Wf && iy > -1.0f && iy < (float)a.Hf) {
+              const float fx0 = floorf(ix), fy0 = floorf(iy);
+              const int x0 = (int)fx0, y0 = (int)fy0;
+              const float ax = ix - fx0, ay = iy - fy0;
+              const float* fm = a.fmap + ((size_t)b * V + v) * a.Hf * a.Wf * F + quarter * CPL;
+#pragma unroll
+              for (int tap = 0; tap < 4; ++tap) {
+                const int xx = x0 + (tap & 1), yy = y0 + (tap >> 1);
+                if (xx < 0 || xx >= a.Wf || yy < 0 || yy >= a.Hf) continue;
+                const float wt = ((tap & 1) ? ax : 1.0f - ax) * ((tap >> 1) ? ay : 1.0f - ay);
+                const f32x4* src = reinterpret_cast<const f32x4*>(fm + ((size_t)yy * a.Wf + xx) * F);
+#pragma unroll
+                for (int qq = 0; qq < CPL / 4; ++qq) {
+                  const f32x4 v4 = src[qq];
+                  f[4 * qq + 0] += wt * v4[0];
+                  f[4 * qq + 1] += wt * v4[1];
+                  f[4 * qq + 2] += wt * v4[2];
+                  f[4 * qq + 3] += wt * v4[3];
+                }
+              }
+            }
+          }
+        }
+        E* dst = act + col * KP + quarter * CPL;
+        float* kp = keep + col * F + quarter * CPL;
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+          dst[c] = (E)f[c];
+          kp[c] = f[c];
+        }
+      }
+      // ---- positional encoding (freq_encoding.cu:37-45 layout), shared by views
+      for (int item = tid; item < NS * kPeSlots; item += kThreads) {
+        const int sl = item / kPeSlots, q = item % kPeSlots;
+        const float t = trow[c0 + sl];
+        float val = 0.0f;
+        if (q < 63) {
+          const int d = q % 3;
+          const float xd = d == 0 ? (ox + t * dx - cx) * ihx
+                                  : (d == 1 ? (oy + t * dy - cy) * ihy : (oz + t * dz - cz) * ihz);
+          if (q < 3) {
+            val = xd;
+          } else {
+            const int col = q / 3 - 1;
+            val = sinf(ldexpf(xd, col >> 1) + (float)(col & 1) * kHalfPi);
+          }
+        }
+        const E ev = (E)val;
+#pragma unroll
+        for (int v = 0; v < kVP; ++v) act[(sl * kVP + v) * KP + F + q] = ev;
+      }
+      if (tid < NS) s_t[tid] = trow[c0 + tid];
+      __syncthreads();
+      // ---- dense layers on MFMA --------------------------------------------
+      f32x4 acc[MTW][NTL];
+      mfma_layer<H, PREC, NTL, KS0>(W0 + w0off, act, lane, acc);
+      for (int l = 1; l < 4; ++l) {
+        __syncthreads();  // every wave finished reading act
+        write_act<H, PREC, NTL>(act, a.bdense + (l - 1) * H, wave, lane, acc);
+        __syncthreads();
+        mfma_layer<H, PREC, NTL, KSH>(W0 + l0 + (size_t)(l - 1) * lh + whoff, act, lane, acc);
+      }
+      // ---- occupancy head: per-column partial dot products ------------------
+      {
+        const int n = lane & 15, g = lane >> 4;
+        const float* bias = a.bdense + 3 * H;
+#pragma unroll
+        for (int nt = 0; nt < NTL; ++nt) {
+          float part = 0.0f;
+#pragma unroll
+          for (int mt = 0; mt < MTW; ++mt) {
+            const int row0 = (H / 4) * wave + 16 * mt + 4 * g;
+            const f32x4 bv = *reinterpret_cast<const f32x4*>(bias + row0);
+            const f32x4 wv = *reinterpret_cast<const f32x4*>(ws + so.sigma_w + row0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) part += wv[i] * fmaxf(acc[mt][nt][i] + bv[i], 0.0f);
+          }
+          part += __shfl_xor(part, 16, 64);
+          part += __shfl_xor(part, 32, 64);
+          if (g == 0) sig[wave * NC + 16 * nt + n] = part;
+        }
+      }
+      __syncthreads();
+      if (tid < NS) {
+        float sum = 0.0f;
+        for (int v = 0; v < V; ++v)
+          for (int w = 0; w < kWaves; ++w) sum += sig[w * NC + tid * kVP + v];
+        const float al = sigmoidf(sum * inv_v + ws[so.sigma_b]);
+        s_alpha[tid] = al;
+        if (a.out_occ) a.out_occ[ray * S + c0 + tid] = al;
+      }
+      __syncthreads();
+      // ---- front-to-back compositing (volume_rendering.cu:713-734) ---------
+      if (tid == 0) {
+        for (int sl = 0; sl < NS; ++sl) {
+          float w = 0.0f;
+          if (!done) {
+            const float al = s_alpha[sl];
+            w = al * T;
+            dsum += w * s_t[sl];
+            wsum += w;
+            asum += w * al;
+            T *= 1.0f - al;
+            if (T < a.t_thres) done = true;
+          }
+          s_w[sl] = w;
+        }
+        s_flag[0] = done ? 1 : 0;
+      }
+      __syncthreads();
+      // ---- per-view tokens: sum_i w_i f_v(x_i) -----------------------------
+      if (tid < kVP * F) {
+        const int v = tid / F, c = tid % F;
+#pragma unroll 4
+        for (int sl = 0; sl < NS; ++sl) tok += s_w[sl] * keep[(sl * kVP + v) * F + c];
+      }
+    }
+    __syncthreads();
+    // ---- outputs: per-view tokens for the Hydra/RGB kernel, depth, sums ----
+    if (tid < kVP * F) a.out_tok[ray * kVP * F + tid] = (tid / F) < V ? tok : 0.0f;
+    if (tid == 0) {
+      const float* zd = a.zdir + b * 3;
+      a.out_depth[ray] = dsum * (zd[0] * dx + zd[1] * dy + zd[2] * dz);
+      if (a.out_aux) {
+        a.out_aux[ray * 2 + 0] = wsum;
+        a.out_aux[ray * 2 + 1] = asum;
+      }
+    }
+  }
+}
+
+// ---- Hydra encoder (2 stacks) + RGB head on the composited tokens ---------
+// (aggregation.py:101-128,164-171,299-307; nerf.py:117-124; infer_mlp.cu
+// hydra_attention_kernel :242-445).  Token t of a workgroup is (ray t/4 of
+// the group, view t%4); LDS holds feature-major rows [row][token] padded to
+// kLd floats so both the row reads and the transposing token load are free of
+// bank conflicts.
+constexpr int kRaysPerGroup = 16;
+constexpr int kTok = kRaysPerGroup * kVP;  // 64 = one token per lane
+constexpr int kLd = kTok + 1;
+
+template <int F>
+struct HydraRows {
+  static constexpr int X = 0;                  // layer input / output [F]
+  static constexpr int QKV = F;                // q, k, v [96]; RGB layer 1 [64]
+  static constexpr int OUT = F + 3 * kHydraInner;  // attention output [F]
+  static constexpr int HID = OUT + F;          // FFN hidden [32]; RGB layer 2 [32]
+  static constexpr int ROWS = HID + kHydraInner;
+};
+
+// rows[o][lane] = act(bias[o] + sum_k W[o][k] in[k]) for this wave's NO
+// outputs starting at o0; W rows are wave-uniform so they are fetched once
+// per wave and shared by the 64 tokens.
+template <int K, int NO, bool RELU>
+__device__ __forceinline__ void token_dense(const float* __restrict__ W, const float* bias,
+                                            const float (&in)[K], int o0, float* rows,
+                                            int lane) {
+#pragma unroll
+  for (int i = 0; i < NO; ++i) {
+    const int o = o0 + i;
+    const float* wr = W + (size_t)o * K;
+    float acc = bias ? bias[o] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc += wr[k] * in[k];
+    rows[o * kLd + lane] = RELU ? fmaxf(acc, 0.0f) : acc;
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void load_rows(const float* rows, int lane, float (&dst)[N]) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) dst[k] = rows[k * kLd + lane];
 }
 
 template <int F, int H>
+__global__ void __launch_bounds__(kThreads)
+hydra_rgb_kernel(const float* __restrict__ tok, const float* __restrict__ t, int S,
+                 const float* __restrict__ ws, int V, long long n_rays,
+                 float* __restrict__ out_rgb) {
+  typedef HydraRows<F> R;
+  constexpr SmallOffsets so = small_offsets(F, H);
+  constexpr int stack_stride = so.q[1] - so.q[0];
+  static_assert(F % kWaves == 0 && kRgbH1 % kWaves == 0 && kRgbH2 % kWaves == 0, "split");
+  __shared__ float sm[R::ROWS * kLd];
+  __shared__ int s_valid[kRaysPerGroup];
+
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const long long ray0 = (long long)blockIdx.x * kRaysPerGroup;
+  const int view = lane & 3;
+  const long long ray = ray0 + (lane >> 2);
+  bool valid = false;
+  if (tid < kTok && ray < n_rays) valid = t[ray * S] >= 0.0f;  // dropped rays have t < 0
+  if (tid < kTok && view == 0) s_valid[lane >> 2] = valid ? 1 : 0;
+  if (!__syncthreads_or(valid)) {
+    if (tid < 3 * kRaysPerGroup && ray0 + tid / 3 < n_rays) out_rgb[ray0 * 3 + tid] = 0.0f;
+    return;
+  }
+  // tokens of the group are one contiguous block of 64 x F floats
+  for (int i = tid; i < kTok * F; i += kThreads) {
+    const int tk = i / F, c = i % F;
+    const long long r = ray0 + tk / kVP;
+    sm[(R::X + c) * kLd + tk] = (r < n_rays && s_valid[tk / kVP]) ? tok[ray0 * kVP * F + i] : 0.0f;
+  }
+  __syncthreads();
+  const float vmask = view < V ? 1.0f : 0.0f;
+
+  for (int st = 0; st < kHydraStacks; ++st) {
+    const float* wq = ws + so.q[0] + st * stack_stride;  // q, k, v rows are contiguous
+    {
+      float x[F];
+      load_rows(sm + R::X * kLd, lane, x);
+      constexpr int NO = 3 * kHydraInner / kWaves;
+      token_dense<F, NO, false>(wq, nullptr, x, wave * NO, sm + R::QKV * kLd, lane);
+    }
+    __syncthreads();
+    float attn[kHydraInner];
+    {
+      // F.normalize(q), F.normalize(k) (eps 1e-12); kv = sum_views khat * v
+      float qq = 0.0f, kk = 0.0f;
+#pragma unroll
+      for (int e = 0; e < kHydraInner; ++e) {
+        const float q = sm[(R::QKV + e) * kLd + lane];
+        const float k = sm[(R::QKV + kHydraInner + e) * kLd + lane];
+        qq += q * q;
+        kk += k * k;
+      }
+      const float nq = 1.0f / fmaxf(sqrtf(qq), 1e-12f);
+      const float nk = vmask / fmaxf(sqrtf(kk), 1e-12f);
+#pragma unroll
+      for (int e = 0; e < kHydraInner; ++e) {
+        const float q = sm[(R::QKV + e) * kLd + lane];
+        const float k = sm[(R::QKV + kHydraInner + e) * kLd + lane];
+        const float v = sm[(R::QKV + 2 * kHydraInner + e) * kLd + lane];
+        float kv = k * nk * v;
+        kv += __shfl_xor(kv, 1, 64);
+        kv += __shfl_xor(kv, 2, 64);
+        attn[e] = q * nq * kv;
+      }
+    }
+    // out = fc(attn) + x
+    {
+      constexpr int NO = F / kWaves;
+      const float* wf = ws + so.fc[0] + st * stack_stride;
+#pragma unroll
+      for (int i = 0; i < NO; ++i) {
+        const int c = wave * NO + i;
+        const float* wr = wf + c * kHydraInner;
+        float acc = 0.0f;
+#pragma unroll
+        for (int e = 0; e < kHydraInner; ++e) acc += wr[e] * attn[e];
+        sm[(R::OUT + c) * kLd + lane] = acc + sm[(R::X + c) * kLd + lane];
+      }
+    }
+    __syncthreads();
+    // FFN hidden: relu(W1 out + b1)
+    {
+      float o[F];
+      load_rows(sm + R::OUT * kLd, lane, o);
+      constexpr int NO = kHydraInner / kWaves;
+      token_dense<F, NO, true>(ws + so.w1[0] + st * stack_stride,
+                               ws + so.b1[0] + st * stack_stride, o, wave * NO,
+                               sm + R::HID * kLd, lane);
+    }
+    __syncthreads();
+    // x = W2 h + b2 + out
+    {
+      float h[kHydraInner];
+      load_rows(sm + R::HID * kLd, lane, h);
+      constexpr int NO = F / kWaves;
+      const float* w2 = ws + so.w2[0] + st * stack_stride;
+      const float* b2 = ws + so.b2[0] + st * stack_stride;
+#pragma unroll
+      for (int i = 0; i < NO; ++i) {
+        const int c = wave * NO + i;
+        const float* wr = w2 + c * kHydraInner;
+        float acc = b2[c];
+#pragma unroll
+        for (int e = 0; e < kHydraInner; ++e) acc += wr[e] * h[e];
+        sm[(R::X + c) * kLd + lane] = acc + sm[(R::OUT + c) * kLd + lane];
+      }
+    }
+    __syncthreads();
+  }
+  // ---- RGB head on the first view's token: F -> 64 -> 32 -> 3 -------------
+  {
+    float x[F];
+    load_rows(sm + R::X * kLd, lane, x);
+    constexpr int NO = kRgbH1 / kWaves;
+    token_dense<F, NO, true>(ws + so.r1w, ws + so.r1b, x, wave * NO, sm + R::QKV * kLd, lane);
+  }
+  __syncthreads();
+  {
+    float r1[kRgbH1];
+    load_rows(sm + R::QKV * kLd, lane, r1);
+    constexpr int NO = kRgbH2 / kWaves;
+    token_dense<kRgbH1, NO, true>(ws + so.r2w, ws + so.r2b, r1, wave * NO, sm + R::HID * kLd,
+                                  lane);
+  }
+  __syncthreads();
+  if (wave == 0 && view == 0 && ray < n_rays) {
+    float r2[kRgbH2];
+    load_rows(sm + R::HID * kLd, lane, r2);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const float* wr = ws + so.r3w + j * kRgbH2;
+      float acc = ws[so.r3b + j];
+#pragma unroll
+      for (int k = 0; k < kRgbH2; ++k) acc += wr[k] * r2[k];
+      out_rgb[ray * 3 + j] = valid ? sigmoidf(acc) : 0.0f;
+    }
+  }
+}
+
+template <int F, int H, int PREC, int NS>
+int launch(const RenderArgs& a, hipStream_t st) {
+  typedef Smem<F, H, PREC, NS> L;
+  if constexpr (L::total > kLdsBytes) {
+    return sailor::set_error(SAILOR_ERR_ARG, "render_rays: %d-sample chunks need %zu bytes of "
+                             "LDS at this precision (limit %zu)", NS, L::total, kLdsBytes);
+  } else {
+  auto kern = render_kernel<F, H, PREC, NS>;
+  static int grid = 0;
+  if (grid == 0) {
+    SAILOR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)L::total));
+    int dev = 0, cus = 0, per_cu = 0;
+    SAILOR_HIP_TRY(hipGetDevice(&dev));
+    SAILOR_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    SAILOR_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads,
+                                                                L::total));
+    grid = cus * std::max(per_cu, 1);
+  }
+  const int blocks = (int)std::min<long long>(grid, a.n_rays);
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(kThreads), L::total, st, a);
+  if (int err = sailor::check_launch("render_rays")) return err;
+  const long long groups = (a.n_rays + kRaysPerGroup - 1) / kRaysPerGroup;
+  hipLaunchKernelGGL((hydra_rgb_kernel<F, H>), dim3((unsigned)groups), dim3(kThreads), 0, st,
+                     a.out_tok, a.t, a.S, a.wsmall, a.V, a.n_rays, a.out_rgb);
+  return sailor::check_launch("render_rays (hydra)");
+  }
+}
+
+template <int F, int H, int NS>
 int dispatch_prec(int prec, const RenderArgs& a, hipStream_t st) {
   switch (prec) {
-    case kF32: return launch<F, H, kF32>(a, st);
-    case kBF16: return launch<F, H, kBF16>(a, st);
-    case kF16: return launch<F, H, kF16>(a, st);
+    case kF32: return launch<F, H, kF32, NS>(a, st);
+    case kBF16: return launch<F, H, kBF16, NS>(a, st);
+    case kF16: return launch<F, H, kF16, NS>(a, st);
   }
   return sailor::set_error(SAILOR_ERR_ARG, "render_rays: precision %d unsupported", prec);
+}
+
+template <int F, int H>
+int dispatch_ns(int ns, int prec, const RenderArgs& a, hipStream_t st) {
+  switch (ns) {
+    case 8: return dispatch_prec<F, H, 8>(prec, a, st);
+    case 16: return dispatch_prec<F, H, 16>(prec, a, st);
+    case 24: return dispatch_prec<F, H, 24>(prec, a, st);
+    case 48: return dispatch_prec<F, H, 48>(prec, a, st);
+  }
+  return sailor::set_error(SAILOR_ERR_ARG, "render_rays: samples per chunk %d not "
+                           "instantiated (8, 16, 24, 48)", ns);
 }
 
 }  // namespace
@@ -504,26 +684,29 @@ extern "C" int sailor_render_rays(const float* t, const float* rays, int B, int 
                                   const float* bbox, const float* zdir, int H,
                                   int precision, const void* w_dense, const float* b_dense,
                                   const float* w_small, float t_thresh, int early_exit,
-                                  float* out_rgb, float* out_depth, float* out_aux,
-                                  float* out_occ, void* stream) {
-  SAILOR_REQUIRE(B > 0 && R >= 0, "render_rays: bad sizes");
-  SAILOR_REQUIRE(S > 0 && S % (4 * kNT) == 0, "render_rays: S must be a multiple of %d",
-                 4 * kNT);
+                                  int chunk_samples, float* out_rgb, float* out_depth,
+                                  float* out_aux, float* out_occ, float* tokens,
+                                  void* stream) {
+  SAILOR_REQUIRE(B > 0 && R >= 0 && S > 0, "render_rays: bad sizes");
+  SAILOR_REQUIRE(tokens != nullptr, "render_rays: token workspace missing");
   SAILOR_REQUIRE(V >= 1 && V <= kVP, "render_rays: 1 to %d views supported", kVP);
   SAILOR_REQUIRE(Hf > 1 && Wf > 1, "render_rays: feature map too small");
+  int ns = chunk_samples;
+  if (ns <= 0) ns = (S % 16 == 0) ? 16 : 8;
+  SAILOR_REQUIRE(S % ns == 0, "render_rays: S=%d is not a multiple of the chunk (%d samples)",
+                 S, ns);
   if (R == 0) return SAILOR_OK;
   RenderArgs a;
   a.t = t; a.rays = rays; a.Kv = Kv; a.RTv = RTv; a.fmap = fmap; a.bbox = bbox; a.zdir = zdir;
   a.wdense = w_dense; a.bdense = b_dense; a.wsmall = w_small;
   a.out_rgb = out_rgb; a.out_depth = out_depth; a.out_aux = out_aux; a.out_occ = out_occ;
+  a.out_tok = tokens;
   a.n_rays = (long long)B * R; a.rays_per_batch = R;
   a.S = S; a.V = V; a.img_w = img_w; a.img_h = img_h; a.Hf = Hf; a.Wf = Wf;
   a.t_thres = t_thresh; a.early_exit = early_exit;
   hipStream_t st = sailor::as_stream(stream);
-  if (F == 64 && H == 256) return dispatch_prec<64, 256>(precision, a, st);
-  if (F == 32 && H == 128) return dispatch_prec<32, 128>(precision, a, st);
-  if (F == 64 && H == 128) return dispatch_prec<64, 128>(precision, a, st);
-  if (F == 32 && H == 256) return dispatch_prec<32, 256>(precision, a, st);
+  if (F == 64 && H == 256) return dispatch_ns<64, 256>(ns, precision, a, st);
+  if (F == 32 && H == 128) return dispatch_ns<32, 128>(ns, precision, a, st);
   return sailor::set_error(SAILOR_ERR_ARG, "render_rays: (F=%d, H=%d) not instantiated "
-                           "(supported: F in {32,64}, H in {128,256})", F, H);
+                           "(supported: (32,128), (64,256))", F, H);
 }

@@ -1,7 +1,7 @@
-// Layout contract between the host-side weight packer and the fused render
-// kernel (mlp.hip).  Shared constants only; no code.
+// Layout contract between the host-side weight packer (sailor_amd/model.py)
+// and the fused render kernel (mlp.hip).  Shared constants only.
 //
-// MFMA operand layouts used (gfx950, wave64):
+// MFMA operand layouts (gfx950, wave64):
 //  fp32 v_mfma_f32_16x16x4f32   A: lane l holds A[l%16][l/16]
 //                               B: lane l holds B[l/16][l%16]
 //  bf16/fp16 v_mfma_f32_16x16x32_{bf16,f16}
@@ -9,21 +9,18 @@
 //                               B: lane l holds B[8*(l/16)+j][l%16]
 //  D (both): lane l holds D[4*(l/16)+i][l%16], i<4.
 //
-// The network is evaluated "transposed": columns (the MFMA N dimension) are
-// (sample, view) pairs, rows (M) are output units.  Column n of an N-tile is
-// sample n/4 (of four) and view n%4.  A layer's D tiles are then directly the
-// next layer's B fragments once the reduction index is permuted: for hidden
-// layers the k index of fragment slot (step s, lane group g, slot j) is
-//   fp32:  s = 4t + i           -> k = 16t + 4g + i
-//   bf16:  s, j<4 / j>=4        -> k = 32s + 4g + j  /  32s + 16 + 4g + (j-4)
-// and for the first layer (inputs [pe(63) | feat(F)], padded to 64 + F):
-//   lane group g owns feature channels [g*F/4, (g+1)*F/4) and PE dims
-//   [16g, 16g+16);
-//   fp32: s < F/4  -> feature channel g*F/4 + s;  else PE dim 16g + s - F/4
-//   bf16: s < F/32 -> feature channel g*F/4 + 8s + j;
-//         else PE dim 16g + 8(s - F/32) + j.
-// The packer (sailor_amd/model.py) writes fragment (layer, step, lane, m-tile)
-// so that one lane's A values for one step are contiguous.
+// The network runs transposed: MFMA columns (N) are (sample, view) pairs of
+// one ray, rows (M) are output units.  A workgroup of 4 waves owns one ray;
+// wave w computes output rows [w*H/4, (w+1)*H/4) for every column of a chunk
+// of samples.  Layer inputs live in LDS as act[column][k] in natural k order
+// (first layer: k < F feature channels, F <= k < F+63 positional encoding,
+// k = F+63 zero padding); B fragments are read from there, so the reduction
+// index of MFMA step s, lane group g (= l/16), slot j is
+//   fp32:  k = 16*(s/4) + 4*g + s%4      (one 16-byte read serves 4 steps)
+//   bf16:  k = 32*s + 8*g + j
+// The packer stores, for every layer and wave w, the A fragments as
+// [w][step][lane][m-tile][slot] so one lane's values of a step are
+// contiguous.
 #pragma once
 
 namespace sailor {
@@ -34,6 +31,7 @@ constexpr int kHydraInner = 32;
 constexpr int kRgbH1 = 64;
 constexpr int kRgbH2 = 32;
 constexpr int kHydraStacks = 2;
+constexpr int kWaves = 4;      // waves per workgroup (they split the rows)
 
 // Offsets (in floats) into the small-weight buffer for feature width F and
 // hidden width H.

@@ -13,7 +13,8 @@ the bilinear sample of view v's feature map at the projection of x.
 
 Weights are Xavier-uniform (seed 3130), biases zero.  `pack()` writes the
 dense layers as MFMA A-fragments in the order the fused kernel consumes them
-(layout in sailor_amd/csrc/mlp_layout.h).
+(layout in sailor_amd/csrc/mlp_layout.h): per layer and per wave (a quarter
+of the output rows each), per MFMA step, per lane.
 """
 from __future__ import annotations
 
@@ -92,9 +93,9 @@ class NerfWeights:
 
     def pack(self, prec="fp32", device="cuda"):
         """-> (w_dense tensor, b_dense [4,H] fp32, w_small fp32) on `device`."""
-        frags = [pack_layer(self.dense_w[0], self.F, self.H, prec, first=True)]
+        frags = [pack_layer(natural_first_layer(self.dense_w[0], self.F), prec)]
         for i in range(1, 4):
-            frags.append(pack_layer(self.dense_w[i], self.F, self.H, prec, first=False))
+            frags.append(pack_layer(self.dense_w[i], prec))
         dense = np.concatenate([f.reshape(-1) for f in frags])
         if prec == "fp32":
             wd = torch.from_numpy(dense.astype(np.float32))
@@ -106,53 +107,39 @@ class NerfWeights:
         return wd.to(device).contiguous(), bd.to(device).contiguous(), ws.to(device).contiguous()
 
 
-def _kmap(prec, F, H, first):
-    """Column index of W for every packed slot [KS, 64, EPL] (-1 = zero)."""
+def natural_first_layer(W0, F):
+    """First-layer weights in the kernel's input order: feature channels,
+    then the 63 PE dims, then one zero column (mlp_layout.h)."""
+    H = W0.shape[0]
+    return np.concatenate([W0[:, PE_DIMS:PE_DIMS + F], W0[:, :PE_DIMS],
+                           np.zeros((H, 1), np.float32)], 1)
+
+
+def kmap(prec, K):
+    """Reduction index of every packed slot [KS, 64 lanes, EPL] for a layer
+    with K inputs (mlp_layout.h)."""
     g = np.arange(64) // 16
-    cpl = F // 4
     if prec == "fp32":
-        if first:
-            ks = (F + PE_SLOTS) // 4
-            s = np.arange(ks)[:, None]
-            fs = cpl
-            feat = PE_DIMS + cpl * g[None, :] + s
-            pe = 16 * g[None, :] + (s - fs)
-            col = np.where(s < fs, feat, np.where(pe < PE_DIMS, pe, -1))
-        else:
-            ks = H // 4
-            s = np.arange(ks)[:, None]
-            col = 16 * (s >> 2) + 4 * g[None, :] + (s & 3)
-        return col[:, :, None]
-    # bf16 / fp16: 8 slots per lane
+        s = np.arange(K // 4)[:, None]
+        return (16 * (s >> 2) + 4 * g[None, :] + (s & 3))[:, :, None]
+    s = np.arange(K // 32)[:, None, None]
     j = np.arange(8)[None, None, :]
-    if first:
-        ks = (F + PE_SLOTS) // 32
-        fs = F // 32
-        s = np.arange(ks)[:, None, None]
-        gg = g[None, :, None]
-        feat = PE_DIMS + cpl * gg + 8 * s + j
-        pe = 16 * gg + 8 * (s - fs) + j
-        col = np.where(s < fs, feat, np.where(pe < PE_DIMS, pe, -1))
-    else:
-        ks = H // 32
-        s = np.arange(ks)[:, None, None]
-        gg = g[None, :, None]
-        col = np.where(j < 4, 32 * s + 4 * gg + j, 32 * s + 16 + 4 * gg + (j - 4))
-    return col
+    return 32 * s + 8 * g[None, :, None] + j
 
 
-def pack_layer(W, F, H, prec, first):
-    """A fragments of layer W [H, K] as [KS, 64 lanes, MT, EPL]."""
-    col = _kmap(prec, F, H, first)                  # [KS, 64, EPL]
-    MT = H // 16
-    Wz = np.concatenate([W, np.zeros((W.shape[0], 1), np.float32)], 1)
-    colz = np.where(col < 0, W.shape[1], col)
+def pack_layer(Wn, prec, waves=4):
+    """A fragments of a natural-order layer Wn [H, K] as
+    [waves, KS, 64 lanes, MT per wave, EPL]."""
+    H, K = Wn.shape
+    col = kmap(prec, K)                                 # [KS, 64, EPL]
+    MTW = H // (16 * waves)
     lane = np.arange(64)
-    rows = 16 * np.arange(MT)[None, None, :, None] + (lane % 16)[None, :, None, None]
-    cols = colz[:, :, None, :]                      # [KS, 64, 1, EPL]
-    rows = np.broadcast_to(rows, (col.shape[0], 64, MT, col.shape[2]))
-    cols = np.broadcast_to(cols, rows.shape)
-    return Wz[rows, cols]
+    w = np.arange(waves)[:, None, None, None, None]
+    mt = np.arange(MTW)[None, None, None, :, None]
+    rows = (H // waves) * w + 16 * mt + (lane % 16)[None, None, :, None, None]
+    cols = col[None, :, :, None, :]
+    rows, cols = np.broadcast_arrays(rows, cols)
+    return Wn[rows, cols]
 
 
 @dataclass

@@ -52,8 +52,9 @@ CONFIGS = {
 
 class FramePipeline:
     def __init__(self, cfg: FrameConfig, prec="fp32", device="cuda", early_exit=False,
-                 weights: NerfWeights = None):
+                 weights: NerfWeights = None, chunk_samples=0):
         self.cfg = cfg
+        self.chunk_samples = chunk_samples
         self.dev = torch.device(device)
         self.prec = prec
         self.early_exit = early_exit
@@ -105,6 +106,7 @@ class FramePipeline:
         self.rgb = torch.empty((1, nt, 3), dtype=_f32, **o)
         self.depth = torch.empty((1, nt), dtype=_f32, **o)
         self.aux = torch.empty((1, nt, 2), dtype=_f32, **o)
+        self.tokens = torch.empty((1, nt, 4, self.mc.F), dtype=_f32, **o)
         self.graph = None
 
     # ------------------------------------------------------------------
@@ -181,8 +183,8 @@ class FramePipeline:
              cfg.target * cfg.target, cfg.S, ptr(self.K), ptr(self.RT), cfg.n_views, cfg.img,
              cfg.img, ptr(self.fmap), mc.Hf, mc.Wf, mc.F, ptr(self.bbox), ptr(self.zdir), mc.H,
              PREC_IDS[m.prec], ptr(m.w_dense), ptr(m.b_dense), ptr(m.w_small), 1e-4,
-             int(self.early_exit), ptr(self.rgb), ptr(self.depth), ptr(self.aux), None,
-             stream_of(self.dev))
+             int(self.early_exit), int(self.chunk_samples), ptr(self.rgb), ptr(self.depth),
+             ptr(self.aux), None, ptr(self.tokens), stream_of(self.dev))
 
     def run(self):
         self.build()

@@ -47,7 +47,7 @@ class PackedModel:
 
 
 def render_rays(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, model: PackedModel,
-                t_thres=1e-4, early_exit=False, want_occ=False, out=None):
+                t_thres=1e-4, early_exit=False, want_occ=False, out=None, chunk_samples=0):
     """t [B,R,S] sorted; rays [B,R,6]; Kv [B,V,3,3]; RTv [B,V,3,4];
     fmap_cl [B,V,Hf,Wf,F] channels-last; bbox [B,3,2]; zdir [B,3].
     Returns dict(rgb [B,R,3], depth [B,R], aux [B,R,2], occ [B,R,S] or None)."""
@@ -62,9 +62,12 @@ def render_rays(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, model: PackedMode
                    depth=torch.empty((B, R), dtype=_f32, device=dev),
                    aux=torch.empty((B, R, 2), dtype=_f32, device=dev),
                    occ=torch.zeros((B, R, S), dtype=_f32, device=dev) if want_occ else None)
+    if out.get("tokens") is None:
+        out["tokens"] = torch.empty((B, R, 4, F), dtype=_f32, device=dev)
     call("sailor_render_rays", ptr(t), ptr(rays), B, R, S, ptr(Kv), ptr(RTv), V, int(img_hw[0]),
          int(img_hw[1]), ptr(fmap_cl), Hf, Wf, F, ptr(bbox), ptr(zdir), model.H,
          PREC_IDS[model.prec], ptr(model.w_dense), ptr(model.b_dense), ptr(model.w_small),
-         float(t_thres), int(bool(early_exit)), ptr(out["rgb"]), ptr(out["depth"]),
-         ptr(out["aux"]), ptr(out["occ"]), stream_of(dev))
+         float(t_thres), int(bool(early_exit)), int(chunk_samples), ptr(out["rgb"]),
+         ptr(out["depth"]),
+         ptr(out["aux"]), ptr(out["occ"]), ptr(out["tokens"]), stream_of(dev))
     return out

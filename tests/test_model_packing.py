@@ -1,60 +1,55 @@
 """CPU checks of the MFMA weight packing (sailor_amd/model.py) against the
-fragment rules the fused kernel uses (sailor_amd/csrc/mlp_layout.h):
-emulating the MFMA lane layouts, packed-A x kernel-built-B must equal W @ x."""
+fragment rules of the fused kernel (sailor_amd/csrc/mlp_layout.h):
+emulating the MFMA lane layouts, packed-A x kernel-read-B must equal W @ x."""
 import numpy as np
 import pytest
 
 from sailor_amd import model as M
 
 
-def kernel_b_index(prec, F, H, first, s, g, j):
-    """Input index the kernel places in B-fragment slot (step s, lane group g,
-    slot j) -- mlp.hip render_kernel layer lambdas (-1 = padding)."""
-    cpl = F // 4
+def kernel_b_index(prec, s, g, j):
+    """LDS row index the kernel reads for B slot (step s, lane group g, slot j)
+    (mlp.hip mfma_layer)."""
     if prec == "fp32":
-        if first:
-            fs = cpl
-            if s < fs:
-                return M.PE_DIMS + cpl * g + s
-            q = 16 * g + (s - fs)
-            return q if q < M.PE_DIMS else -1
         return 16 * (s >> 2) + 4 * g + (s & 3)
-    if first:
-        fs = F // 32
-        if s < fs:
-            return M.PE_DIMS + cpl * g + 8 * s + j
-        q = 16 * g + 8 * (s - fs) + j
-        return q if q < M.PE_DIMS else -1
-    return 32 * s + 4 * g + j if j < 4 else 32 * s + 16 + 4 * g + (j - 4)
+    return 32 * s + 8 * g + j
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
-@pytest.mark.parametrize("F,H", [(32, 128), (64, 256)])
-@pytest.mark.parametrize("first", [True, False])
-def test_packed_layer_times_fragments_equals_matvec(prec, F, H, first):
+@pytest.mark.parametrize("H,K", [(128, 96), (128, 128), (256, 128), (256, 256)])
+def test_packed_layer_times_fragments_equals_matvec(prec, H, K):
     rng = np.random.default_rng(0)
-    K = M.PE_DIMS + F if first else H
-    W = rng.standard_normal((H, K)).astype(np.float64)
+    W = rng.standard_normal((H, K)).astype(np.float32)
     X = rng.standard_normal((K, 16)).astype(np.float64)  # 16 columns = one N-tile
-    A = M.pack_layer(W.astype(np.float32), F, H, prec, first).astype(np.float64)
-    KS, _, MT, EPL = A.shape
+    A = M.pack_layer(W, prec).astype(np.float64)         # [waves, KS, 64, MTW, EPL]
+    waves, KS, _, MTW, EPL = A.shape
     D = np.zeros((H, 16))
-    seen = []
-    for s in range(KS):
-        for lane in range(64):
-            g, m = lane // 16, lane % 16
-            for j in range(EPL):
-                kk = kernel_b_index(prec, F, H, first, s, g, j)
-                seen.append(kk)
-                xval = X[kk] if kk >= 0 else np.zeros(16)
-                for mt in range(MT):
-                    # A[m][k-row] * B[k-row][n] summed over the k-rows of the step
-                    D[16 * mt + m] += A[s, lane, mt, j] * xval
-    np.testing.assert_allclose(D, W @ X, rtol=1e-5, atol=1e-4)
-    # every input appears exactly once per output row
-    vals = [k for k in seen[::1] if k >= 0]
-    counts = np.bincount(vals, minlength=K) // 16  # 16 m-lanes per k-row
-    assert (counts == 1).all()
+    count = np.zeros(K, int)
+    for w in range(waves):
+        for s in range(KS):
+            for lane in range(64):
+                g, m = lane // 16, lane % 16
+                for j in range(EPL):
+                    kk = kernel_b_index(prec, s, g, j)
+                    if w == 0 and m == 0:
+                        count[kk] += 1
+                    for mt in range(MTW):
+                        row = (H // waves) * w + 16 * mt + m
+                        D[row] += A[w, s, lane, mt, j] * X[kk]
+    np.testing.assert_allclose(D, W.astype(np.float64) @ X, rtol=1e-5, atol=1e-4)
+    assert (count == 1).all()  # each input is reduced exactly once per row
+
+
+def test_first_layer_natural_order():
+    rng = np.random.default_rng(1)
+    F, H = 64, 256
+    W0 = rng.standard_normal((H, M.PE_DIMS + F)).astype(np.float32)
+    Wn = M.natural_first_layer(W0, F)
+    pe = rng.standard_normal(M.PE_DIMS)
+    feat = rng.standard_normal(F)
+    x_ref = np.concatenate([pe, feat])            # model input order
+    x_ker = np.concatenate([feat, pe, [0.0]])     # kernel LDS order
+    np.testing.assert_allclose(Wn @ x_ker, W0 @ x_ref, rtol=1e-5, atol=1e-4)
 
 
 def test_small_buffer_matches_kernel_layout():
