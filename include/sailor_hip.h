@@ -227,6 +227,10 @@ int sailor_volume_rendering_training_backward(
  * given in sailor_amd/csrc/mlp_layout.h and written by sailor_amd/model.py. */
 size_t sailor_render_small_weights_floats(int F, int H);
 
+/* Bytes of device workspace sailor_render_rays needs for B x R rays of
+ * feature width F (0 for invalid sizes). */
+size_t sailor_render_workspace_bytes(int B, int R, int F);
+
 /* Fused form of the FastNerf inference path (c_lib/FastNerf/src/infer_mlp.cu:
  * infer_fast_rendering :638-789 -- density_feat_kernel per (point, view),
  * mean over views, density_sigma_kernel, per-view feature compositing,
@@ -252,8 +256,10 @@ size_t sailor_render_small_weights_floats(int F, int H);
  *              workgroup (fp32 with 48 samples) is rejected
  *  out_rgb f32 [B,R,3], out_depth f32 [B,R]; out_aux f32 [B,R,2] (weight sum,
  *  alpha sum) and out_occ f32 [B,R,S] (per-sample occupancy) may be NULL.
- *  tokens   f32 [B,R,4,F] workspace: composited per-view features handed
- *           from the sample kernel to the Hydra/RGB kernel (required).     */
+ *  workspace  device buffer of at least sailor_render_workspace_bytes(B, R,
+ *           F) bytes (list of rays with samples and their composited
+ *           per-view features, handed from the sample kernel to the
+ *           Hydra/RGB kernel); calls that overlap on one workspace race.    */
 int sailor_render_rays(const float* t, const float* rays, int B, int R, int S,
                        const float* Kv, const float* RTv, int V, int img_h,
                        int img_w, const float* fmap, int Hf, int Wf, int F,
@@ -261,7 +267,8 @@ int sailor_render_rays(const float* t, const float* rays, int B, int R, int S,
                        int precision, const void* w_dense, const float* b_dense,
                        const float* w_small, float t_thresh, int early_exit,
                        int chunk_samples, float* out_rgb, float* out_depth,
-                       float* out_aux, float* out_occ, float* tokens, void* stream);
+                       float* out_aux, float* out_occ, void* workspace,
+                       size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Frequency encoding                                                        */

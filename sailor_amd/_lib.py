@@ -38,7 +38,8 @@ _SIGS = {
     "sailor_volume_rendering_training_backward": [P] * 12 + [I, I, I, I, F, I, P, P, P],
     "sailor_render_small_weights_floats": [I, I],
     "sailor_render_rays": [P, P, I, I, I, P, P, I, I, I, P, I, I, I, P, P, I, I, P, P, P, F, I,
-                           I, P, P, P, P, P, P],
+                           I, P, P, P, P, P, SZ, P],
+    "sailor_render_workspace_bytes": [I, I, I],
     "sailor_freq_encode_forward": [P, I, I, I, P, P],
     "sailor_freq_encode_backward": [P, P, I, I, I, P, P],
 }
@@ -46,6 +47,7 @@ _RESTYPES = {
     "sailor_last_error": ctypes.c_char_p,
     "sailor_octree_workspace_bytes": SZ,
     "sailor_render_small_weights_floats": SZ,
+    "sailor_render_workspace_bytes": SZ,
 }
 
 _lib = None

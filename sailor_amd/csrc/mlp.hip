@@ -64,7 +64,9 @@ struct RenderArgs {
   const float* bdense;     // [4, H] biases
   const float* wsmall;     // occupancy head, Hydra, RGB head (mlp_layout.h)
   float* out_rgb;          // [n_rays, 3]
-  float* out_tok;          // [n_rays, 4, F] composited per-view features (workspace)
+  float* out_tok;          // [n_valid, 4, F] composited per-view features (workspace)
+  const int* list;         // [n_valid] rays with samples, in list order (workspace)
+  const unsigned* n_valid; // device count of list entries (workspace)
   float* out_depth;        // [n_rays]
   float* out_aux;          // [n_rays, 2] (weight sum, alpha sum) or null
   float* out_occ;          // [n_rays, S] per-sample occupancy or null
@@ -107,11 +109,18 @@ __device__ __forceinline__ void mfma_layer(const typename Tr<PREC>::E* __restric
   for (int mt = 0; mt < MTW; ++mt)
 #pragma unroll
     for (int nt = 0; nt < NTL; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // Weight (A) fragments come from L2 and are fetched ahead; activation (B)
+  // fragments come from LDS and are read one step ahead.  The scheduling
+  // barriers keep both prefetches where they are written: left alone, the
+  // scheduler sinks the loads next to their first use and exposes the full
+  // memory latency at every step.
   if constexpr (PREC == kF32) {
-    // weights for 4 steps (one 16-byte B read) are fetched one group ahead
+    // one 16-byte B read serves 4 MFMA steps (a group); weights of a group
+    // are fetched one group ahead
     static_assert(KS % 4 == 0, "fp32 steps come in groups of 4");
     constexpr int NG = KS / 4;
     float abuf[2][4][MTW];
+    f32x4 bbuf[2][NTL];
     auto load_group = [&](int q, float (&dst)[4][MTW]) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -125,22 +134,30 @@ __device__ __forceinline__ void mfma_layer(const typename Tr<PREC>::E* __restric
         }
       }
     };
-    load_group(0, abuf[0]);
-#pragma unroll
-    for (int q = 0; q < NG; ++q) {
-      if (q + 1 < NG) load_group(q + 1, abuf[(q + 1) & 1]);
-      f32x4 b4[NTL];
+    auto read_b = [&](int q, f32x4 (&dst)[NTL]) {
 #pragma unroll
       for (int nt = 0; nt < NTL; ++nt)
-        b4[nt] = *reinterpret_cast<const f32x4*>(act + (16 * nt + n) * KP + 16 * q + 4 * g);
+        dst[nt] = *reinterpret_cast<const f32x4*>(act + (16 * nt + n) * KP + 16 * q + 4 * g);
+    };
+    load_group(0, abuf[0]);
+    read_b(0, bbuf[0]);
+#pragma unroll
+    for (int q = 0; q < NG; ++q) {
+      if (q + 1 < NG) {
+        load_group(q + 1, abuf[(q + 1) & 1]);
+        read_b(q + 1, bbuf[(q + 1) & 1]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int nt = 0; nt < NTL; ++nt)
 #pragma unroll
           for (int mt = 0; mt < MTW; ++mt)
-            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(abuf[q & 1][i][mt], b4[nt][i],
-                                                               acc[mt][nt], 0, 0, 0);
+            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(abuf[q & 1][i][mt],
+                                                               bbuf[q & 1][nt][i], acc[mt][nt],
+                                                               0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
     }
   } else {
     // weights are fetched PD steps ahead (PD + 1 rotating buffers); wide
@@ -148,29 +165,40 @@ __device__ __forceinline__ void mfma_layer(const typename Tr<PREC>::E* __restric
     typedef typename Tr<PREC>::V8 V8;
     constexpr int PD = NTL >= 12 ? 1 : 2, NB = PD + 1;
     V8 abuf[NB][MTW];
+    V8 bbuf[2][NTL];
     auto load_step = [&](int s, V8 (&dst)[MTW]) {
       const V8* ap = reinterpret_cast<const V8*>(W + ((size_t)s * 64 + lane) * MTW * 8);
 #pragma unroll
       for (int mt = 0; mt < MTW; ++mt) dst[mt] = ap[mt];
     };
+    auto read_b = [&](int s, V8 (&dst)[NTL]) {
+#pragma unroll
+      for (int nt = 0; nt < NTL; ++nt)
+        dst[nt] = *reinterpret_cast<const V8*>(act + (16 * nt + n) * KP + 32 * s + 8 * g);
+    };
 #pragma unroll
     for (int s = 0; s < PD && s < KS; ++s) load_step(s, abuf[s]);
+    read_b(0, bbuf[0]);
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
       if (s + PD < KS) load_step(s + PD, abuf[(s + PD) % NB]);
+      if (s + 1 < KS) read_b(s + 1, bbuf[(s + 1) & 1]);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int nt = 0; nt < NTL; ++nt) {
-        const V8 b = *reinterpret_cast<const V8*>(act + (16 * nt + n) * KP + 32 * s + 8 * g);
 #pragma unroll
         for (int mt = 0; mt < MTW; ++mt) {
           if constexpr (PREC == kBF16)
-            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(abuf[s % NB][mt], b,
-                                                                  acc[mt][nt], 0, 0, 0);
+            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(abuf[s % NB][mt],
+                                                                  bbuf[s & 1][nt], acc[mt][nt],
+                                                                  0, 0, 0);
           else
-            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(abuf[s % NB][mt], b,
-                                                                 acc[mt][nt], 0, 0, 0);
+            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(abuf[s % NB][mt],
+                                                                 bbuf[s & 1][nt], acc[mt][nt],
+                                                                 0, 0, 0);
         }
       }
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
 }
@@ -255,14 +283,14 @@ render_kernel(RenderArgs a) {
   const size_t whoff = (size_t)wave * KSH * 64 * MTW * EPL;
   const float inv_v = 1.0f / (float)V;
 
-  for (long long ray = blockIdx.x; ray < a.n_rays; ray += gridDim.x) {
+  // every list entry costs about the same, so a static stride over the
+  // compacted list balances the workgroups (dropped rays were finished by the
+  // compaction kernel)
+  const long long n_valid = *a.n_valid;
+  for (long long item = blockIdx.x; item < n_valid; item += gridDim.x) {
     __syncthreads();  // LDS of the previous ray is no longer read
+    const long long ray = a.list[item];
     const float* trow = a.t + ray * S;
-    if (!(trow[0] >= 0.0f)) {  // no full set of samples: the ray is dropped
-      if (tid == 0) a.out_depth[ray] = 0.0f;
-      if (a.out_aux && tid >= 1 && tid < 3) a.out_aux[ray * 2 + tid - 1] = 0.0f;
-      continue;
-    }
     const int b = (int)(ray / a.rays_per_batch);
     const float ox = a.rays[ray * 6 + 0], oy = a.rays[ray * 6 + 1], oz = a.rays[ray * 6 + 2];
     const float dx = a.rays[ray * 6 + 3], dy = a.rays[ray * 6 + 4], dz = a.rays[ray * 6 + 5];
@@ -425,7 +453,7 @@ render_kernel(RenderArgs a) {
     }
     __syncthreads();
     // ---- outputs: per-view tokens for the Hydra/RGB kernel, depth, sums ----
-    if (tid < kVP * F) a.out_tok[ray * kVP * F + tid] = (tid / F) < V ? tok : 0.0f;
+    if (tid < kVP * F) a.out_tok[item * kVP * F + tid] = (tid / F) < V ? tok : 0.0f;
     if (tid == 0) {
       const float* zd = a.zdir + b * 3;
       a.out_depth[ray] = dsum * (zd[0] * dx + zd[1] * dy + zd[2] * dz);
@@ -482,33 +510,30 @@ __device__ __forceinline__ void load_rows(const float* rows, int lane, float (&d
 
 template <int F, int H>
 __global__ void __launch_bounds__(kThreads)
-hydra_rgb_kernel(const float* __restrict__ tok, const float* __restrict__ t, int S,
-                 const float* __restrict__ ws, int V, long long n_rays,
+hydra_rgb_kernel(const float* __restrict__ tok, const int* __restrict__ list,
+                 const unsigned* __restrict__ n_valid_ptr, const float* __restrict__ ws, int V,
                  float* __restrict__ out_rgb) {
   typedef HydraRows<F> R;
   constexpr SmallOffsets so = small_offsets(F, H);
   constexpr int stack_stride = so.q[1] - so.q[0];
   static_assert(F % kWaves == 0 && kRgbH1 % kWaves == 0 && kRgbH2 % kWaves == 0, "split");
   __shared__ float sm[R::ROWS * kLd];
-  __shared__ int s_valid[kRaysPerGroup];
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const long long ray0 = (long long)blockIdx.x * kRaysPerGroup;
+  // the group is list entries [item0, item0 + 16); the grid covers every ray
+  // and groups past the device-side count leave at once
+  const long long n_valid = *n_valid_ptr;
+  const long long item0 = (long long)blockIdx.x * kRaysPerGroup;
+  if (item0 >= n_valid) return;
   const int view = lane & 3;
-  const long long ray = ray0 + (lane >> 2);
-  bool valid = false;
-  if (tid < kTok && ray < n_rays) valid = t[ray * S] >= 0.0f;  // dropped rays have t < 0
-  if (tid < kTok && view == 0) s_valid[lane >> 2] = valid ? 1 : 0;
-  if (!__syncthreads_or(valid)) {
-    if (tid < 3 * kRaysPerGroup && ray0 + tid / 3 < n_rays) out_rgb[ray0 * 3 + tid] = 0.0f;
-    return;
-  }
+  const long long item = item0 + (lane >> 2);
+  const bool valid = item < n_valid;
   // tokens of the group are one contiguous block of 64 x F floats
+  const long long n_floats = (n_valid - item0) * kVP * F;
   for (int i = tid; i < kTok * F; i += kThreads) {
     const int tk = i / F, c = i % F;
-    const long long r = ray0 + tk / kVP;
-    sm[(R::X + c) * kLd + tk] = (r < n_rays && s_valid[tk / kVP]) ? tok[ray0 * kVP * F + i] : 0.0f;
+    sm[(R::X + c) * kLd + tk] = i < n_floats ? tok[item0 * kVP * F + i] : 0.0f;
   }
   __syncthreads();
   const float vmask = view < V ? 1.0f : 0.0f;
@@ -606,7 +631,7 @@ hydra_rgb_kernel(const float* __restrict__ tok, const float* __restrict__ t, int
                                   lane);
   }
   __syncthreads();
-  if (wave == 0 && view == 0 && ray < n_rays) {
+  if (wave == 0 && view == 0 && valid) {
     float r2[kRgbH2];
     load_rows(sm + R::HID * kLd, lane, r2);
 #pragma unroll
@@ -615,7 +640,46 @@ hydra_rgb_kernel(const float* __restrict__ tok, const float* __restrict__ t, int
       float acc = ws[so.r3b + j];
 #pragma unroll
       for (int k = 0; k < kRgbH2; ++k) acc += wr[k] * r2[k];
-      out_rgb[ray * 3 + j] = valid ? sigmoidf(acc) : 0.0f;
+      out_rgb[(long long)list[item] * 3 + j] = sigmoidf(acc);
+    }
+  }
+}
+
+// Workspace: [counter (256 B)] [list: n_rays int32, 256-B aligned]
+// [tokens: n_rays x 4 x F f32].
+size_t workspace_list_offset() { return 256; }
+size_t workspace_token_offset(long long n_rays) {
+  return 256 + ((size_t)n_rays * 4 + 255) / 256 * 256;
+}
+
+__global__ void queue_reset(unsigned* counter) {
+  if (threadIdx.x == 0) counter[0] = 0u;
+}
+
+// Appends every ray with a full set of samples (t[ray, 0] >= 0) to the list
+// (one atomic per wave; list order is arbitrary, results do not depend on it)
+// and writes the outputs of dropped rays (zero colour, depth and sums).
+__global__ void __launch_bounds__(256)
+compact_rays(const float* __restrict__ t, int S, long long n_rays, unsigned* counter,
+             int* list, float* out_rgb, float* out_depth, float* out_aux) {
+  const long long ray = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = ray < n_rays;
+  const bool valid = in && t[ray * S] >= 0.0f;
+  const unsigned long long mask = __ballot(valid);
+  const int lane = threadIdx.x & 63;
+  unsigned base = 0;
+  if (lane == 0 && mask) base = atomicAdd(counter, (unsigned)__popcll(mask));
+  base = __shfl(base, 0, 64);
+  if (valid) {
+    list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int)ray;
+  } else if (in) {
+    out_rgb[ray * 3 + 0] = 0.0f;
+    out_rgb[ray * 3 + 1] = 0.0f;
+    out_rgb[ray * 3 + 2] = 0.0f;
+    out_depth[ray] = 0.0f;
+    if (out_aux) {
+      out_aux[ray * 2 + 0] = 0.0f;
+      out_aux[ray * 2 + 1] = 0.0f;
     }
   }
 }
@@ -645,7 +709,7 @@ int launch(const RenderArgs& a, hipStream_t st) {
   if (int err = sailor::check_launch("render_rays")) return err;
   const long long groups = (a.n_rays + kRaysPerGroup - 1) / kRaysPerGroup;
   hipLaunchKernelGGL((hydra_rgb_kernel<F, H>), dim3((unsigned)groups), dim3(kThreads), 0, st,
-                     a.out_tok, a.t, a.S, a.wsmall, a.V, a.n_rays, a.out_rgb);
+                     a.out_tok, a.list, a.n_valid, a.wsmall, a.V, a.out_rgb);
   return sailor::check_launch("render_rays (hydra)");
   }
 }
@@ -678,6 +742,12 @@ extern "C" size_t sailor_render_small_weights_floats(int F, int H) {
   return (size_t)small_offsets(F, H).total;
 }
 
+extern "C" size_t sailor_render_workspace_bytes(int B, int R, int F) {
+  if (B <= 0 || R < 0 || F <= 0) return 0;
+  const long long n = (long long)B * R;
+  return workspace_token_offset(n) + (size_t)n * kVP * F * sizeof(float);
+}
+
 extern "C" int sailor_render_rays(const float* t, const float* rays, int B, int R, int S,
                                   const float* Kv, const float* RTv, int V, int img_h,
                                   int img_w, const float* fmap, int Hf, int Wf, int F,
@@ -685,10 +755,14 @@ extern "C" int sailor_render_rays(const float* t, const float* rays, int B, int 
                                   int precision, const void* w_dense, const float* b_dense,
                                   const float* w_small, float t_thresh, int early_exit,
                                   int chunk_samples, float* out_rgb, float* out_depth,
-                                  float* out_aux, float* out_occ, float* tokens,
-                                  void* stream) {
+                                  float* out_aux, float* out_occ, void* workspace,
+                                  size_t workspace_bytes, void* stream) {
   SAILOR_REQUIRE(B > 0 && R >= 0 && S > 0, "render_rays: bad sizes");
-  SAILOR_REQUIRE(tokens != nullptr, "render_rays: token workspace missing");
+  SAILOR_REQUIRE((long long)B * R < (1ll << 31), "render_rays: at most 2^31-1 rays per call");
+  SAILOR_REQUIRE(workspace != nullptr &&
+                     workspace_bytes >= sailor_render_workspace_bytes(B, R, F),
+                 "render_rays: workspace of %zu bytes is smaller than the %zu required",
+                 workspace_bytes, sailor_render_workspace_bytes(B, R, F));
   SAILOR_REQUIRE(V >= 1 && V <= kVP, "render_rays: 1 to %d views supported", kVP);
   SAILOR_REQUIRE(Hf > 1 && Wf > 1, "render_rays: feature map too small");
   int ns = chunk_samples;
@@ -700,11 +774,35 @@ extern "C" int sailor_render_rays(const float* t, const float* rays, int B, int 
   a.t = t; a.rays = rays; a.Kv = Kv; a.RTv = RTv; a.fmap = fmap; a.bbox = bbox; a.zdir = zdir;
   a.wdense = w_dense; a.bdense = b_dense; a.wsmall = w_small;
   a.out_rgb = out_rgb; a.out_depth = out_depth; a.out_aux = out_aux; a.out_occ = out_occ;
-  a.out_tok = tokens;
   a.n_rays = (long long)B * R; a.rays_per_batch = R;
+  char* wsb = static_cast<char*>(workspace);
+  unsigned* counter = reinterpret_cast<unsigned*>(wsb);
+  int* list = reinterpret_cast<int*>(wsb + workspace_list_offset());
+  a.out_tok = reinterpret_cast<float*>(wsb + workspace_token_offset(a.n_rays));
+  a.list = list;
+  a.n_valid = counter;
   a.S = S; a.V = V; a.img_w = img_w; a.img_h = img_h; a.Hf = Hf; a.Wf = Wf;
   a.t_thres = t_thresh; a.early_exit = early_exit;
   hipStream_t st = sailor::as_stream(stream);
+  if (!((F == 64 && H == 256) || (F == 32 && H == 128))) {
+    return sailor::set_error(SAILOR_ERR_ARG, "render_rays: (F=%d, H=%d) not instantiated "
+                             "(supported: (32,128), (64,256))", F, H);
+  }
+  if (precision < kF32 || precision > kF16)
+    return sailor::set_error(SAILOR_ERR_ARG, "render_rays: precision %d unsupported", precision);
+  if (ns != 8 && ns != 16 && ns != 24 && ns != 48)
+    return sailor::set_error(SAILOR_ERR_ARG, "render_rays: samples per chunk %d not "
+                             "instantiated (8, 16, 24, 48)", ns);
+  if (precision == kF32 && ns == 48)
+    return sailor::set_error(SAILOR_ERR_ARG, "render_rays: 48-sample chunks do not fit the LDS "
+                             "of a workgroup in fp32");
+  // the counter is cleared by a kernel, not a memset, so the call can be
+  // captured into a graph
+  hipLaunchKernelGGL(queue_reset, dim3(1), dim3(64), 0, st, counter);
+  if (int err = sailor::check_launch("render_rays (reset)")) return err;
+  hipLaunchKernelGGL(compact_rays, dim3((unsigned)((a.n_rays + 255) / 256)), dim3(256), 0, st, t,
+                     S, a.n_rays, counter, list, out_rgb, out_depth, out_aux);
+  if (int err = sailor::check_launch("render_rays (compact)")) return err;
   if (F == 64 && H == 256) return dispatch_ns<64, 256>(ns, precision, a, st);
   if (F == 32 && H == 128) return dispatch_ns<32, 128>(ns, precision, a, st);
   return sailor::set_error(SAILOR_ERR_ARG, "render_rays: (F=%d, H=%d) not instantiated "

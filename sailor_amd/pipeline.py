@@ -106,7 +106,8 @@ class FramePipeline:
         self.rgb = torch.empty((1, nt, 3), dtype=_f32, **o)
         self.depth = torch.empty((1, nt), dtype=_f32, **o)
         self.aux = torch.empty((1, nt, 2), dtype=_f32, **o)
-        self.tokens = torch.empty((1, nt, 4, self.mc.F), dtype=_f32, **o)
+        self.render_ws = torch.empty(_lib.lib().sailor_render_workspace_bytes(1, nt, self.mc.F),
+                                     dtype=torch.uint8, **o)
         self.graph = None
 
     # ------------------------------------------------------------------
@@ -184,7 +185,8 @@ class FramePipeline:
              cfg.img, ptr(self.fmap), mc.Hf, mc.Wf, mc.F, ptr(self.bbox), ptr(self.zdir), mc.H,
              PREC_IDS[m.prec], ptr(m.w_dense), ptr(m.b_dense), ptr(m.w_small), 1e-4,
              int(self.early_exit), int(self.chunk_samples), ptr(self.rgb), ptr(self.depth),
-             ptr(self.aux), None, ptr(self.tokens), stream_of(self.dev))
+             ptr(self.aux), None, ptr(self.render_ws), self.render_ws.numel(),
+             stream_of(self.dev))
 
     def run(self):
         self.build()

@@ -62,12 +62,14 @@ def render_rays(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, model: PackedMode
                    depth=torch.empty((B, R), dtype=_f32, device=dev),
                    aux=torch.empty((B, R, 2), dtype=_f32, device=dev),
                    occ=torch.zeros((B, R, S), dtype=_f32, device=dev) if want_occ else None)
-    if out.get("tokens") is None:
-        out["tokens"] = torch.empty((B, R, 4, F), dtype=_f32, device=dev)
+    ws_bytes = _lib.lib().sailor_render_workspace_bytes(B, R, F)
+    if out.get("workspace") is None or out["workspace"].numel() < ws_bytes:
+        out["workspace"] = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     call("sailor_render_rays", ptr(t), ptr(rays), B, R, S, ptr(Kv), ptr(RTv), V, int(img_hw[0]),
          int(img_hw[1]), ptr(fmap_cl), Hf, Wf, F, ptr(bbox), ptr(zdir), model.H,
          PREC_IDS[model.prec], ptr(model.w_dense), ptr(model.b_dense), ptr(model.w_small),
          float(t_thres), int(bool(early_exit)), int(chunk_samples), ptr(out["rgb"]),
          ptr(out["depth"]),
-         ptr(out["aux"]), ptr(out["occ"]), ptr(out["tokens"]), stream_of(dev))
+         ptr(out["aux"]), ptr(out["occ"]), ptr(out["workspace"]),
+         out["workspace"].numel(), stream_of(dev))
     return out
