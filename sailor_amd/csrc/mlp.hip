@@ -19,8 +19,16 @@
 //  * the per-(sample, view) MLP runs transposed on MFMA: columns are the NC
 //    (sample, view) pairs, each wave owns a quarter of the output rows, so a
 //    weight fragment read from L2 is reused across all NC/16 column tiles;
-//  * layer inputs / activations of the chunk live in LDS (rows padded by 16
-//    bytes so the 16-byte B-fragment reads of 16 columns hit distinct banks);
+//  * layer inputs / activations of the chunk live in LDS as act[col][k],
+//    unpadded rows of H elements whose 16-byte chunks are XOR-swizzled by the
+//    column (chunk c of column col is stored at c ^ (col & 15)).  Measured on
+//    gfx950 with the SQ bank-conflict counter, this is the layout (among
+//    padded strides of 16/32 mod 256 bytes and partial swizzles) for which
+//    both the 16-byte B-fragment reads and the 16-byte stores of the layer
+//    outputs are conflict-free;
+//  * each lane's accumulators hold 4*MTW consecutive output units of one
+//    column (the packer orders the weight rows so), so a layer's outputs are
+//    written back with whole 16-byte stores;
 //    the fp32 accumulators of a layer are written back as the next layer's
 //    input after ReLU (bf16/fp16 forms store the activations in 16 bit);
 //  * feature gather (channels-last maps: 4 lanes fetch the 4 quarters of a
@@ -33,6 +41,8 @@
 //    workgroup: lane = token, so the 4 views of a ray are 4 adjacent lanes
 //    (cross-view sums are two shuffles), the weights are wave-uniform reads
 //    shared by all 64 tokens, and the 4 waves split each layer's outputs.
+#include <type_traits>
+
 #include "common.h"
 #include "mlp_layout.h"
 
@@ -80,188 +90,280 @@ struct RenderArgs {
 template <int PREC> struct Tr;
 template <> struct Tr<kF32> {
   typedef float E;
-  static constexpr int EPL = 1, PAD = 4;
+  static constexpr int EPL = 1, CE = 4;  // CE: elements per 16-byte chunk
 };
 template <> struct Tr<kBF16> {
   typedef __bf16 E;
   typedef bf16x8 V8;
   typedef bf16x4 V4;
-  static constexpr int EPL = 8, PAD = 8;
+  static constexpr int EPL = 8, CE = 8;
 };
 template <> struct Tr<kF16> {
   typedef _Float16 E;
   typedef f16x8 V8;
   typedef f16x4 V4;
-  static constexpr int EPL = 8, PAD = 8;
+  static constexpr int EPL = 8, CE = 8;
 };
 
 __device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
 
-// acc[mt][nt] = W_wave(layer) x act for KS MFMA steps (see mlp_layout.h).
-template <int H, int PREC, int NTL, int KS>
-__device__ __forceinline__ void mfma_layer(const typename Tr<PREC>::E* __restrict__ W,
-                                           const typename Tr<PREC>::E* act, int lane,
-                                           f32x4 (&acc)[H / 64][NTL]) {
-  constexpr int MTW = H / 64;
-  constexpr int KP = H + Tr<PREC>::PAD;
+// Element offset of 16-byte chunk `chunk` of column `col` in the swizzled
+// activation layout.
+template <int H, int PREC>
+__device__ __forceinline__ int act_chunk(int col, int chunk) {
+  return col * H + ((chunk ^ (col & 15)) * Tr<PREC>::CE);
+}
+
+// Output unit of accumulator element i of m-tile mt held by lane group g in
+// wave w: lane (n, g) holds units [w*H/4 + 4*MTW*g, +4*MTW) of column n.
+template <int H>
+__device__ __forceinline__ int unit_of(int wave, int g, int mt, int i) {
+  return (H / 4) * wave + 4 * (H / 64) * g + 4 * mt + i;
+}
+
+// act[col][unit] = relu(acc + bias) for this wave's units (16-byte stores).
+template <int H, int PREC, int NTL>
+__device__ __forceinline__ void write_act(typename Tr<PREC>::E* act, const float* bias,
+                                          int wave, int lane, const f32x4 (&acc)[H / 64][NTL]) {
+  typedef typename Tr<PREC>::E E;
+  constexpr int MTW = H / 64, CE = Tr<PREC>::CE;
+  constexpr int NV = 4 * MTW;  // consecutive units per lane and column
   const int n = lane & 15, g = lane >> 4;
+  const int u0 = unit_of<H>(wave, g, 0, 0);
+  float bv[NV];
 #pragma unroll
-  for (int mt = 0; mt < MTW; ++mt)
+  for (int q = 0; q < NV; q += 4) {
+    const f32x4 b4 = *reinterpret_cast<const f32x4*>(bias + u0 + q);
+    bv[q] = b4[0]; bv[q + 1] = b4[1]; bv[q + 2] = b4[2]; bv[q + 3] = b4[3];
+  }
 #pragma unroll
-    for (int nt = 0; nt < NTL; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-  // Weight (A) fragments come from L2 and are fetched ahead; activation (B)
-  // fragments come from LDS and are read one step ahead.  The scheduling
-  // barriers keep both prefetches where they are written: left alone, the
-  // scheduler sinks the loads next to their first use and exposes the full
-  // memory latency at every step.
-  if constexpr (PREC == kF32) {
-    // one 16-byte B read serves 4 MFMA steps (a group); weights of a group
-    // are fetched one group ahead
-    static_assert(KS % 4 == 0, "fp32 steps come in groups of 4");
-    constexpr int NG = KS / 4;
-    float abuf[2][4][MTW];
-    f32x4 bbuf[2][NTL];
-    auto load_group = [&](int q, float (&dst)[4][MTW]) {
+  for (int nt = 0; nt < NTL; ++nt) {
+    const int col = 16 * nt + n;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float* ap = W + ((size_t)(4 * q + i) * 64 + lane) * MTW;
-        if constexpr (MTW == 4) {
-          const f32x4 v = *reinterpret_cast<const f32x4*>(ap);
-          dst[i][0] = v[0]; dst[i][1] = v[1]; dst[i][2] = v[2]; dst[i][3] = v[3];
-        } else {
+    for (int c = 0; c < NV / CE; ++c) {
+      E* dst = act + act_chunk<H, PREC>(col, u0 / CE + c);
+      if constexpr (PREC == kF32) {
+        f32x4 v;
 #pragma unroll
-          for (int mt = 0; mt < MTW; ++mt) dst[i][mt] = ap[mt];
+        for (int e = 0; e < 4; ++e) {
+          const int q = 4 * c + e;
+          v[e] = fmaxf(acc[q / 4][nt][q % 4] + bv[q], 0.0f);
         }
-      }
-    };
-    auto read_b = [&](int q, f32x4 (&dst)[NTL]) {
+        *reinterpret_cast<f32x4*>(dst) = v;
+      } else {
+        typename Tr<PREC>::V8 h;
 #pragma unroll
-      for (int nt = 0; nt < NTL; ++nt)
-        dst[nt] = *reinterpret_cast<const f32x4*>(act + (16 * nt + n) * KP + 16 * q + 4 * g);
-    };
-    load_group(0, abuf[0]);
-    read_b(0, bbuf[0]);
-#pragma unroll
-    for (int q = 0; q < NG; ++q) {
-      if (q + 1 < NG) {
-        load_group(q + 1, abuf[(q + 1) & 1]);
-        read_b(q + 1, bbuf[(q + 1) & 1]);
+        for (int e = 0; e < 8; ++e) {
+          const int q = 8 * c + e;
+          h[e] = (E)fmaxf(acc[q / 4][nt][q % 4] + bv[q], 0.0f);
+        }
+        *reinterpret_cast<typename Tr<PREC>::V8*>(dst) = h;
       }
-      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+// Per-precision shape of the MFMA work.  A "unit" is fp32: 4 MFMA steps
+// that share one 16-byte B read; 16-bit: one step.  Weight (A) fragments of a
+// unit are fetched PD units ahead (NB = PD + 1 rotating buffers, NB divides
+// the units of a hidden layer so the rotation is the same in every hidden
+// layer); activation (B) fragments are read from LDS one unit ahead.
+template <int H, int PREC, int NTL>
+struct Chain {
+  static constexpr bool F32 = PREC == kF32;
+  static constexpr int MTW = H / 64;
+  static constexpr int KPU = F32 ? 16 : 32;   // reduction index per unit
+  static constexpr int UE = F32 ? 4 : 8;      // elements per lane per m-tile per unit
+  static constexpr int UH = H / KPU;
+  // 16-bit: three units ahead where registers allow (one wave per SIMD, up
+  // to 8 column tiles), else one
+  static constexpr int PD = F32 ? 1 : ((NTL > 4 && NTL < 12) ? 3 : 1);
+  static constexpr int NB = PD + 1;
+  static constexpr int AN = F32 ? 4 : MTW;    // fp32: 4 steps x f32x4 over m-tiles
+  typedef typename Tr<PREC>::E E;
+  typedef typename std::conditional<F32, f32x4, typename Tr<F32 ? kBF16 : PREC>::V8>::type V;
+  static constexpr size_t unit_bytes = (size_t)64 * MTW * UE * sizeof(E);
+  static_assert(UH % NB == 0, "hidden-layer units must fill whole buffer rotations");
+};
+
+template <int H, int PREC, int NTL>
+__device__ __forceinline__ void load_a(const char* base, int j, unsigned lane_off,
+                                       typename Chain<H, PREC, NTL>::V (&dst)[Chain<H, PREC, NTL>::AN]) {
+  typedef Chain<H, PREC, NTL> C;
+  const char* ub = base + (size_t)j * C::unit_bytes + lane_off;
+  if constexpr (C::F32) {
+    // lane data of step i: MTW floats at (i * 64 + lane) * MTW
+    static_assert(C::MTW == 4 || C::MTW == 2, "fp32 A unit layout");
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const char* ap = ub + (size_t)i * 64 * C::MTW * sizeof(float);
+      if constexpr (C::MTW == 4) {
+        dst[i] = *reinterpret_cast<const f32x4*>(ap);
+      } else {
+        const f32x2 v = *reinterpret_cast<const f32x2*>(ap);
+        dst[i] = f32x4{v[0], v[1], 0.f, 0.f};
+      }
+    }
+  } else {
+    const typename C::V* ap = reinterpret_cast<const typename C::V*>(ub);
+#pragma unroll
+    for (int mt = 0; mt < C::MTW; ++mt) dst[mt] = ap[mt];
+  }
+}
+
+template <int H, int PREC, int NTL>
+__device__ __forceinline__ void read_b(const typename Tr<PREC>::E* act, int j, int n, int g,
+                                       typename Chain<H, PREC, NTL>::V (&dst)[NTL]) {
+  typedef Chain<H, PREC, NTL> C;
+  // unit j covers chunks 4j..4j+3; lane group g reads chunk 4j + g
+#pragma unroll
+  for (int nt = 0; nt < NTL; ++nt)
+    dst[nt] = *reinterpret_cast<const typename C::V*>(act + act_chunk<H, PREC>(16 * nt + n,
+                                                                               4 * j + g));
+}
+
+// UL units of one layer whose first PD units are already in the A buffers
+// (unit j in buffer (O + j) % NB) and whose first B unit is in bbuf[0]; the
+// loads that run past the layer's end fetch the first PD units of the layer
+// at `nxt`.  The scheduling barriers keep both prefetches where they are
+// written: left alone, the scheduler sinks the loads next to their first use
+// and exposes the full memory latency at every unit.
+template <int H, int PREC, int NTL, int UL, int O>
+__device__ __forceinline__ void run_layer(
+    const char* cur, const char* nxt, unsigned lane_off, const typename Tr<PREC>::E* act, int n,
+    int g, typename Chain<H, PREC, NTL>::V (&abuf)[Chain<H, PREC, NTL>::NB][Chain<H, PREC, NTL>::AN],
+    typename Chain<H, PREC, NTL>::V (&bbuf)[2][NTL], f32x4 (&acc)[H / 64][NTL]) {
+  typedef Chain<H, PREC, NTL> C;
+  constexpr int NB = C::NB, PD = C::PD, MTW = C::MTW;
+#pragma unroll
+  for (int j = 0; j < UL; ++j) {
+    if (j + PD < UL)
+      load_a<H, PREC, NTL>(cur, j + PD, lane_off, abuf[(O + j + PD) % NB]);
+    else
+      load_a<H, PREC, NTL>(nxt, j + PD - UL, lane_off, abuf[(O + j + PD) % NB]);
+    if (j + 1 < UL) read_b<H, PREC, NTL>(act, j + 1, n, g, bbuf[(j + 1) & 1]);
+    __builtin_amdgcn_sched_barrier(0);
+    const typename C::V (&A)[C::AN] = abuf[(O + j) % NB];
+    const typename C::V (&Bv)[NTL] = bbuf[j & 1];
+    if constexpr (C::F32) {
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int nt = 0; nt < NTL; ++nt)
 #pragma unroll
           for (int mt = 0; mt < MTW; ++mt)
-            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(abuf[q & 1][i][mt],
-                                                               bbuf[q & 1][nt][i], acc[mt][nt],
+            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[i][mt], Bv[nt][i], acc[mt][nt],
                                                                0, 0, 0);
-      __builtin_amdgcn_sched_barrier(0);
-    }
-  } else {
-    // weights are fetched PD steps ahead (PD + 1 rotating buffers); wide
-    // chunks have less register room left beside their accumulators
-    typedef typename Tr<PREC>::V8 V8;
-    constexpr int PD = NTL >= 12 ? 1 : 2, NB = PD + 1;
-    V8 abuf[NB][MTW];
-    V8 bbuf[2][NTL];
-    auto load_step = [&](int s, V8 (&dst)[MTW]) {
-      const V8* ap = reinterpret_cast<const V8*>(W + ((size_t)s * 64 + lane) * MTW * 8);
-#pragma unroll
-      for (int mt = 0; mt < MTW; ++mt) dst[mt] = ap[mt];
-    };
-    auto read_b = [&](int s, V8 (&dst)[NTL]) {
+    } else {
 #pragma unroll
       for (int nt = 0; nt < NTL; ++nt)
-        dst[nt] = *reinterpret_cast<const V8*>(act + (16 * nt + n) * KP + 32 * s + 8 * g);
-    };
-#pragma unroll
-    for (int s = 0; s < PD && s < KS; ++s) load_step(s, abuf[s]);
-    read_b(0, bbuf[0]);
-#pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      if (s + PD < KS) load_step(s + PD, abuf[(s + PD) % NB]);
-      if (s + 1 < KS) read_b(s + 1, bbuf[(s + 1) & 1]);
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int nt = 0; nt < NTL; ++nt) {
 #pragma unroll
         for (int mt = 0; mt < MTW; ++mt) {
           if constexpr (PREC == kBF16)
-            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(abuf[s % NB][mt],
-                                                                  bbuf[s & 1][nt], acc[mt][nt],
+            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt], Bv[nt], acc[mt][nt],
                                                                   0, 0, 0);
           else
-            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(abuf[s % NB][mt],
-                                                                 bbuf[s & 1][nt], acc[mt][nt],
+            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[mt], Bv[nt], acc[mt][nt],
                                                                  0, 0, 0);
         }
-      }
-      __builtin_amdgcn_sched_barrier(0);
     }
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
-// act[col][row] = relu(acc + bias) for this wave's rows.
-template <int H, int PREC, int NTL>
-__device__ __forceinline__ void write_act(typename Tr<PREC>::E* act, const float* bias,
-                                          int wave, int lane, const f32x4 (&acc)[H / 64][NTL]) {
-  constexpr int MTW = H / 64;
-  constexpr int KP = H + Tr<PREC>::PAD;
+// The four dense layers of one chunk: acc = W3 relu(W2 relu(W1 relu(W0 x +
+// b0) + b1) + b2); bias and ReLU of the last layer are applied by the caller.
+// Layer l's weights for wave w are [w][unit][lane][m-tile][slot] (see
+// mlp_layout.h); the weight prefetch runs across layer boundaries so the
+// loads of a layer's first units overlap the previous layer's epilogue.
+template <int H, int PREC, int NTL, int K0>
+__device__ __forceinline__ void dense_chain(const typename Tr<PREC>::E* __restrict__ W,
+                                            typename Tr<PREC>::E* act,
+                                            const float* __restrict__ s_bias, int wave,
+                                            int lane, f32x4 (&acc)[H / 64][NTL]) {
+  typedef Chain<H, PREC, NTL> C;
+  typedef typename C::E E;
+  constexpr int U0 = K0 / C::KPU, UH = C::UH, NB = C::NB, PD = C::PD, MTW = C::MTW;
+  static_assert(K0 % C::KPU == 0, "layer widths must fill whole units");
+  constexpr int O0 = (NB - U0 % NB) % NB;   // so that layer 1 starts at buffer 0
   const int n = lane & 15, g = lane >> 4;
+  // Wave-uniform bases (scalar registers) plus a per-lane byte offset: unit
+  // addresses are compile-time offsets from them.  The empty asm makes the
+  // bases opaque so unit addresses are formed next to their loads instead of
+  // being hoisted out of the ray loop as live vector registers.
+  const char* wb0 = reinterpret_cast<const char*>(W) + (size_t)wave * U0 * C::unit_bytes;
+  const char* wbh = reinterpret_cast<const char*>(W) + (size_t)kWaves * U0 * C::unit_bytes +
+                    (size_t)wave * UH * C::unit_bytes;
+  asm volatile("" : "+s"(wb0), "+s"(wbh));
+  const size_t lh_bytes = (size_t)kWaves * UH * C::unit_bytes;
+  const unsigned lane_off = (unsigned)lane * MTW * (C::F32 ? 1 : 8) * sizeof(E);
+
+  typename C::V abuf[NB][C::AN];
+  typename C::V bbuf[2][NTL];
+  auto zero = [&]() {
 #pragma unroll
-  for (int mt = 0; mt < MTW; ++mt) {
-    const int row0 = (H / 4) * wave + 16 * mt + 4 * g;
-    const f32x4 bv = *reinterpret_cast<const f32x4*>(bias + row0);
+    for (int mt = 0; mt < MTW; ++mt)
 #pragma unroll
-    for (int nt = 0; nt < NTL; ++nt) {
-      f32x4 v;
+      for (int nt = 0; nt < NTL; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  };
+  zero();
 #pragma unroll
-      for (int i = 0; i < 4; ++i) v[i] = fmaxf(acc[mt][nt][i] + bv[i], 0.0f);
-      typename Tr<PREC>::E* dst = act + (16 * nt + n) * KP + row0;
-      if constexpr (PREC == kF32) {
-        *reinterpret_cast<f32x4*>(dst) = v;
-      } else {
-        typename Tr<PREC>::V4 h;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) h[i] = (typename Tr<PREC>::E)v[i];
-        *reinterpret_cast<typename Tr<PREC>::V4*>(dst) = h;
-      }
-    }
+  for (int j = 0; j < PD; ++j) load_a<H, PREC, NTL>(wb0, j, lane_off, abuf[(O0 + j) % NB]);
+  read_b<H, PREC, NTL>(act, 0, n, g, bbuf[0]);
+  run_layer<H, PREC, NTL, U0, O0>(wb0, wbh, lane_off, act, n, g, abuf, bbuf, acc);
+#pragma unroll 1
+  for (int l = 1; l < 4; ++l) {
+    __syncthreads();  // every wave finished reading act
+    write_act<H, PREC, NTL>(act, s_bias + (l - 1) * H, wave, lane, acc);
+    zero();
+    __syncthreads();
+    read_b<H, PREC, NTL>(act, 0, n, g, bbuf[0]);
+    const char* cur = wbh + (size_t)(l - 1) * lh_bytes;
+    const char* nxt = l < 3 ? cur + lh_bytes : cur;  // the last layer re-reads its own head
+    run_layer<H, PREC, NTL, UH, 0>(cur, nxt, lane_off, act, n, g, abuf, bbuf, acc);
   }
 }
 
 template <int F, int H, int PREC, int NS>
 struct Smem {
   static constexpr int NC = NS * kVP;
-  static constexpr int KP = H + Tr<PREC>::PAD;
   static constexpr size_t act = 0;
-  static constexpr size_t act_bytes = (size_t)NC * KP * sizeof(typename Tr<PREC>::E);
-  static constexpr size_t keep = (act + act_bytes + 15) / 16 * 16;      // f32 [NS][4][F]
-  static constexpr size_t sig = keep + (size_t)NS * kVP * F * 4;       // f32 [4][NC]
+  static constexpr size_t act_bytes = (size_t)NC * H * sizeof(typename Tr<PREC>::E);
+  // kept-feature rows of F + 4 floats: an odd number of 16-byte chunks, so
+  // the 16-byte stores of 16 columns are conflict-free
+  static constexpr int KF = F + 4;
+  static constexpr size_t keep = (act + act_bytes + 15) / 16 * 16;      // f32 [NS][4][KF]
+  static constexpr size_t sig = keep + (size_t)NS * kVP * KF * 4;      // f32 [4][NC]
   static constexpr size_t alpha = sig + (size_t)kWaves * NC * 4;       // f32 [NS]
   static constexpr size_t tval = alpha + (size_t)NS * 4;               // f32 [NS]
   static constexpr size_t wts = tval + (size_t)NS * 4;                 // f32 [NS]
   static constexpr size_t flag = wts + (size_t)NS * 4;                 // int [4]
-  static constexpr size_t total = flag + 16;
+  static constexpr size_t bias = flag + 16;                            // f32 [5][H]
+  static constexpr size_t total = bias + (size_t)5 * H * 4;
 };
 
 constexpr size_t kLdsBytes = 160 * 1024;  // LDS per workgroup on gfx950
 
+// Waves per SIMD the register allocation is fitted to: two workgroups per CU
+// let one workgroup's gather / compositing overlap the other's MFMA work.
+constexpr int render_occupancy(int PREC, int NS) { return (PREC != kF32 && NS <= 16) ? 2 : 1; }
+
 template <int F, int H, int PREC, int NS>
 __global__ void __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(render_occupancy(PREC, NS), render_occupancy(PREC, NS))))
 render_kernel(RenderArgs a) {
   typedef typename Tr<PREC>::E E;
   typedef Smem<F, H, PREC, NS> L;
-  constexpr int NC = L::NC, KP = L::KP, NTL = NC / 16, MTW = H / 64;
+  constexpr int NC = L::NC, KF = L::KF, NTL = NC / 16, MTW = H / 64;
+  constexpr int CE = Tr<PREC>::CE;
   constexpr int CPL = F / 4;
   constexpr int K0 = F + kPeSlots;
   constexpr int STEP = PREC == kF32 ? 4 : 32;
   constexpr int KS0 = K0 / STEP, KSH = H / STEP;
   constexpr int EPL = Tr<PREC>::EPL;
   constexpr SmallOffsets so = small_offsets(F, H);
-  static_assert(K0 <= KP, "first-layer input must fit an activation row");
+  static_assert(K0 <= H, "first-layer input must fit an activation row");
+  static_assert(CPL % CE == 0 && F % CE == 0, "feature quarters must fill whole chunks");
   static_assert(kVP * F <= kThreads, "one thread per (view, channel) token entry");
 
   extern __shared__ __attribute__((aligned(16))) char smem[];
@@ -273,14 +375,16 @@ render_kernel(RenderArgs a) {
   float* s_w = reinterpret_cast<float*>(smem + L::wts);
   int* s_flag = reinterpret_cast<int*>(smem + L::flag);
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int S = a.S, V = a.V;
   const float* ws = a.wsmall;
   const E* W0 = reinterpret_cast<const E*>(a.wdense);
-  const size_t l0 = (size_t)kWaves * KS0 * 64 * MTW * EPL;   // elements of layer 0
-  const size_t lh = (size_t)kWaves * KSH * 64 * MTW * EPL;   // elements of a hidden layer
-  const size_t w0off = (size_t)wave * KS0 * 64 * MTW * EPL;
-  const size_t whoff = (size_t)wave * KSH * 64 * MTW * EPL;
+  float* s_bias = reinterpret_cast<float*>(smem + L::bias);
+  // layer biases and the occupancy-head weights stay in LDS for the whole
+  // kernel (the first loop iteration's barrier publishes them)
+  for (int i = tid; i < 4 * H; i += kThreads) s_bias[i] = a.bdense[i];
+  for (int i = tid; i < H; i += kThreads) s_bias[4 * H + i] = ws[so.sigma_w + i];
   const float inv_v = 1.0f / (float)V;
 
   // every list entry costs about the same, so a static stride over the
@@ -308,9 +412,10 @@ render_kernel(RenderArgs a) {
     for (int c0 = 0; c0 < S; c0 += NS) {
       __syncthreads();
       if (a.early_exit && s_flag[0]) break;
-      // ---- first-layer input: bilinear features (4 threads per column) ----
+      // ---- first-layer input: bilinear features (4 threads per column; the
+      // 16 lanes of a quarter-wave take 16 columns of the same quarter) ------
       for (int item = tid; item < NC * 4; item += kThreads) {
-        const int col = item >> 2, quarter = item & 3;
+        const int col = (item & 15) + 16 * (item >> 6), quarter = (item >> 4) & 3;
         const int sl = col >> 2, v = col & 3;
         const float t = trow[c0 + sl];
         const float x = ox + t * dx, y = oy + t * dy, z = oz + t * dz;
@@ -357,57 +462,75 @@ render_kernel(RenderArgs a) {
             }
           }
         }
-        E* dst = act + col * KP + quarter * CPL;
-        float* kp = keep + col * F + quarter * CPL;
+        float* kp = keep + col * KF + quarter * CPL;
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-          dst[c] = (E)f[c];
-          kp[c] = f[c];
-        }
-      }
-      // ---- positional encoding (freq_encoding.cu:37-45 layout), shared by views
-      for (int item = tid; item < NS * kPeSlots; item += kThreads) {
-        const int sl = item / kPeSlots, q = item % kPeSlots;
-        const float t = trow[c0 + sl];
-        float val = 0.0f;
-        if (q < 63) {
-          const int d = q % 3;
-          const float xd = d == 0 ? (ox + t * dx - cx) * ihx
-                                  : (d == 1 ? (oy + t * dy - cy) * ihy : (oz + t * dz - cz) * ihz);
-          if (q < 3) {
-            val = xd;
+        for (int c = 0; c < CPL; c += 4)
+          *reinterpret_cast<f32x4*>(kp + c) = f32x4{f[c], f[c + 1], f[c + 2], f[c + 3]};
+        // layer-0 input k = channel (k < F); 16-byte chunks, swizzled
+#pragma unroll
+        for (int c = 0; c < CPL; c += CE) {
+          E* dst = act + act_chunk<H, PREC>(col, (quarter * CPL + c) / CE);
+          if constexpr (PREC == kF32) {
+            *reinterpret_cast<f32x4*>(dst) = f32x4{f[c], f[c + 1], f[c + 2], f[c + 3]};
           } else {
-            const int col = q / 3 - 1;
-            val = sinf(ldexpf(xd, col >> 1) + (float)(col & 1) * kHalfPi);
+            typename Tr<PREC>::V8 h;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) h[i] = (E)f[c + i];
+            *reinterpret_cast<typename Tr<PREC>::V8*>(dst) = h;
           }
         }
-        const E ev = (E)val;
+      }
+      // ---- positional encoding (freq_encoding.cu:37-45 layout), shared by views;
+      // a thread computes two adjacent slots and stores them as one pair
+      for (int item = tid; item < NS * kPeSlots / 2; item += kThreads) {
+        const int sl = item / (kPeSlots / 2), q0 = 2 * (item % (kPeSlots / 2));
+        const float t = trow[c0 + sl];
+        float val[2];
 #pragma unroll
-        for (int v = 0; v < kVP; ++v) act[(sl * kVP + v) * KP + F + q] = ev;
+        for (int h = 0; h < 2; ++h) {
+          const int q = q0 + h;
+          val[h] = 0.0f;
+          if (q < 63) {
+            const int d = q % 3;
+            const float xd = d == 0 ? (ox + t * dx - cx) * ihx
+                                    : (d == 1 ? (oy + t * dy - cy) * ihy : (oz + t * dz - cz) * ihz);
+            if (q < 3) {
+              val[h] = xd;
+            } else {
+              const int col = q / 3 - 1;
+              val[h] = sinf(ldexpf(xd, col >> 1) + (float)(col & 1) * kHalfPi);
+            }
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < kVP; ++v) {
+          const int col = sl * kVP + v, k = F + q0;  // the pair never straddles a chunk
+          E* d2 = act + act_chunk<H, PREC>(col, k / CE) + k % CE;
+          if constexpr (PREC == kF32) {
+            *reinterpret_cast<f32x2*>(d2) = f32x2{val[0], val[1]};
+          } else {
+            typedef E E2 __attribute__((ext_vector_type(2)));
+            *reinterpret_cast<E2*>(d2) = E2{(E)val[0], (E)val[1]};
+          }
+        }
       }
       if (tid < NS) s_t[tid] = trow[c0 + tid];
       __syncthreads();
       // ---- dense layers on MFMA --------------------------------------------
       f32x4 acc[MTW][NTL];
-      mfma_layer<H, PREC, NTL, KS0>(W0 + w0off, act, lane, acc);
-      for (int l = 1; l < 4; ++l) {
-        __syncthreads();  // every wave finished reading act
-        write_act<H, PREC, NTL>(act, a.bdense + (l - 1) * H, wave, lane, acc);
-        __syncthreads();
-        mfma_layer<H, PREC, NTL, KSH>(W0 + l0 + (size_t)(l - 1) * lh + whoff, act, lane, acc);
-      }
+      dense_chain<H, PREC, NTL, K0>(W0, act, s_bias, wave, lane, acc);
       // ---- occupancy head: per-column partial dot products ------------------
       {
         const int n = lane & 15, g = lane >> 4;
-        const float* bias = a.bdense + 3 * H;
+        const float* bias = s_bias + 3 * H;
 #pragma unroll
         for (int nt = 0; nt < NTL; ++nt) {
           float part = 0.0f;
 #pragma unroll
           for (int mt = 0; mt < MTW; ++mt) {
-            const int row0 = (H / 4) * wave + 16 * mt + 4 * g;
+            const int row0 = unit_of<H>(wave, g, mt, 0);
             const f32x4 bv = *reinterpret_cast<const f32x4*>(bias + row0);
-            const f32x4 wv = *reinterpret_cast<const f32x4*>(ws + so.sigma_w + row0);
+            const f32x4 wv = *reinterpret_cast<const f32x4*>(s_bias + 4 * H + row0);
 #pragma unroll
             for (int i = 0; i < 4; ++i) part += wv[i] * fmaxf(acc[mt][nt][i] + bv[i], 0.0f);
           }
@@ -448,7 +571,7 @@ render_kernel(RenderArgs a) {
       if (tid < kVP * F) {
         const int v = tid / F, c = tid % F;
 #pragma unroll 4
-        for (int sl = 0; sl < NS; ++sl) tok += s_w[sl] * keep[(sl * kVP + v) * F + c];
+        for (int sl = 0; sl < NS; ++sl) tok += s_w[sl] * keep[(sl * kVP + v) * KF + c];
       }
     }
     __syncthreads();
@@ -685,12 +808,13 @@ compact_rays(const float* __restrict__ t, int S, long long n_rays, unsigned* cou
 }
 
 template <int F, int H, int PREC, int NS>
-int launch(const RenderArgs& a, hipStream_t st) {
+int launch(const RenderArgs& a, hipStream_t st, bool dry) {
   typedef Smem<F, H, PREC, NS> L;
   if constexpr (L::total > kLdsBytes) {
     return sailor::set_error(SAILOR_ERR_ARG, "render_rays: %d-sample chunks need %zu bytes of "
                              "LDS at this precision (limit %zu)", NS, L::total, kLdsBytes);
   } else {
+  if (dry) return SAILOR_OK;  // configuration check only
   auto kern = render_kernel<F, H, PREC, NS>;
   static int grid = 0;
   if (grid == 0) {
@@ -715,22 +839,22 @@ int launch(const RenderArgs& a, hipStream_t st) {
 }
 
 template <int F, int H, int NS>
-int dispatch_prec(int prec, const RenderArgs& a, hipStream_t st) {
+int dispatch_prec(int prec, const RenderArgs& a, hipStream_t st, bool dry) {
   switch (prec) {
-    case kF32: return launch<F, H, kF32, NS>(a, st);
-    case kBF16: return launch<F, H, kBF16, NS>(a, st);
-    case kF16: return launch<F, H, kF16, NS>(a, st);
+    case kF32: return launch<F, H, kF32, NS>(a, st, dry);
+    case kBF16: return launch<F, H, kBF16, NS>(a, st, dry);
+    case kF16: return launch<F, H, kF16, NS>(a, st, dry);
   }
   return sailor::set_error(SAILOR_ERR_ARG, "render_rays: precision %d unsupported", prec);
 }
 
 template <int F, int H>
-int dispatch_ns(int ns, int prec, const RenderArgs& a, hipStream_t st) {
+int dispatch_ns(int ns, int prec, const RenderArgs& a, hipStream_t st, bool dry) {
   switch (ns) {
-    case 8: return dispatch_prec<F, H, 8>(prec, a, st);
-    case 16: return dispatch_prec<F, H, 16>(prec, a, st);
-    case 24: return dispatch_prec<F, H, 24>(prec, a, st);
-    case 48: return dispatch_prec<F, H, 48>(prec, a, st);
+    case 8: return dispatch_prec<F, H, 8>(prec, a, st, dry);
+    case 16: return dispatch_prec<F, H, 16>(prec, a, st, dry);
+    case 24: return dispatch_prec<F, H, 24>(prec, a, st, dry);
+    case 48: return dispatch_prec<F, H, 48>(prec, a, st, dry);
   }
   return sailor::set_error(SAILOR_ERR_ARG, "render_rays: samples per chunk %d not "
                            "instantiated (8, 16, 24, 48)", ns);
@@ -793,9 +917,12 @@ extern "C" int sailor_render_rays(const float* t, const float* rays, int B, int 
   if (ns != 8 && ns != 16 && ns != 24 && ns != 48)
     return sailor::set_error(SAILOR_ERR_ARG, "render_rays: samples per chunk %d not "
                              "instantiated (8, 16, 24, 48)", ns);
-  if (precision == kF32 && ns == 48)
-    return sailor::set_error(SAILOR_ERR_ARG, "render_rays: 48-sample chunks do not fit the LDS "
-                             "of a workgroup in fp32");
+  // every configuration error is reported before anything is launched
+  {
+    const int err = (F == 64) ? dispatch_ns<64, 256>(ns, precision, a, st, true)
+                              : dispatch_ns<32, 128>(ns, precision, a, st, true);
+    if (err) return err;
+  }
   // the counter is cleared by a kernel, not a memset, so the call can be
   // captured into a graph
   hipLaunchKernelGGL(queue_reset, dim3(1), dim3(64), 0, st, counter);
@@ -803,8 +930,8 @@ extern "C" int sailor_render_rays(const float* t, const float* rays, int B, int 
   hipLaunchKernelGGL(compact_rays, dim3((unsigned)((a.n_rays + 255) / 256)), dim3(256), 0, st, t,
                      S, a.n_rays, counter, list, out_rgb, out_depth, out_aux);
   if (int err = sailor::check_launch("render_rays (compact)")) return err;
-  if (F == 64 && H == 256) return dispatch_ns<64, 256>(ns, precision, a, st);
-  if (F == 32 && H == 128) return dispatch_ns<32, 128>(ns, precision, a, st);
+  if (F == 64 && H == 256) return dispatch_ns<64, 256>(ns, precision, a, st, false);
+  if (F == 32 && H == 128) return dispatch_ns<32, 128>(ns, precision, a, st, false);
   return sailor::set_error(SAILOR_ERR_ARG, "render_rays: (F=%d, H=%d) not instantiated "
                            "(supported: (32,128), (64,256))", F, H);
 }

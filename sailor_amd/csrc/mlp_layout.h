@@ -11,11 +11,16 @@
 //
 // The network runs transposed: MFMA columns (N) are (sample, view) pairs of
 // one ray, rows (M) are output units.  A workgroup of 4 waves owns one ray;
-// wave w computes output rows [w*H/4, (w+1)*H/4) for every column of a chunk
-// of samples.  Layer inputs live in LDS as act[column][k] in natural k order
-// (first layer: k < F feature channels, F <= k < F+63 positional encoding,
-// k = F+63 zero padding); B fragments are read from there, so the reduction
-// index of MFMA step s, lane group g (= l/16), slot j is
+// wave w computes output units [w*H/4, (w+1)*H/4) for every column of a
+// chunk of samples.  Within a wave, A row m of m-tile mt computes unit
+// w*H/4 + 4*MTW*(m/4) + 4*mt + m%4 (MTW = H/64 m-tiles per wave), so lane
+// group g's accumulators hold 4*MTW consecutive units of its column.
+// Layer inputs live in LDS as act[column][k] in natural k order (first
+// layer: k < F feature channels, F <= k < F+63 positional encoding,
+// k = F+63 zero padding; physically the 16-byte chunks of a column are
+// XOR-swizzled, which the packer does not see); B fragments are read from
+// there, so the reduction index of MFMA step s, lane group g (= l/16), slot
+// j is
 //   fp32:  k = 16*(s/4) + 4*g + s%4      (one 16-byte read serves 4 steps)
 //   bf16:  k = 32*s + 8*g + j
 // The packer stores, for every layer and wave w, the A fragments as

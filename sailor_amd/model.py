@@ -136,7 +136,11 @@ def pack_layer(Wn, prec, waves=4):
     lane = np.arange(64)
     w = np.arange(waves)[:, None, None, None, None]
     mt = np.arange(MTW)[None, None, None, :, None]
-    rows = (H // waves) * w + 16 * mt + (lane % 16)[None, None, :, None, None]
+    # A row m of m-tile mt in wave w computes output unit
+    # (H/waves) w + 4 MTW (m // 4) + 4 mt + m % 4, so that the accumulators of
+    # lane group g hold 4 MTW consecutive units (mlp.hip unit_of)
+    m = (lane % 16)[None, None, :, None, None]
+    rows = (H // waves) * w + 4 * MTW * (m // 4) + 4 * mt + m % 4
     cols = col[None, :, :, None, :]
     rows, cols = np.broadcast_arrays(rows, cols)
     return Wn[rows, cols]

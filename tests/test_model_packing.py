@@ -7,6 +7,34 @@ import pytest
 from sailor_amd import model as M
 
 
+def unit_of(H, w, g, mt, i):
+    """Output unit held by accumulator element i of m-tile mt, lane group g,
+    wave w (mlp.hip unit_of)."""
+    return (H // 4) * w + 4 * (H // 64) * g + 4 * mt + i
+
+
+def test_unit_map_gives_each_lane_group_consecutive_units():
+    for H in (128, 256):
+        MTW = H // 64
+        seen = []
+        for w in range(4):
+            for g in range(4):
+                units = [unit_of(H, w, g, mt, i) for mt in range(MTW) for i in range(4)]
+                assert units == list(range(units[0], units[0] + 4 * MTW))
+                seen += units
+        assert sorted(seen) == list(range(H))
+
+
+@pytest.mark.parametrize("H,CE", [(128, 4), (256, 4), (128, 8), (256, 8)])
+def test_swizzle_is_a_permutation_of_each_column(H, CE):
+    """act_chunk(col, c) = col*H + (c ^ (col & 15)) * CE keeps every chunk of
+    a column inside that column and hits each slot once."""
+    nchunk = H // CE
+    for col in range(64):
+        phys = [col * H + (c ^ (col & 15)) * CE for c in range(nchunk)]
+        assert sorted(phys) == [col * H + CE * c for c in range(nchunk)]
+
+
 def kernel_b_index(prec, s, g, j):
     """LDS row index the kernel reads for B slot (step s, lane group g, slot j)
     (mlp.hip mfma_layer)."""
@@ -34,7 +62,9 @@ def test_packed_layer_times_fragments_equals_matvec(prec, H, K):
                     if w == 0 and m == 0:
                         count[kk] += 1
                     for mt in range(MTW):
-                        row = (H // waves) * w + 16 * mt + m
+                        # D row m = 4g' + i of this m-tile is the unit the
+                        # kernel writes for lane group g', element i
+                        row = unit_of(H, w, m // 4, mt, m % 4)
                         D[row] += A[w, s, lane, mt, j] * X[kk]
     np.testing.assert_allclose(D, W.astype(np.float64) @ X, rtol=1e-5, atol=1e-4)
     assert (count == 1).all()  # each input is reduced exactly once per row
