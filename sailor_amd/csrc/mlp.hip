@@ -52,6 +52,7 @@ using namespace sailor::mlp;
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
@@ -324,6 +325,156 @@ __device__ __forceinline__ void dense_chain(const typename Tr<PREC>::E* __restri
   }
 }
 
+// ---- 16-bit forms: 32x32x16 MFMA ------------------------------------------
+// (lane l holds A[l%32][8(l/32)+j] and B[8(l/32)+j][l%32]; accumulator r holds
+// D[8(r/4) + 4(l/32) + r%4][l%32], checked on gfx950).  The packer orders the
+// weight rows so that accumulator r of m-tile mt in lanes with l/32 = h is
+// output unit unit_of32(w, mt, h, r): 16 consecutive units per lane and
+// column.  A unit of work is one k-step of 16: M32 x NT MFMAs of 32x32x16,
+// which sustain close to the dense peak with few independent accumulators
+// (the 16x16x32 form needs 16 chains per wave for the same rate).
+template <int H>
+__device__ __forceinline__ int unit_of32(int wave, int mt, int h, int r) {
+  return (H / 4) * wave + 32 * mt + 16 * h + r;
+}
+
+template <int H, int PREC, int NT>
+struct Chain16 {
+  static constexpr int M32 = H / 128;   // 32-row m-tiles per wave
+  static constexpr int UH = H / 16;     // k-steps of a hidden layer
+  static constexpr int PD = 3, NB = 4;  // weight prefetch distance / buffers
+  typedef typename Tr<PREC>::V8 V;
+  static constexpr size_t unit_bytes = (size_t)64 * M32 * 8 * 2;
+  static_assert(UH % NB == 0, "hidden-layer steps must fill whole buffer rotations");
+};
+
+template <int H, int PREC, int NT>
+__device__ __forceinline__ void load_a16(const char* base, int j, unsigned lane_off,
+                                         typename Chain16<H, PREC, NT>::V (&dst)[H / 128]) {
+  typedef Chain16<H, PREC, NT> C;
+  const typename C::V* ap =
+      reinterpret_cast<const typename C::V*>(base + (size_t)j * C::unit_bytes + lane_off);
+#pragma unroll
+  for (int mt = 0; mt < C::M32; ++mt) dst[mt] = ap[mt];
+}
+
+// k-step j covers chunks 2j, 2j+1 of a column; lanes l/32 = h read chunk 2j+h
+template <int H, int PREC, int NT>
+__device__ __forceinline__ void read_b16(const typename Tr<PREC>::E* act, int j, int lane,
+                                         typename Chain16<H, PREC, NT>::V (&dst)[NT]) {
+  typedef Chain16<H, PREC, NT> C;
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt)
+    dst[nt] = *reinterpret_cast<const typename C::V*>(
+        act + act_chunk<H, PREC>(32 * nt + (lane & 31), 2 * j + (lane >> 5)));
+}
+
+template <int H, int PREC, int NT, int UL, int O>
+__device__ __forceinline__ void run_layer16(
+    const char* cur, const char* nxt, unsigned lane_off, const typename Tr<PREC>::E* act,
+    int lane, typename Chain16<H, PREC, NT>::V (&abuf)[Chain16<H, PREC, NT>::NB][H / 128],
+    typename Chain16<H, PREC, NT>::V (&bbuf)[2][NT], f32x16 (&acc)[H / 128][NT]) {
+  typedef Chain16<H, PREC, NT> C;
+  constexpr int NB = C::NB, PD = C::PD;
+#pragma unroll
+  for (int j = 0; j < UL; ++j) {
+    if (j + PD < UL)
+      load_a16<H, PREC, NT>(cur, j + PD, lane_off, abuf[(O + j + PD) % NB]);
+    else
+      load_a16<H, PREC, NT>(nxt, j + PD - UL, lane_off, abuf[(O + j + PD) % NB]);
+    if (j + 1 < UL) read_b16<H, PREC, NT>(act, j + 1, lane, bbuf[(j + 1) & 1]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int mt = 0; mt < C::M32; ++mt) {
+        if constexpr (PREC == kBF16)
+          acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(abuf[(O + j) % NB][mt],
+                                                                bbuf[j & 1][nt], acc[mt][nt],
+                                                                0, 0, 0);
+        else
+          acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(abuf[(O + j) % NB][mt],
+                                                               bbuf[j & 1][nt], acc[mt][nt],
+                                                               0, 0, 0);
+      }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// act[col][unit] = relu(acc + bias) for this wave's units (two 16-byte stores
+// per m-tile and column).
+template <int H, int PREC, int NT>
+__device__ __forceinline__ void write_act16(typename Tr<PREC>::E* act, const float* bias,
+                                            int wave, int lane, const f32x16 (&acc)[H / 128][NT]) {
+  typedef typename Tr<PREC>::E E;
+  const int c = lane & 31, h = lane >> 5;
+#pragma unroll
+  for (int mt = 0; mt < H / 128; ++mt) {
+    const int u0 = unit_of32<H>(wave, mt, h, 0);
+    float bv[16];
+#pragma unroll
+    for (int q = 0; q < 16; q += 4) {
+      const f32x4 b4 = *reinterpret_cast<const f32x4*>(bias + u0 + q);
+      bv[q] = b4[0]; bv[q + 1] = b4[1]; bv[q + 2] = b4[2]; bv[q + 3] = b4[3];
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        typename Tr<PREC>::V8 v;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = (E)fmaxf(acc[mt][nt][8 * e + k] + bv[8 * e + k], 0.0f);
+        *reinterpret_cast<typename Tr<PREC>::V8*>(act + act_chunk<H, PREC>(32 * nt + c,
+                                                                           u0 / 8 + e)) = v;
+      }
+    }
+  }
+}
+
+template <int H, int PREC, int NT, int K0>
+__device__ __forceinline__ void dense_chain16(const typename Tr<PREC>::E* __restrict__ W,
+                                              typename Tr<PREC>::E* act,
+                                              const float* __restrict__ s_bias, int wave,
+                                              int lane, f32x16 (&acc)[H / 128][NT]) {
+  typedef Chain16<H, PREC, NT> C;
+  constexpr int U0 = K0 / 16, UH = C::UH, NB = C::NB, PD = C::PD, M32 = C::M32;
+  static_assert(K0 % 16 == 0, "layer widths must fill whole k-steps");
+  constexpr int O0 = (NB - U0 % NB) % NB;   // so that layer 1 starts at buffer 0
+  const char* wb0 = reinterpret_cast<const char*>(W) + (size_t)wave * U0 * C::unit_bytes;
+  const char* wbh = reinterpret_cast<const char*>(W) + (size_t)kWaves * U0 * C::unit_bytes +
+                    (size_t)wave * UH * C::unit_bytes;
+  asm volatile("" : "+s"(wb0), "+s"(wbh));  // see dense_chain
+  const size_t lh_bytes = (size_t)kWaves * UH * C::unit_bytes;
+  const unsigned lane_off = (unsigned)lane * M32 * 16;
+
+  typename C::V abuf[NB][M32];
+  typename C::V bbuf[2][NT];
+  auto zero = [&]() {
+#pragma unroll
+    for (int mt = 0; mt < M32; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[mt][nt][r] = 0.0f;
+  };
+  zero();
+#pragma unroll
+  for (int j = 0; j < PD; ++j) load_a16<H, PREC, NT>(wb0, j, lane_off, abuf[(O0 + j) % NB]);
+  read_b16<H, PREC, NT>(act, 0, lane, bbuf[0]);
+  run_layer16<H, PREC, NT, U0, O0>(wb0, wbh, lane_off, act, lane, abuf, bbuf, acc);
+#pragma unroll 1
+  for (int l = 1; l < 4; ++l) {
+    __syncthreads();  // every wave finished reading act
+    write_act16<H, PREC, NT>(act, s_bias + (l - 1) * H, wave, lane, acc);
+    zero();
+    __syncthreads();
+    read_b16<H, PREC, NT>(act, 0, lane, bbuf[0]);
+    const char* cur = wbh + (size_t)(l - 1) * lh_bytes;
+    const char* nxt = l < 3 ? cur + lh_bytes : cur;  // the last layer re-reads its own head
+    run_layer16<H, PREC, NT, UH, 0>(cur, nxt, lane_off, act, lane, abuf, bbuf, acc);
+  }
+}
+
 template <int F, int H, int PREC, int NS>
 struct Smem {
   static constexpr int NC = NS * kVP;
@@ -517,10 +668,30 @@ render_kernel(RenderArgs a) {
       if (tid < NS) s_t[tid] = trow[c0 + tid];
       __syncthreads();
       // ---- dense layers on MFMA --------------------------------------------
-      f32x4 acc[MTW][NTL];
-      dense_chain<H, PREC, NTL, K0>(W0, act, s_bias, wave, lane, acc);
-      // ---- occupancy head: per-column partial dot products ------------------
-      {
+      // ---- dense layers + occupancy head (per-column partial dot products) --
+      if constexpr (PREC != kF32) {
+        constexpr int NT = NC / 32;
+        static_assert(NC % 32 == 0, "16-bit forms need chunks of a multiple of 8 samples");
+        f32x16 acc[H / 128][NT];
+        dense_chain16<H, PREC, NT, K0>(W0, act, s_bias, wave, lane, acc);
+        const int c = lane & 31, h = lane >> 5;
+        const float* bias = s_bias + 3 * H;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          float part = 0.0f;
+#pragma unroll
+          for (int mt = 0; mt < H / 128; ++mt) {
+            const int u0 = unit_of32<H>(wave, mt, h, 0);
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+              part += s_bias[4 * H + u0 + r] * fmaxf(acc[mt][nt][r] + bias[u0 + r], 0.0f);
+          }
+          part += __shfl_xor(part, 32, 64);
+          if (h == 0) sig[wave * NC + 32 * nt + c] = part;
+        }
+      } else {
+        f32x4 acc[MTW][NTL];
+        dense_chain<H, PREC, NTL, K0>(W0, act, s_bias, wave, lane, acc);
         const int n = lane & 15, g = lane >> 4;
         const float* bias = s_bias + 3 * H;
 #pragma unroll

@@ -116,31 +116,46 @@ def natural_first_layer(W0, F):
 
 
 def kmap(prec, K):
-    """Reduction index of every packed slot [KS, 64 lanes, EPL] for a layer
-    with K inputs (mlp_layout.h)."""
-    g = np.arange(64) // 16
+    """Reduction index of every packed slot [steps, 64 lanes, EPL] for a layer
+    with K inputs (mlp_layout.h).  fp32 (16x16x4 MFMA): one slot per lane and
+    step; 16-bit (32x32x16 MFMA): 8 slots per lane and step."""
     if prec == "fp32":
+        g = np.arange(64) // 16
         s = np.arange(K // 4)[:, None]
         return (16 * (s >> 2) + 4 * g[None, :] + (s & 3))[:, :, None]
-    s = np.arange(K // 32)[:, None, None]
+    h = np.arange(64) // 32
+    u = np.arange(K // 16)[:, None, None]
     j = np.arange(8)[None, None, :]
-    return 32 * s + 8 * g[None, :, None] + j
+    return 16 * u + 8 * h[None, :, None] + j
+
+
+def unit_rows(prec, H, waves=4):
+    """Output unit computed by A row m (= lane % 16 or lane % 32) of m-tile mt
+    in wave w, as [waves, 1, 64 lanes, m-tiles per wave, 1] (mlp.hip
+    unit_of / unit_of32): each lane's accumulators then hold runs of
+    consecutive units of one column."""
+    lane = np.arange(64)
+    w = np.arange(waves)[:, None, None, None, None]
+    if prec == "fp32":
+        MTW = H // (16 * waves)
+        mt = np.arange(MTW)[None, None, None, :, None]
+        m = (lane % 16)[None, None, :, None, None]
+        return (H // waves) * w + 4 * MTW * (m // 4) + 4 * mt + m % 4
+    M32 = H // (32 * waves)
+    mt = np.arange(M32)[None, None, None, :, None]
+    m = (lane % 32)[None, None, :, None, None]
+    # D row m = 8q + 4h + i of a 32x32 tile sits in accumulator 4q + i of the
+    # lanes with l/32 = h; unit 32 mt + 16 h + 4 q + i makes accumulator r of
+    # those lanes unit 32 mt + 16 h + r
+    return (H // waves) * w + 32 * mt + 16 * ((m // 4) % 2) + 4 * (m // 8) + m % 4
 
 
 def pack_layer(Wn, prec, waves=4):
     """A fragments of a natural-order layer Wn [H, K] as
-    [waves, KS, 64 lanes, MT per wave, EPL]."""
+    [waves, steps, 64 lanes, m-tiles per wave, EPL]."""
     H, K = Wn.shape
-    col = kmap(prec, K)                                 # [KS, 64, EPL]
-    MTW = H // (16 * waves)
-    lane = np.arange(64)
-    w = np.arange(waves)[:, None, None, None, None]
-    mt = np.arange(MTW)[None, None, None, :, None]
-    # A row m of m-tile mt in wave w computes output unit
-    # (H/waves) w + 4 MTW (m // 4) + 4 mt + m % 4, so that the accumulators of
-    # lane group g hold 4 MTW consecutive units (mlp.hip unit_of)
-    m = (lane % 16)[None, None, :, None, None]
-    rows = (H // waves) * w + 4 * MTW * (m // 4) + 4 * mt + m % 4
+    col = kmap(prec, K)                                 # [steps, 64, EPL]
+    rows = unit_rows(prec, H, waves)
     cols = col[None, :, :, None, :]
     rows, cols = np.broadcast_arrays(rows, cols)
     return Wn[rows, cols]

@@ -35,39 +35,63 @@ def test_swizzle_is_a_permutation_of_each_column(H, CE):
         assert sorted(phys) == [col * H + CE * c for c in range(nchunk)]
 
 
-def kernel_b_index(prec, s, g, j):
-    """LDS row index the kernel reads for B slot (step s, lane group g, slot j)
-    (mlp.hip mfma_layer)."""
-    if prec == "fp32":
-        return 16 * (s >> 2) + 4 * g + (s & 3)
-    return 32 * s + 8 * g + j
+def unit_of32(H, w, mt, h, r):
+    """Output unit held by accumulator r of 32x32 m-tile mt for lanes with
+    l/32 = h, wave w (mlp.hip unit_of32)."""
+    return (H // 4) * w + 32 * mt + 16 * h + r
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16"])
 @pytest.mark.parametrize("H,K", [(128, 96), (128, 128), (256, 128), (256, 256)])
-def test_packed_layer_times_fragments_equals_matvec(prec, H, K):
+def test_packed_fp32_layer_times_fragments_equals_matvec(H, K):
+    """16x16x4 MFMA: lane l of step s multiplies A[l%16][k] by B[k][l%16]
+    with k = 16(s/4) + 4(l/16) + s%4; D row 4g+i of m-tile mt is unit_of."""
     rng = np.random.default_rng(0)
     W = rng.standard_normal((H, K)).astype(np.float32)
-    X = rng.standard_normal((K, 16)).astype(np.float64)  # 16 columns = one N-tile
-    A = M.pack_layer(W, prec).astype(np.float64)         # [waves, KS, 64, MTW, EPL]
-    waves, KS, _, MTW, EPL = A.shape
+    X = rng.standard_normal((K, 16))                     # 16 columns = one N-tile
+    A = M.pack_layer(W, "fp32").astype(np.float64)       # [waves, steps, 64, MTW, 1]
+    waves, KS, _, MTW, _ = A.shape
     D = np.zeros((H, 16))
     count = np.zeros(K, int)
     for w in range(waves):
         for s in range(KS):
             for lane in range(64):
                 g, m = lane // 16, lane % 16
-                for j in range(EPL):
-                    kk = kernel_b_index(prec, s, g, j)
-                    if w == 0 and m == 0:
-                        count[kk] += 1
-                    for mt in range(MTW):
-                        # D row m = 4g' + i of this m-tile is the unit the
-                        # kernel writes for lane group g', element i
-                        row = unit_of(H, w, m // 4, mt, m % 4)
-                        D[row] += A[w, s, lane, mt, j] * X[kk]
+                kk = 16 * (s >> 2) + 4 * g + (s & 3)
+                if w == 0 and m == 0:
+                    count[kk] += 1
+                for mt in range(MTW):
+                    D[unit_of(H, w, m // 4, mt, m % 4)] += A[w, s, lane, mt, 0] * X[kk]
     np.testing.assert_allclose(D, W.astype(np.float64) @ X, rtol=1e-5, atol=1e-4)
     assert (count == 1).all()  # each input is reduced exactly once per row
+
+
+@pytest.mark.parametrize("H,K", [(128, 96), (128, 128), (256, 128), (256, 256)])
+def test_packed_16bit_layer_times_fragments_equals_matvec(H, K):
+    """32x32x16 MFMA (lane layout checked on gfx950): lane l of step u holds
+    A[l%32][16u + 8(l/32) + j]; D row m = 8q + 4h + i of m-tile mt lands in
+    accumulator 4q + i of lanes with l/32 = h, which the kernel writes as
+    unit_of32(w, mt, h, 4q + i)."""
+    rng = np.random.default_rng(0)
+    W = rng.standard_normal((H, K)).astype(np.float32)
+    X = rng.standard_normal((K, 32))                     # 32 columns = one N-tile
+    A = M.pack_layer(W, "bf16").astype(np.float64)       # [waves, steps, 64, M32, 8]
+    waves, U, _, M32, _ = A.shape
+    assert M32 == H // 128
+    D = np.zeros((H, 32))
+    count = np.zeros(K, int)
+    for w in range(waves):
+        for u in range(U):
+            for lane in range(64):
+                m, h = lane % 32, lane // 32
+                for j in range(8):
+                    kk = 16 * u + 8 * h + j
+                    if w == 0 and m == 0:
+                        count[kk] += 1
+                    for mt in range(M32):
+                        unit = unit_of32(H, w, mt, (m // 4) % 2, 4 * (m // 8) + m % 4)
+                        D[unit] += A[w, u, lane, mt, j] * X[kk]
+    np.testing.assert_allclose(D, W.astype(np.float64) @ X, rtol=1e-5, atol=1e-4)
+    assert (count == 1).all()
 
 
 def test_first_layer_natural_order():
