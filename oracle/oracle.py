@@ -5,22 +5,29 @@ this module.  It wraps the C restatement in sailor_oracle.c (bit-exact
 reference for integer / index outputs, float32 op order of the reference) and
 adds float64 NumPy / torch restatements used as accuracy oracles.
 
-How each part is pinned (the tests that pin it are named in brackets):
-  * bbox / resized bbox: closed form on a fronto-parallel plane; float64
-    brute force  [test_oracle.py::test_bbox_*]
-  * integrate: closed-form TSDF of a plane, view-count weights, colour
-    packing, float64 restatement within tolerance  [test_oracle.py::test_integrate_*]
-  * merge: brute-force OR over blocks  [test_oracle.py::test_merge_bruteforce]
-  * tree: brute-force isolation filter, centre closed form
-    min + vs*((size-1)/2 + size*i)  [test_oracle.py::test_tree_*]
-  * rays: float64 K^-1 R^-1 closed form  [test_oracle.py::test_rays_closed_form]
-  * traversal: float64 slab test against every coarse cell (set of hit cells,
-    entry order), axis-aligned rays  [test_oracle.py::test_traverse_*]
-  * sampling: closed form of noise 0.5, sum of counts == S, samples inside
-    their interval  [test_oracle.py::test_sampling_*]
-  * compositing: constant-alpha closed form, float64 restatement, central
-    finite differences of the gradient  [test_oracle.py::test_render_*]
-  * frequency encoding: float64 sin(2^f x) closed form  [test_oracle.py::test_freq_*]
+How each part is pinned (tests/test_oracle.py; test names in brackets):
+  * inverse3x3: numpy.linalg.inv  [test_inverse3x3_matches_linalg]
+  * bbox / resized bbox: closed form on a fronto-parallel plane, float64
+    brute force over posed views, centred-cube closed form  [test_bbox_*,
+    test_resized_bbox_is_centred_cube]
+  * integrate: closed-form TSDF / weights / colour / occupancy of a plane
+    seen by identical views, rounded running colour mean, float64
+    restatement on the capsule scene  [test_integrate_*]
+  * merge: brute-force OR over blocks  [test_merge_bruteforce]
+  * tree: brute-force 26-neighbour isolation filter, centre closed form
+    min + vs*((size-1)/2 + size*i), raster-ordered contiguous children
+    [test_tree_bruteforce_and_centres]
+  * rays: float64 R^-1 K^-1 closed form, row interleave  [test_rays_*]
+  * traversal: float64 slab test against every allocated cell (hit set,
+    children, valid length), axis-aligned row of cells  [test_traverse_*]
+  * sampling: noise-0.5 closed form, per-interval counts, leftovers and
+    interval containment  [test_sampling_*]
+  * sort: ordering, permutation, tie stability  [test_sort_is_stable_and_ordered]
+  * compositing: geometric series for constant alpha, early termination,
+    dropped rays, density-mode closed form, central finite differences of
+    the backward in both modes  [test_render_*]
+  * frequency encoding: float64 sin / cos closed form and its derivative
+    [test_freq_closed_form_and_gradient]
 The reference ships no golden outputs for this path (its test_data frames
 carry camera parameters only as pickled objects, which are not loaded).
 """
