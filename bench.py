@@ -68,6 +68,67 @@ def stage_times(pipe, reps=5):
     return res
 
 
+def kernel_ms(pipe, stage, reps=10):
+    """Average duration (ms) of one launch of a sailor_render_rays stage
+    (2 = the fused sample kernel, 4 = Hydra/RGB), from HIP events recorded on
+    the stream the kernel is launched on (torch's current stream)."""
+    import torch
+    st = torch.cuda.current_stream()
+    # stage 1 lists the rays in an arbitrary order, stage 2 stores per-view
+    # features by list position and stage 4 reads them: time a stage only on
+    # top of the earlier stages of the same list
+    pipe.shade(1)
+    pipe.shade(2)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record(st)
+    for _ in range(reps):
+        pipe.shade(stage)
+    ev[1].record(st)
+    torch.cuda.synchronize()
+    pipe.shade(7)
+    torch.cuda.synchronize()
+    return ev[0].elapsed_time(ev[1]) / reps
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "render_kernel_traffic_c2.md")
+
+
+def measured_traffic(prec):
+    """HBM bytes per launch of the sample kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (lines `traffic_bytes <prec>: <bytes>`),
+    or None when no summary is present."""
+    import re
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes %s:\s*([0-9.eE+]+)" % prec, text)
+    return float(m.group(1)) if m else None
+
+
+def profile_kernels(cfg):
+    """Short fixed sequence for rocprofv3 counter passes: a 256 MiB device copy
+    (the known-size calibration kernel for FETCH_SIZE / WRITE_SIZE), then
+    three launches of the sample kernel per precision."""
+    import torch
+    from sailor_amd.pipeline import FramePipeline
+    n = 64 * 1024 * 1024
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    for prec in ("fp32", "bf16"):
+        p = FramePipeline.synthetic(cfg, prec=prec)
+        p.run()
+        torch.cuda.synchronize()
+        print(f"profile {prec}: valid rays {p.check()['valid_rays']}", flush=True)
+        p.shade(1)
+        for _ in range(3):
+            p.shade(2)
+        torch.cuda.synchronize()
+        del p
+
+
 def psnr(a, b, mask):
     import numpy as np
     m = mask[..., None].astype(np.float64)
@@ -140,6 +201,9 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bf16", action="store_true")
+    ap.add_argument("--no-early-exit", action="store_true")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="run the fixed launch sequence used for rocprofv3 counter passes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; multi-GPU figures are not produced "
@@ -151,19 +215,25 @@ def main():
 
     cfg = CONFIGS[args.workload]
     torch.cuda.set_device(0)
+    if args.profile_kernels:
+        profile_kernels(cfg)
+        return
     p32 = FramePipeline.synthetic(cfg, prec="fp32")
     p32.run()
     torch.cuda.synchronize()
     info = p32.check()
     sec = time_frames(p32, args.steps, args.warmup)
     stages = stage_times(p32)
-    # roofline of the dominant kernel: shade() is one launch of the fused
-    # radiance kernel; algorithmic FLOPs count evaluated (valid) samples only
+    # roofline of the dominant kernel, the fused sample kernel (stage 2 of
+    # sailor_render_rays): algorithmic FLOPs count evaluated (valid) samples
+    # of the per-view MLP only
     mc = p32.mc
     n_valid = info["valid_rays"]
     flops = n_valid * cfg.S * cfg.n_views * mlp_flops_per_sample_view(mc.F, mc.H)
-    shade_s = stages["shade"] / 1e3
-    achieved = flops / shade_s / 1e12
+    k32 = kernel_ms(p32, 2)
+    stages["sample_kernel"] = k32
+    stages["hydra_kernel"] = kernel_ms(p32, 4)
+    achieved = flops / (k32 / 1e3) / 1e12
     result = {
         "metric": METRIC, "value": 1.0 / sec, "unit": "frames/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
@@ -180,38 +250,43 @@ def main():
         },
         "roofline": {
             "bound": "mfma", "achieved": achieved, "peak": PEAKS["fp32"], "unit": "TFLOP/s",
-            "frac": achieved / PEAKS["fp32"], "traffic": None,
-            "kernel": "render_kernel<64,256,fp32> (sailor_render_rays)",
-            "flops_per_launch": flops, "avg_launch_ms": stages["shade"],
+            "frac": achieved / PEAKS["fp32"], "traffic": measured_traffic("fp32"),
+            "kernel": "render_kernel<64,256,fp32,24> (sailor_render_rays stage 2)",
+            "flops_per_launch": flops, "avg_launch_ms": k32,
         },
     }
     rgb32 = p32.rgb[0].cpu().numpy()
     valid_mask = (p32.sorted[0, :, 0] >= 0).cpu().numpy()
     if not args.no_bf16:
-        for prec in ("bf16",):
+        for prec in ("bf16", "fp16"):
             plo = FramePipeline.synthetic(cfg, prec=prec)
             plo.run()
             torch.cuda.synchronize()
             s_lo = time_frames(plo, args.steps, args.warmup)
             st_lo = stage_times(plo)
+            k_lo = kernel_ms(plo, 2)
+            st_lo["sample_kernel"] = k_lo
+            st_lo["hydra_kernel"] = kernel_ms(plo, 4)
             rgb_lo = plo.rgb[0].cpu().numpy()
             result[prec] = {
                 "frames_per_s": 1.0 / s_lo, "ms_per_step": s_lo * 1e3,
                 "speedup_vs_fp32": sec / s_lo,
                 "psnr_vs_fp32_db": psnr(rgb32, rgb_lo, valid_mask),
                 "stage_ms": st_lo,
-                "shade_tflops": flops / (st_lo["shade"] / 1e3) / 1e12,
-                "shade_frac_of_peak": flops / (st_lo["shade"] / 1e3) / 1e12 / PEAKS[prec],
+                "sample_kernel_tflops": flops / (k_lo / 1e3) / 1e12,
+                "sample_kernel_frac_of_peak": flops / (k_lo / 1e3) / 1e12 / PEAKS[prec],
+                "traffic": measured_traffic(prec),
             }
             del plo
-    pe = FramePipeline.synthetic(cfg, prec="fp32", early_exit=True)
-    pe.run()
-    torch.cuda.synchronize()
-    result["fp32_early_exit"] = {
-        "frames_per_s": 1.0 / time_frames(pe, args.steps, args.warmup),
-        "identical_image": bool(np.array_equal(pe.rgb[0].cpu().numpy(), rgb32)),
-    }
-    del pe
+    if not args.no_early_exit:
+        pe = FramePipeline.synthetic(cfg, prec="fp32", early_exit=True)
+        pe.run()
+        torch.cuda.synchronize()
+        result["fp32_early_exit"] = {
+            "frames_per_s": 1.0 / time_frames(pe, args.steps, args.warmup),
+            "identical_image": bool(np.array_equal(pe.rgb[0].cpu().numpy(), rgb32)),
+        }
+        del pe
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(p32)
     else:

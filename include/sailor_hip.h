@@ -251,7 +251,8 @@ size_t sailor_render_workspace_bytes(int B, int R, int F);
  *  early_exit  nonzero: stop evaluating a ray once its transmittance is
  *              below t_thresh (outputs are unchanged)
  *  chunk_samples  samples evaluated together per workgroup step (8, 16, 24
- *              or 48; 0 = default); S must be a multiple of it; a chunk
+ *              or 48; 0 = fp32: 24, 16-bit: 16, or the largest of 16 and 8
+ *              that divides S); S must be a multiple of it; a chunk
  *              whose activations do not fit the 160 KiB of LDS of a
  *              workgroup (fp32 with 48 samples) is rejected
  *  out_rgb f32 [B,R,3], out_depth f32 [B,R]; out_aux f32 [B,R,2] (weight sum,
@@ -269,6 +270,24 @@ int sailor_render_rays(const float* t, const float* rays, int B, int R, int S,
                        int chunk_samples, float* out_rgb, float* out_depth,
                        float* out_aux, float* out_occ, void* workspace,
                        size_t workspace_bytes, void* stream);
+
+/* sailor_render_rays split into its three launches, for profiling and for
+ * callers that interleave other work: `stages` is a mask of 1 (reset the ray
+ * list, list the rays with samples, write the outputs of dropped rays), 2
+ * (the sample kernel: gather, encoding, MLP, occupancy, compositing; depth,
+ * sums, occupancy and per-view features) and 4 (Hydra + RGB head: colour).
+ * A stage reads the workspace left by the earlier stages of a call with the
+ * same inputs; stages = 7 is sailor_render_rays.  Same arguments otherwise;
+ * no counterpart in the reference (which runs the path as one call). */
+int sailor_render_rays_stages(int stages, const float* t, const float* rays, int B, int R,
+                              int S, const float* Kv, const float* RTv, int V, int img_h,
+                              int img_w, const float* fmap, int Hf, int Wf, int F,
+                              const float* bbox, const float* zdir, int H, int precision,
+                              const void* w_dense, const float* b_dense,
+                              const float* w_small, float t_thresh, int early_exit,
+                              int chunk_samples, float* out_rgb, float* out_depth,
+                              float* out_aux, float* out_occ, void* workspace,
+                              size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Frequency encoding                                                        */

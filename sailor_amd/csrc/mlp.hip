@@ -86,6 +86,7 @@ struct RenderArgs {
   int S, V, img_w, img_h, Hf, Wf;
   float t_thres;
   int early_exit;
+  int stages;              // 1: compaction, 2: sample kernel, 4: Hydra/RGB kernel
 };
 
 template <int PREC> struct Tr;
@@ -1000,8 +1001,11 @@ int launch(const RenderArgs& a, hipStream_t st, bool dry) {
     grid = cus * std::max(per_cu, 1);
   }
   const int blocks = (int)std::min<long long>(grid, a.n_rays);
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(kThreads), L::total, st, a);
-  if (int err = sailor::check_launch("render_rays")) return err;
+  if (a.stages & 2) {
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kThreads), L::total, st, a);
+    if (int err = sailor::check_launch("render_rays")) return err;
+  }
+  if (!(a.stages & 4)) return SAILOR_OK;
   const long long groups = (a.n_rays + kRaysPerGroup - 1) / kRaysPerGroup;
   hipLaunchKernelGGL((hydra_rgb_kernel<F, H>), dim3((unsigned)groups), dim3(kThreads), 0, st,
                      a.out_tok, a.list, a.n_valid, a.wsmall, a.V, a.out_rgb);
@@ -1043,15 +1047,14 @@ extern "C" size_t sailor_render_workspace_bytes(int B, int R, int F) {
   return workspace_token_offset(n) + (size_t)n * kVP * F * sizeof(float);
 }
 
-extern "C" int sailor_render_rays(const float* t, const float* rays, int B, int R, int S,
-                                  const float* Kv, const float* RTv, int V, int img_h,
-                                  int img_w, const float* fmap, int Hf, int Wf, int F,
-                                  const float* bbox, const float* zdir, int H,
-                                  int precision, const void* w_dense, const float* b_dense,
-                                  const float* w_small, float t_thresh, int early_exit,
-                                  int chunk_samples, float* out_rgb, float* out_depth,
-                                  float* out_aux, float* out_occ, void* workspace,
-                                  size_t workspace_bytes, void* stream) {
+namespace {
+int render_stages(int stages, const float* t, const float* rays, int B, int R, int S,
+                  const float* Kv, const float* RTv, int V, int img_h, int img_w,
+                  const float* fmap, int Hf, int Wf, int F, const float* bbox,
+                  const float* zdir, int H, int precision, const void* w_dense,
+                  const float* b_dense, const float* w_small, float t_thresh, int early_exit,
+                  int chunk_samples, float* out_rgb, float* out_depth, float* out_aux,
+                  float* out_occ, void* workspace, size_t workspace_bytes, void* stream) {
   SAILOR_REQUIRE(B > 0 && R >= 0 && S > 0, "render_rays: bad sizes");
   SAILOR_REQUIRE((long long)B * R < (1ll << 31), "render_rays: at most 2^31-1 rays per call");
   SAILOR_REQUIRE(workspace != nullptr &&
@@ -1061,7 +1064,12 @@ extern "C" int sailor_render_rays(const float* t, const float* rays, int B, int 
   SAILOR_REQUIRE(V >= 1 && V <= kVP, "render_rays: 1 to %d views supported", kVP);
   SAILOR_REQUIRE(Hf > 1 && Wf > 1, "render_rays: feature map too small");
   int ns = chunk_samples;
-  if (ns <= 0) ns = (S % 16 == 0) ? 16 : 8;
+  if (ns <= 0) {
+    // measured on MI355X (C2 frame): fp32 24-sample chunks, 16-bit 16-sample
+    // chunks (two workgroups per CU)
+    if (precision == kF32 && S % 24 == 0) ns = 24;
+    else ns = (S % 16 == 0) ? 16 : 8;
+  }
   SAILOR_REQUIRE(S % ns == 0, "render_rays: S=%d is not a multiple of the chunk (%d samples)",
                  S, ns);
   if (R == 0) return SAILOR_OK;
@@ -1078,7 +1086,9 @@ extern "C" int sailor_render_rays(const float* t, const float* rays, int B, int 
   a.n_valid = counter;
   a.S = S; a.V = V; a.img_w = img_w; a.img_h = img_h; a.Hf = Hf; a.Wf = Wf;
   a.t_thres = t_thresh; a.early_exit = early_exit;
+  a.stages = stages;
   hipStream_t st = sailor::as_stream(stream);
+  SAILOR_REQUIRE(stages >= 1 && stages <= 7, "render_rays: stage mask %d out of range", stages);
   if (!((F == 64 && H == 256) || (F == 32 && H == 128))) {
     return sailor::set_error(SAILOR_ERR_ARG, "render_rays: (F=%d, H=%d) not instantiated "
                              "(supported: (32,128), (64,256))", F, H);
@@ -1096,13 +1106,48 @@ extern "C" int sailor_render_rays(const float* t, const float* rays, int B, int 
   }
   // the counter is cleared by a kernel, not a memset, so the call can be
   // captured into a graph
-  hipLaunchKernelGGL(queue_reset, dim3(1), dim3(64), 0, st, counter);
-  if (int err = sailor::check_launch("render_rays (reset)")) return err;
-  hipLaunchKernelGGL(compact_rays, dim3((unsigned)((a.n_rays + 255) / 256)), dim3(256), 0, st, t,
-                     S, a.n_rays, counter, list, out_rgb, out_depth, out_aux);
-  if (int err = sailor::check_launch("render_rays (compact)")) return err;
+  if (stages & 1) {
+    hipLaunchKernelGGL(queue_reset, dim3(1), dim3(64), 0, st, counter);
+    if (int err = sailor::check_launch("render_rays (reset)")) return err;
+    hipLaunchKernelGGL(compact_rays, dim3((unsigned)((a.n_rays + 255) / 256)), dim3(256), 0, st,
+                       t, S, a.n_rays, counter, list, out_rgb, out_depth, out_aux);
+    if (int err = sailor::check_launch("render_rays (compact)")) return err;
+  }
+  if (!(stages & 6)) return SAILOR_OK;
   if (F == 64 && H == 256) return dispatch_ns<64, 256>(ns, precision, a, st, false);
   if (F == 32 && H == 128) return dispatch_ns<32, 128>(ns, precision, a, st, false);
   return sailor::set_error(SAILOR_ERR_ARG, "render_rays: (F=%d, H=%d) not instantiated "
                            "(supported: (32,128), (64,256))", F, H);
+}
+}  // namespace
+
+extern "C" int sailor_render_rays(const float* t, const float* rays, int B, int R, int S,
+                                  const float* Kv, const float* RTv, int V, int img_h,
+                                  int img_w, const float* fmap, int Hf, int Wf, int F,
+                                  const float* bbox, const float* zdir, int H,
+                                  int precision, const void* w_dense, const float* b_dense,
+                                  const float* w_small, float t_thresh, int early_exit,
+                                  int chunk_samples, float* out_rgb, float* out_depth,
+                                  float* out_aux, float* out_occ, void* workspace,
+                                  size_t workspace_bytes, void* stream) {
+  return render_stages(7, t, rays, B, R, S, Kv, RTv, V, img_h, img_w, fmap, Hf, Wf, F, bbox,
+                       zdir, H, precision, w_dense, b_dense, w_small, t_thresh, early_exit,
+                       chunk_samples, out_rgb, out_depth, out_aux, out_occ, workspace,
+                       workspace_bytes, stream);
+}
+
+extern "C" int sailor_render_rays_stages(int stages, const float* t, const float* rays, int B,
+                                         int R, int S, const float* Kv, const float* RTv, int V,
+                                         int img_h, int img_w, const float* fmap, int Hf, int Wf,
+                                         int F, const float* bbox, const float* zdir, int H,
+                                         int precision, const void* w_dense,
+                                         const float* b_dense, const float* w_small,
+                                         float t_thresh, int early_exit, int chunk_samples,
+                                         float* out_rgb, float* out_depth, float* out_aux,
+                                         float* out_occ, void* workspace, size_t workspace_bytes,
+                                         void* stream) {
+  return render_stages(stages, t, rays, B, R, S, Kv, RTv, V, img_h, img_w, fmap, Hf, Wf, F,
+                       bbox, zdir, H, precision, w_dense, b_dense, w_small, t_thresh,
+                       early_exit, chunk_samples, out_rgb, out_depth, out_aux, out_occ,
+                       workspace, workspace_bytes, stream);
 }

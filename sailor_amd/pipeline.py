@@ -176,11 +176,12 @@ class FramePipeline:
         call("sailor_sort_samplings", ptr(self.st), cfg.target * cfg.target, cfg.S,
              ptr(self.sorted), ptr(self.order), st)
 
-    def shade(self):
-        """Fused radiance evaluation + compositing (forward_ray_nerf / FastNerf)."""
+    def shade(self, stages=7):
+        """Fused radiance evaluation + compositing (forward_ray_nerf / FastNerf);
+        `stages` selects launches of sailor_render_rays_stages (7 = all)."""
         cfg, mc, m = self.cfg, self.mc, self.model
         from .model import PREC_IDS
-        call("sailor_render_rays", ptr(self.sorted), ptr(self.rays), 1,
+        call("sailor_render_rays_stages", int(stages), ptr(self.sorted), ptr(self.rays), 1,
              cfg.target * cfg.target, cfg.S, ptr(self.K), ptr(self.RT), cfg.n_views, cfg.img,
              cfg.img, ptr(self.fmap), mc.Hf, mc.Wf, mc.F, ptr(self.bbox), ptr(self.zdir), mc.H,
              PREC_IDS[m.prec], ptr(m.w_dense), ptr(m.b_dense), ptr(m.w_small), 1e-4,
