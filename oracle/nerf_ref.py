@@ -28,6 +28,31 @@ def freq_encode(x, deg=10):
     return torch.cat(outs, -1)
 
 
+def gather_features(x, Kv, RTv, fmaps, img_hw):
+    """Per-view features at world points x [..., 3]: proj_persp + grid_sample
+    (align_corners=True, zero padding); Kv [V,3,3], RTv [V,3,4], fmaps
+    [V,F,Hf,Wf].  Returns a list over views of [..., F]."""
+    img_h, img_w = img_hw
+    feats = []
+    shape = x.shape[:-1]
+    xf = x.reshape(1, -1, 3)
+    for v in range(Kv.shape[0]):
+        Rv, Tv = RTv[v, :, :3], RTv[v, :, 3]
+        p = xf @ Rv.T + Tv
+        z = p[..., 2:3]
+        inv = z < 0.01
+        zz = torch.where(inv, torch.ones_like(z), z)
+        q = p / zz
+        scr = q @ Kv[v].T
+        res = torch.tensor([img_w // 2, img_h // 2], dtype=x.dtype, device=x.device)
+        grid = (scr[..., :2] - res) / res
+        grid = torch.where(inv.expand_as(grid), torch.full_like(grid, -2.0), grid)
+        smp = Fn.grid_sample(fmaps[v][None], grid[None], mode="bilinear",
+                             padding_mode="zeros", align_corners=True)[0]  # [F, 1, n]
+        feats.append(smp[:, 0].T.reshape(*shape, -1))
+    return feats
+
+
 def render_rays(weights, t, rays, Kv, RTv, fmaps, bbox, zdir, img_hw, t_thres=1e-4,
                 dtype=torch.float64, device="cpu"):
     """weights: sailor_amd.model.NerfWeights; t [B,R,S] sorted; rays [B,R,6];
@@ -38,7 +63,6 @@ def render_rays(weights, t, rays, Kv, RTv, fmaps, bbox, zdir, img_hw, t_thres=1e
     t, rays, Kv, RTv, fmaps, bbox, zdir = map(T_, (t, rays, Kv, RTv, fmaps, bbox, zdir))
     B, R, S = t.shape
     V = Kv.shape[1]
-    img_h, img_w = img_hw
     W = [T_(w) for w in weights.dense_w]
     Bs = [T_(b) for b in weights.dense_b]
     rgb_out = torch.zeros((B, R, 3), dtype=dtype, device=device)
@@ -57,21 +81,7 @@ def render_rays(weights, t, rays, Kv, RTv, fmaps, bbox, zdir, img_hw, t_thres=1e
         c = 0.5 * (bbox[b, :, 0] + bbox[b, :, 1])
         ih = 2.0 / (bbox[b, :, 1] - bbox[b, :, 0])
         pe = freq_encode((x - c) * ih)                   # [N, S, 63]
-        feats = []
-        for v in range(V):
-            Rv, Tv = RTv[b, v, :, :3], RTv[b, v, :, 3]
-            p = x @ Rv.T + Tv                            # [N, S, 3]
-            z = p[..., 2:3]
-            inv = z < 0.01
-            zz = torch.where(inv, torch.ones_like(z), z)
-            q = p / zz
-            scr = q @ Kv[b, v].T
-            res = torch.tensor([img_w // 2, img_h // 2], dtype=dtype, device=device)
-            grid = (scr[..., :2] - res) / res
-            grid = torch.where(inv.expand_as(grid), torch.full_like(grid, -2.0), grid)
-            smp = Fn.grid_sample(fmaps[b, v][None], grid[None], mode="bilinear",
-                                 padding_mode="zeros", align_corners=True)[0]  # [F, N, S]
-            feats.append(smp.permute(1, 2, 0))           # [N, S, F]
+        feats = gather_features(x, Kv[b], RTv[b], fmaps[b], img_hw)  # V x [N, S, F]
         h_sum = 0
         for v in range(V):
             h = torch.cat([pe, feats[v]], -1)

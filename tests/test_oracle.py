@@ -579,3 +579,71 @@ def test_freq_closed_form_and_gradient():
         c = gy[:, D + (2 * f + 1) * D:D + (2 * f + 2) * D]
         ref += 2.0 ** f * (s * np.cos(2.0 ** f * x64) - c * np.sin(2.0 ** f * x64))
     np.testing.assert_allclose(gx, ref, rtol=1e-4, atol=1e-4)
+
+
+# ---------------------------------------------------------------- model restatement
+def test_nerf_ref_gather_is_bilinear_align_corners():
+    """proj_persp + grid_sample(align_corners=True, zeros) equals the explicit
+    bilinear formula ix = (u - W/2)/(W/2) -> (gx + 1)/2 (Wf - 1) with zero taps
+    outside the map, and points behind the camera (z < 0.01) read zero."""
+    import torch
+    from oracle import nerf_ref
+    rng = np.random.default_rng(4)
+    Fc, Hf, Wf, img = 5, 9, 11, (64, 80)
+    fm = rng.standard_normal((1, Fc, Hf, Wf))
+    K = np.array([[[70.0, 0, 40.0], [0, 70.0, 32.0], [0, 0, 1]]])
+    RT = np.concatenate([_rot_y(0.2), [[0.1], [-0.05], [1.5]]], 1)[None]
+    x = rng.uniform(-0.8, 0.8, (40, 3))
+    x[:3, 2] = -1.8                                     # behind the camera
+    got = nerf_ref.gather_features(torch.tensor(x), torch.tensor(K), torch.tensor(RT),
+                                   torch.tensor(fm), img)[0].numpy()
+    for n in range(len(x)):
+        p = RT[0, :, :3] @ x[n] + RT[0, :, 3]
+        if p[2] < 0.01:
+            assert np.all(got[n] == 0)
+            continue
+        u = K[0] @ (p / p[2])
+        gx = (u[0] - img[1] // 2) / (img[1] // 2)
+        gy = (u[1] - img[0] // 2) / (img[0] // 2)
+        ix, iy = (gx + 1) / 2 * (Wf - 1), (gy + 1) / 2 * (Hf - 1)
+        x0, y0 = int(np.floor(ix)), int(np.floor(iy))
+        ax, ay = ix - x0, iy - y0
+        exp = np.zeros(Fc)
+        for dy, wy in ((0, 1 - ay), (1, ay)):
+            for dx, wx in ((0, 1 - ax), (1, ax)):
+                xx, yy = x0 + dx, y0 + dy
+                if 0 <= xx < Wf and 0 <= yy < Hf:
+                    exp += wx * wy * fm[0, :, yy, xx]
+        np.testing.assert_allclose(got[n], exp, atol=1e-12)
+
+
+def test_nerf_ref_encoding_and_compositing_agree_with_oracle():
+    """The float64 model restatement's frequency encoding is the closed form,
+    and its compositing (depth, weight sum, alpha sum given its own per-sample
+    occupancies) agrees with the C oracle's recurrence."""
+    import torch
+    from oracle import nerf_ref
+    from sailor_amd import model as M
+    x = torch.tensor(np.random.default_rng(0).uniform(-1, 1, (7, 3)))
+    pe = nerf_ref.freq_encode(x).numpy()
+    np.testing.assert_allclose(pe, O.freq_fwd(x.numpy().astype(np.float32), 10), atol=2e-3)
+    w = M.NerfWeights.init(32, 128)
+    rng = np.random.default_rng(1)
+    B, R, S, V = 1, 6, 8, 4
+    t = np.sort(rng.uniform(2.0, 3.0, (B, R, S)), -1)
+    rays = np.zeros((B, R, 6))
+    rays[..., :2] = rng.uniform(-0.2, 0.2, (B, R, 2))
+    rays[..., 2] = 2.5
+    rays[..., 5] = -1.0
+    K = np.tile(np.array([[100., 0, 64], [0, 100., 64], [0, 0, 1]]), (B, V, 1, 1))
+    RT = np.tile(np.concatenate([np.eye(3), np.array([[0], [0], [3.0]])], 1), (B, V, 1, 1))
+    fm = rng.standard_normal((B, V, 32, 16, 16))
+    bbox = np.array([[[-1, 1], [-1, 1], [-1, 1]]], np.float64)
+    zdir = np.array([[0, 0, 1.0]])
+    rgb, dep, ws, asum, occ = nerf_ref.render_rays(w, t, rays, K, RT, fm, bbox, zdir, (128, 128))
+    feats = np.zeros((B, R, S, 1), np.float32)
+    of, od, ow, oa = O.render_occ(t.astype(np.float32), occ.numpy().astype(np.float32), feats)
+    np.testing.assert_allclose(ws.numpy(), ow, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(asum.numpy(), oa, rtol=1e-5, atol=1e-6)
+    # depth is t composited times (d . zdir) = -1 for these rays
+    np.testing.assert_allclose(dep.numpy(), -od, rtol=1e-5, atol=1e-6)
