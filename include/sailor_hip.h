@@ -243,7 +243,9 @@ size_t sailor_render_workspace_bytes(int B, int R, int F);
  *  t        f32 [B,R,S] sorted sample distances (-1 padded rows are dropped)
  *  rays     f32 [B,R,6] origin and unit direction
  *  Kv, RTv  f32 [B,V,3,3], [B,V,3,4] source cameras; img_h x img_w images
- *  fmap     f32 [B,V,Hf,Wf,F] channels-last per-view feature maps
+ *  fmap     [B,V,Hf,Wf,F] channels-last per-view feature maps, in fp32
+ *           (fmap_precision 0) or in the form's 16-bit type (fmap_precision
+ *           == precision; FastNerf keeps its features in half precision)
  *  bbox     f32 [B,3,2] volume box (frame of the positional encoding)
  *  zdir     f32 [B,3] third row of the target rotation (depth = t * zdir.dir)
  *  precision 0 = fp32 (MFMA f32), 1 = bf16, 2 = fp16 (MFMA, fp32 accumulate)
@@ -263,7 +265,8 @@ size_t sailor_render_workspace_bytes(int B, int R, int F);
  *           Hydra/RGB kernel); calls that overlap on one workspace race.    */
 int sailor_render_rays(const float* t, const float* rays, int B, int R, int S,
                        const float* Kv, const float* RTv, int V, int img_h,
-                       int img_w, const float* fmap, int Hf, int Wf, int F,
+                       int img_w, const void* fmap, int fmap_precision, int Hf,
+                       int Wf, int F,
                        const float* bbox, const float* zdir, int H,
                        int precision, const void* w_dense, const float* b_dense,
                        const float* w_small, float t_thresh, int early_exit,
@@ -281,7 +284,8 @@ int sailor_render_rays(const float* t, const float* rays, int B, int R, int S,
  * no counterpart in the reference (which runs the path as one call). */
 int sailor_render_rays_stages(int stages, const float* t, const float* rays, int B, int R,
                               int S, const float* Kv, const float* RTv, int V, int img_h,
-                              int img_w, const float* fmap, int Hf, int Wf, int F,
+                              int img_w, const void* fmap, int fmap_precision, int Hf,
+                              int Wf, int F,
                               const float* bbox, const float* zdir, int H, int precision,
                               const void* w_dense, const float* b_dense,
                               const float* w_small, float t_thresh, int early_exit,

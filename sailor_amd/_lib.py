@@ -37,11 +37,11 @@ _SIGS = {
     "sailor_volume_rendering_training_forward": [P, P, P, P, I, I, I, I, F, I, P, P, P, P, P],
     "sailor_volume_rendering_training_backward": [P] * 12 + [I, I, I, I, F, I, P, P, P],
     "sailor_render_small_weights_floats": [I, I],
-    "sailor_render_rays": [P, P, I, I, I, P, P, I, I, I, P, I, I, I, P, P, I, I, P, P, P, F, I,
-                           I, P, P, P, P, P, SZ, P],
+    "sailor_render_rays": [P, P, I, I, I, P, P, I, I, I, P, I, I, I, I, P, P, I, I, P, P, P, F,
+                           I, I, P, P, P, P, P, SZ, P],
     "sailor_render_workspace_bytes": [I, I, I],
-    "sailor_render_rays_stages": [I, P, P, I, I, I, P, P, I, I, I, P, I, I, I, P, P, I, I, P, P,
-                                  P, F, I, I, P, P, P, P, P, SZ, P],
+    "sailor_render_rays_stages": [I, P, P, I, I, I, P, P, I, I, I, P, I, I, I, I, P, P, I, I, P,
+                                  P, P, F, I, I, P, P, P, P, P, SZ, P],
     "sailor_freq_encode_forward": [P, I, I, I, P, P],
     "sailor_freq_encode_backward": [P, P, I, I, I, P, P],
 }

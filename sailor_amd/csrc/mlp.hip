@@ -68,7 +68,8 @@ struct RenderArgs {
   const float* rays;       // [n_rays, 6] origin, direction
   const float* Kv;         // [B, V, 3, 3] source intrinsics
   const float* RTv;        // [B, V, 3, 4] source extrinsics
-  const float* fmap;       // [B, V, Hf, Wf, F] channels-last feature maps
+  const void* fmap;        // [B, V, Hf, Wf, F] channels-last maps, fp32 or the form's 16 bit
+  int fmap_half;           // maps stored in the form's 16-bit type
   const float* bbox;       // [B, 3, 2] volume box (positional-encoding frame)
   const float* zdir;       // [B, 3] third row of the target rotation
   const void* wdense;      // packed dense layers
@@ -500,7 +501,7 @@ constexpr size_t kLdsBytes = 160 * 1024;  // LDS per workgroup on gfx950
 // let one workgroup's gather / compositing overlap the other's MFMA work.
 constexpr int render_occupancy(int PREC, int NS) { return (PREC != kF32 && NS <= 16) ? 2 : 1; }
 
-template <int F, int H, int PREC, int NS>
+template <int F, int H, int PREC, int NS, bool HALF_MAP>
 __global__ void __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(render_occupancy(PREC, NS), render_occupancy(PREC, NS))))
 render_kernel(RenderArgs a) {
@@ -595,20 +596,34 @@ render_kernel(RenderArgs a) {
               const float fx0 = floorf(ix), fy0 = floorf(iy);
               const int x0 = (int)fx0, y0 = (int)fy0;
               const float ax = ix - fx0, ay = iy - fy0;
-              const float* fm = a.fmap + ((size_t)b * V + v) * a.Hf * a.Wf * F + quarter * CPL;
+              const size_t base = ((size_t)b * V + v) * a.Hf * a.Wf * F + quarter * CPL;
 #pragma unroll
               for (int tap = 0; tap < 4; ++tap) {
                 const int xx = x0 + (tap & 1), yy = y0 + (tap >> 1);
                 if (xx < 0 || xx >= a.Wf || yy < 0 || yy >= a.Hf) continue;
                 const float wt = ((tap & 1) ? ax : 1.0f - ax) * ((tap >> 1) ? ay : 1.0f - ay);
-                const f32x4* src = reinterpret_cast<const f32x4*>(fm + ((size_t)yy * a.Wf + xx) * F);
+                const size_t off = base + ((size_t)yy * a.Wf + xx) * F;
+                if constexpr (HALF_MAP) {
+                  // the form's 16-bit maps: one 16-byte load per 8 channels
+                  typedef typename Tr<PREC>::V8 V8;
+                  const V8* src = reinterpret_cast<const V8*>(static_cast<const E*>(a.fmap) + off);
 #pragma unroll
-                for (int qq = 0; qq < CPL / 4; ++qq) {
-                  const f32x4 v4 = src[qq];
-                  f[4 * qq + 0] += wt * v4[0];
-                  f[4 * qq + 1] += wt * v4[1];
-                  f[4 * qq + 2] += wt * v4[2];
-                  f[4 * qq + 3] += wt * v4[3];
+                  for (int qq = 0; qq < CPL / 8; ++qq) {
+                    const V8 v8 = src[qq];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) f[8 * qq + e] += wt * (float)v8[e];
+                  }
+                } else {
+                  const f32x4* src =
+                      reinterpret_cast<const f32x4*>(static_cast<const float*>(a.fmap) + off);
+#pragma unroll
+                  for (int qq = 0; qq < CPL / 4; ++qq) {
+                    const f32x4 v4 = src[qq];
+                    f[4 * qq + 0] += wt * v4[0];
+                    f[4 * qq + 1] += wt * v4[1];
+                    f[4 * qq + 2] += wt * v4[2];
+                    f[4 * qq + 3] += wt * v4[3];
+                  }
                 }
               }
             }
@@ -979,6 +994,9 @@ compact_rays(const float* __restrict__ t, int S, long long n_rays, unsigned* cou
   }
 }
 
+template <int F, int H, int PREC, int NS, bool HALF_MAP>
+int launch_kernels(const RenderArgs& a, hipStream_t st);
+
 template <int F, int H, int PREC, int NS>
 int launch(const RenderArgs& a, hipStream_t st, bool dry) {
   typedef Smem<F, H, PREC, NS> L;
@@ -987,7 +1005,17 @@ int launch(const RenderArgs& a, hipStream_t st, bool dry) {
                              "LDS at this precision (limit %zu)", NS, L::total, kLdsBytes);
   } else {
   if (dry) return SAILOR_OK;  // configuration check only
-  auto kern = render_kernel<F, H, PREC, NS>;
+  if constexpr (PREC != kF32) {
+    if (a.fmap_half) return launch_kernels<F, H, PREC, NS, true>(a, st);
+  }
+  return launch_kernels<F, H, PREC, NS, false>(a, st);
+  }
+}
+
+template <int F, int H, int PREC, int NS, bool HALF_MAP>
+int launch_kernels(const RenderArgs& a, hipStream_t st) {
+  typedef Smem<F, H, PREC, NS> L;
+  auto kern = render_kernel<F, H, PREC, NS, HALF_MAP>;
   static int grid = 0;
   if (grid == 0) {
     SAILOR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -1010,7 +1038,6 @@ int launch(const RenderArgs& a, hipStream_t st, bool dry) {
   hipLaunchKernelGGL((hydra_rgb_kernel<F, H>), dim3((unsigned)groups), dim3(kThreads), 0, st,
                      a.out_tok, a.list, a.n_valid, a.wsmall, a.V, a.out_rgb);
   return sailor::check_launch("render_rays (hydra)");
-  }
 }
 
 template <int F, int H, int NS>
@@ -1050,8 +1077,9 @@ extern "C" size_t sailor_render_workspace_bytes(int B, int R, int F) {
 namespace {
 int render_stages(int stages, const float* t, const float* rays, int B, int R, int S,
                   const float* Kv, const float* RTv, int V, int img_h, int img_w,
-                  const float* fmap, int Hf, int Wf, int F, const float* bbox,
-                  const float* zdir, int H, int precision, const void* w_dense,
+                  const void* fmap, int fmap_precision, int Hf, int Wf, int F,
+                  const float* bbox, const float* zdir, int H, int precision,
+                  const void* w_dense,
                   const float* b_dense, const float* w_small, float t_thresh, int early_exit,
                   int chunk_samples, float* out_rgb, float* out_depth, float* out_aux,
                   float* out_occ, void* workspace, size_t workspace_bytes, void* stream) {
@@ -1074,7 +1102,10 @@ int render_stages(int stages, const float* t, const float* rays, int B, int R, i
                  S, ns);
   if (R == 0) return SAILOR_OK;
   RenderArgs a;
+  SAILOR_REQUIRE(fmap_precision == kF32 || fmap_precision == precision,
+                 "render_rays: feature maps must be fp32 or the form's 16-bit type");
   a.t = t; a.rays = rays; a.Kv = Kv; a.RTv = RTv; a.fmap = fmap; a.bbox = bbox; a.zdir = zdir;
+  a.fmap_half = fmap_precision != kF32;
   a.wdense = w_dense; a.bdense = b_dense; a.wsmall = w_small;
   a.out_rgb = out_rgb; a.out_depth = out_depth; a.out_aux = out_aux; a.out_occ = out_occ;
   a.n_rays = (long long)B * R; a.rays_per_batch = R;
@@ -1123,31 +1154,32 @@ int render_stages(int stages, const float* t, const float* rays, int B, int R, i
 
 extern "C" int sailor_render_rays(const float* t, const float* rays, int B, int R, int S,
                                   const float* Kv, const float* RTv, int V, int img_h,
-                                  int img_w, const float* fmap, int Hf, int Wf, int F,
-                                  const float* bbox, const float* zdir, int H,
+                                  int img_w, const void* fmap, int fmap_precision, int Hf,
+                                  int Wf, int F, const float* bbox, const float* zdir, int H,
                                   int precision, const void* w_dense, const float* b_dense,
                                   const float* w_small, float t_thresh, int early_exit,
                                   int chunk_samples, float* out_rgb, float* out_depth,
                                   float* out_aux, float* out_occ, void* workspace,
                                   size_t workspace_bytes, void* stream) {
-  return render_stages(7, t, rays, B, R, S, Kv, RTv, V, img_h, img_w, fmap, Hf, Wf, F, bbox,
-                       zdir, H, precision, w_dense, b_dense, w_small, t_thresh, early_exit,
-                       chunk_samples, out_rgb, out_depth, out_aux, out_occ, workspace,
-                       workspace_bytes, stream);
+  return render_stages(7, t, rays, B, R, S, Kv, RTv, V, img_h, img_w, fmap, fmap_precision, Hf,
+                       Wf, F, bbox, zdir, H, precision, w_dense, b_dense, w_small, t_thresh,
+                       early_exit, chunk_samples, out_rgb, out_depth, out_aux, out_occ,
+                       workspace, workspace_bytes, stream);
 }
 
 extern "C" int sailor_render_rays_stages(int stages, const float* t, const float* rays, int B,
                                          int R, int S, const float* Kv, const float* RTv, int V,
-                                         int img_h, int img_w, const float* fmap, int Hf, int Wf,
-                                         int F, const float* bbox, const float* zdir, int H,
+                                         int img_h, int img_w, const void* fmap,
+                                         int fmap_precision, int Hf, int Wf, int F,
+                                         const float* bbox, const float* zdir, int H,
                                          int precision, const void* w_dense,
                                          const float* b_dense, const float* w_small,
                                          float t_thresh, int early_exit, int chunk_samples,
                                          float* out_rgb, float* out_depth, float* out_aux,
                                          float* out_occ, void* workspace, size_t workspace_bytes,
                                          void* stream) {
-  return render_stages(stages, t, rays, B, R, S, Kv, RTv, V, img_h, img_w, fmap, Hf, Wf, F,
-                       bbox, zdir, H, precision, w_dense, b_dense, w_small, t_thresh,
-                       early_exit, chunk_samples, out_rgb, out_depth, out_aux, out_occ,
-                       workspace, workspace_bytes, stream);
+  return render_stages(stages, t, rays, B, R, S, Kv, RTv, V, img_h, img_w, fmap,
+                       fmap_precision, Hf, Wf, F, bbox, zdir, H, precision, w_dense, b_dense,
+                       w_small, t_thresh, early_exit, chunk_samples, out_rgb, out_depth,
+                       out_aux, out_occ, workspace, workspace_bytes, stream);
 }

@@ -52,7 +52,7 @@ CONFIGS = {
 
 class FramePipeline:
     def __init__(self, cfg: FrameConfig, prec="fp32", device="cuda", early_exit=False,
-                 weights: NerfWeights = None, chunk_samples=0):
+                 weights: NerfWeights = None, chunk_samples=0, half_maps=None):
         self.cfg = cfg
         self.chunk_samples = chunk_samples
         self.dev = torch.device(device)
@@ -75,6 +75,14 @@ class FramePipeline:
         self.masks = torch.empty((1, V, n, n), dtype=_i32, **o)
         self.fixed_bbox = torch.empty((1, 3, 2), dtype=_f32, **o)
         self.fmap = torch.empty((1, V, mc.Hf, mc.Wf, mc.F), dtype=_f32, **o)
+        # 16-bit forms read the feature maps in their own 16-bit type (as
+        # FastNerf keeps features in half precision); the conversion runs in
+        # build(), inside the frame
+        self.half_maps = (prec != "fp32") if half_maps is None else bool(half_maps)
+        self.fmap_lo = None
+        if self.half_maps:
+            dt = torch.bfloat16 if prec == "bf16" else torch.float16
+            self.fmap_lo = torch.empty((1, V, mc.Hf, mc.Wf, mc.F), dtype=dt, **o)
         self.occ0 = torch.empty((1, d, d, d), dtype=_i32, **o)
         self.occ1 = torch.empty((1, dc, dc, dc), dtype=_i32, **o)
         self.num = torch.zeros((2, 1), dtype=_i32, **o)
@@ -151,6 +159,8 @@ class FramePipeline:
         cfg, V, n = self.cfg, self.cfg.n_views, self.cfg.img
         d, dc = cfg.dim, self.dc
         st = stream_of(self.dev)
+        if self.fmap_lo is not None:
+            self.fmap_lo.copy_(self.fmap)
         self.num.zero_()
         call("sailor_integrate", ptr(self.occ0), ptr(self.num[0]), ptr(self.K), ptr(self.RT),
              ptr(self.R), ptr(self.T), ptr(self.colors), ptr(self.depths), ptr(self.masks), 1,
@@ -183,7 +193,8 @@ class FramePipeline:
         from .model import PREC_IDS
         call("sailor_render_rays_stages", int(stages), ptr(self.sorted), ptr(self.rays), 1,
              cfg.target * cfg.target, cfg.S, ptr(self.K), ptr(self.RT), cfg.n_views, cfg.img,
-             cfg.img, ptr(self.fmap), mc.Hf, mc.Wf, mc.F, ptr(self.bbox), ptr(self.zdir), mc.H,
+             cfg.img, ptr(self.fmap_lo if self.half_maps else self.fmap),
+             PREC_IDS[m.prec] if self.half_maps else 0, mc.Hf, mc.Wf, mc.F, ptr(self.bbox), ptr(self.zdir), mc.H,
              PREC_IDS[m.prec], ptr(m.w_dense), ptr(m.b_dense), ptr(m.w_small), 1e-4,
              int(self.early_exit), int(self.chunk_samples), ptr(self.rgb), ptr(self.depth),
              ptr(self.aux), None, ptr(self.render_ws), self.render_ws.numel(),

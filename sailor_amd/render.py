@@ -49,7 +49,8 @@ class PackedModel:
 def render_rays(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, model: PackedModel,
                 t_thres=1e-4, early_exit=False, want_occ=False, out=None, chunk_samples=0):
     """t [B,R,S] sorted; rays [B,R,6]; Kv [B,V,3,3]; RTv [B,V,3,4];
-    fmap_cl [B,V,Hf,Wf,F] channels-last; bbox [B,3,2]; zdir [B,3].
+    fmap_cl [B,V,Hf,Wf,F] channels-last, float32 or the model's 16-bit dtype;
+    bbox [B,3,2]; zdir [B,3].
     Returns dict(rgb [B,R,3], depth [B,R], aux [B,R,2], occ [B,R,S] or None)."""
     dev = t.device
     B, R, S = t.shape
@@ -57,6 +58,10 @@ def render_rays(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, model: PackedMode
     Hf, Wf, F = fmap_cl.shape[2], fmap_cl.shape[3], fmap_cl.shape[4]
     if F != model.F:
         raise RuntimeError(f"feature maps have {F} channels, model expects {model.F}")
+    map_prec = {torch.float32: 0, torch.bfloat16: PREC_IDS["bf16"],
+                torch.float16: PREC_IDS["fp16"]}.get(fmap_cl.dtype)
+    if map_prec is None:
+        raise RuntimeError(f"feature maps of dtype {fmap_cl.dtype} are not supported")
     if out is None:
         out = dict(rgb=torch.empty((B, R, 3), dtype=_f32, device=dev),
                    depth=torch.empty((B, R), dtype=_f32, device=dev),
@@ -66,7 +71,7 @@ def render_rays(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, model: PackedMode
     if out.get("workspace") is None or out["workspace"].numel() < ws_bytes:
         out["workspace"] = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     call("sailor_render_rays", ptr(t), ptr(rays), B, R, S, ptr(Kv), ptr(RTv), V, int(img_hw[0]),
-         int(img_hw[1]), ptr(fmap_cl), Hf, Wf, F, ptr(bbox), ptr(zdir), model.H,
+         int(img_hw[1]), ptr(fmap_cl), map_prec, Hf, Wf, F, ptr(bbox), ptr(zdir), model.H,
          PREC_IDS[model.prec], ptr(model.w_dense), ptr(model.b_dense), ptr(model.w_small),
          float(t_thres), int(bool(early_exit)), int(chunk_samples), ptr(out["rgb"]),
          ptr(out["depth"]),
