@@ -129,6 +129,32 @@ def profile_kernels(cfg):
         del p
 
 
+def precision_form(cfg, steps, warmup):
+    """Config-3 block: the C2 scene rendered at 1024^2 (1.05 M rays, 48
+    samples) in FP32, BF16 and FP16; frames/s and PSNR against the FP32
+    render."""
+    import torch
+    from sailor_amd.pipeline import FramePipeline
+    out = {"workload": f"{cfg.name}: C2 scene, {cfg.target}^2 render, {cfg.S} samples/ray"}
+    ref = None
+    for prec in ("fp32", "bf16", "fp16"):
+        p = FramePipeline.synthetic(cfg, prec=prec)
+        p.run()
+        torch.cuda.synchronize()
+        sec = time_frames(p, steps, warmup)
+        rgb = p.rgb[0].cpu().numpy()
+        if ref is None:
+            ref = rgb
+            mask = (p.sorted[0, :, 0] >= 0).cpu().numpy()
+            out["valid_rays"] = int(mask.sum())
+        out[prec] = {"frames_per_s": 1.0 / sec, "ms_per_step": sec * 1e3}
+        if prec != "fp32":
+            out[prec]["psnr_vs_fp32_db"] = psnr(ref, rgb, mask)
+            out[prec]["speedup_vs_fp32"] = out["fp32"]["ms_per_step"] / (sec * 1e3)
+        del p
+    return out
+
+
 def psnr(a, b, mask):
     import numpy as np
     m = mask[..., None].astype(np.float64)
@@ -202,6 +228,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bf16", action="store_true")
     ap.add_argument("--no-early-exit", action="store_true")
+    ap.add_argument("--no-c3", action="store_true",
+                    help="skip the 1024^2 precision-form block")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="run the fixed launch sequence used for rocprofv3 counter passes")
     args = ap.parse_args()
@@ -287,6 +315,9 @@ def main():
             "identical_image": bool(np.array_equal(pe.rgb[0].cpu().numpy(), rgb32)),
         }
         del pe
+    if not args.no_c3 and args.workload == "c2":
+        result["c3_precision_form"] = precision_form(CONFIGS["c3"], max(3, args.steps // 4),
+                                                     args.warmup)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(p32)
     else:
