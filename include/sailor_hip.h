@@ -293,6 +293,20 @@ int sailor_render_rays_stages(int stages, const float* t, const float* rays, int
                               float* out_aux, float* out_occ, void* workspace,
                               size_t workspace_bytes, void* stream);
 
+/* Density query at arbitrary points: replaces infer_density_mlp
+ * (c_lib/FastNerf/src/infer_mlp.cu:583-634, the GUI's occupancy query after
+ * fusion) for the fused model: per point, the per-view MLP on the gathered
+ * features and the encoding, the view mean and the occupancy head.
+ *  points f32 [B,N,3]; other inputs as sailor_render_rays (zdir, t and the
+ *  compositing arguments do not apply); chunk_samples = points per group
+ *  (0 = default).  out_density f32 [B,N]. */
+int sailor_density_points(const float* points, int B, int N, const float* Kv,
+                          const float* RTv, int V, int img_h, int img_w, const void* fmap,
+                          int fmap_precision, int Hf, int Wf, int F, const float* bbox, int H,
+                          int precision, const void* w_dense, const float* b_dense,
+                          const float* w_small, int chunk_samples, float* out_density,
+                          void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Frequency encoding                                                        */
 /* ------------------------------------------------------------------------ */

@@ -3,6 +3,7 @@ INFRASTRUCTURE ONLY (imported by tests/ and bench.py's CPU baseline).
 
 Written from the reference modules, not from the HIP kernel:
   projection + grid_sample   utils_render/utils_render.py:380-410 (proj_persp),
+  density query              c_lib/FastNerf/src/infer_mlp.cu:583-634
                              SRONet/BasicRenNet.py:245-259 (F.grid_sample,
                              align_corners=True, zero padding)
   frequency encoding         c_lib/VoxelEncoding/src/freq_encoding.cu:37-45
@@ -51,6 +52,30 @@ def gather_features(x, Kv, RTv, fmaps, img_hw):
                              padding_mode="zeros", align_corners=True)[0]  # [F, 1, n]
         feats.append(smp[:, 0].T.reshape(*shape, -1))
     return feats
+
+
+def density(weights, x, Kv, RTv, fmaps, bbox, img_hw, dtype=torch.float64, device="cpu"):
+    """Occupancy at points x [B,N,3] (infer_density_mlp, infer_mlp.cu:583-634,
+    for this model): per-view MLP on [PE | gathered features], view mean,
+    occupancy head.  Returns [B,N]."""
+    T_ = lambda a: torch.as_tensor(a).to(device=device, dtype=dtype)
+    x, Kv, RTv, fmaps, bbox = map(T_, (x, Kv, RTv, fmaps, bbox))
+    W = [T_(w) for w in weights.dense_w]
+    Bs = [T_(b) for b in weights.dense_b]
+    out = []
+    for b in range(x.shape[0]):
+        c = 0.5 * (bbox[b, :, 0] + bbox[b, :, 1])
+        ih = 2.0 / (bbox[b, :, 1] - bbox[b, :, 0])
+        pe = freq_encode((x[b] - c) * ih)
+        feats = gather_features(x[b], Kv[b], RTv[b], fmaps[b], img_hw)
+        h_sum = 0
+        for f in feats:
+            h = torch.cat([pe, f], -1)
+            for li in range(4):
+                h = torch.relu(h @ W[li].T + Bs[li])
+            h_sum = h_sum + h
+        out.append(torch.sigmoid((h_sum / len(feats)) @ T_(weights.sigma_w) + weights.sigma_b))
+    return torch.stack(out)
 
 
 def render_rays(weights, t, rays, Kv, RTv, fmaps, bbox, zdir, img_hw, t_thres=1e-4,

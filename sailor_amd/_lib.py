@@ -40,6 +40,7 @@ _SIGS = {
     "sailor_render_rays": [P, P, I, I, I, P, P, I, I, I, P, I, I, I, I, P, P, I, I, P, P, P, F,
                            I, I, P, P, P, P, P, SZ, P],
     "sailor_render_workspace_bytes": [I, I, I],
+    "sailor_density_points": [P, I, I, P, P, I, I, I, P, I, I, I, I, P, I, I, P, P, P, I, P, P],
     "sailor_render_rays_stages": [I, P, P, I, I, I, P, P, I, I, I, P, I, I, I, I, P, P, I, I, P,
                                   P, P, F, I, I, P, P, P, P, P, SZ, P],
     "sailor_freq_encode_forward": [P, I, I, I, P, P],

@@ -88,6 +88,10 @@ struct RenderArgs {
   float t_thres;
   int early_exit;
   int stages;              // 1: compaction, 2: sample kernel, 4: Hydra/RGB kernel
+  // point mode (density query): points [B, n_points, 3]; groups of NS points
+  // take the place of rays and out_occ receives one density per point
+  const float* points;
+  long long n_points;
 };
 
 template <int PREC> struct Tr;
@@ -492,7 +496,8 @@ struct Smem {
   static constexpr size_t wts = tval + (size_t)NS * 4;                 // f32 [NS]
   static constexpr size_t flag = wts + (size_t)NS * 4;                 // int [4]
   static constexpr size_t bias = flag + 16;                            // f32 [5][H]
-  static constexpr size_t total = bias + (size_t)5 * H * 4;
+  static constexpr size_t pos = bias + (size_t)5 * H * 4;              // f32 [NS][4]
+  static constexpr size_t total = pos + (size_t)NS * 16;
 };
 
 constexpr size_t kLdsBytes = 160 * 1024;  // LDS per workgroup on gfx950
@@ -501,7 +506,7 @@ constexpr size_t kLdsBytes = 160 * 1024;  // LDS per workgroup on gfx950
 // let one workgroup's gather / compositing overlap the other's MFMA work.
 constexpr int render_occupancy(int PREC, int NS) { return (PREC != kF32 && NS <= 16) ? 2 : 1; }
 
-template <int F, int H, int PREC, int NS, bool HALF_MAP>
+template <int F, int H, int PREC, int NS, bool HALF_MAP, bool POINTS>
 __global__ void __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(render_occupancy(PREC, NS), render_occupancy(PREC, NS))))
 render_kernel(RenderArgs a) {
@@ -527,6 +532,7 @@ render_kernel(RenderArgs a) {
   float* s_t = reinterpret_cast<float*>(smem + L::tval);
   float* s_w = reinterpret_cast<float*>(smem + L::wts);
   int* s_flag = reinterpret_cast<int*>(smem + L::flag);
+  float* s_pos = reinterpret_cast<float*>(smem + L::pos);
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -542,15 +548,18 @@ render_kernel(RenderArgs a) {
 
   // every list entry costs about the same, so a static stride over the
   // compacted list balances the workgroups (dropped rays were finished by the
-  // compaction kernel)
-  const long long n_valid = *a.n_valid;
-  for (long long item = blockIdx.x; item < n_valid; item += gridDim.x) {
+  // compaction kernel).  Point mode: item = a group of NS points.
+  const long long n_items = POINTS ? a.n_rays : (long long)*a.n_valid;
+  for (long long item = blockIdx.x; item < n_items; item += gridDim.x) {
     __syncthreads();  // LDS of the previous ray is no longer read
-    const long long ray = a.list[item];
-    const float* trow = a.t + ray * S;
+    const long long ray = POINTS ? item : (long long)a.list[item];
     const int b = (int)(ray / a.rays_per_batch);
-    const float ox = a.rays[ray * 6 + 0], oy = a.rays[ray * 6 + 1], oz = a.rays[ray * 6 + 2];
-    const float dx = a.rays[ray * 6 + 3], dy = a.rays[ray * 6 + 4], dz = a.rays[ray * 6 + 5];
+    const long long p0 = (ray % a.rays_per_batch) * NS;  // point mode: first point
+    const float* trow = POINTS ? nullptr : a.t + ray * S;
+    const float* rr = POINTS ? a.points : a.rays + ray * 6;
+    const float ox = rr[0], oy = rr[1], oz = rr[2];
+    const float dx = POINTS ? 0.0f : rr[3], dy = POINTS ? 0.0f : rr[4],
+                dz = POINTS ? 0.0f : rr[5];
     const float* bb = a.bbox + b * 6;
     const float cx = 0.5f * (bb[0] + bb[1]), cy = 0.5f * (bb[2] + bb[3]),
                 cz = 0.5f * (bb[4] + bb[5]);
@@ -565,13 +574,27 @@ render_kernel(RenderArgs a) {
     for (int c0 = 0; c0 < S; c0 += NS) {
       __syncthreads();
       if (a.early_exit && s_flag[0]) break;
+      // sample positions of the chunk: o + t d, or the points themselves
+      if (tid < NS) {
+        float px, py, pz, tv = 0.0f;
+        if constexpr (POINTS) {
+          const long long pi = p0 + tid < a.n_points ? p0 + tid : a.n_points - 1;
+          const float* q = a.points + ((long long)b * a.n_points + pi) * 3;
+          px = q[0]; py = q[1]; pz = q[2];
+        } else {
+          tv = trow[c0 + tid];
+          px = ox + tv * dx; py = oy + tv * dy; pz = oz + tv * dz;
+        }
+        s_pos[4 * tid + 0] = px; s_pos[4 * tid + 1] = py; s_pos[4 * tid + 2] = pz;
+        s_t[tid] = tv;
+      }
+      __syncthreads();
       // ---- first-layer input: bilinear features (4 threads per column; the
       // 16 lanes of a quarter-wave take 16 columns of the same quarter) ------
       for (int item = tid; item < NC * 4; item += kThreads) {
         const int col = (item & 15) + 16 * (item >> 6), quarter = (item >> 4) & 3;
         const int sl = col >> 2, v = col & 3;
-        const float t = trow[c0 + sl];
-        const float x = ox + t * dx, y = oy + t * dy, z = oz + t * dz;
+        const float x = s_pos[4 * sl], y = s_pos[4 * sl + 1], z = s_pos[4 * sl + 2];
         float f[CPL];
 #pragma unroll
         for (int c = 0; c < CPL; ++c) f[c] = 0.0f;
@@ -651,7 +674,6 @@ render_kernel(RenderArgs a) {
       // a thread computes two adjacent slots and stores them as one pair
       for (int item = tid; item < NS * kPeSlots / 2; item += kThreads) {
         const int sl = item / (kPeSlots / 2), q0 = 2 * (item % (kPeSlots / 2));
-        const float t = trow[c0 + sl];
         float val[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -659,8 +681,9 @@ render_kernel(RenderArgs a) {
           val[h] = 0.0f;
           if (q < 63) {
             const int d = q % 3;
-            const float xd = d == 0 ? (ox + t * dx - cx) * ihx
-                                    : (d == 1 ? (oy + t * dy - cy) * ihy : (oz + t * dz - cz) * ihz);
+            const float xd = d == 0 ? (s_pos[4 * sl] - cx) * ihx
+                                    : (d == 1 ? (s_pos[4 * sl + 1] - cy) * ihy
+                                              : (s_pos[4 * sl + 2] - cz) * ihz);
             if (q < 3) {
               val[h] = xd;
             } else {
@@ -681,7 +704,6 @@ render_kernel(RenderArgs a) {
           }
         }
       }
-      if (tid < NS) s_t[tid] = trow[c0 + tid];
       __syncthreads();
       // ---- dense layers on MFMA --------------------------------------------
       // ---- dense layers + occupancy head (per-column partial dot products) --
@@ -732,9 +754,14 @@ render_kernel(RenderArgs a) {
         for (int v = 0; v < V; ++v)
           for (int w = 0; w < kWaves; ++w) sum += sig[w * NC + tid * kVP + v];
         const float al = sigmoidf(sum * inv_v + ws[so.sigma_b]);
-        s_alpha[tid] = al;
-        if (a.out_occ) a.out_occ[ray * S + c0 + tid] = al;
+        if constexpr (POINTS) {
+          if (p0 + tid < a.n_points) a.out_occ[(long long)b * a.n_points + p0 + tid] = al;
+        } else {
+          s_alpha[tid] = al;
+          if (a.out_occ) a.out_occ[ray * S + c0 + tid] = al;
+        }
       }
+      if constexpr (POINTS) continue;  // density query: no compositing (S == NS)
       __syncthreads();
       // ---- front-to-back compositing (volume_rendering.cu:713-734) ---------
       if (tid == 0) {
@@ -762,14 +789,16 @@ render_kernel(RenderArgs a) {
       }
     }
     __syncthreads();
-    // ---- outputs: per-view tokens for the Hydra/RGB kernel, depth, sums ----
-    if (tid < kVP * F) a.out_tok[item * kVP * F + tid] = (tid / F) < V ? tok : 0.0f;
-    if (tid == 0) {
-      const float* zd = a.zdir + b * 3;
-      a.out_depth[ray] = dsum * (zd[0] * dx + zd[1] * dy + zd[2] * dz);
-      if (a.out_aux) {
-        a.out_aux[ray * 2 + 0] = wsum;
-        a.out_aux[ray * 2 + 1] = asum;
+    if constexpr (!POINTS) {
+      // ---- outputs: per-view tokens for the Hydra/RGB kernel, depth, sums --
+      if (tid < kVP * F) a.out_tok[item * kVP * F + tid] = (tid / F) < V ? tok : 0.0f;
+      if (tid == 0) {
+        const float* zd = a.zdir + b * 3;
+        a.out_depth[ray] = dsum * (zd[0] * dx + zd[1] * dy + zd[2] * dz);
+        if (a.out_aux) {
+          a.out_aux[ray * 2 + 0] = wsum;
+          a.out_aux[ray * 2 + 1] = asum;
+        }
       }
     }
   }
@@ -994,7 +1023,7 @@ compact_rays(const float* __restrict__ t, int S, long long n_rays, unsigned* cou
   }
 }
 
-template <int F, int H, int PREC, int NS, bool HALF_MAP>
+template <int F, int H, int PREC, int NS, bool HALF_MAP, bool POINTS>
 int launch_kernels(const RenderArgs& a, hipStream_t st);
 
 template <int F, int H, int PREC, int NS>
@@ -1006,16 +1035,20 @@ int launch(const RenderArgs& a, hipStream_t st, bool dry) {
   } else {
   if (dry) return SAILOR_OK;  // configuration check only
   if constexpr (PREC != kF32) {
-    if (a.fmap_half) return launch_kernels<F, H, PREC, NS, true>(a, st);
+    if (a.fmap_half) {
+      return a.points ? launch_kernels<F, H, PREC, NS, true, true>(a, st)
+                      : launch_kernels<F, H, PREC, NS, true, false>(a, st);
+    }
   }
-  return launch_kernels<F, H, PREC, NS, false>(a, st);
+  return a.points ? launch_kernels<F, H, PREC, NS, false, true>(a, st)
+                  : launch_kernels<F, H, PREC, NS, false, false>(a, st);
   }
 }
 
-template <int F, int H, int PREC, int NS, bool HALF_MAP>
+template <int F, int H, int PREC, int NS, bool HALF_MAP, bool POINTS>
 int launch_kernels(const RenderArgs& a, hipStream_t st) {
   typedef Smem<F, H, PREC, NS> L;
-  auto kern = render_kernel<F, H, PREC, NS, HALF_MAP>;
+  auto kern = render_kernel<F, H, PREC, NS, HALF_MAP, POINTS>;
   static int grid = 0;
   if (grid == 0) {
     SAILOR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -1033,7 +1066,7 @@ int launch_kernels(const RenderArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(kThreads), L::total, st, a);
     if (int err = sailor::check_launch("render_rays")) return err;
   }
-  if (!(a.stages & 4)) return SAILOR_OK;
+  if (POINTS || !(a.stages & 4)) return SAILOR_OK;
   const long long groups = (a.n_rays + kRaysPerGroup - 1) / kRaysPerGroup;
   hipLaunchKernelGGL((hydra_rgb_kernel<F, H>), dim3((unsigned)groups), dim3(kThreads), 0, st,
                      a.out_tok, a.list, a.n_valid, a.wsmall, a.V, a.out_rgb);
@@ -1101,7 +1134,7 @@ int render_stages(int stages, const float* t, const float* rays, int B, int R, i
   SAILOR_REQUIRE(S % ns == 0, "render_rays: S=%d is not a multiple of the chunk (%d samples)",
                  S, ns);
   if (R == 0) return SAILOR_OK;
-  RenderArgs a;
+  RenderArgs a{};
   SAILOR_REQUIRE(fmap_precision == kF32 || fmap_precision == precision,
                  "render_rays: feature maps must be fp32 or the form's 16-bit type");
   a.t = t; a.rays = rays; a.Kv = Kv; a.RTv = RTv; a.fmap = fmap; a.bbox = bbox; a.zdir = zdir;
@@ -1151,6 +1184,50 @@ int render_stages(int stages, const float* t, const float* rays, int B, int R, i
                            "(supported: (32,128), (64,256))", F, H);
 }
 }  // namespace
+
+extern "C" int sailor_density_points(const float* points, int B, int N, const float* Kv,
+                                     const float* RTv, int V, int img_h, int img_w,
+                                     const void* fmap, int fmap_precision, int Hf, int Wf,
+                                     int F, const float* bbox, int H, int precision,
+                                     const void* w_dense, const float* b_dense,
+                                     const float* w_small, int chunk_samples,
+                                     float* out_density, void* stream) {
+  SAILOR_REQUIRE(B > 0 && N >= 0, "density_points: bad sizes");
+  SAILOR_REQUIRE(V >= 1 && V <= kVP, "density_points: 1 to %d views supported", kVP);
+  SAILOR_REQUIRE(Hf > 1 && Wf > 1, "density_points: feature map too small");
+  SAILOR_REQUIRE(fmap_precision == kF32 || fmap_precision == precision,
+                 "density_points: feature maps must be fp32 or the form's 16-bit type");
+  if (!((F == 64 && H == 256) || (F == 32 && H == 128))) {
+    return sailor::set_error(SAILOR_ERR_ARG, "density_points: (F=%d, H=%d) not instantiated "
+                             "(supported: (32,128), (64,256))", F, H);
+  }
+  if (precision < kF32 || precision > kF16)
+    return sailor::set_error(SAILOR_ERR_ARG, "density_points: precision %d unsupported",
+                             precision);
+  int ns = chunk_samples > 0 ? chunk_samples : (precision == kF32 ? 24 : 16);
+  if (ns != 8 && ns != 16 && ns != 24 && ns != 48)
+    return sailor::set_error(SAILOR_ERR_ARG, "density_points: points per group %d not "
+                             "instantiated (8, 16, 24, 48)", ns);
+  if (N == 0) return SAILOR_OK;
+  RenderArgs a{};
+  a.points = points;
+  a.n_points = N;
+  a.rays_per_batch = (N + ns - 1) / ns;
+  a.n_rays = (long long)B * a.rays_per_batch;
+  a.Kv = Kv; a.RTv = RTv; a.fmap = fmap; a.fmap_half = fmap_precision != kF32; a.bbox = bbox;
+  a.wdense = w_dense; a.bdense = b_dense; a.wsmall = w_small;
+  a.out_occ = out_density;
+  a.S = ns; a.V = V; a.img_w = img_w; a.img_h = img_h; a.Hf = Hf; a.Wf = Wf;
+  a.stages = 2;
+  hipStream_t st = sailor::as_stream(stream);
+  {
+    const int err = (F == 64) ? dispatch_ns<64, 256>(ns, precision, a, st, true)
+                              : dispatch_ns<32, 128>(ns, precision, a, st, true);
+    if (err) return err;
+  }
+  return (F == 64) ? dispatch_ns<64, 256>(ns, precision, a, st, false)
+                   : dispatch_ns<32, 128>(ns, precision, a, st, false);
+}
 
 extern "C" int sailor_render_rays(const float* t, const float* rays, int B, int R, int S,
                                   const float* Kv, const float* RTv, int V, int img_h,

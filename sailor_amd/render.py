@@ -78,3 +78,24 @@ def render_rays(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, model: PackedMode
          ptr(out["aux"]), ptr(out["occ"]), ptr(out["workspace"]),
          out["workspace"].numel(), stream_of(dev))
     return out
+
+
+def density_points(points, Kv, RTv, img_hw, fmap_cl, bbox, model: PackedModel, out=None,
+                   chunk_samples=0):
+    """Occupancy of the fused model at points [B,N,3] (infer_density path):
+    per-view MLP on gathered features + encoding, view mean, occupancy head.
+    Returns [B,N] float32."""
+    B, N = points.shape[0], points.shape[1]
+    V = Kv.shape[1]
+    Hf, Wf, F = fmap_cl.shape[2], fmap_cl.shape[3], fmap_cl.shape[4]
+    map_prec = {torch.float32: 0, torch.bfloat16: PREC_IDS["bf16"],
+                torch.float16: PREC_IDS["fp16"]}.get(fmap_cl.dtype)
+    if map_prec is None:
+        raise RuntimeError(f"feature maps of dtype {fmap_cl.dtype} are not supported")
+    if out is None:
+        out = torch.empty((B, N), dtype=_f32, device=points.device)
+    call("sailor_density_points", ptr(points), B, N, ptr(Kv), ptr(RTv), V, int(img_hw[0]),
+         int(img_hw[1]), ptr(fmap_cl), map_prec, Hf, Wf, F, ptr(bbox), model.H,
+         PREC_IDS[model.prec], ptr(model.w_dense), ptr(model.b_dense), ptr(model.w_small),
+         int(chunk_samples), ptr(out), stream_of(points.device))
+    return out

@@ -145,3 +145,34 @@ def test_early_exit_gives_same_image():
     ref = pipe("fp32")
     np.testing.assert_allclose(p.rgb.cpu().numpy(), ref.rgb.cpu().numpy(), atol=0)
     np.testing.assert_allclose(p.depth.cpu().numpy(), ref.depth.cpu().numpy(), atol=0)
+
+
+@pytest.mark.parametrize("prec,N,chunk", [("fp32", 1000, 0), ("fp32", 37, 8),
+                                          ("bf16", 1000, 0), ("fp16", 531, 24)])
+def test_density_query_matches_float64(prec, N, chunk):
+    """sailor_density_points (infer_density path) on two batches of points
+    in and around the scene box, including a partial last group of points,
+    vs the float64 restatement.  fp32: max |d alpha| <= 1e-5; 16-bit forms
+    (16-bit maps and MLP): <= 2e-2."""
+    from sailor_amd.model import NerfWeights
+    from sailor_amd.render import PackedModel, density_points
+    p = pipe()
+    rng = np.random.default_rng(7 + N)
+    B = 2
+    bbox = p.bbox.cpu().numpy()
+    lo, hi = bbox[0, :, 0], bbox[0, :, 1]
+    pts = rng.uniform(lo, hi, (B, N, 3)).astype(np.float32)
+    K = np.repeat(p.K.cpu().numpy(), B, 0)
+    RT = np.repeat(p.RT.cpu().numpy(), B, 0)
+    fm_cl = np.repeat(p.fmap.cpu().numpy(), B, 0)
+    bb = np.repeat(bbox, B, 0)
+    model = PackedModel(p.weights, prec, "cuda")
+    dt = {"fp32": torch.float32, "bf16": torch.bfloat16, "fp16": torch.float16}[prec]
+    cu = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    got = density_points(cu(pts), cu(K), cu(RT), (p.cfg.img, p.cfg.img), cu(fm_cl).to(dt),
+                         cu(bb), model, chunk_samples=chunk)
+    torch.cuda.synchronize()
+    ref = nerf_ref.density(p.weights, pts, K, RT, np.transpose(fm_cl, (0, 1, 4, 2, 3)), bb,
+                           (p.cfg.img, p.cfg.img)).numpy()
+    err = np.max(np.abs(got.cpu().numpy() - ref))
+    assert err <= (1e-5 if prec == "fp32" else 2e-2), err
