@@ -293,6 +293,27 @@ int sailor_render_rays_stages(int stages, const float* t, const float* rays, int
                               float* out_aux, float* out_occ, void* workspace,
                               size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Differentiation of the fused render                                       */
+/* ------------------------------------------------------------------------ */
+
+/* Floats of the per-workgroup partial-gradient buffer the backward kernels
+ * use (one small-weight gradient per workgroup of the device). */
+size_t sailor_render_backward_partials_floats(int F, int H);
+
+/* Backward of the Hydra encoder + RGB head of sailor_render_rays (the
+ * autograd of aggregation.py:101-128,164-171,299-307 and nerf.py:117-124):
+ * from dL/d rgb [B*R, 3] (ray order) and the forward's workspace (ray list
+ * and per-view tokens) to dL/d token [n_valid, 4, F] (list order, in
+ * grad_tokens of B*R*4*F floats) and the gradient of the Hydra and RGB-head
+ * entries of the small-weight buffer (grad_small, small_weights_floats(F, H)
+ * floats, overwritten; other entries zero).  partials: scratch of
+ * partials_floats floats (sailor_render_backward_partials_floats). */
+int sailor_render_hydra_backward(const float* grad_rgb, int B, int R, int V, int F, int H,
+                                 const float* w_small, const void* workspace,
+                                 size_t workspace_bytes, float* grad_tokens, float* grad_small,
+                                 float* partials, size_t partials_floats, void* stream);
+
 /* Density query at arbitrary points: replaces infer_density_mlp
  * (c_lib/FastNerf/src/infer_mlp.cu:583-634, the GUI's occupancy query after
  * fusion) for the fused model: per point, the per-view MLP on the gathered

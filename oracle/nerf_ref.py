@@ -54,6 +54,26 @@ def gather_features(x, Kv, RTv, fmaps, img_hw):
     return feats
 
 
+def hydra_rgb(hydra, rgbw, toks):
+    """Hydra encoder stacks over the view tokens toks [N, V, F] and the RGB
+    head on the first view's token (aggregation.py:101-128,164-171,299-307;
+    nerf.py:117-124).  hydra: list of dicts of tensors (q, k, v, fc, w1, b1,
+    w2, b2); rgbw: [(w1, b1), (w2, b2), (w3, b3)].  Returns rgb [N, 3]."""
+    x_tok = toks
+    for st in hydra:
+        q = Fn.normalize(x_tok @ st["q"].T, dim=-1)
+        k = Fn.normalize(x_tok @ st["k"].T, dim=-1)
+        vv = x_tok @ st["v"].T
+        kv = (k * vv).sum(dim=-2, keepdim=True)
+        out = (q * kv) @ st["fc"].T + x_tok
+        x_tok = torch.relu(out @ st["w1"].T + st["b1"]) @ st["w2"].T + st["b2"] + out
+    h = x_tok[:, 0]
+    (w1, b1), (w2, b2), (w3, b3) = rgbw
+    h = torch.relu(h @ w1.T + b1)
+    h = torch.relu(h @ w2.T + b2)
+    return torch.sigmoid(h @ w3.T + b3)
+
+
 def density(weights, x, Kv, RTv, fmaps, bbox, img_hw, dtype=torch.float64, device="cpu"):
     """Occupancy at points x [B,N,3] (infer_density_mlp, infer_mlp.cu:583-634,
     for this model): per-view MLP on [PE | gathered features], view mean,
@@ -133,20 +153,9 @@ def render_rays(weights, t, rays, Kv, RTv, fmaps, bbox, zdir, img_hw, t_thres=1e
                 toks[:, v] += w[:, None] * feats[v][:, s]
             Tt = torch.where(alive, Tt * (1 - a), Tt)
             alive = alive & ~(Tt < t_thres)
-        x_tok = toks
-        for st in weights.hydra:
-            q = Fn.normalize(x_tok @ T_(st["q"]).T, dim=-1)
-            k = Fn.normalize(x_tok @ T_(st["k"]).T, dim=-1)
-            vv = x_tok @ T_(st["v"]).T
-            kv = (k * vv).sum(dim=-2, keepdim=True)
-            out = (q * kv) @ T_(st["fc"]).T + x_tok
-            x_tok = torch.relu(out @ T_(st["w1"]).T + T_(st["b1"])) @ T_(st["w2"]).T + \
-                T_(st["b2"]) + out
-        h = x_tok[:, 0]
-        (w1, b1), (w2, b2), (w3, b3) = weights.rgb
-        h = torch.relu(h @ T_(w1).T + T_(b1))
-        h = torch.relu(h @ T_(w2).T + T_(b2))
-        rgb = torch.sigmoid(h @ T_(w3).T + T_(b3))
+        hyd = [{k: T_(v) for k, v in st.items()} for st in weights.hydra]
+        rgbw = [(T_(w_), T_(b_)) for w_, b_ in weights.rgb]
+        rgb = hydra_rgb(hyd, rgbw, toks)
         rgb_out[b, idx] = rgb
         dep_out[b, idx] = dsum * (d @ zdir[b])
         ws_out[b, idx] = wsum

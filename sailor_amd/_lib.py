@@ -40,6 +40,8 @@ _SIGS = {
     "sailor_render_rays": [P, P, I, I, I, P, P, I, I, I, P, I, I, I, I, P, P, I, I, P, P, P, F,
                            I, I, P, P, P, P, P, SZ, P],
     "sailor_render_workspace_bytes": [I, I, I],
+    "sailor_render_backward_partials_floats": [I, I],
+    "sailor_render_hydra_backward": [P, I, I, I, I, I, P, P, SZ, P, P, P, SZ, P],
     "sailor_density_points": [P, I, I, P, P, I, I, I, P, I, I, I, I, P, I, I, P, P, P, I, P, P],
     "sailor_render_rays_stages": [I, P, P, I, I, I, P, P, I, I, I, P, I, I, I, I, P, P, I, I, P,
                                   P, P, F, I, I, P, P, P, P, P, SZ, P],
@@ -51,6 +53,7 @@ _RESTYPES = {
     "sailor_octree_workspace_bytes": SZ,
     "sailor_render_small_weights_floats": SZ,
     "sailor_render_workspace_bytes": SZ,
+    "sailor_render_backward_partials_floats": SZ,
 }
 
 _lib = None

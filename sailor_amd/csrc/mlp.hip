@@ -826,7 +826,7 @@ struct HydraRows {
 // rows[o][lane] = act(bias[o] + sum_k W[o][k] in[k]) for this wave's NO
 // outputs starting at o0; W rows are wave-uniform so they are fetched once
 // per wave and shared by the 64 tokens.
-template <int K, int NO, bool RELU>
+template <int K, int NO, bool RELU, int LD = kLd>
 __device__ __forceinline__ void token_dense(const float* __restrict__ W, const float* bias,
                                             const float (&in)[K], int o0, float* rows,
                                             int lane) {
@@ -837,14 +837,14 @@ __device__ __forceinline__ void token_dense(const float* __restrict__ W, const f
     float acc = bias ? bias[o] : 0.0f;
 #pragma unroll
     for (int k = 0; k < K; ++k) acc += wr[k] * in[k];
-    rows[o * kLd + lane] = RELU ? fmaxf(acc, 0.0f) : acc;
+    rows[o * LD + lane] = RELU ? fmaxf(acc, 0.0f) : acc;
   }
 }
 
-template <int N>
+template <int N, int LD = kLd>
 __device__ __forceinline__ void load_rows(const float* rows, int lane, float (&dst)[N]) {
 #pragma unroll
-  for (int k = 0; k < N; ++k) dst[k] = rows[k * kLd + lane];
+  for (int k = 0; k < N; ++k) dst[k] = rows[k * LD + lane];
 }
 
 template <int F, int H>
@@ -982,6 +982,390 @@ hydra_rgb_kernel(const float* __restrict__ tok, const int* __restrict__ list,
       out_rgb[(long long)list[item] * 3 + j] = sigmoidf(acc);
     }
   }
+}
+
+
+// ---- Backward of the Hydra encoder + RGB head ------------------------------
+// For each group of 16 rays (list order) the forward is recomputed from the
+// stored tokens (stack inputs kept, stack internals recomputed during the
+// backward walk), then dL/d rgb is propagated back to dL/d token.  Weight
+// gradients are sums over the 64 tokens of outer products, computed from LDS
+// rows with one output entry per thread, and accumulated in this workgroup's
+// own partial buffer in global memory (no atomics); a second kernel adds the
+// partials of all workgroups.
+constexpr int kLdB = 68;  // 16-byte aligned rows of 64 tokens
+
+template <int F>
+struct HydraBwdRows {
+  static constexpr int X0 = 0, X1 = F, X2 = 2 * F;    // stack inputs, final tokens
+  static constexpr int QKV = 3 * F;      // q, k, v then dq, dk, dv [96]; RGB r1 [64] + r2 [32]
+  static constexpr int OUT = QKV + 3 * kHydraInner;   // attention output [F]
+  static constexpr int HID = OUT + F;    // FFN hidden, then attention values [32]
+  static constexpr int DX = HID + kHydraInner;        // dL/d stack output, then input [F]
+  static constexpr int DOUT = DX + F;                 // [F]
+  static constexpr int DHID = DOUT + F;               // dL/d hidden, then d attention [32]
+  static constexpr int ROWS = DHID + kHydraInner;
+  static constexpr size_t bytes = (size_t)ROWS * kLdB * sizeof(float);
+};
+
+// gp[o*K + c] += sum_t d[o][t] in[c][t];  gb[o] += sum_t d[o][t]
+template <int O, int K>
+__device__ __forceinline__ void accum_outer(const float* drows, const float* inrows,
+                                            float* __restrict__ gp, float* __restrict__ gb,
+                                            int tid) {
+  for (int e = tid; e < O * K; e += kThreads) {
+    const int o = e / K, c = e % K;
+    const f32x4* dr = reinterpret_cast<const f32x4*>(drows + o * kLdB);
+    const f32x4* ir = reinterpret_cast<const f32x4*>(inrows + c * kLdB);
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < kTok / 4; ++q) acc += dr[q] * ir[q];
+    gp[e] += (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  }
+  if (gb) {
+    for (int o = tid; o < O; o += kThreads) {
+      const f32x4* dr = reinterpret_cast<const f32x4*>(drows + o * kLdB);
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int q = 0; q < kTok / 4; ++q) acc += dr[q];
+      gb[o] += (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    }
+  }
+}
+
+// rows[k][lane] = (add ? rows[k][lane] : 0) + sum_o W[o][k] d[o] for this wave's
+// NK outputs k0.. (W is [O][K], wave-uniform)
+template <int O, int K, int NK>
+__device__ __forceinline__ void token_dense_t(const float* __restrict__ W, const float (&d)[O],
+                                              int k0, float* rows, const float* addrows,
+                                              const float* maskrows, int lane) {
+#pragma unroll
+  for (int i = 0; i < NK; ++i) {
+    const int k = k0 + i;
+    float acc = addrows ? addrows[k * kLdB + lane] : 0.0f;
+#pragma unroll
+    for (int o = 0; o < O; ++o) acc += W[(size_t)o * K + k] * d[o];
+    if (maskrows && !(maskrows[k * kLdB + lane] > 0.0f)) acc = 0.0f;
+    rows[k * kLdB + lane] = acc;
+  }
+}
+
+// Forward of one Hydra stack from rows xin into QKV / OUT / HID (and xout if
+// given); all waves take part, synchronised inside.
+template <int F, int H>
+__device__ __forceinline__ void hydra_stack_fwd(const float* ws, int st, float* sm, int xin,
+                                                int xout, int lane, int wave, int V) {
+  typedef HydraBwdRows<F> R;
+  constexpr SmallOffsets so = small_offsets(F, H);
+  constexpr int stride = so.q[1] - so.q[0];
+  const float vmask = (lane & 3) < V ? 1.0f : 0.0f;
+  {
+    float x[F];
+    load_rows<F, kLdB>(sm + xin * kLdB, lane, x);
+    constexpr int NO = 3 * kHydraInner / kWaves;
+    token_dense<F, NO, false, kLdB>(ws + so.q[0] + st * stride, nullptr, x, wave * NO,
+                                    sm + R::QKV * kLdB, lane);
+  }
+  __syncthreads();
+  float attn[kHydraInner];
+  {
+    float qq = 0.0f, kk = 0.0f;
+#pragma unroll
+    for (int e = 0; e < kHydraInner; ++e) {
+      const float q = sm[(R::QKV + e) * kLdB + lane];
+      const float k = sm[(R::QKV + kHydraInner + e) * kLdB + lane];
+      qq += q * q;
+      kk += k * k;
+    }
+    const float nq = 1.0f / fmaxf(sqrtf(qq), 1e-12f);
+    const float nk = vmask / fmaxf(sqrtf(kk), 1e-12f);
+#pragma unroll
+    for (int e = 0; e < kHydraInner; ++e) {
+      const float q = sm[(R::QKV + e) * kLdB + lane];
+      const float k = sm[(R::QKV + kHydraInner + e) * kLdB + lane];
+      const float v = sm[(R::QKV + 2 * kHydraInner + e) * kLdB + lane];
+      float kv = k * nk * v;
+      kv += __shfl_xor(kv, 1, 64);
+      kv += __shfl_xor(kv, 2, 64);
+      attn[e] = q * nq * kv;
+    }
+  }
+  {
+    constexpr int NO = F / kWaves;
+    const float* wf = ws + so.fc[0] + st * stride;
+#pragma unroll
+    for (int i = 0; i < NO; ++i) {
+      const int c = wave * NO + i;
+      float acc = 0.0f;
+#pragma unroll
+      for (int e = 0; e < kHydraInner; ++e) acc += wf[c * kHydraInner + e] * attn[e];
+      sm[(R::OUT + c) * kLdB + lane] = acc + sm[(xin + c) * kLdB + lane];
+    }
+  }
+  __syncthreads();
+  {
+    float o[F];
+    load_rows<F, kLdB>(sm + R::OUT * kLdB, lane, o);
+    constexpr int NO = kHydraInner / kWaves;
+    token_dense<F, NO, true, kLdB>(ws + so.w1[0] + st * stride, ws + so.b1[0] + st * stride, o,
+                                   wave * NO, sm + R::HID * kLdB, lane);
+  }
+  __syncthreads();
+  if (xout >= 0) {
+    float h[kHydraInner];
+    load_rows<kHydraInner, kLdB>(sm + R::HID * kLdB, lane, h);
+    constexpr int NO = F / kWaves;
+    const float* w2 = ws + so.w2[0] + st * stride;
+    const float* b2 = ws + so.b2[0] + st * stride;
+#pragma unroll
+    for (int i = 0; i < NO; ++i) {
+      const int c = wave * NO + i;
+      float acc = b2[c];
+#pragma unroll
+      for (int e = 0; e < kHydraInner; ++e) acc += w2[c * kHydraInner + e] * h[e];
+      sm[(xout + c) * kLdB + lane] = acc + sm[(R::OUT + c) * kLdB + lane];
+    }
+    __syncthreads();
+  }
+}
+
+template <int F, int H>
+__global__ void __launch_bounds__(kThreads)
+hydra_rgb_backward_kernel(const float* __restrict__ tok, const int* __restrict__ list,
+                          const unsigned* __restrict__ n_valid_ptr, const float* __restrict__ ws,
+                          int V, const float* __restrict__ grad_rgb, float* __restrict__ dtok,
+                          float* __restrict__ gpart) {
+  typedef HydraBwdRows<F> R;
+  constexpr SmallOffsets so = small_offsets(F, H);
+  constexpr int stride = so.q[1] - so.q[0];
+  extern __shared__ __attribute__((aligned(16))) char smem_b[];
+  float* sm = reinterpret_cast<float*>(smem_b);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  float* g = gpart + (size_t)blockIdx.x * so.total;  // this workgroup's partial gradients
+  const long long n_valid = *n_valid_ptr;
+  for (long long item0 = (long long)blockIdx.x * kRaysPerGroup; item0 < n_valid;
+       item0 += (long long)gridDim.x * kRaysPerGroup) {
+    __syncthreads();
+    const long long n_floats = (n_valid - item0) * kVP * F;
+    for (int i = tid; i < kTok * F; i += kThreads) {
+      const int tk = i / F, c = i % F;
+      sm[(R::X0 + c) * kLdB + tk] = i < n_floats ? tok[item0 * kVP * F + i] : 0.0f;
+    }
+    __syncthreads();
+    // ---- forward recompute: stacks (inputs kept), RGB head ----------------
+    hydra_stack_fwd<F, H>(ws, 0, sm, R::X0, R::X1, lane, wave, V);
+    hydra_stack_fwd<F, H>(ws, 1, sm, R::X1, R::X2, lane, wave, V);
+    {
+      float x[F];
+      load_rows<F, kLdB>(sm + R::X2 * kLdB, lane, x);
+      constexpr int NO = kRgbH1 / kWaves;
+      token_dense<F, NO, true, kLdB>(ws + so.r1w, ws + so.r1b, x, wave * NO,
+                                     sm + R::QKV * kLdB, lane);
+    }
+    __syncthreads();
+    {
+      float r1[kRgbH1];
+      load_rows<kRgbH1, kLdB>(sm + R::QKV * kLdB, lane, r1);
+      constexpr int NO = kRgbH2 / kWaves;
+      token_dense<kRgbH1, NO, true, kLdB>(ws + so.r2w, ws + so.r2b, r1, wave * NO,
+                                          sm + (R::QKV + kRgbH1) * kLdB, lane);
+    }
+    __syncthreads();
+    // dz3 = dL/drgb * rgb (1 - rgb) on the first view's token of each valid ray
+    if (wave == 0) {
+      const long long item = item0 + (lane >> 2);
+      const bool on = (lane & 3) == 0 && item < n_valid;
+      float r2[kRgbH2];
+      load_rows<kRgbH2, kLdB>(sm + (R::QKV + kRgbH1) * kLdB, lane, r2);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        float z = ws[so.r3b + j];
+#pragma unroll
+        for (int k = 0; k < kRgbH2; ++k) z += ws[so.r3w + j * kRgbH2 + k] * r2[k];
+        const float rgb = sigmoidf(z);
+        const float gr = on ? grad_rgb[(long long)list[item] * 3 + j] : 0.0f;
+        sm[(R::DHID + j) * kLdB + lane] = gr * rgb * (1.0f - rgb);
+      }
+    }
+    __syncthreads();
+    accum_outer<3, kRgbH2>(sm + R::DHID * kLdB, sm + (R::QKV + kRgbH1) * kLdB, g + so.r3w,
+                           g + so.r3b, tid);
+    {  // dr2 = (Wr3^T dz3) . [r2 > 0]  -> DOUT rows [0, 32)
+      float dz[3];
+      load_rows<3, kLdB>(sm + R::DHID * kLdB, lane, dz);
+      constexpr int NK = kRgbH2 / kWaves;
+      token_dense_t<3, kRgbH2, NK>(ws + so.r3w, dz, wave * NK, sm + R::DOUT * kLdB, nullptr,
+                                   sm + (R::QKV + kRgbH1) * kLdB, lane);
+    }
+    __syncthreads();
+    accum_outer<kRgbH2, kRgbH1>(sm + R::DOUT * kLdB, sm + R::QKV * kLdB, g + so.r2w,
+                                g + so.r2b, tid);
+    {  // dr1 = (Wr2^T dr2) . [r1 > 0]  -> DX rows [0, 64)
+      float dr2[kRgbH2];
+      load_rows<kRgbH2, kLdB>(sm + R::DOUT * kLdB, lane, dr2);
+      constexpr int NK = kRgbH1 / kWaves;
+      token_dense_t<kRgbH2, kRgbH1, NK>(ws + so.r2w, dr2, wave * NK, sm + R::DX * kLdB, nullptr,
+                                        sm + R::QKV * kLdB, lane);
+    }
+    __syncthreads();
+    accum_outer<kRgbH1, F>(sm + R::DX * kLdB, sm + R::X2 * kLdB, g + so.r1w, g + so.r1b, tid);
+    {  // dx2 = Wr1^T dr1 -> DX rows
+      float dr1[kRgbH1];
+      load_rows<kRgbH1, kLdB>(sm + R::DX * kLdB, lane, dr1);
+      __syncthreads();
+      constexpr int NK = F / kWaves;
+      token_dense_t<kRgbH1, F, NK>(ws + so.r1w, dr1, wave * NK, sm + R::DX * kLdB, nullptr,
+                                   nullptr, lane);
+    }
+    __syncthreads();
+    // ---- stacks, last to first ----------------------------------------------
+    for (int st = kHydraStacks - 1; st >= 0; --st) {
+      const int xin = st == 0 ? R::X0 : R::X1;
+      hydra_stack_fwd<F, H>(ws, st, sm, xin, -1, lane, wave, V);
+      const float* w1 = ws + so.w1[0] + st * stride;
+      const float* w2 = ws + so.w2[0] + st * stride;
+      const float* wf = ws + so.fc[0] + st * stride;
+      const float* wq = ws + so.q[0] + st * stride;
+      // x' = W2 hid + b2 + out
+      accum_outer<F, kHydraInner>(sm + R::DX * kLdB, sm + R::HID * kLdB, g + so.w2[0] + st * stride,
+                                  g + so.b2[0] + st * stride, tid);
+      {
+        float dxp[F];
+        load_rows<F, kLdB>(sm + R::DX * kLdB, lane, dxp);
+        constexpr int NK = kHydraInner / kWaves;
+        token_dense_t<F, kHydraInner, NK>(w2, dxp, wave * NK, sm + R::DHID * kLdB, nullptr,
+                                          sm + R::HID * kLdB, lane);
+      }
+      __syncthreads();
+      // hid = relu(W1 out + b1)
+      accum_outer<kHydraInner, F>(sm + R::DHID * kLdB, sm + R::OUT * kLdB,
+                                  g + so.w1[0] + st * stride, g + so.b1[0] + st * stride, tid);
+      {
+        float dh[kHydraInner];
+        load_rows<kHydraInner, kLdB>(sm + R::DHID * kLdB, lane, dh);
+        constexpr int NK = F / kWaves;
+        token_dense_t<kHydraInner, F, NK>(w1, dh, wave * NK, sm + R::DOUT * kLdB,
+                                          sm + R::DX * kLdB, nullptr, lane);
+      }
+      __syncthreads();
+      // out = fc(attn) + x: attention values into the HID rows (wave 0)
+      const float vmask = (lane & 3) < V ? 1.0f : 0.0f;
+      float nq, nk;
+      {
+        float qq = 0.0f, kk = 0.0f;
+#pragma unroll
+        for (int e = 0; e < kHydraInner; ++e) {
+          const float q = sm[(R::QKV + e) * kLdB + lane];
+          const float k = sm[(R::QKV + kHydraInner + e) * kLdB + lane];
+          qq += q * q;
+          kk += k * k;
+        }
+        nq = 1.0f / fmaxf(sqrtf(qq), 1e-12f);
+        nk = vmask / fmaxf(sqrtf(kk), 1e-12f);
+      }
+      if (wave == 0) {
+#pragma unroll
+        for (int e = 0; e < kHydraInner; ++e) {
+          const float q = sm[(R::QKV + e) * kLdB + lane];
+          const float k = sm[(R::QKV + kHydraInner + e) * kLdB + lane];
+          const float v = sm[(R::QKV + 2 * kHydraInner + e) * kLdB + lane];
+          float kv = k * nk * v;
+          kv += __shfl_xor(kv, 1, 64);
+          kv += __shfl_xor(kv, 2, 64);
+          sm[(R::HID + e) * kLdB + lane] = q * nq * kv;
+        }
+      }
+      __syncthreads();
+      accum_outer<F, kHydraInner>(sm + R::DOUT * kLdB, sm + R::HID * kLdB,
+                                  g + so.fc[0] + st * stride, nullptr, tid);
+      {  // d attn = Wfc^T dout -> DHID rows
+        float dout[F];
+        load_rows<F, kLdB>(sm + R::DOUT * kLdB, lane, dout);
+        constexpr int NK = kHydraInner / kWaves;
+        token_dense_t<F, kHydraInner, NK>(wf, dout, wave * NK, sm + R::DHID * kLdB, nullptr,
+                                          nullptr, lane);
+      }
+      __syncthreads();
+      // attention and normalisation backward, per token (wave 0): dq, dk, dv
+      // overwrite q, k, v in the QKV rows
+      if (wave == 0) {
+        // two passes over the 32 attention units: the first accumulates
+        // q_hat . dq_hat and k_hat . dk_hat, the second writes dq, dk, dv
+        // (recomputing kv and dkv, which are cross-token shuffles)
+        float qd = 0.0f, kd = 0.0f;
+        for (int pass = 0; pass < 2; ++pass) {
+          const float qn = 1.0f / nq, kn = vmask > 0.0f ? vmask / nk : 0.0f;
+          const bool q_big = qn > 1e-12f, k_big = kn > 1e-12f;
+#pragma unroll 4
+          for (int e = 0; e < kHydraInner; ++e) {
+            const float q = sm[(R::QKV + e) * kLdB + lane];
+            const float k = sm[(R::QKV + kHydraInner + e) * kLdB + lane];
+            const float v = sm[(R::QKV + 2 * kHydraInner + e) * kLdB + lane];
+            const float da = sm[(R::DHID + e) * kLdB + lane];
+            float kv = k * nk * v;
+            kv += __shfl_xor(kv, 1, 64);
+            kv += __shfl_xor(kv, 2, 64);
+            // dq_hat = da * kv; dkv = sum over the ray's tokens of da * q_hat
+            const float dqh = da * kv;
+            float dkv = da * q * nq;
+            dkv += __shfl_xor(dkv, 1, 64);
+            dkv += __shfl_xor(dkv, 2, 64);
+            const float dkh = dkv * v;
+            if (pass == 0) {
+              qd += q * nq * dqh;
+              kd += k * nk * dkh;
+            } else {
+              // F.normalize backward: dx = n (dxhat - xhat (xhat . dxhat)) when
+              // |x| > eps, n dxhat otherwise (n = 1/max(|x|, eps)); a masked
+              // view's k_hat is 0 and gets no gradient
+              const float dq = q_big ? nq * (dqh - q * nq * qd) : nq * dqh;
+              const float dk =
+                  vmask > 0.0f ? (k_big ? nk * (dkh - k * nk * kd) : nk * dkh) : 0.0f;
+              const float dv = dkv * k * nk;
+              __builtin_amdgcn_wave_barrier();
+              sm[(R::QKV + e) * kLdB + lane] = dq;
+              sm[(R::QKV + kHydraInner + e) * kLdB + lane] = dk;
+              sm[(R::QKV + 2 * kHydraInner + e) * kLdB + lane] = dv;
+            }
+          }
+        }
+      }
+      __syncthreads();
+      accum_outer<3 * kHydraInner, F>(sm + R::QKV * kLdB, sm + xin * kLdB,
+                                      g + so.q[0] + st * stride, nullptr, tid);
+      {  // dx = dout + Wq^T dq + Wk^T dk + Wv^T dv -> DX rows
+        float dqkv[3 * kHydraInner];
+        load_rows<3 * kHydraInner, kLdB>(sm + R::QKV * kLdB, lane, dqkv);
+        constexpr int NK = F / kWaves;
+        token_dense_t<3 * kHydraInner, F, NK>(wq, dqkv, wave * NK, sm + R::DX * kLdB,
+                                              sm + R::DOUT * kLdB, nullptr, lane);
+      }
+      __syncthreads();
+    }
+    // ---- dL/d token (list order) ------------------------------------------
+    for (int i = tid; i < kTok * F; i += kThreads) {
+      const int tk = i / F, c = i % F;
+      if (i < n_floats) dtok[item0 * kVP * F + i] = sm[(R::DX + c) * kLdB + tk];
+    }
+  }
+}
+
+// out[e] = sum over parts of part[p][e]
+__global__ void sum_partials(const float* __restrict__ part, int parts, int n,
+                             float* __restrict__ out, int accumulate) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  float s = accumulate ? out[e] : 0.0f;
+  for (int p = 0; p < parts; ++p) s += part[(size_t)p * n + e];
+  out[e] = s;
+}
+
+// zero a float buffer (graph-capturable, no memset)
+__global__ void fill_zero(float* __restrict__ p, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0.0f;
 }
 
 // Workspace: [counter (256 B)] [list: n_rays int32, 256-B aligned]
@@ -1183,7 +1567,73 @@ int render_stages(int stages, const float* t, const float* rays, int B, int R, i
   return sailor::set_error(SAILOR_ERR_ARG, "render_rays: (F=%d, H=%d) not instantiated "
                            "(supported: (32,128), (64,256))", F, H);
 }
+
+template <int F, int H>
+int hydra_backward_launch(const float* grad_rgb, const float* tok, const int* list,
+                          const unsigned* counter, const float* w_small, int V,
+                          float* grad_tokens, float* partials, int parts, float* grad_small,
+                          hipStream_t st) {
+  constexpr SmallOffsets so = small_offsets(F, H);
+  typedef HydraBwdRows<F> R;
+  auto kern = hydra_rgb_backward_kernel<F, H>;
+  static bool init = false;
+  if (!init) {
+    SAILOR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)R::bytes));
+    init = true;
+  }
+  const long long nz = (long long)parts * so.total;
+  hipLaunchKernelGGL(fill_zero, dim3((unsigned)std::min<long long>((nz + 255) / 256, 4096)),
+                     dim3(256), 0, st, partials, nz);
+  if (int err = sailor::check_launch("render_backward (zero)")) return err;
+  hipLaunchKernelGGL(kern, dim3(parts), dim3(kThreads), R::bytes, st, tok, list, counter,
+                     w_small, V, grad_rgb, grad_tokens, partials);
+  if (int err = sailor::check_launch("render_backward (hydra)")) return err;
+  hipLaunchKernelGGL(sum_partials, dim3((so.total + 255) / 256), dim3(256), 0, st, partials,
+                     parts, so.total, grad_small, 0);
+  return sailor::check_launch("render_backward (sum)");
+}
+
 }  // namespace
+
+extern "C" size_t sailor_render_backward_partials_floats(int F, int H) {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus = 256;
+  return (size_t)cus * small_offsets(F, H).total;
+}
+
+extern "C" int sailor_render_hydra_backward(const float* grad_rgb, int B, int R, int V, int F,
+                                            int H, const float* w_small, const void* workspace,
+                                            size_t workspace_bytes, float* grad_tokens,
+                                            float* grad_small, float* partials,
+                                            size_t partials_floats, void* stream) {
+  SAILOR_REQUIRE(B > 0 && R >= 0, "render_hydra_backward: bad sizes");
+  SAILOR_REQUIRE(V >= 1 && V <= kVP, "render_hydra_backward: 1 to %d views supported", kVP);
+  SAILOR_REQUIRE(workspace != nullptr &&
+                     workspace_bytes >= sailor_render_workspace_bytes(B, R, F),
+                 "render_hydra_backward: workspace too small");
+  if (!((F == 64 && H == 256) || (F == 32 && H == 128)))
+    return sailor::set_error(SAILOR_ERR_ARG, "render_hydra_backward: (F=%d, H=%d) not "
+                             "instantiated", F, H);
+  const size_t per = (size_t)small_offsets(F, H).total;
+  const int parts = (int)std::min<size_t>(partials_floats / per, 1024);
+  SAILOR_REQUIRE(parts >= 1, "render_hydra_backward: partial buffer holds %zu floats, at least "
+                 "%zu needed", partials_floats, per);
+  const long long n = (long long)B * R;
+  if (n == 0) return SAILOR_OK;
+  const char* wsb = static_cast<const char*>(workspace);
+  const unsigned* counter = reinterpret_cast<const unsigned*>(wsb);
+  const int* list = reinterpret_cast<const int*>(wsb + workspace_list_offset());
+  const float* tok = reinterpret_cast<const float*>(wsb + workspace_token_offset(n));
+  hipStream_t st = sailor::as_stream(stream);
+  return F == 64 ? hydra_backward_launch<64, 256>(grad_rgb, tok, list, counter, w_small, V,
+                                                  grad_tokens, partials, parts, grad_small, st)
+                 : hydra_backward_launch<32, 128>(grad_rgb, tok, list, counter, w_small, V,
+                                                  grad_tokens, partials, parts, grad_small, st);
+}
 
 extern "C" int sailor_density_points(const float* points, int B, int N, const float* Kv,
                                      const float* RTv, int V, int img_h, int img_w,
