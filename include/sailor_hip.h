@@ -314,6 +314,30 @@ int sailor_render_hydra_backward(const float* grad_rgb, int B, int R, int V, int
                                  size_t workspace_bytes, float* grad_tokens, float* grad_small,
                                  float* partials, size_t partials_floats, void* stream);
 
+/* Backward of the sample kernel of sailor_render_rays (compositing reverse
+ * scan + per-view MLP, per-ray recompute): for the list entries
+ * [item_begin, item_end) of the forward's workspace (entries past the
+ * listed-ray count are skipped, their columns left untouched) it writes, per column c = (item - item_begin) * S
+ * * 4 + sample * 4 + view, the inputs A_l and deltas D_l of the four dense
+ * layers (col_bufs = {A0, A1, A2, A3, D0, D1, D2, D3}, each [C, H] in the
+ * form's type, C >= (item_end - item_begin) * S * 4, F + 64 <= H; A0 holds the first layer's inputs in kernel order
+ * [features | encoding | 0] in its first F + 64 columns), so that
+ * dW_l = D_l^T A_l and db_l = sum_c D_l; and adds the occupancy-head
+ * gradient to grad_small.  Inputs as sailor_render_rays plus w_dense_t
+ * (the packed W_3^T, W_2^T, W_1^T), occ (the forward's per-sample
+ * occupancy, [B*R, S]) and grad_tokens (sailor_render_hydra_backward).
+ * fmap is fp32; S <= 64, multiple of 4 (fp32) or 8 (16-bit). */
+int sailor_render_sample_backward(const float* t, const float* rays, int B, int R, int S,
+                                  const float* Kv, const float* RTv, int V, int img_h,
+                                  int img_w, const float* fmap, int Hf, int Wf, int F,
+                                  const float* bbox, int H, int precision, const void* w_dense,
+                                  const void* w_dense_t, const float* b_dense,
+                                  const float* w_small, float t_thresh, const float* occ,
+                                  const float* grad_tokens, const void* workspace,
+                                  size_t workspace_bytes, long long item_begin,
+                                  long long item_end, void* const* col_bufs, float* grad_small,
+                                  float* partials, size_t partials_floats, void* stream);
+
 /* Density query at arbitrary points: replaces infer_density_mlp
  * (c_lib/FastNerf/src/infer_mlp.cu:583-634, the GUI's occupancy query after
  * fusion) for the fused model: per point, the per-view MLP on the gathered

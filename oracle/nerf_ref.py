@@ -98,8 +98,17 @@ def density(weights, x, Kv, RTv, fmaps, bbox, img_hw, dtype=torch.float64, devic
     return torch.stack(out)
 
 
+def params_of(weights, dtype=torch.float64, device="cpu", requires_grad=False):
+    """The model's parameters as tensors (leaves when requires_grad)."""
+    T_ = lambda a: torch.tensor(a, dtype=dtype, device=device, requires_grad=requires_grad)
+    return dict(dense_w=[T_(w) for w in weights.dense_w], dense_b=[T_(b) for b in weights.dense_b],
+                sigma_w=T_(weights.sigma_w), sigma_b=T_(float(weights.sigma_b)),
+                hydra=[{k: T_(v) for k, v in st.items()} for st in weights.hydra],
+                rgb=[(T_(w), T_(b)) for w, b in weights.rgb])
+
+
 def render_rays(weights, t, rays, Kv, RTv, fmaps, bbox, zdir, img_hw, t_thres=1e-4,
-                dtype=torch.float64, device="cpu"):
+                dtype=torch.float64, device="cpu", params=None):
     """weights: sailor_amd.model.NerfWeights; t [B,R,S] sorted; rays [B,R,6];
     Kv [B,V,3,3]; RTv [B,V,3,4]; fmaps [B,V,F,Hf,Wf] (channel-first);
     bbox [B,3,2]; zdir [B,3].  Returns rgb [B,R,3], depth [B,R], ws [B,R],
@@ -108,8 +117,8 @@ def render_rays(weights, t, rays, Kv, RTv, fmaps, bbox, zdir, img_hw, t_thres=1e
     t, rays, Kv, RTv, fmaps, bbox, zdir = map(T_, (t, rays, Kv, RTv, fmaps, bbox, zdir))
     B, R, S = t.shape
     V = Kv.shape[1]
-    W = [T_(w) for w in weights.dense_w]
-    Bs = [T_(b) for b in weights.dense_b]
+    P = params if params is not None else params_of(weights, dtype, device)
+    W, Bs = P["dense_w"], P["dense_b"]
     rgb_out = torch.zeros((B, R, 3), dtype=dtype, device=device)
     dep_out = torch.zeros((B, R), dtype=dtype, device=device)
     ws_out = torch.zeros((B, R), dtype=dtype, device=device)
@@ -134,7 +143,7 @@ def render_rays(weights, t, rays, Kv, RTv, fmaps, bbox, zdir, img_hw, t_thres=1e
                 h = torch.relu(h @ W[li].T + Bs[li])
             h_sum = h_sum + h
         h_mean = h_sum / V
-        alpha = torch.sigmoid(h_mean @ T_(weights.sigma_w) + weights.sigma_b)  # [N, S]
+        alpha = torch.sigmoid(h_mean @ P["sigma_w"] + P["sigma_b"])  # [N, S]
         # sequential front-to-back compositing with early termination
         N = tb.shape[0]
         Tt = torch.ones(N, dtype=dtype, device=device)
@@ -153,9 +162,7 @@ def render_rays(weights, t, rays, Kv, RTv, fmaps, bbox, zdir, img_hw, t_thres=1e
                 toks[:, v] += w[:, None] * feats[v][:, s]
             Tt = torch.where(alive, Tt * (1 - a), Tt)
             alive = alive & ~(Tt < t_thres)
-        hyd = [{k: T_(v) for k, v in st.items()} for st in weights.hydra]
-        rgbw = [(T_(w_), T_(b_)) for w_, b_ in weights.rgb]
-        rgb = hydra_rgb(hyd, rgbw, toks)
+        rgb = hydra_rgb(P["hydra"], P["rgb"], toks)
         rgb_out[b, idx] = rgb
         dep_out[b, idx] = dsum * (d @ zdir[b])
         ws_out[b, idx] = wsum

@@ -17,6 +17,7 @@ P = ctypes.c_void_p
 I = ctypes.c_int
 F = ctypes.c_float
 SZ = ctypes.c_size_t
+LL = ctypes.c_longlong
 
 # name -> argtypes (restype int unless listed in _RESTYPES)
 _SIGS = {
@@ -42,6 +43,8 @@ _SIGS = {
     "sailor_render_workspace_bytes": [I, I, I],
     "sailor_render_backward_partials_floats": [I, I],
     "sailor_render_hydra_backward": [P, I, I, I, I, I, P, P, SZ, P, P, P, SZ, P],
+    "sailor_render_sample_backward": [P, P, I, I, I, P, P, I, I, I, P, I, I, I, P, I, I, P, P, P,
+                                      P, F, P, P, P, SZ, LL, LL, P, P, P, SZ, P],
     "sailor_density_points": [P, I, I, P, P, I, I, I, P, I, I, I, I, P, I, I, P, P, P, I, P, P],
     "sailor_render_rays_stages": [I, P, P, I, I, I, P, P, I, I, I, P, I, I, I, I, P, P, I, I, P,
                                   P, P, F, I, I, P, P, P, P, P, SZ, P],

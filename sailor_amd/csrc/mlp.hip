@@ -502,6 +502,69 @@ struct Smem {
 
 constexpr size_t kLdsBytes = 160 * 1024;  // LDS per workgroup on gfx950
 
+// Bilinear feature gather of one quarter of the F channels (CPL = F/4) of
+// view v at world point (x, y, z): proj_persp (utils_render.py:380-410; depth
+// < 0.01 reads nothing) + grid_sample(align_corners=True, zero padding)
+// (SRONet/BasicRenNet.py:245-259) on channels-last maps, one 16-byte load
+// per 4 fp32 or 8 16-bit channels of a tap.
+template <int F, int PREC, bool HALF_MAP>
+__device__ __forceinline__ void gather_quarter(const RenderArgs& a, int b, int v, float x,
+                                               float y, float z, int quarter,
+                                               float (&f)[F / 4]) {
+  typedef typename Tr<PREC>::E E;
+  constexpr int CPL = F / 4;
+  const int V = a.V;
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) f[c] = 0.0f;
+  if (v >= V) return;
+  const float* RT = a.RTv + ((size_t)b * V + v) * 12;
+  const float* K = a.Kv + ((size_t)b * V + v) * 9;
+  float px = RT[0] * x + RT[1] * y + RT[2] * z + RT[3];
+  float py = RT[4] * x + RT[5] * y + RT[6] * z + RT[7];
+  const float pz = RT[8] * x + RT[9] * y + RT[10] * z + RT[11];
+  if (!(pz >= 0.01f)) return;
+  px /= pz;
+  py /= pz;
+  const float u = K[0] * px + K[1] * py + K[2];
+  const float w = K[3] * px + K[4] * py + K[5];
+  const float hw = (float)(a.img_w / 2), hh2 = (float)(a.img_h / 2);
+  const float gx = (u - hw) / hw, gy = (w - hh2) / hh2;
+  const float ix = (gx + 1.0f) * 0.5f * (float)(a.Wf - 1);
+  const float iy = (gy + 1.0f) * 0.5f * (float)(a.Hf - 1);
+  if (!(ix > -1.0f && ix < (float)a.Wf && iy > -1.0f && iy < (float)a.Hf)) return;
+  const float fx0 = floorf(ix), fy0 = floorf(iy);
+  const int x0 = (int)fx0, y0 = (int)fy0;
+  const float ax = ix - fx0, ay = iy - fy0;
+  const size_t base = ((size_t)b * V + v) * a.Hf * a.Wf * F + quarter * CPL;
+#pragma unroll
+  for (int tap = 0; tap < 4; ++tap) {
+    const int xx = x0 + (tap & 1), yy = y0 + (tap >> 1);
+    if (xx < 0 || xx >= a.Wf || yy < 0 || yy >= a.Hf) continue;
+    const float wt = ((tap & 1) ? ax : 1.0f - ax) * ((tap >> 1) ? ay : 1.0f - ay);
+    const size_t off = base + ((size_t)yy * a.Wf + xx) * F;
+    if constexpr (HALF_MAP) {
+      typedef typename Tr<PREC>::V8 V8;
+      const V8* src = reinterpret_cast<const V8*>(static_cast<const E*>(a.fmap) + off);
+#pragma unroll
+      for (int qq = 0; qq < CPL / 8; ++qq) {
+        const V8 v8 = src[qq];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) f[8 * qq + e] += wt * (float)v8[e];
+      }
+    } else {
+      const f32x4* src = reinterpret_cast<const f32x4*>(static_cast<const float*>(a.fmap) + off);
+#pragma unroll
+      for (int qq = 0; qq < CPL / 4; ++qq) {
+        const f32x4 v4 = src[qq];
+        f[4 * qq + 0] += wt * v4[0];
+        f[4 * qq + 1] += wt * v4[1];
+        f[4 * qq + 2] += wt * v4[2];
+        f[4 * qq + 3] += wt * v4[3];
+      }
+    }
+  }
+}
+
 // Waves per SIMD the register allocation is fitted to: two workgroups per CU
 // let one workgroup's gather / compositing overlap the other's MFMA work.
 constexpr int render_occupancy(int PREC, int NS) { return (PREC != kF32 && NS <= 16) ? 2 : 1; }
@@ -596,62 +659,7 @@ render_kernel(RenderArgs a) {
         const int sl = col >> 2, v = col & 3;
         const float x = s_pos[4 * sl], y = s_pos[4 * sl + 1], z = s_pos[4 * sl + 2];
         float f[CPL];
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) f[c] = 0.0f;
-        if (v < V) {
-          const float* RT = a.RTv + ((size_t)b * V + v) * 12;
-          const float* K = a.Kv + ((size_t)b * V + v) * 9;
-          float px = RT[0] * x + RT[1] * y + RT[2] * z + RT[3];
-          float py = RT[4] * x + RT[5] * y + RT[6] * z + RT[7];
-          float pz = RT[8] * x + RT[9] * y + RT[10] * z + RT[11];
-          // proj_persp (utils_render.py:380-410): depth < 0.01 -> outside
-          if (pz >= 0.01f) {
-            px /= pz;
-            py /= pz;
-            const float u = K[0] * px + K[1] * py + K[2];
-            const float w = K[3] * px + K[4] * py + K[5];
-            const float hw = (float)(a.img_w / 2), hh2 = (float)(a.img_h / 2);
-            const float gx = (u - hw) / hw, gy = (w - hh2) / hh2;
-            // grid_sample, align_corners=True, zero padding
-            const float ix = (gx + 1.0f) * 0.5f * (float)(a.Wf - 1);
-            const float iy = (gy + 1.0f) * 0.5f * (float)(a.Hf - 1);
-            if (ix > -1.0f && ix < (float)a.Wf && iy > -1.0f && iy < (float)a.Hf) {
-              const float fx0 = floorf(ix), fy0 = floorf(iy);
-              const int x0 = (int)fx0, y0 = (int)fy0;
-              const float ax = ix - fx0, ay = iy - fy0;
-              const size_t base = ((size_t)b * V + v) * a.Hf * a.Wf * F + quarter * CPL;
-#pragma unroll
-              for (int tap = 0; tap < 4; ++tap) {
-                const int xx = x0 + (tap & 1), yy = y0 + (tap >> 1);
-                if (xx < 0 || xx >= a.Wf || yy < 0 || yy >= a.Hf) continue;
-                const float wt = ((tap & 1) ? ax : 1.0f - ax) * ((tap >> 1) ? ay : 1.0f - ay);
-                const size_t off = base + ((size_t)yy * a.Wf + xx) * F;
-                if constexpr (HALF_MAP) {
-                  // the form's 16-bit maps: one 16-byte load per 8 channels
-                  typedef typename Tr<PREC>::V8 V8;
-                  const V8* src = reinterpret_cast<const V8*>(static_cast<const E*>(a.fmap) + off);
-#pragma unroll
-                  for (int qq = 0; qq < CPL / 8; ++qq) {
-                    const V8 v8 = src[qq];
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) f[8 * qq + e] += wt * (float)v8[e];
-                  }
-                } else {
-                  const f32x4* src =
-                      reinterpret_cast<const f32x4*>(static_cast<const float*>(a.fmap) + off);
-#pragma unroll
-                  for (int qq = 0; qq < CPL / 4; ++qq) {
-                    const f32x4 v4 = src[qq];
-                    f[4 * qq + 0] += wt * v4[0];
-                    f[4 * qq + 1] += wt * v4[1];
-                    f[4 * qq + 2] += wt * v4[2];
-                    f[4 * qq + 3] += wt * v4[3];
-                  }
-                }
-              }
-            }
-          }
-        }
+        gather_quarter<F, PREC, HALF_MAP>(a, b, v, x, y, z, quarter, f);
         float* kp = keep + col * KF + quarter * CPL;
 #pragma unroll
         for (int c = 0; c < CPL; c += 4)
@@ -984,6 +992,479 @@ hydra_rgb_kernel(const float* __restrict__ tok, const int* __restrict__ list,
   }
 }
 
+
+
+// ---- Backward of the sample kernel: compositing reverse scan + MLP ---------
+// Per ray (list order), with per-ray recompute instead of stored activations:
+//  1. the features of all S samples x 4 views are gathered again (LDS);
+//  2. g_i = dL/dw_i = sum_v dL/dtoken_v . f_v(x_i) (the tokens are
+//     sum_i w_i f_v(x_i));
+//  3. the transmittance recurrence of the forward (same order, same early
+//     termination) is replayed and walked back: with
+//     U_i = sum_{k>i} g_k alpha_k prod_{i<j<k}(1 - alpha_j),
+//     dL/dalpha_i = T_i (g_i - U_i) and U_{i-1} = g_i alpha_i + (1 - alpha_i) U_i
+//     (no division by 1 - alpha);  dz_i = dL/dalpha_i alpha_i (1 - alpha_i);
+//  4. per chunk of NS samples the MLP forward is recomputed keeping every
+//     layer's output in LDS, delta_3 = dz / V sigma_w [pre_3 > 0] and
+//     delta_{l-1} = W_l^T delta_l [h_{l-1} > 0] run on MFMA with the packed
+//     transposed weights;
+//  5. the layer inputs A_l and deltas D_l of every column go to HBM for the
+//     weight-gradient GEMMs dW_l = D_l^T A_l (sums over all columns), and
+//     the occupancy-head gradients go to this workgroup's partial buffer.
+// Column c of a batch is (item - item_begin) * S * 4 + sample * 4 + view.
+struct BackArgs {
+  RenderArgs r;            // forward arguments (t, rays, cameras, maps, weights, ...)
+  const void* wdense_t;    // packed W_3^T, W_2^T, W_1^T (layers of input width H)
+  const float* occ;        // [n_rays, S] forward occupancies
+  const float* dtok;       // [n_valid, 4, F] dL/d token (list order)
+  long long item_begin, item_end;
+  void* A0;                // [C, H] layer-0 inputs (kernel order, K0 used)
+  void* A1;                // [C, H] h0
+  void* A2;                // [C, H] h1
+  void* A3;                // [C, H] h2
+  void* D0;                // [C, H] delta of layer 0 .. 3
+  void* D1;
+  void* D2;
+  void* D3;
+  float* gpart;            // [gridDim.x, small_total] partial small-weight gradients
+};
+
+
+// dst[col][unit] = acc (times [mask[col][unit] > 0] when a mask buffer is
+// given) for this wave's units, fp32 accumulator layout (16-byte stores).
+template <int H, int PREC, int NTL>
+__device__ __forceinline__ void write_vals(typename Tr<PREC>::E* dst, const f32x4 (&acc)[H / 64][NTL],
+                                           const typename Tr<PREC>::E* mask, int wave, int lane) {
+  typedef typename Tr<PREC>::E E;
+  constexpr int MTW = H / 64, CE = Tr<PREC>::CE, NV = 4 * MTW;
+  static_assert(CE == 4, "fp32 layout");
+  const int n = lane & 15, g = lane >> 4;
+  const int u0 = unit_of<H>(wave, g, 0, 0);
+#pragma unroll
+  for (int nt = 0; nt < NTL; ++nt) {
+    const int col = 16 * nt + n;
+#pragma unroll
+    for (int c = 0; c < NV / CE; ++c) {
+      const int off = act_chunk<H, PREC>(col, u0 / CE + c);
+      f32x4 v;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int q = 4 * c + e;
+        v[e] = acc[q / 4][nt][q % 4];
+      }
+      if (mask) {
+        const f32x4 m = *reinterpret_cast<const f32x4*>(mask + off);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = m[e] > 0.0f ? v[e] : 0.0f;
+      }
+      *reinterpret_cast<f32x4*>(dst + off) = v;
+    }
+  }
+}
+
+// The same for the 32x32 accumulator layout of the 16-bit forms.
+template <int H, int PREC, int NT>
+__device__ __forceinline__ void write_vals16(typename Tr<PREC>::E* dst,
+                                             const f32x16 (&acc)[H / 128][NT],
+                                             const typename Tr<PREC>::E* mask, int wave,
+                                             int lane) {
+  typedef typename Tr<PREC>::E E;
+  typedef typename Tr<PREC>::V8 V8;
+  const int c = lane & 31, h = lane >> 5;
+#pragma unroll
+  for (int mt = 0; mt < H / 128; ++mt) {
+    const int u0 = unit_of32<H>(wave, mt, h, 0);
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int off = act_chunk<H, PREC>(32 * nt + c, u0 / 8 + e);
+        V8 v;
+        V8 m;
+        if (mask) m = *reinterpret_cast<const V8*>(mask + off);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const float x = acc[mt][nt][8 * e + k];
+          v[k] = (E)((!mask || (float)m[k] > 0.0f) ? x : 0.0f);
+        }
+        *reinterpret_cast<V8*>(dst + off) = v;
+      }
+    }
+  }
+}
+
+constexpr int kBackMaxS = 64;
+
+template <int F, int H, int PREC, int NS>
+struct SmemB {
+  typedef typename Tr<PREC>::E E;
+  static constexpr int NC = NS * kVP;
+  static constexpr size_t buf = (size_t)NC * H * sizeof(E);   // one activation buffer
+  static constexpr size_t x0 = 0;
+  static constexpr size_t h0 = x0 + buf, h1 = h0 + buf, h2 = h1 + buf, dl = h2 + buf;
+  static constexpr int KF = F + 4;
+  static constexpr size_t keep = dl + buf;                         // f32 [SMAX][4][KF]
+  static constexpr size_t alpha = keep + (size_t)kBackMaxS * kVP * KF * 4;
+  static constexpr size_t tr = alpha + kBackMaxS * 4;             // T before each sample
+  static constexpr size_t gw = tr + kBackMaxS * 4;                // dL/dw
+  static constexpr size_t dz = gw + kBackMaxS * 4;                // dL/d pre-sigmoid
+  static constexpr size_t pos = dz + kBackMaxS * 4;               // f32 [SMAX][4]
+  static constexpr size_t tval = pos + kBackMaxS * 16;
+  static constexpr size_t bias = tval + kBackMaxS * 4;            // f32 [5][H]
+  static constexpr size_t total = bias + (size_t)5 * H * 4;
+};
+
+// D = W_layer x in[col] for one layer of input width K on MFMA, fp32 form
+// (NTL column tiles of 16).  Wl: the layer's packed fragments for all waves.
+template <int H, int PREC, int NTL, int K>
+__device__ __forceinline__ void one_layer(const typename Tr<PREC>::E* Wl,
+                                          const typename Tr<PREC>::E* in, int wave, int lane,
+                                          f32x4 (&acc)[H / 64][NTL]) {
+  typedef Chain<H, PREC, NTL> C;
+  constexpr int UL = K / C::KPU, NB = C::NB, PD = C::PD;
+  static_assert(K % C::KPU == 0, "layer width must fill whole units");
+  const int n = lane & 15, g = lane >> 4;
+  const char* wb = reinterpret_cast<const char*>(Wl) + (size_t)wave * UL * C::unit_bytes;
+  asm volatile("" : "+s"(wb));
+  const unsigned lane_off = (unsigned)lane * C::MTW * sizeof(typename C::E);
+  typename C::V abuf[NB][C::AN];
+  typename C::V bbuf[2][NTL];
+#pragma unroll
+  for (int mt = 0; mt < C::MTW; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NTL; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int j = 0; j < PD; ++j) load_a<H, PREC, NTL>(wb, j, lane_off, abuf[j % NB]);
+  read_b<H, PREC, NTL>(in, 0, n, g, bbuf[0]);
+  run_layer<H, PREC, NTL, UL, 0>(wb, wb, lane_off, in, n, g, abuf, bbuf, acc);
+}
+
+// The same for the 16-bit forms (32x32x16 MFMA, NT column tiles of 32).
+template <int H, int PREC, int NT, int K>
+__device__ __forceinline__ void one_layer16(const typename Tr<PREC>::E* Wl,
+                                            const typename Tr<PREC>::E* in, int wave, int lane,
+                                            f32x16 (&acc)[H / 128][NT]) {
+  typedef Chain16<H, PREC, NT> C;
+  constexpr int UL = K / 16, NB = C::NB, PD = C::PD;
+  static_assert(K % 16 == 0, "layer width must fill whole k-steps");
+  const char* wb = reinterpret_cast<const char*>(Wl) + (size_t)wave * UL * C::unit_bytes;
+  asm volatile("" : "+s"(wb));
+  const unsigned lane_off = (unsigned)lane * C::M32 * 16;
+  typename C::V abuf[NB][C::M32];
+  typename C::V bbuf[2][NT];
+#pragma unroll
+  for (int mt = 0; mt < C::M32; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[mt][nt][r] = 0.0f;
+#pragma unroll
+  for (int j = 0; j < PD; ++j) load_a16<H, PREC, NT>(wb, j, lane_off, abuf[j % NB]);
+  read_b16<H, PREC, NT>(in, 0, lane, bbuf[0]);
+  run_layer16<H, PREC, NT, UL, 0>(wb, wb, lane_off, in, lane, abuf, bbuf, acc);
+}
+
+// Copies NC columns x K elements of a swizzled LDS buffer to rows of a
+// [C, H]-strided matrix in global memory (16-byte chunks).
+template <int H, int PREC, int NC, int K>
+__device__ __forceinline__ void store_cols(const typename Tr<PREC>::E* buf, void* dst,
+                                           long long col0, int tid) {
+  typedef typename Tr<PREC>::E E;
+  constexpr int CE = Tr<PREC>::CE, NCH = K / CE;
+  E* out = static_cast<E*>(dst);
+  for (int it = tid; it < NC * NCH; it += kThreads) {
+    const int col = it / NCH, ch = it % NCH;
+    *reinterpret_cast<f32x4*>(out + (col0 + col) * H + ch * CE) =
+        *reinterpret_cast<const f32x4*>(buf + act_chunk<H, PREC>(col, ch));
+  }
+}
+
+template <int F, int H, int PREC, int NS>
+__global__ void __launch_bounds__(kThreads)
+render_backward_kernel(BackArgs ba) {
+  typedef typename Tr<PREC>::E E;
+  typedef SmemB<F, H, PREC, NS> L;
+  const RenderArgs& a = ba.r;
+  constexpr int NC = L::NC, KF = L::KF, MTW = H / 64, CE = Tr<PREC>::CE;
+  constexpr int CPL = F / 4, K0 = F + kPeSlots;
+  constexpr SmallOffsets so = small_offsets(F, H);
+  constexpr bool F32 = PREC == kF32;
+  constexpr int NTL = NC / 16, NT32 = NC / 32;
+  static_assert(F32 || NC % 32 == 0, "16-bit forms need chunks of a multiple of 8 samples");
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  E* x0 = reinterpret_cast<E*>(smem + L::x0);
+  auto hb = [&](int l) { return reinterpret_cast<E*>(smem + L::h0 + l * L::buf); };
+  E* dl = reinterpret_cast<E*>(smem + L::dl);
+  float* keep = reinterpret_cast<float*>(smem + L::keep);
+  float* s_alpha = reinterpret_cast<float*>(smem + L::alpha);
+  float* s_T = reinterpret_cast<float*>(smem + L::tr);
+  float* s_g = reinterpret_cast<float*>(smem + L::gw);
+  float* s_dz = reinterpret_cast<float*>(smem + L::dz);
+  float* s_pos = reinterpret_cast<float*>(smem + L::pos);
+  float* s_bias = reinterpret_cast<float*>(smem + L::bias);
+
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int S = a.S, V = a.V;
+  const float* ws = a.wsmall;
+  const E* W = reinterpret_cast<const E*>(a.wdense);
+  const E* WT = reinterpret_cast<const E*>(ba.wdense_t);
+  // packed layer sizes (elements, all waves): layer 0 has K0 inputs, the
+  // others H; the transposed layers all have H inputs
+  const size_t l0e = (size_t)H * K0, lhe = (size_t)H * H;
+  for (int i = tid; i < 4 * H; i += kThreads) s_bias[i] = a.bdense[i];
+  for (int i = tid; i < H; i += kThreads) s_bias[4 * H + i] = ws[so.sigma_w + i];
+  const float inv_v = 1.0f / (float)V;
+  // occupancy-head gradient partials held per lane: the units this lane's
+  // accumulators cover, summed over its columns
+  constexpr int NU = F32 ? 4 * MTW : 16 * (H / 128);
+  float gsw[NU];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) gsw[u] = 0.0f;
+  float gsb = 0.0f;
+
+  const long long item_end = std::min<long long>(ba.item_end, (long long)*a.n_valid);
+  for (long long item = ba.item_begin + blockIdx.x; item < item_end; item += gridDim.x) {
+    __syncthreads();
+    const long long ray = a.list[item];
+    const int b = (int)(ray / a.rays_per_batch);
+    const float* trow = a.t + ray * S;
+    const float* rr = a.rays + ray * 6;
+    const float* bb = a.bbox + b * 6;
+    const float cx = 0.5f * (bb[0] + bb[1]), cy = 0.5f * (bb[2] + bb[3]),
+                cz = 0.5f * (bb[4] + bb[5]);
+    const float ihx = 2.0f / (bb[1] - bb[0]), ihy = 2.0f / (bb[3] - bb[2]),
+                ihz = 2.0f / (bb[5] - bb[4]);
+    if (tid < S) {
+      const float tv = trow[tid];
+      s_pos[4 * tid + 0] = rr[0] + tv * rr[3];
+      s_pos[4 * tid + 1] = rr[1] + tv * rr[4];
+      s_pos[4 * tid + 2] = rr[2] + tv * rr[5];
+      s_alpha[tid] = ba.occ[ray * S + tid];
+    }
+    __syncthreads();
+    // 1. features of all samples and views
+    for (int it = tid; it < S * kVP * 4; it += kThreads) {
+      const int col = it >> 2, quarter = it & 3;
+      const int sl = col >> 2, v = col & 3;
+      float f[CPL];
+      gather_quarter<F, PREC, false>(a, b, v, s_pos[4 * sl], s_pos[4 * sl + 1],
+                                     s_pos[4 * sl + 2], quarter, f);
+      float* kp = keep + col * KF + quarter * CPL;
+#pragma unroll
+      for (int c = 0; c < CPL; c += 4)
+        *reinterpret_cast<f32x4*>(kp + c) = f32x4{f[c], f[c + 1], f[c + 2], f[c + 3]};
+    }
+    __syncthreads();
+    // 2. g_i = dL/dtoken . f(x_i): lane holds dtoken entries lane + 64 j
+    {
+      float dt[kVP * F / 64];
+#pragma unroll
+      for (int j = 0; j < kVP * F / 64; ++j) dt[j] = ba.dtok[item * kVP * F + lane + 64 * j];
+      for (int i = wave; i < S; i += kWaves) {
+        float part = 0.0f;
+#pragma unroll
+        for (int j = 0; j < kVP * F / 64; ++j) {
+          const int e = lane + 64 * j, v = e / F, c = e % F;
+          part += dt[j] * keep[(i * kVP + v) * KF + c];
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+        if (lane == 0) s_g[i] = part;
+      }
+    }
+    __syncthreads();
+    // 3. compositing replayed and walked back
+    if (tid == 0) {
+      float T = 1.0f;
+      int last = -1;
+      for (int i = 0; i < S; ++i) {
+        s_T[i] = T;
+        last = i;
+        T *= 1.0f - s_alpha[i];
+        if (T < a.t_thres) break;
+      }
+      float U = 0.0f;
+      for (int i = S - 1; i >= 0; --i) {
+        float d = 0.0f;
+        if (i <= last) {
+          const float al = s_alpha[i];
+          d = s_T[i] * (s_g[i] - U) * al * (1.0f - al);
+          U = s_g[i] * al + (1.0f - al) * U;
+        }
+        s_dz[i] = d;
+        gsb += d;
+      }
+    }
+    __syncthreads();
+    // 4./5. per chunk
+    for (int c0 = 0; c0 < S; c0 += NS) {
+      const long long col0 = (item - ba.item_begin) * S * kVP + (long long)c0 * kVP;
+      // layer-0 input: features (from keep) and positional encoding
+      for (int it = tid; it < NC * (F / CE); it += kThreads) {
+        const int col = it / (F / CE), ch = it % (F / CE);
+        const float* kp = keep + ((c0 + (col >> 2)) * kVP + (col & 3)) * KF + ch * CE;
+        E* d = x0 + act_chunk<H, PREC>(col, ch);
+        if constexpr (F32) {
+          *reinterpret_cast<f32x4*>(d) = *reinterpret_cast<const f32x4*>(kp);
+        } else {
+          typename Tr<PREC>::V8 h;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) h[e] = (E)kp[e];
+          *reinterpret_cast<typename Tr<PREC>::V8*>(d) = h;
+        }
+      }
+      for (int it = tid; it < NS * kPeSlots / 2; it += kThreads) {
+        const int sl = it / (kPeSlots / 2), q0 = 2 * (it % (kPeSlots / 2));
+        const float* ps = s_pos + 4 * (c0 + sl);
+        float val[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int q = q0 + h;
+          val[h] = 0.0f;
+          if (q < 63) {
+            const int d = q % 3;
+            const float xd = d == 0 ? (ps[0] - cx) * ihx
+                                    : (d == 1 ? (ps[1] - cy) * ihy : (ps[2] - cz) * ihz);
+            if (q < 3) {
+              val[h] = xd;
+            } else {
+              const int col = q / 3 - 1;
+              val[h] = sinf(ldexpf(xd, col >> 1) + (float)(col & 1) * kHalfPi);
+            }
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < kVP; ++v) {
+          const int col = sl * kVP + v, k = F + q0;
+          E* d2 = x0 + act_chunk<H, PREC>(col, k / CE) + k % CE;
+          if constexpr (F32) {
+            *reinterpret_cast<f32x2*>(d2) = f32x2{val[0], val[1]};
+          } else {
+            typedef E E2 __attribute__((ext_vector_type(2)));
+            *reinterpret_cast<E2*>(d2) = E2{(E)val[0], (E)val[1]};
+          }
+        }
+      }
+      __syncthreads();
+      store_cols<H, PREC, NC, K0>(x0, ba.A0, col0, tid);
+      if constexpr (F32) {
+        f32x4 acc[MTW][NTL];
+        // forward, keeping every layer's output
+        one_layer<H, PREC, NTL, K0>(W, x0, wave, lane, acc);
+        write_act<H, PREC, NTL>(hb(0), s_bias, wave, lane, acc);
+        __syncthreads();
+        one_layer<H, PREC, NTL, H>(W + l0e, hb(0), wave, lane, acc);
+        write_act<H, PREC, NTL>(hb(1), s_bias + H, wave, lane, acc);
+        __syncthreads();
+        one_layer<H, PREC, NTL, H>(W + l0e + lhe, hb(1), wave, lane, acc);
+        write_act<H, PREC, NTL>(hb(2), s_bias + 2 * H, wave, lane, acc);
+        __syncthreads();
+        one_layer<H, PREC, NTL, H>(W + l0e + 2 * lhe, hb(2), wave, lane, acc);
+        // delta_3 = dz / V sigma_w [pre_3 > 0] (real views); sigma_w gradient
+        {
+          const int n = lane & 15, g = lane >> 4;
+#pragma unroll
+          for (int nt = 0; nt < NTL; ++nt) {
+            const int col = 16 * nt + n;
+            const bool real = (col & 3) < V;
+            const float dzc = real ? s_dz[c0 + (col >> 2)] * inv_v : 0.0f;
+#pragma unroll
+            for (int mt = 0; mt < MTW; ++mt)
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                const int u = unit_of<H>(wave, g, mt, i);
+                const float pre = acc[mt][nt][i] + s_bias[3 * H + u];
+                gsw[4 * mt + i] += dzc * fmaxf(pre, 0.0f);
+                acc[mt][nt][i] = pre > 0.0f ? dzc * s_bias[4 * H + u] : 0.0f;
+              }
+          }
+        }
+        write_vals<H, PREC, NTL>(dl, acc, nullptr, wave, lane);
+        __syncthreads();
+        store_cols<H, PREC, NC, H>(dl, ba.D3, col0, tid);
+        store_cols<H, PREC, NC, H>(hb(0), ba.A1, col0, tid);
+        store_cols<H, PREC, NC, H>(hb(1), ba.A2, col0, tid);
+        store_cols<H, PREC, NC, H>(hb(2), ba.A3, col0, tid);
+        void* Ds[3] = {ba.D2, ba.D1, ba.D0};
+        for (int l = 0; l < 3; ++l) {   // delta_{2-l} = W_{3-l}^T delta [h_{2-l} > 0]
+          one_layer<H, PREC, NTL, H>(WT + l * lhe, dl, wave, lane, acc);
+          __syncthreads();
+          write_vals<H, PREC, NTL>(dl, acc, hb(2 - l), wave, lane);
+          __syncthreads();
+          store_cols<H, PREC, NC, H>(dl, Ds[l], col0, tid);
+        }
+      } else {
+        f32x16 acc[H / 128][NT32];
+        one_layer16<H, PREC, NT32, K0>(W, x0, wave, lane, acc);
+        write_act16<H, PREC, NT32>(hb(0), s_bias, wave, lane, acc);
+        __syncthreads();
+        one_layer16<H, PREC, NT32, H>(W + l0e, hb(0), wave, lane, acc);
+        write_act16<H, PREC, NT32>(hb(1), s_bias + H, wave, lane, acc);
+        __syncthreads();
+        one_layer16<H, PREC, NT32, H>(W + l0e + lhe, hb(1), wave, lane, acc);
+        write_act16<H, PREC, NT32>(hb(2), s_bias + 2 * H, wave, lane, acc);
+        __syncthreads();
+        one_layer16<H, PREC, NT32, H>(W + l0e + 2 * lhe, hb(2), wave, lane, acc);
+        {
+          const int c = lane & 31, h = lane >> 5;
+#pragma unroll
+          for (int nt = 0; nt < NT32; ++nt) {
+            const int col = 32 * nt + c;
+            const bool real = (col & 3) < V;
+            const float dzc = real ? s_dz[c0 + (col >> 2)] * inv_v : 0.0f;
+#pragma unroll
+            for (int mt = 0; mt < H / 128; ++mt)
+#pragma unroll
+              for (int r = 0; r < 16; ++r) {
+                const int u = unit_of32<H>(wave, mt, h, r);
+                const float pre = acc[mt][nt][r] + s_bias[3 * H + u];
+                gsw[16 * mt + r] += dzc * fmaxf(pre, 0.0f);
+                acc[mt][nt][r] = pre > 0.0f ? dzc * s_bias[4 * H + u] : 0.0f;
+              }
+          }
+        }
+        write_vals16<H, PREC, NT32>(dl, acc, nullptr, wave, lane);
+        __syncthreads();
+        store_cols<H, PREC, NC, H>(dl, ba.D3, col0, tid);
+        store_cols<H, PREC, NC, H>(hb(0), ba.A1, col0, tid);
+        store_cols<H, PREC, NC, H>(hb(1), ba.A2, col0, tid);
+        store_cols<H, PREC, NC, H>(hb(2), ba.A3, col0, tid);
+        void* Ds[3] = {ba.D2, ba.D1, ba.D0};
+        for (int l = 0; l < 3; ++l) {
+          one_layer16<H, PREC, NT32, H>(WT + l * lhe, dl, wave, lane, acc);
+          __syncthreads();
+          write_vals16<H, PREC, NT32>(dl, acc, hb(2 - l), wave, lane);
+          __syncthreads();
+          store_cols<H, PREC, NC, H>(dl, Ds[l], col0, tid);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // occupancy-head gradient partials of this workgroup
+  float* gp = ba.gpart + (size_t)blockIdx.x * so.total;
+  if constexpr (F32) {
+    const int n = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int k = 0; k < NU; ++k) {
+      float v = gsw[k];
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 64);
+      if (n == 0) gp[so.sigma_w + unit_of<H>(wave, g, k / 4, k % 4)] += v;
+    }
+  } else {
+    const int c = lane & 31, h = lane >> 5;
+#pragma unroll
+    for (int k = 0; k < NU; ++k) {
+      float v = gsw[k];
+#pragma unroll
+      for (int off = 1; off < 32; off <<= 1) v += __shfl_xor(v, off, 64);
+      if (c == 0) gp[so.sigma_w + unit_of32<H>(wave, k / 16, h, k % 16)] += v;
+    }
+  }
+  if (tid == 0) gp[so.sigma_b] += gsb;
+}
 
 // ---- Backward of the Hydra encoder + RGB head ------------------------------
 // For each group of 16 rays (list order) the forward is recomputed from the
@@ -1633,6 +2114,93 @@ extern "C" int sailor_render_hydra_backward(const float* grad_rgb, int B, int R,
                                                   grad_tokens, partials, parts, grad_small, st)
                  : hydra_backward_launch<32, 128>(grad_rgb, tok, list, counter, w_small, V,
                                                   grad_tokens, partials, parts, grad_small, st);
+}
+
+template <int F, int H, int PREC>
+int sample_backward_launch(BackArgs& ba, int parts, float* grad_small, hipStream_t st) {
+  constexpr int NS = PREC == kF32 ? 4 : 8;
+  constexpr SmallOffsets so = small_offsets(F, H);
+  typedef SmemB<F, H, PREC, NS> L;
+  static_assert(L::total <= kLdsBytes, "backward LDS");
+  SAILOR_REQUIRE(ba.r.S % NS == 0 && ba.r.S <= kBackMaxS,
+                 "render_sample_backward: S=%d must be a multiple of %d and at most %d",
+                 ba.r.S, NS, kBackMaxS);
+  auto kern = render_backward_kernel<F, H, PREC, NS>;
+  static bool init = false;
+  if (!init) {
+    SAILOR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)L::total));
+    init = true;
+  }
+  const long long nz = (long long)parts * so.total;
+  hipLaunchKernelGGL(fill_zero, dim3((unsigned)std::min<long long>((nz + 255) / 256, 4096)),
+                     dim3(256), 0, st, ba.gpart, nz);
+  if (int err = sailor::check_launch("render_sample_backward (zero)")) return err;
+  hipLaunchKernelGGL(kern, dim3(parts), dim3(kThreads), L::total, st, ba);
+  if (int err = sailor::check_launch("render_sample_backward")) return err;
+  hipLaunchKernelGGL(sum_partials, dim3((so.total + 255) / 256), dim3(256), 0, st, ba.gpart,
+                     parts, so.total, grad_small, 1);
+  return sailor::check_launch("render_sample_backward (sum)");
+}
+
+extern "C" int sailor_render_sample_backward(
+    const float* t, const float* rays, int B, int R, int S, const float* Kv, const float* RTv,
+    int V, int img_h, int img_w, const float* fmap, int Hf, int Wf, int F, const float* bbox,
+    int H, int precision, const void* w_dense, const void* w_dense_t, const float* b_dense,
+    const float* w_small, float t_thresh, const float* occ, const float* grad_tokens,
+    const void* workspace, size_t workspace_bytes, long long item_begin, long long item_end,
+    void* const* col_bufs, float* grad_small, float* partials, size_t partials_floats,
+    void* stream) {
+  SAILOR_REQUIRE(B > 0 && R >= 0 && S > 0, "render_sample_backward: bad sizes");
+  SAILOR_REQUIRE(V >= 1 && V <= kVP, "render_sample_backward: 1 to %d views supported", kVP);
+  SAILOR_REQUIRE(workspace != nullptr &&
+                     workspace_bytes >= sailor_render_workspace_bytes(B, R, F),
+                 "render_sample_backward: workspace too small");
+  SAILOR_REQUIRE(col_bufs != nullptr && item_begin >= 0 && item_end >= item_begin,
+                 "render_sample_backward: bad column batch");
+  if (!((F == 64 && H == 256) || (F == 32 && H == 128)))
+    return sailor::set_error(SAILOR_ERR_ARG, "render_sample_backward: (F=%d, H=%d) not "
+                             "instantiated", F, H);
+  if (precision < kF32 || precision > kF16)
+    return sailor::set_error(SAILOR_ERR_ARG, "render_sample_backward: precision %d unsupported",
+                             precision);
+  const size_t per = (size_t)small_offsets(F, H).total;
+  const int parts = (int)std::min<size_t>(partials_floats / per, 1024);
+  SAILOR_REQUIRE(parts >= 1, "render_sample_backward: partial buffer too small");
+  const long long n = (long long)B * R;
+  if (n == 0 || item_end == item_begin) return SAILOR_OK;
+  BackArgs ba{};
+  RenderArgs& a = ba.r;
+  a.t = t; a.rays = rays; a.Kv = Kv; a.RTv = RTv; a.fmap = fmap; a.bbox = bbox;
+  a.wdense = w_dense; a.bdense = b_dense; a.wsmall = w_small;
+  a.n_rays = n; a.rays_per_batch = R;
+  const char* wsb = static_cast<const char*>(workspace);
+  a.n_valid = reinterpret_cast<const unsigned*>(wsb);
+  a.list = reinterpret_cast<const int*>(wsb + workspace_list_offset());
+  a.S = S; a.V = V; a.img_w = img_w; a.img_h = img_h; a.Hf = Hf; a.Wf = Wf;
+  a.t_thres = t_thresh;
+  ba.wdense_t = w_dense_t;
+  ba.occ = occ;
+  ba.dtok = grad_tokens;
+  ba.item_begin = item_begin;
+  ba.item_end = item_end;
+  ba.A0 = col_bufs[0]; ba.A1 = col_bufs[1]; ba.A2 = col_bufs[2]; ba.A3 = col_bufs[3];
+  ba.D0 = col_bufs[4]; ba.D1 = col_bufs[5]; ba.D2 = col_bufs[6]; ba.D3 = col_bufs[7];
+  ba.gpart = partials;
+  hipStream_t st = sailor::as_stream(stream);
+  if (F == 64) {
+    switch (precision) {
+      case kF32: return sample_backward_launch<64, 256, kF32>(ba, parts, grad_small, st);
+      case kBF16: return sample_backward_launch<64, 256, kBF16>(ba, parts, grad_small, st);
+      default: return sample_backward_launch<64, 256, kF16>(ba, parts, grad_small, st);
+    }
+  }
+  switch (precision) {
+    case kF32: return sample_backward_launch<32, 128, kF32>(ba, parts, grad_small, st);
+    case kBF16: return sample_backward_launch<32, 128, kBF16>(ba, parts, grad_small, st);
+    default: return sample_backward_launch<32, 128, kF16>(ba, parts, grad_small, st);
+  }
 }
 
 extern "C" int sailor_density_points(const float* points, int B, int N, const float* Kv,

@@ -91,6 +91,32 @@ class NerfWeights:
         parts += [w1, b1, w2, b2, w3, np.concatenate([b3, np.zeros(1, np.float32)])]
         return np.concatenate([np.asarray(p, np.float32).reshape(-1) for p in parts])
 
+    def split_small(self, buf):
+        """Inverse of small_buffer: views of a flat buffer (numpy or torch) in
+        the layout of this model's parameters, as dict(sigma_w, sigma_b,
+        hydra=[dict per stack], rgb=[(W, b)] x3)."""
+        pos = 0
+
+        def take(shape):
+            nonlocal pos
+            n = int(np.prod(shape))
+            v = buf[pos:pos + n].reshape(shape)
+            pos += n
+            return v
+        out = dict(sigma_w=take(self.sigma_w.shape))
+        out["sigma_b"] = take((4,))[0]
+        out["hydra"] = [{k: take(st[k].shape) for k in ("q", "k", "v", "fc", "w1", "b1", "w2",
+                                                        "b2")} for st in self.hydra]
+        rgb = []
+        for i, (w, b) in enumerate(self.rgb):
+            wv = take(w.shape)
+            bv = take((4,))[:3] if i == 2 else take(b.shape)
+            rgb.append((wv, bv))
+        out["rgb"] = rgb
+        if pos != len(buf):
+            raise RuntimeError(f"small buffer has {len(buf)} floats, layout uses {pos}")
+        return out
+
     def pack(self, prec="fp32", device="cuda"):
         """-> (w_dense tensor, b_dense [4,H] fp32, w_small fp32) on `device`."""
         frags = [pack_layer(natural_first_layer(self.dense_w[0], self.F), prec)]

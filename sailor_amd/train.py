@@ -8,6 +8,8 @@ way sailor_render_rays writes it: [counter (256 B)] [list: B*R int32,
 """
 from __future__ import annotations
 
+import ctypes
+
 import torch
 
 from . import _lib
@@ -45,3 +47,129 @@ def hydra_backward(grad_rgb, B, R, V, model, workspace, scratch: BackwardScratch
          ptr(scratch.grad_small), ptr(scratch.partials), scratch.partials.numel(),
          stream_of(grad_rgb.device))
     return scratch.grad_tokens, scratch.grad_small
+
+
+def pack_transposed(weights, prec, device):
+    """Packed A fragments of W_3^T, W_2^T, W_1^T (the delta propagation's
+    layers: input width H), in the forward's packing."""
+    import numpy as np
+    from .model import pack_layer
+    frags = [pack_layer(np.ascontiguousarray(weights.dense_w[l].T), prec) for l in (3, 2, 1)]
+    dense = np.concatenate([f.reshape(-1) for f in frags]).astype(np.float32)
+    t = torch.from_numpy(dense)
+    if prec != "fp32":
+        t = t.to(torch.bfloat16 if prec == "bf16" else torch.float16)
+    return t.to(device).contiguous()
+
+
+def _mm_t(D, A):
+    """D^T A with float32 output (16-bit inputs accumulate in fp32)."""
+    if D.dtype == torch.float32:
+        return D.t().mm(A)
+    try:
+        return torch.mm(D.t(), A, out_dtype=torch.float32)
+    except (TypeError, RuntimeError):
+        return D.t().float().mm(A.float())
+
+
+def sample_backward(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, model, wt, occ, workspace,
+                    grad_tokens, grad_small, partials, n_valid, t_thres=1e-4,
+                    batch_items=4096):
+    """Compositing + MLP backward over the listed rays in batches of
+    batch_items; returns (dW [4] in kernel input order, db [4]) as float32
+    and adds the occupancy-head gradient to grad_small."""
+    B, R, S = t.shape
+    V = Kv.shape[1]
+    Hf, Wf, F = fmap_cl.shape[2], fmap_cl.shape[3], fmap_cl.shape[4]
+    H = model.H
+    dev = t.device
+    dt = {"fp32": torch.float32, "bf16": torch.bfloat16, "fp16": torch.float16}[model.prec]
+    from .model import PREC_IDS
+    cols = min(batch_items, max(n_valid, 1)) * S * 4
+    bufs = [torch.empty((cols, H), dtype=dt, device=dev) for _ in range(8)]
+    arr = (ctypes.c_void_p * 8)(*[b.data_ptr() for b in bufs])
+    K0 = F + 64
+    dW = [torch.zeros((H, K0 if l == 0 else H), dtype=torch.float32, device=dev)
+          for l in range(4)]
+    db = [torch.zeros(H, dtype=torch.float32, device=dev) for _ in range(4)]
+    for i0 in range(0, n_valid, batch_items):
+        i1 = min(i0 + batch_items, n_valid)
+        call("sailor_render_sample_backward", ptr(t), ptr(rays), B, R, S, ptr(Kv), ptr(RTv), V,
+             int(img_hw[0]), int(img_hw[1]), ptr(fmap_cl), Hf, Wf, F, ptr(bbox), H,
+             PREC_IDS[model.prec], ptr(model.w_dense), ptr(wt), ptr(model.b_dense),
+             ptr(model.w_small), float(t_thres), ptr(occ), ptr(grad_tokens), ptr(workspace),
+             workspace.numel(), i0, i1, arr, ptr(grad_small), ptr(partials), partials.numel(),
+             stream_of(dev))
+        c = (i1 - i0) * S * 4
+        A = [bufs[0][:c, :K0], bufs[1][:c], bufs[2][:c], bufs[3][:c]]
+        D = [bufs[4][:c], bufs[5][:c], bufs[6][:c], bufs[7][:c]]
+        for l in range(4):
+            dW[l] += _mm_t(D[l], A[l])
+            db[l] += D[l].float().sum(0)
+    return dW, db
+
+
+def natural_first_layer_grad(dW0, F):
+    """Kernel-order first-layer gradient [H, F + 64] -> the model's [H, 63 + F]
+    ([encoding | features])."""
+    return torch.cat([dW0[:, F:F + 63], dW0[:, :F]], 1)
+
+
+class RenderTrainer:
+    """Forward and backward of the fused render for batches of rays, with
+    parameter gradients in the layout of NerfWeights.
+
+    forward() runs sailor_render_rays keeping the occupancies; backward(g)
+    takes dL/d rgb and runs (1) the Hydra + RGB head backward, (2) the
+    compositing + per-view MLP backward in item batches, each followed by
+    the weight-gradient GEMMs D_l^T A_l on the GPU.  The reference's
+    differentiation of this path is torch autograd through SRONet/nerf.py
+    and _volume_render_torch.forward_occ (voxel_encoding_helper.py:166-186).
+    """
+
+    def __init__(self, weights, prec="fp32", device="cuda", batch_items=4096):
+        from .render import PackedModel
+        self.weights, self.prec, self.dev = weights, prec, torch.device(device)
+        self.model = PackedModel(weights, prec, self.dev)
+        self.wt = pack_transposed(weights, prec, self.dev)
+        self.batch_items = batch_items
+        self._out = None
+        self._scratch = None
+        self._ctx = None
+
+    def forward(self, t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, t_thres=1e-4):
+        from . import render
+        B, R, _ = t.shape
+        if self._out is not None and self._out["rgb"].shape[:2] != (B, R):
+            self._out = None
+        self._out = render.render_rays(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir,
+                                       self.model, t_thres, early_exit=False, want_occ=True,
+                                       out=self._out)
+        self._ctx = (t, rays, Kv, RTv, img_hw, fmap_cl, bbox, t_thres)
+        return self._out
+
+    def backward(self, grad_rgb):
+        """dL/d rgb [B, R, 3] -> dict of float32 gradients shaped like the
+        model's parameters (dense_w, dense_b, sigma_w, sigma_b, hydra, rgb).
+        Synchronises once (to read the number of listed rays)."""
+        if self._ctx is None:
+            raise RuntimeError("backward() before forward()")
+        t, rays, Kv, RTv, img_hw, fmap_cl, bbox, t_thres = self._ctx
+        if fmap_cl.dtype != torch.float32:
+            raise RuntimeError("the backward reads float32 feature maps")
+        B, R, _ = t.shape
+        V, F, H = Kv.shape[1], self.model.F, self.model.H
+        if self._scratch is None or self._scratch.grad_tokens.shape[0] != B * R:
+            self._scratch = BackwardScratch(B, R, F, H, self.dev)
+        ws = self._out["workspace"]
+        g = grad_rgb.reshape(B * R, 3).contiguous().float()
+        hydra_backward(g, B, R, V, self.model, ws, self._scratch)
+        n_valid = int(ws[:4].view(torch.int32)[0].item())
+        dW, db = sample_backward(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, self.model, self.wt,
+                                 self._out["occ"], ws, self._scratch.grad_tokens,
+                                 self._scratch.grad_small, self._scratch.partials, n_valid,
+                                 t_thres, self.batch_items)
+        small = self.weights.split_small(self._scratch.grad_small)
+        return dict(dense_w=[natural_first_layer_grad(dW[0], F)] + dW[1:], dense_b=db,
+                    sigma_w=small["sigma_w"], sigma_b=small["sigma_b"], hydra=small["hydra"],
+                    rgb=small["rgb"])

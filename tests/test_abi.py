@@ -58,7 +58,7 @@ def test_argument_errors_are_reported_not_fatal():
 
 
 def _header_params():
-    """name -> list of parameter kinds ('P', 'I', 'F', 'SZ') parsed from include/*.h."""
+    """name -> list of parameter kinds ('P', 'I', 'F', 'SZ', 'LL') parsed from include/*.h."""
     out = {}
     for fn in os.listdir(os.path.join(ROOT, "include")):
         if not fn.endswith(".h"):
@@ -75,6 +75,8 @@ def _header_params():
                     kinds.append("P")
                 elif p.startswith("size_t"):
                     kinds.append("SZ")
+                elif p.startswith("long long"):
+                    kinds.append("LL")
                 elif p.startswith("float"):
                     kinds.append("F")
                 elif p.startswith("int"):
@@ -87,7 +89,7 @@ def _header_params():
 
 def test_ctypes_signatures_match_header():
     from sailor_amd import _lib
-    kind = {_lib.P: "P", _lib.I: "I", _lib.F: "F", _lib.SZ: "SZ"}
+    kind = {_lib.P: "P", _lib.I: "I", _lib.F: "F", _lib.SZ: "SZ", _lib.LL: "LL"}
     hdr = _header_params()
     bad = []
     for name, params in hdr.items():

@@ -134,3 +134,18 @@ def test_reference_model_runs_on_cpu():
     rgb, dep, ws, asum, occ = nerf_ref.render_rays(w, t, rays, K, RT, fm, bbox, zdir, (128, 128))
     assert torch.all(rgb[0, 0] == 0) and torch.all(rgb[0, 1:] > 0)
     assert torch.all((ws[0, 1:] > 0) & (ws[0, 1:] <= 1 + 1e-9))
+
+
+def test_split_small_inverts_small_buffer():
+    from sailor_amd.model import NerfWeights
+    w = NerfWeights.init(32, 128, perturb_std=0.01)
+    buf = w.small_buffer()
+    sp = w.split_small(buf)
+    np.testing.assert_array_equal(sp["sigma_w"], w.sigma_w)
+    assert sp["sigma_b"] == np.float32(w.sigma_b)
+    for a, b in zip(sp["hydra"], w.hydra):
+        for k in b:
+            np.testing.assert_array_equal(a[k], b[k])
+    for (a, b), (c, d) in zip(sp["rgb"], w.rgb):
+        np.testing.assert_array_equal(a, c)
+        np.testing.assert_array_equal(b, d)
