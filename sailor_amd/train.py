@@ -74,10 +74,11 @@ def _mm_t(D, A):
 
 def sample_backward(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, model, wt, occ, workspace,
                     grad_tokens, grad_small, partials, n_valid, t_thres=1e-4,
-                    batch_items=4096):
+                    batch_items=4096, columns=None):
     """Compositing + MLP backward over the listed rays in batches of
     batch_items; returns (dW [4] in kernel input order, db [4]) as float32
-    and adds the occupancy-head gradient to grad_small."""
+    and adds the occupancy-head gradient to grad_small.  A list `columns`
+    receives (A [4], D [4]) column matrices of the last batch."""
     B, R, S = t.shape
     V = Kv.shape[1]
     Hf, Wf, F = fmap_cl.shape[2], fmap_cl.shape[3], fmap_cl.shape[4]
@@ -92,6 +93,8 @@ def sample_backward(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, model, wt, occ, wor
     dW = [torch.zeros((H, K0 if l == 0 else H), dtype=torch.float32, device=dev)
           for l in range(4)]
     db = [torch.zeros(H, dtype=torch.float32, device=dev) for _ in range(4)]
+    if F + 64 > H:
+        raise RuntimeError(f"first-layer width {F + 64} exceeds the column stride {H}")
     for i0 in range(0, n_valid, batch_items):
         i1 = min(i0 + batch_items, n_valid)
         call("sailor_render_sample_backward", ptr(t), ptr(rays), B, R, S, ptr(Kv), ptr(RTv), V,
@@ -106,6 +109,8 @@ def sample_backward(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, model, wt, occ, wor
         for l in range(4):
             dW[l] += _mm_t(D[l], A[l])
             db[l] += D[l].float().sum(0)
+        if columns is not None:
+            columns[:] = [A, D]
     return dW, db
 
 
@@ -148,10 +153,12 @@ class RenderTrainer:
         self._ctx = (t, rays, Kv, RTv, img_hw, fmap_cl, bbox, t_thres)
         return self._out
 
-    def backward(self, grad_rgb):
+    def backward(self, grad_rgb, keep_columns=False):
         """dL/d rgb [B, R, 3] -> dict of float32 gradients shaped like the
         model's parameters (dense_w, dense_b, sigma_w, sigma_b, hydra, rgb).
-        Synchronises once (to read the number of listed rays)."""
+        Synchronises once (to read the number of listed rays).  With
+        keep_columns, self.columns holds the last item batch's (A, D) column
+        matrices (column (item, sample, view), item in list order)."""
         if self._ctx is None:
             raise RuntimeError("backward() before forward()")
         t, rays, Kv, RTv, img_hw, fmap_cl, bbox, t_thres = self._ctx
@@ -165,10 +172,12 @@ class RenderTrainer:
         g = grad_rgb.reshape(B * R, 3).contiguous().float()
         hydra_backward(g, B, R, V, self.model, ws, self._scratch)
         n_valid = int(ws[:4].view(torch.int32)[0].item())
+        cols = [] if keep_columns else None
         dW, db = sample_backward(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, self.model, self.wt,
                                  self._out["occ"], ws, self._scratch.grad_tokens,
                                  self._scratch.grad_small, self._scratch.partials, n_valid,
-                                 t_thres, self.batch_items)
+                                 t_thres, self.batch_items, columns=cols)
+        self.columns = cols
         small = self.weights.split_small(self._scratch.grad_small)
         return dict(dense_w=[natural_first_layer_grad(dW[0], F)] + dW[1:], dense_b=db,
                     sigma_w=small["sigma_w"], sigma_b=small["sigma_b"], hydra=small["hydra"],

@@ -93,51 +93,139 @@ def _flat_grads(gr):
     return out
 
 
-@pytest.mark.parametrize("prec,batch_items", [("fp32", 128), ("bf16", 4096)])
-def test_render_backward_matches_autograd(prec, batch_items):
-    """All parameter gradients of L = sum(g . rgb) from the fused backward vs
-    torch autograd through the float64 restatement; g is non-zero on a random
-    subset of the listed rays (the others must contribute exactly nothing).
-    Tolerance: fp32 max error <= 1e-4 of the tensor's largest entry; bf16
-    relative L2 error <= 1e-2."""
+def _oracle_grads(w, p, sub, g, img_hw, dtype, trace=None, mm_dtype=None):
+    """Gradients of L = sum(g . rgb) over rays `sub` by torch autograd through
+    nerf_ref.render_rays in `dtype` (float64: the target; float32: what fp32
+    autograd of the same model reaches)."""
+    P = nerf_ref.params_of(w, dtype=dtype, requires_grad=True)
+    rgb = nerf_ref.render_rays(
+        w, p.sorted[:, sub].cpu().numpy(), p.rays[:, sub].cpu().numpy(), p.K.cpu().numpy(),
+        p.RT.cpu().numpy(), p.fmaps_np[None], p.bbox.cpu().numpy(), p.zdir.cpu().numpy(),
+        img_hw, dtype=dtype, params=P, trace=trace, mm_dtype=mm_dtype)[0]
+    (rgb * torch.from_numpy(g[:, sub]).to(dtype)).sum().backward()
+    grads = dict(dense_w=[x.grad for x in P["dense_w"]], dense_b=[x.grad for x in P["dense_b"]],
+                 sigma_w=P["sigma_w"].grad, sigma_b=P["sigma_b"].grad,
+                 hydra=[{k: v.grad for k, v in st.items()} for st in P["hydra"]],
+                 rgb=[(a.grad, b.grad) for a, b in P["rgb"]])
+    return rgb.detach(), grads
+
+
+def _trainer_case(prec, batch_items):
     from sailor_amd.model import NerfWeights
     from sailor_amd.train import RenderTrainer
     p = pipe()
-    cfg = p.cfg
     w = NerfWeights.init(p.mc.F, p.mc.H, perturb_std=0.01)
     w.dense_b = [np.random.default_rng(40 + l).normal(0, 0.01, p.mc.H).astype(np.float32)
                  for l in range(4)]
     tr = RenderTrainer(w, prec, "cuda", batch_items=batch_items)
-    n_rays = cfg.target * cfg.target
-    img_hw = (cfg.img, cfg.img)
+    img_hw = (p.cfg.img, p.cfg.img)
     out = tr.forward(p.sorted, p.rays, p.K, p.RT, img_hw, p.fmap, p.bbox, p.zdir)
     torch.cuda.synchronize()
     valid = torch.nonzero(p.sorted[0, :, 0] >= 0).flatten().cpu().numpy()
     assert valid.size > 100
     rng = np.random.default_rng(9)
     sub = np.sort(rng.choice(valid, size=min(160, valid.size), replace=False))
-    g = np.zeros((1, n_rays, 3), np.float32)
+    g = np.zeros((1, p.cfg.target ** 2, 3), np.float32)
     g[0, sub] = rng.normal(size=(sub.size, 3)).astype(np.float32)
-    grads = tr.backward(torch.from_numpy(g).cuda())
-    torch.cuda.synchronize()
+    return p, w, tr, out, sub, g, img_hw
 
-    P = nerf_ref.params_of(w, requires_grad=True)
-    rgb, _, _, _, occ = nerf_ref.render_rays(
-        w, p.sorted[:, sub].cpu().numpy(), p.rays[:, sub].cpu().numpy(), p.K.cpu().numpy(),
-        p.RT.cpu().numpy(), p.fmaps_np[None], p.bbox.cpu().numpy(), p.zdir.cpu().numpy(),
-        img_hw, params=P)
-    if prec == "fp32":
-        np.testing.assert_allclose(out["rgb"][0, sub].double().cpu().numpy(),
-                                   rgb[0].detach().numpy(), atol=2e-5)
-    (rgb * torch.from_numpy(g[:, sub]).double()).sum().backward()
-    ref = dict(dense_w=[x.grad for x in P["dense_w"]], dense_b=[x.grad for x in P["dense_b"]],
-               sigma_w=P["sigma_w"].grad, sigma_b=P["sigma_b"].grad,
-               hydra=[{k: v.grad for k, v in st.items()} for st in P["hydra"]],
-               rgb=[(a.grad, b.grad) for a, b in P["rgb"]])
-    bad = []
-    for (name, got), (_, want) in zip(_flat_grads(grads), _flat_grads(ref)):
+
+def _compare(grads, ref, tols):
+    rows, bad = [], []
+    for i, ((name, got), (_, want)) in enumerate(zip(_flat_grads(grads), _flat_grads(ref))):
         got = got.detach().double().cpu().numpy() if torch.is_tensor(got) else got
         mx, l2 = _grad_errors(got, want.numpy())
-        if (prec == "fp32" and mx > 1e-4) or (prec != "fp32" and l2 > 1e-2):
-            bad.append((name, mx, l2))
-    assert not bad, bad
+        rows.append(f"{name}: l2 {l2:.2e} max {mx:.2e} tol {tols[i]:.2e}")
+        if l2 > tols[i]:
+            bad.append(name)
+    return rows, bad
+
+
+def test_render_backward_fp32_columns_and_gradients():
+    """fp32 differentiation form vs torch autograd of the float64 restatement,
+    L = sum(g . rgb) with g non-zero on a random subset of the listed rays.
+
+    1. Per column (ray, sample, view): the layer inputs A_l and deltas D_l the
+       kernel hands to the weight-gradient GEMMs equal the restatement's
+       layer inputs and dL/d pre-activation.  A delta may differ only where
+       a ReLU mask flips: every column whose delta differs by more than 1e-4
+       of the largest must hold a pre-activation within 5e-4 of zero at that
+       layer or a later one (the float32 encoding alone moves the first
+       layer's input by up to ~1e-4), and such columns must be rare (< 1%).
+    2. Over the rays free of such flips, every parameter gradient (dense
+       layers, occupancy head, Hydra, RGB head) is within relative L2 error
+       max(1e-4, 3 x that of float32 autograd of the same restatement): the
+       gradient sums cancel heavily, so float32 autograd itself sits up to
+       ~1e-3 off the float64 value on some tensors."""
+    p, w, tr, out, sub, g, img_hw = _trainer_case("fp32", 100000)
+    F, H, S = p.mc.F, p.mc.H, p.cfg.S
+    tr.backward(torch.from_numpy(g).cuda(), keep_columns=True)
+    torch.cuda.synchronize()
+    A, D = [[x.double().cpu().numpy() for x in part] for part in tr.columns]
+    from sailor_amd.train import workspace_views
+    n_valid, lst, _ = workspace_views(out["workspace"], p.cfg.target ** 2, F)
+    pos = {int(r): i for i, r in enumerate(lst.cpu().numpy())}
+    idx = np.array([pos[int(r)] for r in sub])
+    trace = {}
+    rgb64, _ = _oracle_grads(w, p, sub, g, img_hw, torch.float64, trace)
+    np.testing.assert_allclose(out["rgb"][0, sub].double().cpu().numpy(), rgb64[0].numpy(),
+                               atol=2e-5)
+    cols = lambda M, K: M.reshape(n_valid, S, 4, -1)[idx, :, :, :K]      # [N, S, 4, K]
+    x0 = torch.stack(trace["x0"], 2).detach().numpy()                     # [N, S, 4, 63+F]
+    pre = [torch.stack([trace["pre"][v][l] for v in range(4)], 2).detach().numpy()
+           for l in range(4)]
+    dpre = [torch.stack([trace["pre"][v][l].grad for v in range(4)], 2).numpy()
+            for l in range(4)]
+    a0 = cols(A[0], F + 64)
+    np.testing.assert_allclose(np.concatenate([a0[..., F:F + 63], a0[..., :F]], -1), x0,
+                               atol=2e-4)
+    for l in range(1, 4):
+        ref = np.maximum(pre[l - 1], 0.0)
+        np.testing.assert_allclose(cols(A[l], H), ref, atol=2e-4 * np.abs(ref).max())
+    near0 = [np.abs(x).min(-1) < 5e-4 for x in pre]                     # [N, S, 4]
+    flip_rays = np.zeros(sub.size, bool)
+    n_cols = sub.size * S * 4
+    for l in range(4):
+        ref = dpre[l]
+        err = np.abs(cols(D[l], H) - ref).max(-1)
+        off = err > 1e-4 * np.abs(ref).max()
+        explained = np.logical_or.reduce(near0[l:])
+        assert not (off & ~explained).any(), (l, np.argwhere(off & ~explained)[:5])
+        assert off.sum() < 0.01 * n_cols, (l, off.sum())
+        flip_rays |= off.any((1, 2))
+    clean = sub[~flip_rays]
+    assert clean.size >= 0.8 * sub.size, (clean.size, sub.size)
+    g2 = np.zeros_like(g)
+    g2[0, clean] = g[0, clean]
+    tr.batch_items = 128       # several item batches
+    grads = tr.backward(torch.from_numpy(g2).cuda())
+    torch.cuda.synchronize()
+    _, ref = _oracle_grads(w, p, clean, g2, img_hw, torch.float64)
+    _, ref32 = _oracle_grads(w, p, clean, g2, img_hw, torch.float32)
+    base = [_grad_errors(b.double().numpy(), r.numpy())[1]
+            for (_, b), (_, r) in zip(_flat_grads(ref32), _flat_grads(ref))]
+    rows, bad = _compare(grads, ref, [max(1e-4, 3 * b) for b in base])
+    print(f"flip rays {flip_rays.sum()} of {sub.size}")
+    print("\n".join(rows))
+    assert not bad, "\n".join(rows)
+
+
+def test_render_backward_bf16_matches_autograd():
+    """bf16 form (bf16 MFMA, fp32 accumulation) vs torch autograd of the
+    float64 restatement: every parameter gradient within relative L2 error
+    max(1e-2, 3 x that of mixed-precision autograd of the same restatement
+    with bf16-rounded layer inputs and weights).  The sums of this loss
+    cancel heavily (random-sign dL/d rgb), which makes bf16 rounding of the
+    forward alone move some gradients by several percent.  Several item
+    batches (256 rays each) exercise the batching."""
+    p, w, tr, out, sub, g, img_hw = _trainer_case("bf16", 256)
+    grads = tr.backward(torch.from_numpy(g).cuda())
+    torch.cuda.synchronize()
+    _, ref = _oracle_grads(w, p, sub, g, img_hw, torch.float64)
+    _, emu = _oracle_grads(w, p, sub, g, img_hw, torch.float64, mm_dtype=torch.bfloat16)
+    base = [_grad_errors(b.numpy(), r.numpy())[1]
+            for (_, b), (_, r) in zip(_flat_grads(emu), _flat_grads(ref))]
+    rows, bad = _compare(grads, ref, [max(1e-2, 3 * b) for b in base])
+    rows = [f"{r} (bf16-input autograd l2 {b:.2e})" for r, b in zip(rows, base)]
+    print("\n".join(rows))
+    assert not bad, "\n".join(rows)
