@@ -155,6 +155,85 @@ def precision_form(cfg, steps, warmup):
     return out
 
 
+def _median_ms(fn, steps):
+    """Median over `steps` calls of fn's GPU time (HIP events on the current
+    stream, which the library's launches use)."""
+    import numpy as np
+    import torch
+    st = torch.cuda.current_stream()
+    ts = []
+    for _ in range(steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        fn()
+        e1.record(st)
+        e1.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return float(np.median(ts))
+
+
+def training_form(steps, warmup, with_cpu=True):
+    """Config-4 block (differentiation form): 8 target views x 256^2 rays of
+    the config-1 scene and model (4 x 1024^2 source views, 64 samples/ray).
+    One step = fused forward (occupancies kept) + MSE gradient against an
+    FP32 render made with perturbed weights + backward (Hydra/RGB head,
+    compositing reverse scan, per-view MLP with per-ray recompute, weight-
+    gradient GEMMs).  Mrays/s = all 524,288 rays / median(fwd + bwd)."""
+    import numpy as np
+    import torch
+    from sailor_amd.model import NerfWeights
+    from sailor_amd.pipeline import CONFIGS
+    from sailor_amd.train import RenderTrainer, TrainingBatch, mse_grad
+    cfg = CONFIGS["c4"]
+    tb = TrainingBatch(cfg, 8)
+    mc = tb.pipe.mc
+    w = NerfWeights.init(mc.F, mc.H)
+    w_tgt = NerfWeights.init(mc.F, mc.H, perturb_std=0.01)
+    target = RenderTrainer(w_tgt, "fp32").forward(*tb.inputs())["rgb"].clone()
+    out = {"workload": (f"{cfg.name}: 8 target views x {cfg.target}^2 rays, {cfg.n_views} x "
+                        f"{cfg.img}^2 source views, {cfg.dim}^3 grid, {cfg.S} samples/ray, "
+                        f"MLP {mc.H}x4, F={mc.F}; MSE to an FP32 render with weights + N(0,0.01)"),
+           "rays": tb.n_rays, "valid_rays": tb.n_valid}
+    for prec in ("fp32", "bf16"):
+        tr = RenderTrainer(w, prec)
+
+        def fwd():
+            return tr.forward(*tb.inputs())
+
+        def step():
+            o = fwd()
+            return tr.backward(mse_grad(o["rgb"], target, tb.n_valid))
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        f_ms = _median_ms(fwd, steps)
+        s_ms = _median_ms(step, steps)
+        out[prec] = {"mrays_per_s": tb.n_rays / (s_ms / 1e3) / 1e6, "fwd_ms": f_ms,
+                     "fwd_bwd_ms": s_ms, "fwd_bwd_over_fwd": s_ms / f_ms}
+        del tr
+    if with_cpu:
+        from oracle import nerf_ref
+        valid = torch.nonzero(tb.t[0, :, 0] >= 0).flatten().cpu().numpy()
+        sub = valid[:: max(1, valid.size // 64)][:64]
+        p = tb.pipe
+        P = nerf_ref.params_of(w, dtype=torch.float32, requires_grad=True)
+        args = (tb.t[:, sub].cpu().numpy(), tb.rays[:, sub].cpu().numpy(), p.K.cpu().numpy(),
+                p.RT.cpu().numpy(), p.fmaps_np[None], p.bbox.cpu().numpy(),
+                p.zdir.cpu().numpy(), tb.img_hw)
+        tgt = target[:, sub].cpu()
+        t0 = time.perf_counter()
+        rgb = nerf_ref.render_rays(w, *args, dtype=torch.float32, params=P)[0]
+        ((rgb - tgt) ** 2).mean().backward()
+        dt = time.perf_counter() - t0
+        t_all = dt * tb.n_valid / len(sub)
+        out["cpu_baseline"] = {
+            "value": tb.n_rays / t_all / 1e6, "unit": "Mrays/s (fwd+bwd)",
+            "cores": torch.get_num_threads(), "kind": "port",
+            "sample": (f"PyTorch-CPU fp32 autograd of the restatement on {len(sub)} of "
+                       f"{tb.n_valid} valid rays ({dt:.2f} s), extrapolated per valid ray")}
+    return out
+
+
 def psnr(a, b, mask):
     import numpy as np
     m = mask[..., None].astype(np.float64)
@@ -230,6 +309,10 @@ def main():
     ap.add_argument("--no-early-exit", action="store_true")
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the 1024^2 precision-form block")
+    ap.add_argument("--no-c4", action="store_true",
+                    help="skip the differentiation-form (fwd+bwd) block")
+    ap.add_argument("--training-only", action="store_true",
+                    help="run only the differentiation-form block and print it")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="run the fixed launch sequence used for rocprofv3 counter passes")
     args = ap.parse_args()
@@ -245,6 +328,9 @@ def main():
     torch.cuda.set_device(0)
     if args.profile_kernels:
         profile_kernels(cfg)
+        return
+    if args.training_only:
+        print(json.dumps(training_form(args.steps, args.warmup, with_cpu=not args.no_cpu)))
         return
     p32 = FramePipeline.synthetic(cfg, prec="fp32")
     p32.run()
@@ -318,6 +404,9 @@ def main():
     if not args.no_c3 and args.workload == "c2":
         result["c3_precision_form"] = precision_form(CONFIGS["c3"], max(3, args.steps // 4),
                                                      args.warmup)
+    if not args.no_c4 and args.workload == "c2":
+        result["c4_training_form"] = training_form(max(3, args.steps // 4), args.warmup,
+                                                   with_cpu=not args.no_cpu)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(p32)
     else:

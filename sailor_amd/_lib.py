@@ -45,6 +45,9 @@ _SIGS = {
     "sailor_render_hydra_backward": [P, I, I, I, I, I, P, P, SZ, P, P, P, SZ, P],
     "sailor_render_sample_backward": [P, P, I, I, I, P, P, I, I, I, P, I, I, I, P, I, I, P, P, P,
                                       P, F, P, P, P, SZ, LL, LL, P, P, P, SZ, P],
+    "sailor_render_dense_grad_floats": [I, I],
+    "sailor_render_sample_backward_fused": [P, P, I, I, I, P, P, I, I, I, P, I, I, I, P, I, I, P,
+                                            P, P, P, F, P, P, P, SZ, P, P, P, SZ, P],
     "sailor_density_points": [P, I, I, P, P, I, I, I, P, I, I, I, I, P, I, I, P, P, P, I, P, P],
     "sailor_render_rays_stages": [I, P, P, I, I, I, P, P, I, I, I, P, I, I, I, I, P, P, I, I, P,
                                   P, P, F, I, I, P, P, P, P, P, SZ, P],
@@ -57,6 +60,7 @@ _RESTYPES = {
     "sailor_render_small_weights_floats": SZ,
     "sailor_render_workspace_bytes": SZ,
     "sailor_render_backward_partials_floats": SZ,
+    "sailor_render_dense_grad_floats": SZ,
 }
 
 _lib = None

@@ -1027,6 +1027,7 @@ struct BackArgs {
   void* D2;
   void* D3;
   float* gpart;            // [gridDim.x, small_total] partial small-weight gradients
+  float* gdense;           // [gridDim.x, dense_grad_total] partial dense gradients (DW form)
 };
 
 
@@ -1179,8 +1180,188 @@ __device__ __forceinline__ void store_cols(const typename Tr<PREC>::E* buf, void
   }
 }
 
-template <int F, int H, int PREC, int NS>
-__global__ void __launch_bounds__(kThreads)
+// ---- Weight gradients accumulated inside the backward kernel ---------------
+// dW_l[u][k] = sum_c D_l[c][u] A_l[c][k] over the columns c a workgroup
+// processes, db_l[u] = sum_c D_l[c][u].  Each wave owns 32 output units (H =
+// 128) and keeps its rows of all four dW_l in MFMA accumulators for the whole
+// kernel (240 registers per lane); per chunk, both operands are read from the
+// LDS activation layout with the column index as the MFMA reduction index.
+// Layout of a dense-gradient buffer (floats): dW_0 [H][F + 64] (kernel input
+// order), dW_1..3 [H][H], db [4][H].
+__host__ __device__ constexpr int dense_grad_total(int F, int H) {
+  return H * (F + kPeSlots) + 3 * H * H + 4 * H;
+}
+__host__ __device__ constexpr int dense_grad_offset(int F, int H, int l) {
+  return l == 0 ? 0 : H * (F + kPeSlots) + (l - 1) * H * H;
+}
+
+// 16-bit forms, v_mfma_f32_32x32x16: A operand = D^T (lane: unit 32w + l%32,
+// columns c0 + 8(l/32) + j), B operand = A_l (columns c0 + 8(l/32) + j,
+// input 32n + l%32); accumulator r of tile n holds
+// dW[32w + 8(r/4) + 4(l/32) + r%4][32n + l%32].
+template <int H, int PREC, int NC, int K>
+__device__ __forceinline__ void accum_dw16(const typename Tr<PREC>::E* D,
+                                           const typename Tr<PREC>::E* A,
+                                           f32x16 (&dw)[K / 32], float& db, int wave, int lane) {
+  typedef typename Tr<PREC>::V8 V8;
+  constexpr int CE = Tr<PREC>::CE;
+  static_assert(H == 128 && NC % 16 == 0 && K % 32 == 0, "32 units per wave, whole tiles");
+  const int r = lane & 31, half = lane >> 5;
+  const int u = 32 * wave + r;
+#pragma unroll
+  for (int s = 0; s < NC / 16; ++s) {
+    const int c0 = 16 * s + 8 * half;
+    V8 a;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[j] = D[act_chunk<H, PREC>(c0 + j, u / CE) + u % CE];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) db += (float)a[j];
+#pragma unroll
+    for (int n = 0; n < K / 32; ++n) {
+      const int k = 32 * n + r;
+      V8 b;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) b[j] = A[act_chunk<H, PREC>(c0 + j, k / CE) + k % CE];
+      if constexpr (PREC == kBF16)
+        dw[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, dw[n], 0, 0, 0);
+      else
+        dw[n] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, dw[n], 0, 0, 0);
+    }
+  }
+}
+
+// fp32 form, v_mfma_f32_16x16x4f32: two 16-unit tiles per wave; A operand =
+// D[c0 + l/16][32w + 16mt + l%16], B operand = A_l[c0 + l/16][16n + l%16];
+// accumulator i of tile (mt, n) holds dW[32w + 16mt + 4(l/16) + i][16n + l%16].
+template <int H, int NC, int K>
+__device__ __forceinline__ void accum_dw32(const float* D, const float* A,
+                                           f32x4 (&dw)[2][K / 16], float (&db)[2], int wave,
+                                           int lane) {
+  static_assert(H == 128 && NC % 4 == 0 && K % 16 == 0, "32 units per wave, whole tiles");
+  const int r = lane & 15, q = lane >> 4;
+#pragma unroll
+  for (int s = 0; s < NC / 4; ++s) {
+    const int c = 4 * s + q;
+    float a[2];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+      const int u = 32 * wave + 16 * mt + r;
+      a[mt] = D[act_chunk<H, kF32>(c, u / 4) + u % 4];
+      db[mt] += a[mt];
+    }
+#pragma unroll
+    for (int n = 0; n < K / 16; ++n) {
+      const int k = 16 * n + r;
+      const float b = A[act_chunk<H, kF32>(c, k / 4) + k % 4];
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt)
+        dw[mt][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[mt], b, dw[mt][n], 0, 0, 0);
+    }
+  }
+}
+
+// Per-wave register accumulators of the four layers' weight gradients.
+template <int F, int H, int PREC> struct DenseAcc;
+template <int F, int H> struct DenseAcc<F, H, kF32> {
+  static constexpr int K0 = F + kPeSlots;
+  f32x4 w0[2][K0 / 16], w1[2][H / 16], w2[2][H / 16], w3[2][H / 16];
+  float b[4][2];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+#pragma unroll
+      for (int n = 0; n < K0 / 16; ++n) w0[mt][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int n = 0; n < H / 16; ++n) {
+        w1[mt][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+        w2[mt][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+        w3[mt][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+#pragma unroll
+      for (int l = 0; l < 4; ++l) b[l][mt] = 0.0f;
+    }
+  }
+  template <int NTW>
+  __device__ __forceinline__ void store_tiles(float* dst, const f32x4 (&w)[2][NTW], int K,
+                                              int wave, int lane) {
+    const int r = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int n = 0; n < NTW; ++n)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          dst[(size_t)(32 * wave + 16 * mt + 4 * q + i) * K + 16 * n + r] = w[mt][n][i];
+  }
+  // writes this wave's rows of the workgroup's partial dense gradient
+  __device__ __forceinline__ void store(float* gd, int wave, int lane) {
+    store_tiles(gd + dense_grad_offset(F, H, 0), w0, K0, wave, lane);
+    store_tiles(gd + dense_grad_offset(F, H, 1), w1, H, wave, lane);
+    store_tiles(gd + dense_grad_offset(F, H, 2), w2, H, wave, lane);
+    store_tiles(gd + dense_grad_offset(F, H, 3), w3, H, wave, lane);
+    const int r = lane & 15, q = lane >> 4;
+    float* gb = gd + dense_grad_offset(F, H, 4);
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt) {
+        float v = b[l][mt];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        if (q == 0) gb[l * H + 32 * wave + 16 * mt + r] = v;
+      }
+  }
+};
+template <int F, int H, int PREC> struct DenseAcc16 {
+  static constexpr int K0 = F + kPeSlots;
+  f32x16 w0[K0 / 32], w1[H / 32], w2[H / 32], w3[H / 32];
+  float b[4];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+#pragma unroll
+      for (int n = 0; n < K0 / 32; ++n) w0[n][i] = 0.0f;
+#pragma unroll
+      for (int n = 0; n < H / 32; ++n) {
+        w1[n][i] = 0.0f;
+        w2[n][i] = 0.0f;
+        w3[n][i] = 0.0f;
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < 4; ++l) b[l] = 0.0f;
+  }
+  template <int NTW>
+  __device__ __forceinline__ void store_tiles(float* dst, const f32x16 (&w)[NTW], int K,
+                                              int wave, int lane) {
+    const int r = lane & 31, half = lane >> 5;
+#pragma unroll
+    for (int n = 0; n < NTW; ++n)
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        dst[(size_t)(32 * wave + 8 * (i / 4) + 4 * half + i % 4) * K + 32 * n + r] = w[n][i];
+  }
+  __device__ __forceinline__ void store(float* gd, int wave, int lane) {
+    store_tiles(gd + dense_grad_offset(F, H, 0), w0, K0, wave, lane);
+    store_tiles(gd + dense_grad_offset(F, H, 1), w1, H, wave, lane);
+    store_tiles(gd + dense_grad_offset(F, H, 2), w2, H, wave, lane);
+    store_tiles(gd + dense_grad_offset(F, H, 3), w3, H, wave, lane);
+    const int r = lane & 31, half = lane >> 5;
+    float* gb = gd + dense_grad_offset(F, H, 4);
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      const float v = b[l] + __shfl_xor(b[l], 32, 64);
+      if (half == 0) gb[l * H + 32 * wave + r] = v;
+    }
+  }
+};
+template <int F, int H> struct DenseAcc<F, H, kBF16> : DenseAcc16<F, H, kBF16> {};
+template <int F, int H> struct DenseAcc<F, H, kF16> : DenseAcc16<F, H, kF16> {};
+
+// DW: weight gradients accumulated in registers (H = 128 forms) instead of
+// column matrices written for separate GEMMs.
+template <int F, int H, int PREC, int NS, bool DW>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 1)))
 render_backward_kernel(BackArgs ba) {
   typedef typename Tr<PREC>::E E;
   typedef SmemB<F, H, PREC, NS> L;
@@ -1222,6 +1403,8 @@ render_backward_kernel(BackArgs ba) {
 #pragma unroll
   for (int u = 0; u < NU; ++u) gsw[u] = 0.0f;
   float gsb = 0.0f;
+  [[maybe_unused]] DenseAcc<F, H, PREC> dacc;
+  if constexpr (DW) dacc.zero();
 
   const long long item_end = std::min<long long>(ba.item_end, (long long)*a.n_valid);
   for (long long item = ba.item_begin + blockIdx.x; item < item_end; item += gridDim.x) {
@@ -1347,7 +1530,7 @@ render_backward_kernel(BackArgs ba) {
         }
       }
       __syncthreads();
-      store_cols<H, PREC, NC, K0>(x0, ba.A0, col0, tid);
+      if constexpr (!DW) store_cols<H, PREC, NC, K0>(x0, ba.A0, col0, tid);
       if constexpr (F32) {
         f32x4 acc[MTW][NTL];
         // forward, keeping every layer's output
@@ -1382,17 +1565,28 @@ render_backward_kernel(BackArgs ba) {
         }
         write_vals<H, PREC, NTL>(dl, acc, nullptr, wave, lane);
         __syncthreads();
-        store_cols<H, PREC, NC, H>(dl, ba.D3, col0, tid);
-        store_cols<H, PREC, NC, H>(hb(0), ba.A1, col0, tid);
-        store_cols<H, PREC, NC, H>(hb(1), ba.A2, col0, tid);
-        store_cols<H, PREC, NC, H>(hb(2), ba.A3, col0, tid);
+        if constexpr (DW) {
+          accum_dw32<H, NC, H>(dl, hb(2), dacc.w3, dacc.b[3], wave, lane);
+        } else {
+          store_cols<H, PREC, NC, H>(dl, ba.D3, col0, tid);
+          store_cols<H, PREC, NC, H>(hb(0), ba.A1, col0, tid);
+          store_cols<H, PREC, NC, H>(hb(1), ba.A2, col0, tid);
+          store_cols<H, PREC, NC, H>(hb(2), ba.A3, col0, tid);
+        }
         void* Ds[3] = {ba.D2, ba.D1, ba.D0};
+#pragma unroll
         for (int l = 0; l < 3; ++l) {   // delta_{2-l} = W_{3-l}^T delta [h_{2-l} > 0]
           one_layer<H, PREC, NTL, H>(WT + l * lhe, dl, wave, lane, acc);
           __syncthreads();
           write_vals<H, PREC, NTL>(dl, acc, hb(2 - l), wave, lane);
           __syncthreads();
-          store_cols<H, PREC, NC, H>(dl, Ds[l], col0, tid);
+          if constexpr (DW) {
+            if (l == 0) accum_dw32<H, NC, H>(dl, hb(1), dacc.w2, dacc.b[2], wave, lane);
+            else if (l == 1) accum_dw32<H, NC, H>(dl, hb(0), dacc.w1, dacc.b[1], wave, lane);
+            else accum_dw32<H, NC, K0>(dl, x0, dacc.w0, dacc.b[0], wave, lane);
+          } else {
+            store_cols<H, PREC, NC, H>(dl, Ds[l], col0, tid);
+          }
         }
       } else {
         f32x16 acc[H / 128][NT32];
@@ -1426,17 +1620,28 @@ render_backward_kernel(BackArgs ba) {
         }
         write_vals16<H, PREC, NT32>(dl, acc, nullptr, wave, lane);
         __syncthreads();
-        store_cols<H, PREC, NC, H>(dl, ba.D3, col0, tid);
-        store_cols<H, PREC, NC, H>(hb(0), ba.A1, col0, tid);
-        store_cols<H, PREC, NC, H>(hb(1), ba.A2, col0, tid);
-        store_cols<H, PREC, NC, H>(hb(2), ba.A3, col0, tid);
+        if constexpr (DW) {
+          accum_dw16<H, PREC, NC, H>(dl, hb(2), dacc.w3, dacc.b[3], wave, lane);
+        } else {
+          store_cols<H, PREC, NC, H>(dl, ba.D3, col0, tid);
+          store_cols<H, PREC, NC, H>(hb(0), ba.A1, col0, tid);
+          store_cols<H, PREC, NC, H>(hb(1), ba.A2, col0, tid);
+          store_cols<H, PREC, NC, H>(hb(2), ba.A3, col0, tid);
+        }
         void* Ds[3] = {ba.D2, ba.D1, ba.D0};
+#pragma unroll
         for (int l = 0; l < 3; ++l) {
           one_layer16<H, PREC, NT32, H>(WT + l * lhe, dl, wave, lane, acc);
           __syncthreads();
           write_vals16<H, PREC, NT32>(dl, acc, hb(2 - l), wave, lane);
           __syncthreads();
-          store_cols<H, PREC, NC, H>(dl, Ds[l], col0, tid);
+          if constexpr (DW) {
+            if (l == 0) accum_dw16<H, PREC, NC, H>(dl, hb(1), dacc.w2, dacc.b[2], wave, lane);
+            else if (l == 1) accum_dw16<H, PREC, NC, H>(dl, hb(0), dacc.w1, dacc.b[1], wave, lane);
+            else accum_dw16<H, PREC, NC, K0>(dl, x0, dacc.w0, dacc.b[0], wave, lane);
+          } else {
+            store_cols<H, PREC, NC, H>(dl, Ds[l], col0, tid);
+          }
         }
       }
       __syncthreads();
@@ -1464,6 +1669,8 @@ render_backward_kernel(BackArgs ba) {
     }
   }
   if (tid == 0) gp[so.sigma_b] += gsb;
+  if constexpr (DW)
+    dacc.store(ba.gdense + (size_t)blockIdx.x * dense_grad_total(F, H), wave, lane);
 }
 
 // ---- Backward of the Hydra encoder + RGB head ------------------------------
@@ -2083,7 +2290,7 @@ extern "C" size_t sailor_render_backward_partials_floats(int F, int H) {
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     cus = 256;
-  return (size_t)cus * small_offsets(F, H).total;
+  return (size_t)cus * (small_offsets(F, H).total + dense_grad_total(F, H));
 }
 
 extern "C" int sailor_render_hydra_backward(const float* grad_rgb, int B, int R, int V, int F,
@@ -2116,8 +2323,11 @@ extern "C" int sailor_render_hydra_backward(const float* grad_rgb, int B, int R,
                                                   grad_tokens, partials, parts, grad_small, st);
 }
 
-template <int F, int H, int PREC>
-int sample_backward_launch(BackArgs& ba, int parts, float* grad_small, hipStream_t st) {
+namespace {
+
+template <int F, int H, int PREC, bool DW>
+int sample_backward_launch(BackArgs& ba, int parts, float* grad_small, float* grad_dense,
+                           hipStream_t st) {
   constexpr int NS = PREC == kF32 ? 4 : 8;
   constexpr SmallOffsets so = small_offsets(F, H);
   typedef SmemB<F, H, PREC, NS> L;
@@ -2125,7 +2335,7 @@ int sample_backward_launch(BackArgs& ba, int parts, float* grad_small, hipStream
   SAILOR_REQUIRE(ba.r.S % NS == 0 && ba.r.S <= kBackMaxS,
                  "render_sample_backward: S=%d must be a multiple of %d and at most %d",
                  ba.r.S, NS, kBackMaxS);
-  auto kern = render_backward_kernel<F, H, PREC, NS>;
+  auto kern = render_backward_kernel<F, H, PREC, NS, DW>;
   static bool init = false;
   if (!init) {
     SAILOR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -2141,36 +2351,61 @@ int sample_backward_launch(BackArgs& ba, int parts, float* grad_small, hipStream
   if (int err = sailor::check_launch("render_sample_backward")) return err;
   hipLaunchKernelGGL(sum_partials, dim3((so.total + 255) / 256), dim3(256), 0, st, ba.gpart,
                      parts, so.total, grad_small, 1);
-  return sailor::check_launch("render_sample_backward (sum)");
+  if (int err = sailor::check_launch("render_sample_backward (sum)")) return err;
+  if constexpr (DW) {
+    constexpr int nd = dense_grad_total(F, H);
+    hipLaunchKernelGGL(sum_partials, dim3((nd + 255) / 256), dim3(256), 0, st, ba.gdense, parts,
+                       nd, grad_dense, 0);
+    return sailor::check_launch("render_sample_backward (dense sum)");
+  }
+  return SAILOR_OK;
 }
 
-extern "C" int sailor_render_sample_backward(
-    const float* t, const float* rays, int B, int R, int S, const float* Kv, const float* RTv,
-    int V, int img_h, int img_w, const float* fmap, int Hf, int Wf, int F, const float* bbox,
-    int H, int precision, const void* w_dense, const void* w_dense_t, const float* b_dense,
-    const float* w_small, float t_thresh, const float* occ, const float* grad_tokens,
-    const void* workspace, size_t workspace_bytes, long long item_begin, long long item_end,
-    void* const* col_bufs, float* grad_small, float* partials, size_t partials_floats,
-    void* stream) {
+// Shared argument checks and dispatch of the two backward entry points:
+// column form (col_bufs given) or in-kernel weight gradients (grad_dense).
+int sample_backward_impl(const float* t, const float* rays, int B, int R, int S,
+                         const float* Kv, const float* RTv, int V, int img_h, int img_w,
+                         const float* fmap, int Hf, int Wf, int F, const float* bbox, int H,
+                         int precision, const void* w_dense, const void* w_dense_t,
+                         const float* b_dense, const float* w_small, float t_thresh,
+                         const float* occ, const float* grad_tokens, const void* workspace,
+                         size_t workspace_bytes, long long item_begin, long long item_end,
+                         void* const* col_bufs, float* grad_dense, float* grad_small,
+                         float* partials, size_t partials_floats, void* stream) {
+  const bool dw = col_bufs == nullptr;
   SAILOR_REQUIRE(B > 0 && R >= 0 && S > 0, "render_sample_backward: bad sizes");
   SAILOR_REQUIRE(V >= 1 && V <= kVP, "render_sample_backward: 1 to %d views supported", kVP);
   SAILOR_REQUIRE(workspace != nullptr &&
                      workspace_bytes >= sailor_render_workspace_bytes(B, R, F),
                  "render_sample_backward: workspace too small");
-  SAILOR_REQUIRE(col_bufs != nullptr && item_begin >= 0 && item_end >= item_begin,
-                 "render_sample_backward: bad column batch");
+  SAILOR_REQUIRE(item_begin >= 0 && item_end >= item_begin,
+                 "render_sample_backward: bad item range");
+  SAILOR_REQUIRE(!dw || grad_dense != nullptr, "render_sample_backward: no gradient output");
   if (!((F == 64 && H == 256) || (F == 32 && H == 128)))
     return sailor::set_error(SAILOR_ERR_ARG, "render_sample_backward: (F=%d, H=%d) not "
                              "instantiated", F, H);
+  if (dw && H != 128)
+    return sailor::set_error(SAILOR_ERR_ARG, "render_sample_backward_fused: in-kernel weight "
+                             "gradients need H = 128 (got %d); use the column form", H);
   if (precision < kF32 || precision > kF16)
     return sailor::set_error(SAILOR_ERR_ARG, "render_sample_backward: precision %d unsupported",
                              precision);
-  const size_t per = (size_t)small_offsets(F, H).total;
+  const size_t per = (size_t)small_offsets(F, H).total + (dw ? dense_grad_total(F, H) : 0);
   const int parts = (int)std::min<size_t>(partials_floats / per, 1024);
   SAILOR_REQUIRE(parts >= 1, "render_sample_backward: partial buffer too small");
   const long long n = (long long)B * R;
-  if (n == 0 || item_end == item_begin) return SAILOR_OK;
   BackArgs ba{};
+  ba.gpart = partials;
+  ba.gdense = partials + (size_t)parts * small_offsets(F, H).total;
+  hipStream_t st = sailor::as_stream(stream);
+  if (n == 0 || item_end == item_begin) {
+    if (!dw) return SAILOR_OK;
+    // no columns: the dense gradient is zero
+    const long long nd = dense_grad_total(F, H);
+    hipLaunchKernelGGL(fill_zero, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, st,
+                       grad_dense, nd);
+    return sailor::check_launch("render_sample_backward (zero)");
+  }
   RenderArgs& a = ba.r;
   a.t = t; a.rays = rays; a.Kv = Kv; a.RTv = RTv; a.fmap = fmap; a.bbox = bbox;
   a.wdense = w_dense; a.bdense = b_dense; a.wsmall = w_small;
@@ -2185,22 +2420,74 @@ extern "C" int sailor_render_sample_backward(
   ba.dtok = grad_tokens;
   ba.item_begin = item_begin;
   ba.item_end = item_end;
-  ba.A0 = col_bufs[0]; ba.A1 = col_bufs[1]; ba.A2 = col_bufs[2]; ba.A3 = col_bufs[3];
-  ba.D0 = col_bufs[4]; ba.D1 = col_bufs[5]; ba.D2 = col_bufs[6]; ba.D3 = col_bufs[7];
-  ba.gpart = partials;
-  hipStream_t st = sailor::as_stream(stream);
+  if (!dw) {
+    ba.A0 = col_bufs[0]; ba.A1 = col_bufs[1]; ba.A2 = col_bufs[2]; ba.A3 = col_bufs[3];
+    ba.D0 = col_bufs[4]; ba.D1 = col_bufs[5]; ba.D2 = col_bufs[6]; ba.D3 = col_bufs[7];
+  }
+  if (dw) {
+    switch (precision) {
+      case kF32: return sample_backward_launch<32, 128, kF32, true>(ba, parts, grad_small,
+                                                                    grad_dense, st);
+      case kBF16: return sample_backward_launch<32, 128, kBF16, true>(ba, parts, grad_small,
+                                                                      grad_dense, st);
+      default: return sample_backward_launch<32, 128, kF16, true>(ba, parts, grad_small,
+                                                                  grad_dense, st);
+    }
+  }
   if (F == 64) {
     switch (precision) {
-      case kF32: return sample_backward_launch<64, 256, kF32>(ba, parts, grad_small, st);
-      case kBF16: return sample_backward_launch<64, 256, kBF16>(ba, parts, grad_small, st);
-      default: return sample_backward_launch<64, 256, kF16>(ba, parts, grad_small, st);
+      case kF32: return sample_backward_launch<64, 256, kF32, false>(ba, parts, grad_small,
+                                                                     nullptr, st);
+      case kBF16: return sample_backward_launch<64, 256, kBF16, false>(ba, parts, grad_small,
+                                                                       nullptr, st);
+      default: return sample_backward_launch<64, 256, kF16, false>(ba, parts, grad_small,
+                                                                   nullptr, st);
     }
   }
   switch (precision) {
-    case kF32: return sample_backward_launch<32, 128, kF32>(ba, parts, grad_small, st);
-    case kBF16: return sample_backward_launch<32, 128, kBF16>(ba, parts, grad_small, st);
-    default: return sample_backward_launch<32, 128, kF16>(ba, parts, grad_small, st);
+    case kF32: return sample_backward_launch<32, 128, kF32, false>(ba, parts, grad_small,
+                                                                   nullptr, st);
+    case kBF16: return sample_backward_launch<32, 128, kBF16, false>(ba, parts, grad_small,
+                                                                     nullptr, st);
+    default: return sample_backward_launch<32, 128, kF16, false>(ba, parts, grad_small,
+                                                                 nullptr, st);
   }
+}
+
+}  // namespace
+
+extern "C" size_t sailor_render_dense_grad_floats(int F, int H) {
+  return (size_t)dense_grad_total(F, H);
+}
+
+extern "C" int sailor_render_sample_backward(
+    const float* t, const float* rays, int B, int R, int S, const float* Kv, const float* RTv,
+    int V, int img_h, int img_w, const float* fmap, int Hf, int Wf, int F, const float* bbox,
+    int H, int precision, const void* w_dense, const void* w_dense_t, const float* b_dense,
+    const float* w_small, float t_thresh, const float* occ, const float* grad_tokens,
+    const void* workspace, size_t workspace_bytes, long long item_begin, long long item_end,
+    void* const* col_bufs, float* grad_small, float* partials, size_t partials_floats,
+    void* stream) {
+  SAILOR_REQUIRE(col_bufs != nullptr, "render_sample_backward: no column buffers");
+  for (int i = 0; i < 8; ++i)
+    SAILOR_REQUIRE(col_bufs[i] != nullptr, "render_sample_backward: column buffer %d is null", i);
+  return sample_backward_impl(t, rays, B, R, S, Kv, RTv, V, img_h, img_w, fmap, Hf, Wf, F, bbox,
+                              H, precision, w_dense, w_dense_t, b_dense, w_small, t_thresh, occ,
+                              grad_tokens, workspace, workspace_bytes, item_begin, item_end,
+                              col_bufs, nullptr, grad_small, partials, partials_floats, stream);
+}
+
+extern "C" int sailor_render_sample_backward_fused(
+    const float* t, const float* rays, int B, int R, int S, const float* Kv, const float* RTv,
+    int V, int img_h, int img_w, const float* fmap, int Hf, int Wf, int F, const float* bbox,
+    int H, int precision, const void* w_dense, const void* w_dense_t, const float* b_dense,
+    const float* w_small, float t_thresh, const float* occ, const float* grad_tokens,
+    const void* workspace, size_t workspace_bytes, float* grad_dense, float* grad_small,
+    float* partials, size_t partials_floats, void* stream) {
+  return sample_backward_impl(t, rays, B, R, S, Kv, RTv, V, img_h, img_w, fmap, Hf, Wf, F, bbox,
+                              H, precision, w_dense, w_dense_t, b_dense, w_small, t_thresh, occ,
+                              grad_tokens, workspace, workspace_bytes, 0, (long long)B * R,
+                              nullptr, grad_dense, grad_small, partials, partials_floats, stream);
 }
 
 extern "C" int sailor_density_points(const float* points, int B, int N, const float* Kv,

@@ -33,6 +33,8 @@ class BackwardScratch:
         L = _lib.lib()
         self.F, self.H = F, H
         self.grad_tokens = torch.empty((B * R, 4, F), dtype=torch.float32, device=device)
+        self.grad_dense = torch.empty(L.sailor_render_dense_grad_floats(F, H),
+                                      dtype=torch.float32, device=device)
         self.grad_small = torch.empty(L.sailor_render_small_weights_floats(F, H),
                                       dtype=torch.float32, device=device)
         self.partials = torch.empty(L.sailor_render_backward_partials_floats(F, H),
@@ -114,6 +116,32 @@ def sample_backward(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, model, wt, occ, wor
     return dW, db
 
 
+def sample_backward_fused(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, model, wt, occ, workspace,
+                          grad_tokens, grad_small, partials, grad_dense, t_thres=1e-4):
+    """Compositing + MLP backward with the weight gradients accumulated
+    inside the kernel (H = 128 models); returns (dW [4] in kernel input
+    order, db [4]) as views of grad_dense and adds the occupancy-head
+    gradient to grad_small."""
+    B, R, S = t.shape
+    V = Kv.shape[1]
+    Hf, Wf, F = fmap_cl.shape[2], fmap_cl.shape[3], fmap_cl.shape[4]
+    H = model.H
+    from .model import PREC_IDS
+    call("sailor_render_sample_backward_fused", ptr(t), ptr(rays), B, R, S, ptr(Kv), ptr(RTv),
+         V, int(img_hw[0]), int(img_hw[1]), ptr(fmap_cl), Hf, Wf, F, ptr(bbox), H,
+         PREC_IDS[model.prec], ptr(model.w_dense), ptr(wt), ptr(model.b_dense),
+         ptr(model.w_small), float(t_thres), ptr(occ), ptr(grad_tokens), ptr(workspace),
+         workspace.numel(), ptr(grad_dense), ptr(grad_small), ptr(partials), partials.numel(),
+         stream_of(t.device))
+    K0 = F + 64
+    o1 = H * K0
+    dW = [grad_dense[:o1].view(H, K0)] + [grad_dense[o1 + l * H * H:o1 + (l + 1) * H * H]
+                                          .view(H, H) for l in range(3)]
+    ob = o1 + 3 * H * H
+    db = [grad_dense[ob + l * H:ob + (l + 1) * H] for l in range(4)]
+    return dW, db
+
+
 def natural_first_layer_grad(dW0, F):
     """Kernel-order first-layer gradient [H, F + 64] -> the model's [H, 63 + F]
     ([encoding | features])."""
@@ -132,12 +160,13 @@ class RenderTrainer:
     and _volume_render_torch.forward_occ (voxel_encoding_helper.py:166-186).
     """
 
-    def __init__(self, weights, prec="fp32", device="cuda", batch_items=4096):
+    def __init__(self, weights, prec="fp32", device="cuda", batch_items=16384, fused=True):
         from .render import PackedModel
         self.weights, self.prec, self.dev = weights, prec, torch.device(device)
         self.model = PackedModel(weights, prec, self.dev)
         self.wt = pack_transposed(weights, prec, self.dev)
         self.batch_items = batch_items
+        self.fused = fused
         self._out = None
         self._scratch = None
         self._ctx = None
@@ -156,9 +185,12 @@ class RenderTrainer:
     def backward(self, grad_rgb, keep_columns=False):
         """dL/d rgb [B, R, 3] -> dict of float32 gradients shaped like the
         model's parameters (dense_w, dense_b, sigma_w, sigma_b, hydra, rgb).
-        Synchronises once (to read the number of listed rays).  With
-        keep_columns, self.columns holds the last item batch's (A, D) column
-        matrices (column (item, sample, view), item in list order)."""
+        H = 128 models with `fused` accumulate the weight gradients inside the
+        kernel (no host synchronisation); otherwise the column form runs in
+        item batches followed by GEMMs (synchronises once to read the number
+        of listed rays).  With keep_columns (column form), self.columns holds
+        the last item batch's (A, D) column matrices (column (item, sample,
+        view), item in list order)."""
         if self._ctx is None:
             raise RuntimeError("backward() before forward()")
         t, rays, Kv, RTv, img_hw, fmap_cl, bbox, t_thres = self._ctx
@@ -171,14 +203,68 @@ class RenderTrainer:
         ws = self._out["workspace"]
         g = grad_rgb.reshape(B * R, 3).contiguous().float()
         hydra_backward(g, B, R, V, self.model, ws, self._scratch)
-        n_valid = int(ws[:4].view(torch.int32)[0].item())
-        cols = [] if keep_columns else None
-        dW, db = sample_backward(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, self.model, self.wt,
-                                 self._out["occ"], ws, self._scratch.grad_tokens,
-                                 self._scratch.grad_small, self._scratch.partials, n_valid,
-                                 t_thres, self.batch_items, columns=cols)
-        self.columns = cols
+        sc = self._scratch
+        if self.fused and H == 128 and not keep_columns:
+            dW, db = sample_backward_fused(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, self.model,
+                                           self.wt, self._out["occ"], ws, sc.grad_tokens,
+                                           sc.grad_small, sc.partials, sc.grad_dense, t_thres)
+            self.columns = None
+        else:
+            n_valid = int(ws[:4].view(torch.int32)[0].item())
+            cols = [] if keep_columns else None
+            dW, db = sample_backward(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, self.model,
+                                     self.wt, self._out["occ"], ws, sc.grad_tokens,
+                                     sc.grad_small, sc.partials, n_valid, t_thres,
+                                     self.batch_items, columns=cols)
+            self.columns = cols
         small = self.weights.split_small(self._scratch.grad_small)
         return dict(dense_w=[natural_first_layer_grad(dW[0], F)] + dW[1:], dense_b=db,
                     sigma_w=small["sigma_w"], sigma_b=small["sigma_b"], hydra=small["hydra"],
                     rgb=small["rgb"])
+
+
+class TrainingBatch:
+    """Rays of one training batch: n_targets target views (ring cameras
+    between the source views) of cfg.target^2 rays each, traced through the
+    tree of one fused frame (TSDF -> tree -> DDA -> samples -> sort) and
+    concatenated into one [1, n_targets * target^2] ray batch, as the
+    reference's training step batches target views (SRONet/BasicRenNet.py
+    forward_ray_intersect / forward_ray_nerf)."""
+
+    def __init__(self, cfg, n_targets=8, device="cuda"):
+        import numpy as np
+        from . import scenes
+        from .pipeline import FramePipeline
+        self.cfg = cfg
+        p = FramePipeline.synthetic(cfg, prec="fp32", device=device)
+        p.build()
+        nt = cfg.target * cfg.target
+        n = n_targets * nt
+        self.t = torch.empty((1, n, cfg.S), dtype=torch.float32, device=device)
+        self.rays = torch.empty((1, n, 6), dtype=torch.float32, device=device)
+        cams = scenes.ring_cameras(n_targets, cfg.target_fx, cfg.target, cfg.target,
+                                   phase=np.pi / n_targets)
+        cu = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32))
+        for i, cam in enumerate(cams):
+            p.Kt.copy_(cu(cam.K[None]))
+            p.Rt.copy_(cu(cam.R[None]))
+            p.Tt.copy_(cu(cam.T[None]))
+            p.trace()
+            self.t[0, i * nt:(i + 1) * nt].copy_(p.sorted[0])
+            self.rays[0, i * nt:(i + 1) * nt].copy_(p.rays[0])
+        torch.cuda.synchronize()
+        self.pipe = p
+        self.n_rays = n
+        self.n_valid = int((self.t[0, :, 0] >= 0).sum().item())
+        self.img_hw = (cfg.img, cfg.img)
+
+    def inputs(self):
+        """Positional arguments of RenderTrainer.forward."""
+        p = self.pipe
+        return (self.t, self.rays, p.K, p.RT, self.img_hw, p.fmap, p.bbox, p.zdir)
+
+
+def mse_grad(rgb, target, n_valid):
+    """dL/d rgb of L = mean over the valid rays' channels of (rgb - target)^2
+    (dropped rays render 0 in both)."""
+    return (rgb - target) * (2.0 / (3.0 * max(n_valid, 1)))
