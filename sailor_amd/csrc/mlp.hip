@@ -1195,6 +1195,30 @@ __host__ __device__ constexpr int dense_grad_offset(int F, int H, int l) {
   return l == 0 ? 0 : H * (F + kPeSlots) + (l - 1) * H * H;
 }
 
+// 8 consecutive columns c0 + 8(l/32) + 0..7 of feature f0 + l%32 of a 16-bit
+// activation buffer, as one lane's MFMA operand, with two transposing LDS
+// reads (ds_read_b64_tr_b16).  That instruction works on groups of 16 lanes:
+// lane L of a group supplies 4 consecutive 16-bit values, and lane i of the
+// group receives element i%4 of the values supplied by lanes 4j + i/4,
+// j = 0..3.  So lane L = 4j + q points at features f + 4q .. f + 4q + 3 of
+// column c + j, and lane i receives feature f + i of columns c .. c + 3.
+template <int H, int PREC>
+__device__ __forceinline__ typename Tr<PREC>::V8 col_operand(const typename Tr<PREC>::E* buf,
+                                                             int c0, int f0, int lane) {
+  typedef short s4 __attribute__((ext_vector_type(4)));
+  typedef short s8 __attribute__((ext_vector_type(8)));
+  typedef __attribute__((address_space(3))) s4 lds_s4;
+  const int L = lane & 15, g = lane >> 4;
+  const int f = f0 + 16 * (g & 1) + 4 * (L & 3);
+  const int c = c0 + 8 * (g >> 1) + (L >> 2);
+  const typename Tr<PREC>::E* p0 = buf + act_chunk<H, PREC>(c, f >> 3) + (f & 7);
+  const typename Tr<PREC>::E* p1 = buf + act_chunk<H, PREC>(c + 4, f >> 3) + (f & 7);
+  const s4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s4*)(p0));
+  const s4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s4*)(p1));
+  const s8 v = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_bit_cast(typename Tr<PREC>::V8, v);
+}
+
 // 16-bit forms, v_mfma_f32_32x32x16: A operand = D^T (lane: unit 32w + l%32,
 // columns c0 + 8(l/32) + j), B operand = A_l (columns c0 + 8(l/32) + j,
 // input 32n + l%32); accumulator r of tile n holds
@@ -1204,24 +1228,15 @@ __device__ __forceinline__ void accum_dw16(const typename Tr<PREC>::E* D,
                                            const typename Tr<PREC>::E* A,
                                            f32x16 (&dw)[K / 32], float& db, int wave, int lane) {
   typedef typename Tr<PREC>::V8 V8;
-  constexpr int CE = Tr<PREC>::CE;
   static_assert(H == 128 && NC % 16 == 0 && K % 32 == 0, "32 units per wave, whole tiles");
-  const int r = lane & 31, half = lane >> 5;
-  const int u = 32 * wave + r;
 #pragma unroll
   for (int s = 0; s < NC / 16; ++s) {
-    const int c0 = 16 * s + 8 * half;
-    V8 a;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) a[j] = D[act_chunk<H, PREC>(c0 + j, u / CE) + u % CE];
+    const V8 a = col_operand<H, PREC>(D, 16 * s, 32 * wave, lane);
 #pragma unroll
     for (int j = 0; j < 8; ++j) db += (float)a[j];
 #pragma unroll
     for (int n = 0; n < K / 32; ++n) {
-      const int k = 32 * n + r;
-      V8 b;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) b[j] = A[act_chunk<H, PREC>(c0 + j, k / CE) + k % CE];
+      const V8 b = col_operand<H, PREC>(A, 16 * s, 32 * n, lane);
       if constexpr (PREC == kBF16)
         dw[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, dw[n], 0, 0, 0);
       else
