@@ -1723,7 +1723,9 @@ __device__ __forceinline__ void accum_outer(const float* drows, const float* inr
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int q = 0; q < kTok / 4; ++q) acc += dr[q] * ir[q];
-    gp[e] += (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    // no-return atomic add: the workgroup's own slab, one writer per entry,
+    // so it only saves the load-to-store round trip of a read-modify-write
+    unsafeAtomicAdd(gp + e, (acc[0] + acc[1]) + (acc[2] + acc[3]));
   }
   if (gb) {
     for (int o = tid; o < O; o += kThreads) {
@@ -1731,25 +1733,33 @@ __device__ __forceinline__ void accum_outer(const float* drows, const float* inr
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int q = 0; q < kTok / 4; ++q) acc += dr[q];
-      gb[o] += (acc[0] + acc[1]) + (acc[2] + acc[3]);
+      unsafeAtomicAdd(gb + o, (acc[0] + acc[1]) + (acc[2] + acc[3]));
     }
   }
 }
 
 // rows[k][lane] = (add ? rows[k][lane] : 0) + sum_o W[o][k] d[o] for this wave's
-// NK outputs k0.. (W is [O][K], wave-uniform)
+// NK outputs k0.. (W is [O][K], wave-uniform).  The o loop is outermost so
+// each step reads the NK contiguous weights W[o][k0 .. k0 + NK) at once.
 template <int O, int K, int NK>
 __device__ __forceinline__ void token_dense_t(const float* __restrict__ W, const float (&d)[O],
                                               int k0, float* rows, const float* addrows,
                                               const float* maskrows, int lane) {
+  float acc[NK];
+#pragma unroll
+  for (int i = 0; i < NK; ++i) acc[i] = addrows ? addrows[(k0 + i) * kLdB + lane] : 0.0f;
+#pragma unroll
+  for (int o = 0; o < O; ++o) {
+    const float* wr = W + (size_t)o * K + k0;
+#pragma unroll
+    for (int i = 0; i < NK; ++i) acc[i] += wr[i] * d[o];
+  }
 #pragma unroll
   for (int i = 0; i < NK; ++i) {
     const int k = k0 + i;
-    float acc = addrows ? addrows[k * kLdB + lane] : 0.0f;
-#pragma unroll
-    for (int o = 0; o < O; ++o) acc += W[(size_t)o * K + k] * d[o];
-    if (maskrows && !(maskrows[k * kLdB + lane] > 0.0f)) acc = 0.0f;
-    rows[k * kLdB + lane] = acc;
+    float v = acc[i];
+    if (maskrows && !(maskrows[k * kLdB + lane] > 0.0f)) v = 0.0f;
+    rows[k * kLdB + lane] = v;
   }
 }
 
@@ -2343,7 +2353,7 @@ namespace {
 template <int F, int H, int PREC, bool DW>
 int sample_backward_launch(BackArgs& ba, int parts, float* grad_small, float* grad_dense,
                            hipStream_t st) {
-  constexpr int NS = PREC == kF32 ? 4 : 8;
+  constexpr int NS = PREC == kF32 ? 4 : (DW ? 16 : 8);
   constexpr SmallOffsets so = small_offsets(F, H);
   typedef SmemB<F, H, PREC, NS> L;
   static_assert(L::total <= kLdsBytes, "backward LDS");

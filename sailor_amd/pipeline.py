@@ -47,9 +47,11 @@ CONFIGS = {
     "c1": FrameConfig("c1", 4, 512, 600.0, 128, 8, 256, 300.0, 64, "c1"),
     "c2": FrameConfig("c2", 4, 1024, 1200.0, 256, 8, 512, 600.0, 48, "c2"),
     "c3": FrameConfig("c3", 4, 1024, 1200.0, 256, 8, 1024, 1200.0, 48, "c2"),
-    # differentiation form: the config-1 scene and model seen from 4 x 1024^2
-    # source views; its batch is 8 target views of 256^2 rays (train.py)
-    "c4": FrameConfig("c4", 4, 1024, 1200.0, 128, 8, 256, 300.0, 64, "c1"),
+    # differentiation form: the full-size scene (4 x 1024^2 views, 256^3 grid,
+    # 48 samples) with the hidden-128 model, whose weight gradients the
+    # backward kernel accumulates on chip; its batch is 8 target views of
+    # 256^2 rays (train.py)
+    "c4": FrameConfig("c4", 4, 1024, 1200.0, 256, 8, 256, 300.0, 48, "c1"),
 }
 
 

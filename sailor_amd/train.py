@@ -64,14 +64,26 @@ def pack_transposed(weights, prec, device):
     return t.to(device).contiguous()
 
 
-def _mm_t(D, A):
-    """D^T A with float32 output (16-bit inputs accumulate in fp32)."""
+def _mm_t(D, A, split=64):
+    """D^T A with float32 output over a long column dimension: the columns
+    are cut into `split` batches (one batched GEMM, then a sum) so the
+    reduction spreads over the whole device; 16-bit inputs accumulate in
+    fp32."""
+    C = D.shape[0]
+    s = split if C >= 64 * split else 1
+    c = C // s * s
+    Db = D[:c].reshape(s, c // s, D.shape[1]).transpose(1, 2)
+    Ab = A[:c].reshape(s, c // s, A.shape[1])
     if D.dtype == torch.float32:
-        return D.t().mm(A)
-    try:
-        return torch.mm(D.t(), A, out_dtype=torch.float32)
-    except (TypeError, RuntimeError):
-        return D.t().float().mm(A.float())
+        out = torch.bmm(Db, Ab).sum(0)
+    else:
+        try:
+            out = torch.bmm(Db, Ab, out_dtype=torch.float32).sum(0)
+        except (TypeError, RuntimeError):
+            out = torch.bmm(Db.float(), Ab.float()).sum(0)
+    if c < C:
+        out = out + _mm_t(D[c:], A[c:], 1)
+    return out
 
 
 def sample_backward(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, model, wt, occ, workspace,
@@ -97,6 +109,7 @@ def sample_backward(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, model, wt, occ, wor
     db = [torch.zeros(H, dtype=torch.float32, device=dev) for _ in range(4)]
     if F + 64 > H:
         raise RuntimeError(f"first-layer width {F + 64} exceeds the column stride {H}")
+    ones = torch.ones(cols, dtype=dt, device=dev)
     for i0 in range(0, n_valid, batch_items):
         i1 = min(i0 + batch_items, n_valid)
         call("sailor_render_sample_backward", ptr(t), ptr(rays), B, R, S, ptr(Kv), ptr(RTv), V,
@@ -110,7 +123,7 @@ def sample_backward(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, model, wt, occ, wor
         D = [bufs[4][:c], bufs[5][:c], bufs[6][:c], bufs[7][:c]]
         for l in range(4):
             dW[l] += _mm_t(D[l], A[l])
-            db[l] += D[l].float().sum(0)
+            db[l] += _mm_t(D[l], ones[:c, None])[:, 0]
         if columns is not None:
             columns[:] = [A, D]
     return dW, db
