@@ -326,7 +326,8 @@ int sailor_render_hydra_backward(const float* grad_rgb, int B, int R, int V, int
  * gradient to grad_small.  Inputs as sailor_render_rays plus w_dense_t
  * (the packed W_3^T, W_2^T, W_1^T), occ (the forward's per-sample
  * occupancy, [B*R, S]) and grad_tokens (sailor_render_hydra_backward).
- * fmap is fp32; S <= 64, multiple of 4 (fp32) or 8 (16-bit). */
+ * fmap is fp32; S <= 64, multiple of 4 (fp32) or 8 (16-bit; 16 for the
+ * fused variant below). */
 int sailor_render_sample_backward(const float* t, const float* rays, int B, int R, int S,
                                   const float* Kv, const float* RTv, int V, int img_h,
                                   int img_w, const float* fmap, int Hf, int Wf, int F,
@@ -337,6 +338,35 @@ int sailor_render_sample_backward(const float* t, const float* rays, int B, int 
                                   size_t workspace_bytes, long long item_begin,
                                   long long item_end, void* const* col_bufs, float* grad_small,
                                   float* partials, size_t partials_floats, void* stream);
+
+/* Floats of a dense-layer gradient buffer: dW_0 [H][F + 64] (first-layer
+ * inputs in kernel order [features | encoding | 0]), dW_1..3 [H][H],
+ * db [4][H]. */
+size_t sailor_render_dense_grad_floats(int F, int H);
+
+/* The same backward over all listed rays with the weight gradients summed
+ * inside the kernel (MFMA accumulators held across the workgroup's rays,
+ * one partial per workgroup, then a sum): writes grad_dense
+ * (sailor_render_dense_grad_floats(F, H) floats, overwritten) and adds the
+ * occupancy-head gradient to grad_small.  No column buffers and no host
+ * synchronisation.  H = 128 models only (the H = 256 gradient does not fit
+ * on chip; use the column form).  partials: at least
+ * sailor_render_backward_partials_floats(F, H) floats.  Together with
+ * sailor_render_hydra_backward this is the differentiation of the fused
+ * render that the reference obtains from torch autograd through
+ * SRONet/nerf.py:139-186 and _volume_render_torch.forward_occ
+ * (voxel_encoding_helper.py:166-186). */
+int sailor_render_sample_backward_fused(const float* t, const float* rays, int B, int R, int S,
+                                        const float* Kv, const float* RTv, int V, int img_h,
+                                        int img_w, const float* fmap, int Hf, int Wf, int F,
+                                        const float* bbox, int H, int precision,
+                                        const void* w_dense, const void* w_dense_t,
+                                        const float* b_dense, const float* w_small,
+                                        float t_thresh, const float* occ,
+                                        const float* grad_tokens, const void* workspace,
+                                        size_t workspace_bytes, float* grad_dense,
+                                        float* grad_small, float* partials,
+                                        size_t partials_floats, void* stream);
 
 /* Density query at arbitrary points: replaces infer_density_mlp
  * (c_lib/FastNerf/src/infer_mlp.cu:583-634, the GUI's occupancy query after

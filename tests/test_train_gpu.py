@@ -259,3 +259,61 @@ def test_fused_weight_gradients_match_column_form(prec):
             if e > 1e-5:
                 bad.append((key, l, e))
     assert not bad, bad
+
+
+def test_fused_gradients_match_central_differences():
+    """Fused fp32 gradients of L = sum(g . rgb) vs central finite differences
+    of the float64 restatement on 12 random parameters: dense weights and
+    biases of every layer, occupancy head, Hydra and RGB head.  h = 1e-5
+    rather than 1e-3: a bias step moves that unit's pre-activation in every
+    column, and with h = 1e-3 the ReLU kinks crossed inside +-h bias the
+    difference by several percent; in float64 the smaller step costs no
+    accuracy.  Bar: |fd - grad| <= 2e-3 * max|grad| of the tensor + 1e-7 (a
+    column whose float32 pre-activation has the other sign than the float64
+    one contributes a different ReLU derivative; the per-column test above
+    bounds those columns)."""
+    from sailor_amd.model import NerfWeights
+    p, w, tr, out, sub, g, img_hw = _trainer_case("fp32", 100000)
+    sub = sub[:48]
+    g2 = np.zeros_like(g)
+    g2[0, sub] = g[0, sub]
+    grads = tr.backward(torch.from_numpy(g2).cuda())
+    torch.cuda.synchronize()
+    args = (p.sorted[:, sub].cpu().numpy(), p.rays[:, sub].cpu().numpy(), p.K.cpu().numpy(),
+            p.RT.cpu().numpy(), p.fmaps_np[None], p.bbox.cpu().numpy(), p.zdir.cpu().numpy(),
+            img_hw)
+    gt = torch.from_numpy(g2[:, sub]).double()
+
+    def loss(P):
+        with torch.no_grad():
+            return float((nerf_ref.render_rays(w, *args, params=P)[0] * gt).sum())
+    rng = np.random.default_rng(17)
+    picks = [("dense_w", 0), ("dense_w", 1), ("dense_w", 2), ("dense_w", 3), ("dense_b", 1),
+             ("dense_b", 3), ("sigma_w", None), ("hydra0", "q"), ("hydra1", "w2"),
+             ("hydra1", "b1"), ("rgb", 0), ("rgb", 2)]
+    bad = []
+    for name, sel in picks:
+        P = nerf_ref.params_of(w)
+        if name in ("dense_w", "dense_b"):
+            ref_t, got_t = P[name][sel], grads[name][sel]
+        elif name == "sigma_w":
+            ref_t, got_t = P["sigma_w"], grads["sigma_w"]
+        elif name.startswith("hydra"):
+            s = int(name[-1])
+            ref_t, got_t = P["hydra"][s][sel], grads["hydra"][s][sel]
+        else:
+            ref_t, got_t = P["rgb"][sel][0], grads["rgb"][sel][0]
+        got_t = got_t.detach().double().cpu().numpy()
+        idx = tuple(int(rng.integers(0, n)) for n in ref_t.shape)
+        h = 1e-5
+        orig = float(ref_t[idx])
+        ref_t[idx] = orig + h
+        lp = loss(P)
+        ref_t[idx] = orig - h
+        lm = loss(P)
+        ref_t[idx] = orig
+        fd = (lp - lm) / (2 * h)
+        tol = 2e-3 * np.abs(got_t).max() + 1e-7
+        if abs(fd - got_t[idx]) > tol:
+            bad.append((name, sel, idx, fd, float(got_t[idx]), tol))
+    assert not bad, bad
