@@ -214,7 +214,7 @@ def training_form(steps, warmup, with_cpu=True):
     if with_cpu:
         from oracle import nerf_ref
         valid = torch.nonzero(tb.t[0, :, 0] >= 0).flatten().cpu().numpy()
-        sub = valid[:: max(1, valid.size // 64)][:64]
+        sub = valid[:: max(1, valid.size // 2048)][:2048]
         p = tb.pipe
         P = nerf_ref.params_of(w, dtype=torch.float32, requires_grad=True)
         args = (tb.t[:, sub].cpu().numpy(), tb.rays[:, sub].cpu().numpy(), p.K.cpu().numpy(),
