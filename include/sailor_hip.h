@@ -393,6 +393,51 @@ int sailor_freq_encode_forward(const float* x, int N, int D, int deg, float* y,
 int sailor_freq_encode_backward(const float* grad_y, const float* y, int N,
                                 int D, int deg, float* grad_x, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* FastNerf inference surface (InferNerf, c_lib/FastNerf/src/binding.cpp)   */
+/* ------------------------------------------------------------------------ */
+/* Parameters are the reference's packed half vectors (gui/gui_render.py:
+ * 658-685): density 22521 (feature MLP 18->64x4 | occupancy MLP 64->64->64->1
+ * at 13696 | geometry MLP 18->16->8 at 22081), colour 15807 (two Hydra stacks
+ * of 5822 | RGB MLP 30->64->32->3 at 11644), blending 15075; each linear
+ * layer is W [out][in] row-major then b [out].  Arithmetic is fp32 on the
+ * half parameters (the reference accumulates in half). */
+
+/* Workspace of sailor_infer_nerf: per-sample occupancy + the texture views'
+ * 8 geometry features per sample. */
+size_t sailor_infer_nerf_workspace_bytes(int B, int R, int S, int tex_views);
+
+/* Replaces infer_fast_rendering (c_lib/FastNerf/src/infer_mlp.cu:638-789),
+ * bound as InferNerf.infer_nerf (binding.cpp:8).
+ *  sorted_dists f32 [B,R,S]; per view, in (batch, view, ray, sample) order:
+ *  points_z, points_tpsdf f32 [B*Vg*R*S], feats_geo f32 [B*Vg*R*S,16];
+ *  rgbs, dirs f32 [B*Vt*R*S,3], feats_rgb f32 [B*Vt*R*S,16];
+ *  1 <= tex_views <= min(geo_views, 4).
+ *  out_depth [B,R] = sum_i w_i t_i, out_rgb [B,R,3], out_feats [B,R,30]
+ *  (the first view's token after the Hydra stacks).  Kept from the
+ *  reference: q and k normalised without epsilon (a ray with no samples gets
+ *  0/0), samples with t < 0 skipped, rays whose last t < 0 composite to 0,
+ *  compositing stops after T < 1e-4. */
+int sailor_infer_nerf(const float* sorted_dists, const float* points_z,
+                      const float* points_tpsdf, const float* feats_geo, const float* rgbs,
+                      const float* feats_rgb, const float* dirs, const void* params_density,
+                      const void* params_color, int B, int R, int S, int geo_views,
+                      int tex_views, float* out_depth, float* out_rgb, float* out_feats,
+                      void* workspace, size_t workspace_bytes, void* stream);
+
+/* Replaces infer_density_mlp (infer_mlp.cu:583-634), bound as
+ * InferNerf.infer_density: points_z, points_tpsdf [B*V*N], feats_geo
+ * [B*V*N,16] in (batch, view, point) order -> out [B,N] occupancy. */
+int sailor_infer_density(const float* points_z, const float* points_tpsdf,
+                         const float* feats_geo, const void* params_density, int B, int N,
+                         int V, float* out, void* stream);
+
+/* Replaces infer_upsampling (infer_mlp.cu:793-821), bound as
+ * InferNerf.infer_blending: feats_neighbors [N,40], ray_feats [N,30] ->
+ * out [N,3] blend weights. */
+int sailor_infer_blending(const float* feats_neighbors, const float* ray_feats,
+                          const void* params_blend, int N, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,10 @@ _SIGS = {
     "sailor_density_points": [P, I, I, P, P, I, I, I, P, I, I, I, I, P, I, I, P, P, P, I, P, P],
     "sailor_render_rays_stages": [I, P, P, I, I, I, P, P, I, I, I, P, I, I, I, I, P, P, I, I, P,
                                   P, P, F, I, I, P, P, P, P, P, SZ, P],
+    "sailor_infer_nerf_workspace_bytes": [I, I, I, I],
+    "sailor_infer_nerf": [P, P, P, P, P, P, P, P, P, I, I, I, I, I, P, P, P, P, SZ, P],
+    "sailor_infer_density": [P, P, P, P, I, I, I, P, P],
+    "sailor_infer_blending": [P, P, P, I, P, P],
     "sailor_freq_encode_forward": [P, I, I, I, P, P],
     "sailor_freq_encode_backward": [P, P, I, I, I, P, P],
 }
@@ -61,6 +65,7 @@ _RESTYPES = {
     "sailor_render_workspace_bytes": SZ,
     "sailor_render_backward_partials_floats": SZ,
     "sailor_render_dense_grad_floats": SZ,
+    "sailor_infer_nerf_workspace_bytes": SZ,
 }
 
 _lib = None

@@ -1,0 +1,406 @@
+// FastNerf inference surface (InferNerf.infer_nerf / infer_density /
+// infer_blending, c_lib/FastNerf/src/binding.cpp) on MI355X.
+//
+// The reference evaluates its small MLPs one point per thread in half
+// precision with 128-entry per-thread half arrays and per-call stream
+// creation (infer_mlp.cu:120-506, 638-789).  Here:
+//  * every parameter group is staged once per workgroup into LDS as the
+//    given half values; all arithmetic is fp32 (v_fma_mix: half weight x
+//    fp32 activation), so the result is the exact value of the half
+//    parameters to fp32 rounding rather than a half-accumulated one;
+//  * one thread per (point, all views): the per-view feature MLP, the view
+//    mean and the occupancy MLP run back to back, the geometry features of
+//    the texture views go to a workspace, no [N, 64] intermediate;
+//  * one thread per (ray, texture view): compositing of the 30-channel
+//    features along the ray, then the two Hydra stacks with the cross-view
+//    sums as 2 lane shuffles (4 lanes per ray) and the RGB head on the lane
+//    of view 0; no host synchronisation, everything on the caller's stream.
+#include "common.h"
+
+namespace {
+
+typedef _Float16 half_t;
+constexpr int kBlock = 256;
+constexpr int kDensityParams = 22521, kFeatParams = 13696, kGeoOffset = 22081;
+constexpr int kColorParams = 15807, kStackParams = 5822, kRgbOffset = 11644;
+constexpr int kBlendParams = 15075;
+constexpr int kTexSlots = 4;  // texture views per ray (the reference's NUM_VIEWS)
+
+// out[o] = act(b[o] + sum_i W[o][i] in[i]); W [OUT][IN] then b [OUT] at w
+// (half, LDS), in/out fp32 registers.  ACT: 0 none, 1 relu, 2 sigmoid.
+template <int IN, int OUT, int ACT, bool BIAS = true>
+__device__ __forceinline__ void dense(const half_t* __restrict__ w, const float (&in)[IN],
+                                      float (&out)[OUT]) {
+#pragma unroll
+  for (int o = 0; o < OUT; ++o) {
+    float acc = BIAS ? (float)w[OUT * IN + o] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < IN; ++i) acc = fmaf((float)w[o * IN + i], in[i], acc);
+    if (ACT == 1) acc = fmaxf(acc, 0.0f);
+    if (ACT == 2) acc = 1.0f / (1.0f + expf(-acc));
+    out[o] = acc;
+  }
+}
+template <int IN, int OUT>
+__device__ __forceinline__ constexpr int dense_size(bool bias = true) {
+  return IN * OUT + (bias ? OUT : 0);
+}
+
+// The same with the outputs written to this thread's LDS row (o is a
+// runtime loop, so only the input vector and one output live in registers;
+// unrolling both loops makes the compiler hoist thousands of weight loads).
+template <int IN, int OUT, int ACT, bool BIAS = true>
+__device__ __forceinline__ void dense_row(const half_t* __restrict__ w, const float (&in)[IN],
+                                          float* row) {
+#pragma unroll 2
+  for (int o = 0; o < OUT; ++o) {
+    float acc = BIAS ? (float)w[OUT * IN + o] : 0.0f;
+    const half_t* wr = w + o * IN;
+#pragma unroll
+    for (int i = 0; i < IN; ++i) acc = fmaf((float)wr[i], in[i], acc);
+    if (ACT == 1) acc = fmaxf(acc, 0.0f);
+    if (ACT == 2) acc = 1.0f / (1.0f + expf(-acc));
+    row[o] = acc;
+  }
+}
+template <int N>
+__device__ __forceinline__ void from_row(const float* row, float (&out)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) out[i] = row[i];
+}
+
+// Per-thread LDS rows: kRow floats each (odd stride: lane t's element o sits
+// in bank (t + o) mod 64, so same-o accesses of a wave are conflict-free).
+constexpr int kRow = 65;
+constexpr size_t rows_bytes() { return (size_t)kBlock * kRow * sizeof(float); }
+
+__device__ __forceinline__ void stage(half_t* dst, const half_t* __restrict__ src, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+  __syncthreads();
+}
+
+// Per point p (flattened (b, point)): feature MLP of every view (inputs z,
+// tpsdf, geo[16] at (b, v, point)), mean over views, occupancy MLP -> occ[p];
+// with geo_out, the geometry MLP of views < n_geo_out -> geo_out[(b, v,
+// point)][8] (infer_mlp.cu:120-240, 583-634, 681-712).
+__global__ void __launch_bounds__(kBlock)
+density_kernel(const half_t* __restrict__ params, const float* __restrict__ z,
+               const float* __restrict__ tpsdf, const float* __restrict__ geo, int B,
+               long long n_pts, int V, float* __restrict__ occ, float* __restrict__ geo_out,
+               int n_geo_out) {
+  extern __shared__ __attribute__((aligned(16))) char smem_i[];
+  float* row = reinterpret_cast<float*>(smem_i) + threadIdx.x * kRow;
+  half_t* w = reinterpret_cast<half_t*>(smem_i + rows_bytes());
+  stage(w, params, kDensityParams);
+  const long long total = (long long)B * n_pts;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < total;
+       p += (long long)gridDim.x * blockDim.x) {
+    const long long b = p / n_pts, q = p - b * n_pts;
+    float sum[64];
+#pragma unroll
+    for (int i = 0; i < 64; ++i) sum[i] = 0.0f;
+    for (int v = 0; v < V; ++v) {
+      const long long e = (b * V + v) * n_pts + q;
+      float x[18];
+      x[0] = z[e];
+      x[1] = tpsdf[e];
+      const float4* g4 = reinterpret_cast<const float4*>(geo + e * 16);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const float4 t = g4[c];
+        x[2 + 4 * c] = t.x;
+        x[3 + 4 * c] = t.y;
+        x[4 + 4 * c] = t.z;
+        x[5 + 4 * c] = t.w;
+      }
+      float h[64];
+      const half_t* wp = w;
+      dense_row<18, 64, 1>(wp, x, row);
+      from_row(row, h);
+      wp += dense_size<18, 64>();
+#pragma unroll 1
+      for (int l = 0; l < 3; ++l) {
+        dense_row<64, 64, 1>(wp, h, row);
+        from_row(row, h);
+        wp += dense_size<64, 64>();
+      }
+#pragma unroll
+      for (int i = 0; i < 64; ++i) sum[i] += h[i];
+      if (geo_out && v < n_geo_out) {
+        float g16[16], g8[8];
+        dense<18, 16, 1>(w + kGeoOffset, x, g16);
+        dense<16, 8, 1>(w + kGeoOffset + dense_size<18, 16>(), g16, g8);
+        float4* o4 = reinterpret_cast<float4*>(geo_out + ((b * n_geo_out + v) * n_pts + q) * 8);
+        o4[0] = make_float4(g8[0], g8[1], g8[2], g8[3]);
+        o4[1] = make_float4(g8[4], g8[5], g8[6], g8[7]);
+      }
+    }
+    const float inv = 1.0f / (float)V;
+#pragma unroll
+    for (int i = 0; i < 64; ++i) sum[i] *= inv;
+    float h[64], o[1];
+    const half_t* wp = w + kFeatParams;
+    dense_row<64, 64, 1>(wp, sum, row);
+    from_row(row, h);
+    wp += dense_size<64, 64>();
+    dense_row<64, 64, 1>(wp, h, row);
+    from_row(row, h);
+    wp += dense_size<64, 64>();
+    dense<64, 1, 2>(wp, h, o);
+    occ[p] = o[0];
+  }
+}
+
+// One lane per (ray, texture view): composite the view's 30 features
+// [rgb feats 16 | rgb 3 | geometry features 8 | direction 3] with the
+// occupancies (volume_rendering_occ_kernel, infer_mlp.cu:508-581), then the
+// Hydra stacks over the ray's views and the RGB head (hydra_attention_kernel,
+// :243-445).  Lanes 4r .. 4r+3 of a wave hold ray r's views.
+__global__ void __launch_bounds__(kBlock)
+color_kernel(const half_t* __restrict__ params, const float* __restrict__ t,
+             const float* __restrict__ occ, const float* __restrict__ feats_rgb,
+             const float* __restrict__ rgbs, const float* __restrict__ geo8,
+             const float* __restrict__ dirs, int B, int R, int S, int Vt, float t_thres,
+             float* __restrict__ out_depth, float* __restrict__ out_rgb,
+             float* __restrict__ out_feats) {
+  extern __shared__ __attribute__((aligned(16))) char smem_i[];
+  float* row = reinterpret_cast<float*>(smem_i) + threadIdx.x * kRow;
+  half_t* w = reinterpret_cast<half_t*>(smem_i + rows_bytes());
+  stage(w, params, kColorParams);
+  const long long n_rays = (long long)B * R;
+  const long long lanes = n_rays * kTexSlots;
+  // whole waves iterate together (shuffles need every lane of a ray)
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < lanes;
+       base += (long long)gridDim.x * blockDim.x) {
+    const long long id = base + threadIdx.x;
+    const long long ray = id / kTexSlots;
+    const int v = (int)(id % kTexSlots);
+    const bool live = ray < n_rays && v < Vt;
+    float x[30];
+#pragma unroll
+    for (int c = 0; c < 30; ++c) x[c] = 0.0f;
+    float depth = 0.0f;
+    if (live) {
+      const long long b = ray / R, r = ray - b * R;
+      const float* tr = t + ray * S;
+      const float* oc = occ + ray * S;
+      if (!(tr[S - 1] < 0.0f)) {
+        const long long e0 = ((b * Vt + v) * R + r) * S;
+        float T = 1.0f;
+        for (int s = 0; s < S; ++s) {
+          const float ts = tr[s];
+          if (ts < 0.0f) continue;
+          const float a = oc[s];
+          const float wgt = a * T;
+          const long long e = e0 + s;
+          const float4* f4 = reinterpret_cast<const float4*>(feats_rgb + e * 16);
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const float4 f = f4[c];
+            x[4 * c + 0] += wgt * f.x;
+            x[4 * c + 1] += wgt * f.y;
+            x[4 * c + 2] += wgt * f.z;
+            x[4 * c + 3] += wgt * f.w;
+          }
+#pragma unroll
+          for (int c = 0; c < 3; ++c) x[16 + c] += wgt * rgbs[e * 3 + c];
+          const float4* g4 = reinterpret_cast<const float4*>(geo8 + e * 8);
+          const float4 ga = g4[0], gb = g4[1];
+          x[19] += wgt * ga.x; x[20] += wgt * ga.y; x[21] += wgt * ga.z; x[22] += wgt * ga.w;
+          x[23] += wgt * gb.x; x[24] += wgt * gb.y; x[25] += wgt * gb.z; x[26] += wgt * gb.w;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) x[27 + c] += wgt * dirs[e * 3 + c];
+          depth += wgt * ts;
+          T *= 1.0f - a;
+          if (T < t_thres) break;
+        }
+      }
+    }
+    // ---- Hydra stacks (all lanes take part in the shuffles) --------------
+    for (int st = 0; st < 2; ++st) {
+      const half_t* ws = w + st * kStackParams;
+      float q[32], k[32], vv[32];
+      dense_row<30, 32, 0, false>(ws, x, row);
+      from_row(row, q);
+      dense_row<30, 32, 0, false>(ws + 960, x, row);
+      from_row(row, k);
+      dense_row<30, 32, 0, false>(ws + 1920, x, row);
+      from_row(row, vv);
+      float qq = 0.0f, kk = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 32; ++i) {
+        qq += q[i] * q[i];
+        kk += k[i] * k[i];
+      }
+      // no epsilon, as the reference (a zero token gives 0/0)
+      const float rq = 1.0f / sqrtf(qq), rk = 1.0f / sqrtf(kk);
+#pragma unroll
+      for (int i = 0; i < 32; ++i) {
+        float kv = live ? k[i] * rk * vv[i] : 0.0f;
+        kv += __shfl_xor(kv, 1, 64);
+        kv += __shfl_xor(kv, 2, 64);
+        q[i] = q[i] * rq * kv;
+      }
+      float o[30], res[30], h[32];
+      dense_row<32, 30, 0, false>(ws + 2880, q, row);
+      from_row(row, o);
+#pragma unroll
+      for (int c = 0; c < 30; ++c) res[c] = x[c] + o[c];
+      dense_row<30, 32, 1>(ws + 3840, res, row);
+      from_row(row, h);
+      dense_row<32, 30, 0>(ws + 3840 + dense_size<30, 32>(), h, row);
+      from_row(row, x);
+#pragma unroll
+      for (int c = 0; c < 30; ++c) x[c] += res[c];
+    }
+    if (live && v == 0) {
+      float h0[64], h1[32], rgb[3];
+      const half_t* wr = w + kRgbOffset;
+      dense_row<30, 64, 1>(wr, x, row);
+      from_row(row, h0);
+      dense_row<64, 32, 1>(wr + dense_size<30, 64>(), h0, row);
+      from_row(row, h1);
+      dense<32, 3, 2>(wr + dense_size<30, 64>() + dense_size<64, 32>(), h1, rgb);
+      out_depth[ray] = depth;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) out_rgb[ray * 3 + c] = rgb[c];
+#pragma unroll
+      for (int c = 0; c < 30; ++c) out_feats[ray * 30 + c] = x[c];
+    }
+  }
+}
+
+// upsampling_kernel (infer_mlp.cu:449-506): [N, 40] neighbour features and
+// [N, 30] ray features -> [N, 3] blend weights.
+__global__ void __launch_bounds__(kBlock)
+blend_kernel(const half_t* __restrict__ params, const float* __restrict__ nf,
+             const float* __restrict__ rf, long long n, float* __restrict__ out) {
+  extern __shared__ __attribute__((aligned(16))) char smem_i[];
+  float* row = reinterpret_cast<float*>(smem_i) + threadIdx.x * kRow;
+  half_t* w = reinterpret_cast<half_t*>(smem_i + rows_bytes());
+  stage(w, params, kBlendParams);
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (long long)gridDim.x * blockDim.x) {
+    float x[40], h[64], c[62], h2[32], o[3];
+#pragma unroll
+    for (int i = 0; i < 40; ++i) x[i] = nf[p * 40 + i];
+    const half_t* wp = w;
+    dense_row<40, 64, 1>(wp, x, row);
+    from_row(row, h);
+    wp += dense_size<40, 64>();
+    dense_row<64, 64, 1>(wp, h, row);
+    from_row(row, h);
+    wp += dense_size<64, 64>();
+    dense_row<64, 32, 1>(wp, h, row);
+    from_row(row, h2);
+    wp += dense_size<64, 32>();
+#pragma unroll
+    for (int i = 0; i < 32; ++i) c[i] = h2[i];
+#pragma unroll
+    for (int i = 0; i < 30; ++i) c[32 + i] = rf[p * 30 + i];
+    dense_row<62, 64, 1>(wp, c, row);
+    from_row(row, h);
+    wp += dense_size<62, 64>();
+    dense_row<64, 32, 1>(wp, h, row);
+    from_row(row, h2);
+    wp += dense_size<64, 32>();
+    dense<32, 3, 2>(wp, h2, o);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[p * 3 + i] = o[i];
+  }
+}
+
+// Dynamic LDS of a kernel: per-thread rows + the parameter group (half).
+template <typename K>
+int lds_setup(K kern, size_t bytes) {
+  SAILOR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return SAILOR_OK;
+}
+constexpr size_t kDensityLds = rows_bytes() + kDensityParams * sizeof(half_t);
+constexpr size_t kColorLds = rows_bytes() + kColorParams * sizeof(half_t);
+constexpr size_t kBlendLds = rows_bytes() + kBlendParams * sizeof(half_t);
+
+int setup_all() {
+  static int done = -1;
+  if (done == SAILOR_OK) return SAILOR_OK;
+  if (int e = lds_setup(density_kernel, kDensityLds)) return e;
+  if (int e = lds_setup(color_kernel, kColorLds)) return e;
+  if (int e = lds_setup(blend_kernel, kBlendLds)) return e;
+  done = SAILOR_OK;
+  return SAILOR_OK;
+}
+
+unsigned grid_for(long long work) {
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus = 256;
+  const long long blocks = (work + kBlock - 1) / kBlock;
+  return (unsigned)std::max<long long>(1, std::min<long long>(blocks, (long long)cus * 8));
+}
+
+}  // namespace
+
+extern "C" size_t sailor_infer_nerf_workspace_bytes(int B, int R, int S, int tex_views) {
+  if (B <= 0 || R < 0 || S <= 0 || tex_views <= 0) return 0;
+  const size_t n = (size_t)B * R * S;
+  return (n * sizeof(float) + 255) / 256 * 256 + n * tex_views * 8 * sizeof(float);
+}
+
+extern "C" int sailor_infer_nerf(const float* sorted_dists, const float* points_z,
+                                 const float* points_tpsdf, const float* feats_geo,
+                                 const float* rgbs, const float* feats_rgb, const float* dirs,
+                                 const void* params_density, const void* params_color, int B,
+                                 int R, int S, int geo_views, int tex_views, float* out_depth,
+                                 float* out_rgb, float* out_feats, void* workspace,
+                                 size_t workspace_bytes, void* stream) {
+  SAILOR_REQUIRE(B > 0 && R >= 0 && S > 0, "infer_nerf: bad sizes B=%d R=%d S=%d", B, R, S);
+  SAILOR_REQUIRE(geo_views >= 1 && tex_views >= 1 && tex_views <= kTexSlots &&
+                     tex_views <= geo_views,
+                 "infer_nerf: need 1 <= tex_views <= min(geo_views, %d) (got %d, %d)", kTexSlots,
+                 tex_views, geo_views);
+  SAILOR_REQUIRE(workspace != nullptr &&
+                     workspace_bytes >= sailor_infer_nerf_workspace_bytes(B, R, S, tex_views),
+                 "infer_nerf: workspace too small");
+  const long long n_pts = (long long)R * S;
+  if (R == 0) return SAILOR_OK;
+  hipStream_t st = sailor::as_stream(stream);
+  float* occ = static_cast<float*>(workspace);
+  float* geo8 = reinterpret_cast<float*>(static_cast<char*>(workspace) +
+                                         ((size_t)B * n_pts * sizeof(float) + 255) / 256 * 256);
+  if (int err = setup_all()) return err;
+  hipLaunchKernelGGL(density_kernel, dim3(grid_for((long long)B * n_pts)), dim3(kBlock),
+                     kDensityLds, st,
+                     static_cast<const half_t*>(params_density), points_z, points_tpsdf,
+                     feats_geo, B, n_pts, geo_views, occ, geo8, tex_views);
+  if (int err = sailor::check_launch("infer_nerf (density)")) return err;
+  hipLaunchKernelGGL(color_kernel, dim3(grid_for((long long)B * R * kTexSlots)), dim3(kBlock),
+                     kColorLds, st, static_cast<const half_t*>(params_color), sorted_dists, occ, feats_rgb,
+                     rgbs, geo8, dirs, B, R, S, tex_views, 1e-4f, out_depth, out_rgb, out_feats);
+  return sailor::check_launch("infer_nerf (colour)");
+}
+
+extern "C" int sailor_infer_density(const float* points_z, const float* points_tpsdf,
+                                    const float* feats_geo, const void* params_density, int B,
+                                    int N, int V, float* out, void* stream) {
+  SAILOR_REQUIRE(B > 0 && N >= 0 && V >= 1, "infer_density: bad sizes B=%d N=%d V=%d", B, N, V);
+  if (N == 0) return SAILOR_OK;
+  if (int err = setup_all()) return err;
+  hipLaunchKernelGGL(density_kernel, dim3(grid_for((long long)B * N)), dim3(kBlock), kDensityLds,
+                     sailor::as_stream(stream), static_cast<const half_t*>(params_density),
+                     points_z, points_tpsdf, feats_geo, B, (long long)N, V, out, nullptr, 0);
+  return sailor::check_launch("infer_density");
+}
+
+extern "C" int sailor_infer_blending(const float* feats_neighbors, const float* ray_feats,
+                                     const void* params_blend, int N, float* out, void* stream) {
+  SAILOR_REQUIRE(N >= 0, "infer_blending: bad size N=%d", N);
+  if (N == 0) return SAILOR_OK;
+  if (int err = setup_all()) return err;
+  hipLaunchKernelGGL(blend_kernel, dim3(grid_for(N)), dim3(kBlock), kBlendLds,
+                     sailor::as_stream(stream),
+                     static_cast<const half_t*>(params_blend), feats_neighbors, ray_feats,
+                     (long long)N, out);
+  return sailor::check_launch("infer_blending");
+}
