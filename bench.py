@@ -234,6 +234,44 @@ def training_form(steps, warmup, with_cpu=True):
     return out
 
 
+def fastnerf_infer(steps):
+    """InferNerf.infer_nerf (FastNerf inference surface, packed half
+    parameters) on one GUI-sized call: 65,536 rays x 48 samples x 4 geometry
+    and 4 texture views, synthetic inputs.  Algorithmic FLOPs: per (view,
+    sample) 2*(18*64 + 3*64^2) feature MLP + 2*(18*16 + 16*8) geometry MLP;
+    per sample 2*(2*64^2 + 64) occupancy MLP; per (ray, view) two Hydra stacks
+    2*2*(3*30*32 + 32*30 + 30*32 + 32*30); per ray 2*(30*64 + 64*32 + 32*3)."""
+    import torch
+    from sailor_amd import InferNerf
+    from sailor_amd.model import fastnerf_params
+    B, R, S, Vg, Vt = 1, 65536, 48, 4, 4
+    pd, pc, _ = fastnerf_params()
+    dev = "cuda"
+    t = torch.sort(torch.rand((B, R, S), device=dev) * 2 + 1, -1)[0]
+    n_g, n_t = B * Vg * R * S, B * Vt * R * S
+    z = torch.rand((n_g, 1), device=dev) * 2 - 1
+    ps = torch.rand((n_g, 1), device=dev) * 2 - 1
+    geo = torch.randn((n_g, 16), device=dev)
+    rgb = torch.rand((n_t, 3), device=dev)
+    frgb = torch.randn((n_t, 16), device=dev)
+    dirs = torch.randn((n_t, 3), device=dev)
+    pdt = torch.from_numpy(pd).to(dev)
+    pct = torch.from_numpy(pc).to(dev)
+
+    def go():
+        return InferNerf.infer_nerf(t, z, ps, geo, rgb, frgb, dirs, pdt, pct, B, R, S, Vg, Vt, 0)
+    go()
+    torch.cuda.synchronize()
+    ms = _median_ms(go, steps)
+    flops = (Vg * R * S * 2 * (18 * 64 + 3 * 64 * 64) + Vt * R * S * 2 * (18 * 16 + 16 * 8)
+             + R * S * 2 * (2 * 64 * 64 + 64) + Vt * R * 2 * 2 * (3 * 30 * 32 + 3 * 32 * 30)
+             + R * 2 * (30 * 64 + 64 * 32 + 32 * 3))
+    return {"workload": f"infer_nerf: {R} rays x {S} samples, {Vg} geometry / {Vt} texture views",
+            "ms": ms, "rays_per_s": R / (ms / 1e3), "flops": flops,
+            "vector_tflops": flops / (ms / 1e3) / 1e12,
+            "frac_of_fp32_vector_peak": flops / (ms / 1e3) / 1e12 / 157.3}
+
+
 def psnr(a, b, mask):
     import numpy as np
     m = mask[..., None].astype(np.float64)
@@ -311,6 +349,10 @@ def main():
                     help="skip the 1024^2 precision-form block")
     ap.add_argument("--no-c4", action="store_true",
                     help="skip the differentiation-form (fwd+bwd) block")
+    ap.add_argument("--no-infer", action="store_true",
+                    help="skip the FastNerf inference-surface block")
+    ap.add_argument("--infer-only", action="store_true",
+                    help="run only the FastNerf inference-surface block and print it")
     ap.add_argument("--training-only", action="store_true",
                     help="run only the differentiation-form block and print it")
     ap.add_argument("--profile-kernels", action="store_true",
@@ -328,6 +370,9 @@ def main():
     torch.cuda.set_device(0)
     if args.profile_kernels:
         profile_kernels(cfg)
+        return
+    if args.infer_only:
+        print(json.dumps(fastnerf_infer(args.steps)))
         return
     if args.training_only:
         print(json.dumps(training_form(args.steps, args.warmup, with_cpu=not args.no_cpu)))
@@ -407,6 +452,8 @@ def main():
     if not args.no_c4 and args.workload == "c2":
         result["c4_training_form"] = training_form(max(3, args.steps // 4), args.warmup,
                                                    with_cpu=not args.no_cpu)
+    if not args.no_infer and args.workload == "c2":
+        result["fastnerf_infer"] = fastnerf_infer(max(3, args.steps // 4))
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(p32)
     else:

@@ -27,7 +27,9 @@ parameters, so it is the exact value the reference approximates.  Kept
 quirks: q and k are divided by their norm with no epsilon; compositing skips
 samples with negative depth, returns zeros for a ray whose last sample depth
 is negative, and stops after the sample that takes T below 1e-4; the depth
-output is sum_i w_i t_i.
+output is sum_i w_i t_i; ReLU is max(x, 0) with NaN -> 0, so a ray without
+samples (0/0 in the normalisation) gets NaN colour features but a finite
+colour.
 """
 from __future__ import annotations
 
@@ -95,6 +97,12 @@ def unpack_blend(p):
     return layers
 
 
+def _relu(x):
+    """max(x, 0.0f) as the kernels write it (infer_mlp.cu:42-45): like fmaxf it
+    returns 0 for NaN, where torch.relu would propagate the NaN."""
+    return torch.where(x > 0, x, torch.zeros_like(x))
+
+
 def _t(a, dtype):
     return torch.as_tensor(np.asarray(a, dtype=np.float64)).to(dtype)
 
@@ -119,16 +127,16 @@ def density_feats(z, tpsdf, geo, pd, dtype=torch.float64):
     x = torch.cat([_t(z, dtype)[:, None], _t(tpsdf, dtype)[:, None], _t(geo, dtype)], 1)
     h = x
     for wb in feats:
-        h = torch.relu(_lin(h, wb))
-    g = torch.relu(_lin(torch.relu(_lin(x, geo_l[0])), geo_l[1]))
+        h = _relu(_lin(h, wb))
+    g = _relu(_lin(_relu(_lin(x, geo_l[0])), geo_l[1]))
     return h, g
 
 
 def occupancy(mean_feats, pd, dtype=torch.float64):
     """density_sigma_kernel on [N, 64] view-averaged features -> [N]."""
     _, sigma, _ = unpack_density(_params(pd, dtype))
-    h = torch.relu(_lin(mean_feats, sigma[0]))
-    h = torch.relu(_lin(h, sigma[1]))
+    h = _relu(_lin(mean_feats, sigma[0]))
+    h = _relu(_lin(h, sigma[1]))
     return torch.sigmoid(_lin(h, sigma[2]))[:, 0]
 
 
@@ -182,11 +190,11 @@ def hydra_rgb(tokens, pc, dtype=torch.float64):
         kv = (k * v).sum(1, keepdim=True)
         out = (q * kv) @ st["fc"].T
         res = x + out
-        h = torch.relu(res @ st["w1"].T + st["b1"])
+        h = _relu(res @ st["w1"].T + st["b1"])
         x = h @ st["w2"].T + st["b2"] + res
     f = x[:, 0]
-    h = torch.relu(_lin(f, rgb[0]))
-    h = torch.relu(_lin(h, rgb[1]))
+    h = _relu(_lin(f, rgb[0]))
+    h = _relu(_lin(h, rgb[1]))
     return f, torch.sigmoid(_lin(h, rgb[2]))
 
 
@@ -219,10 +227,10 @@ def infer_blending(feats_neighbors, ray_feats, pb, dtype=torch.float64):
     """upsampling_kernel: [N, 40], [N, 30] -> blend weights [N, 3]."""
     L = unpack_blend(_params(pb, dtype))
     x = _t(feats_neighbors, dtype)
-    h = torch.relu(_lin(x, L[0]))
-    h = torch.relu(_lin(h, L[1]))
-    h = torch.relu(_lin(h, L[2]))
+    h = _relu(_lin(x, L[0]))
+    h = _relu(_lin(h, L[1]))
+    h = _relu(_lin(h, L[2]))
     h = torch.cat([h, _t(ray_feats, dtype)], 1)
-    h = torch.relu(_lin(h, L[3]))
-    h = torch.relu(_lin(h, L[4]))
+    h = _relu(_lin(h, L[3]))
+    h = _relu(_lin(h, L[4]))
     return torch.sigmoid(_lin(h, L[5]))

@@ -199,3 +199,28 @@ CONFIG_MODELS = {
     "c1": ModelConfig(F=32, H=128, Hf=128, Wf=128),
     "c2": ModelConfig(F=64, H=256, Hf=256, Wf=256),
 }
+
+
+def _packed_layers(rng, layers):
+    """Packed [W (out x in) row-major | b (out)] blocks in order, float16."""
+    parts = []
+    for n_in, n_out, bias in layers:
+        a = np.sqrt(6.0 / (n_in + n_out))
+        parts.append(rng.uniform(-a, a, n_out * n_in))
+        if bias:
+            parts.append(rng.normal(0, 0.05, n_out))
+    return np.concatenate(parts).astype(np.float16)
+
+
+def fastnerf_params(seed=11):
+    """Synthetic FastNerf parameter vectors in the reference's packed half
+    format (gui/gui_render.py:658-685): density (22521), colour (15807) and
+    blending (15075) halves, Xavier-uniform weights and N(0, 0.05) biases."""
+    rng = np.random.default_rng(seed)
+    pd = _packed_layers(rng, [(18, 64, 1)] + [(64, 64, 1)] * 3
+                        + [(64, 64, 1), (64, 64, 1), (64, 1, 1), (18, 16, 1), (16, 8, 1)])
+    stack = [(30, 32, 0)] * 3 + [(32, 30, 0), (30, 32, 1), (32, 30, 1)]
+    pc = _packed_layers(rng, stack + stack + [(30, 64, 1), (64, 32, 1), (32, 3, 1)])
+    pb = _packed_layers(rng, [(40, 64, 1), (64, 64, 1), (64, 32, 1), (62, 64, 1), (64, 32, 1),
+                              (32, 3, 1)])
+    return pd, pc, pb

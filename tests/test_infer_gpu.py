@@ -2,8 +2,8 @@
 drop-in for the reference's InferNerf module) against the float64
 restatement oracle/fastnerf_ref.py on seeded inputs with the reference's
 packed half parameters.  Tolerance: fp32 arithmetic vs float64 on the same
-half parameters, |d| <= 1e-4 + 1e-4 |ref| (NaN where the reference's
-epsilon-free normalisation gives 0/0, i.e. rays without samples)."""
+half parameters, |d| <= 1e-4 + 1e-4 |ref| (NaN colour features where the
+reference's epsilon-free normalisation gives 0/0: rays without samples)."""
 import numpy as np
 import pytest
 import torch
@@ -31,8 +31,12 @@ def test_infer_nerf_matches_float64(B, R, S, Vg, Vt):
     for got, ref in ((d, rd), (c, rc), (f, rf)):
         np.testing.assert_allclose(got.double().cpu().numpy(), ref.numpy(), rtol=1e-4,
                                    atol=1e-4, equal_nan=True)
+    # rays without samples: 0/0 in the normalisation gives NaN features; the
+    # colour head's max(x, 0) turns them into a finite colour, as in the
+    # reference
     dropped = t[..., -1] < 0
-    assert np.isnan(c.cpu().numpy()[dropped]).all() and not np.isnan(c.cpu().numpy()[~dropped]).any()
+    assert np.isnan(f.cpu().numpy()[dropped]).all()
+    assert not np.isnan(f.cpu().numpy()[~dropped]).any() and np.isfinite(c.cpu().numpy()).all()
 
 
 def test_infer_density_matches_float64():
