@@ -228,8 +228,12 @@ int sailor_volume_rendering_training_backward(
 size_t sailor_render_small_weights_floats(int F, int H);
 
 /* Bytes of device workspace sailor_render_rays needs for B x R rays of
- * feature width F (0 for invalid sizes). */
+ * feature width F seen from up to 4 views (0 for invalid sizes). */
 size_t sailor_render_workspace_bytes(int B, int R, int F);
+
+/* The same for V views (1 <= V <= 8): 5 to 8 views use 8 view slots per
+ * sample, so the per-ray token block doubles. */
+size_t sailor_render_workspace_bytes_views(int B, int R, int F, int V);
 
 /* Fused form of the FastNerf inference path (c_lib/FastNerf/src/infer_mlp.cu:
  * infer_fast_rendering :638-789 -- density_feat_kernel per (point, view),
@@ -242,7 +246,8 @@ size_t sailor_render_workspace_bytes(int B, int R, int F);
  * {128, 256}; Hydra inner width 32, RGB head 64-32-3.
  *  t        f32 [B,R,S] sorted sample distances (-1 padded rows are dropped)
  *  rays     f32 [B,R,6] origin and unit direction
- *  Kv, RTv  f32 [B,V,3,3], [B,V,3,4] source cameras; img_h x img_w images
+ *  Kv, RTv  f32 [B,V,3,3], [B,V,3,4] source cameras (1 <= V <= 8); img_h x
+ *           img_w images
  *  fmap     [B,V,Hf,Wf,F] channels-last per-view feature maps, in fp32
  *           (fmap_precision 0) or in the form's 16-bit type (fmap_precision
  *           == precision; FastNerf keeps its features in half precision)
@@ -254,13 +259,15 @@ size_t sailor_render_workspace_bytes(int B, int R, int F);
  *              below t_thresh (outputs are unchanged)
  *  chunk_samples  samples evaluated together per workgroup step (8, 16, 24
  *              or 48; 0 = fp32: 24, 16-bit: 16, or the largest of 16 and 8
- *              that divides S); S must be a multiple of it; a chunk
+ *              that divides S; with more than 4 views: fp32 12 (8 when S
+ *              is not a multiple of 12), 16-bit 8, and the argument is
+ *              ignored); S must be a multiple of it; a chunk
  *              whose activations do not fit the 160 KiB of LDS of a
  *              workgroup (fp32 with 48 samples) is rejected
  *  out_rgb f32 [B,R,3], out_depth f32 [B,R]; out_aux f32 [B,R,2] (weight sum,
  *  alpha sum) and out_occ f32 [B,R,S] (per-sample occupancy) may be NULL.
- *  workspace  device buffer of at least sailor_render_workspace_bytes(B, R,
- *           F) bytes (list of rays with samples and their composited
+ *  workspace  device buffer of at least sailor_render_workspace_bytes_views(B,
+ *           R, F, V) bytes (list of rays with samples and their composited
  *           per-view features, handed from the sample kernel to the
  *           Hydra/RGB kernel); calls that overlap on one workspace race.    */
 int sailor_render_rays(const float* t, const float* rays, int B, int R, int S,

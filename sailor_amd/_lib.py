@@ -41,6 +41,7 @@ _SIGS = {
     "sailor_render_rays": [P, P, I, I, I, P, P, I, I, I, P, I, I, I, I, P, P, I, I, P, P, P, F,
                            I, I, P, P, P, P, P, SZ, P],
     "sailor_render_workspace_bytes": [I, I, I],
+    "sailor_render_workspace_bytes_views": [I, I, I, I],
     "sailor_render_backward_partials_floats": [I, I],
     "sailor_render_hydra_backward": [P, I, I, I, I, I, P, P, SZ, P, P, P, SZ, P],
     "sailor_render_sample_backward": [P, P, I, I, I, P, P, I, I, I, P, I, I, I, P, I, I, P, P, P,
@@ -63,6 +64,7 @@ _RESTYPES = {
     "sailor_octree_workspace_bytes": SZ,
     "sailor_render_small_weights_floats": SZ,
     "sailor_render_workspace_bytes": SZ,
+    "sailor_render_workspace_bytes_views": SZ,
     "sailor_render_backward_partials_floats": SZ,
     "sailor_render_dense_grad_floats": SZ,
     "sailor_infer_nerf_workspace_bytes": SZ,

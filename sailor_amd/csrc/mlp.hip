@@ -60,7 +60,8 @@ typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 
 enum Prec { kF32 = 0, kBF16 = 1, kF16 = 2 };
 constexpr int kThreads = kWaves * 64;
-constexpr int kVP = 4;  // view slots per sample
+constexpr int kVP = 4;       // view slots per sample (training form, point mode)
+constexpr int kMaxViews = 8;  // view slots of the 8-view rendering instantiations
 constexpr float kHalfPi = 1.5707963267948966f;
 
 struct RenderArgs {
@@ -481,16 +482,16 @@ __device__ __forceinline__ void dense_chain16(const typename Tr<PREC>::E* __rest
   }
 }
 
-template <int F, int H, int PREC, int NS>
+template <int F, int H, int PREC, int NS, int VP = kVP>
 struct Smem {
-  static constexpr int NC = NS * kVP;
+  static constexpr int NC = NS * VP;
   static constexpr size_t act = 0;
   static constexpr size_t act_bytes = (size_t)NC * H * sizeof(typename Tr<PREC>::E);
   // kept-feature rows of F + 4 floats: an odd number of 16-byte chunks, so
   // the 16-byte stores of 16 columns are conflict-free
   static constexpr int KF = F + 4;
-  static constexpr size_t keep = (act + act_bytes + 15) / 16 * 16;      // f32 [NS][4][KF]
-  static constexpr size_t sig = keep + (size_t)NS * kVP * KF * 4;      // f32 [4][NC]
+  static constexpr size_t keep = (act + act_bytes + 15) / 16 * 16;      // f32 [NS][VP][KF]
+  static constexpr size_t sig = keep + (size_t)NS * VP * KF * 4;       // f32 [4][NC]
   static constexpr size_t alpha = sig + (size_t)kWaves * NC * 4;       // f32 [NS]
   static constexpr size_t tval = alpha + (size_t)NS * 4;               // f32 [NS]
   static constexpr size_t wts = tval + (size_t)NS * 4;                 // f32 [NS]
@@ -567,14 +568,18 @@ __device__ __forceinline__ void gather_quarter(const RenderArgs& a, int b, int v
 
 // Waves per SIMD the register allocation is fitted to: two workgroups per CU
 // let one workgroup's gather / compositing overlap the other's MFMA work.
-constexpr int render_occupancy(int PREC, int NS) { return (PREC != kF32 && NS <= 16) ? 2 : 1; }
+// (16-bit forms with at most 64 columns per chunk).
+constexpr int render_occupancy(int PREC, int NC) { return (PREC != kF32 && NC <= 64) ? 2 : 1; }
 
-template <int F, int H, int PREC, int NS, bool HALF_MAP, bool POINTS>
+// VP view slots per sample: 4 (up to 4 views) or 8 (up to 8 views); the
+// columns of a chunk are (sample, view slot) pairs, col = sample * VP + slot.
+template <int F, int H, int PREC, int NS, bool HALF_MAP, bool POINTS, int VP>
 __global__ void __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(render_occupancy(PREC, NS), render_occupancy(PREC, NS))))
+__attribute__((amdgpu_waves_per_eu(render_occupancy(PREC, NS * VP),
+                                   render_occupancy(PREC, NS * VP))))
 render_kernel(RenderArgs a) {
   typedef typename Tr<PREC>::E E;
-  typedef Smem<F, H, PREC, NS> L;
+  typedef Smem<F, H, PREC, NS, VP> L;
   constexpr int NC = L::NC, KF = L::KF, NTL = NC / 16, MTW = H / 64;
   constexpr int CE = Tr<PREC>::CE;
   constexpr int CPL = F / 4;
@@ -585,7 +590,8 @@ render_kernel(RenderArgs a) {
   constexpr SmallOffsets so = small_offsets(F, H);
   static_assert(K0 <= H, "first-layer input must fit an activation row");
   static_assert(CPL % CE == 0 && F % CE == 0, "feature quarters must fill whole chunks");
-  static_assert(kVP * F <= kThreads, "one thread per (view, channel) token entry");
+  constexpr int TE = (VP * F + kThreads - 1) / kThreads;  // token entries per thread
+  static_assert(VP == 4 || VP == 8, "4 or 8 view slots");
 
   extern __shared__ __attribute__((aligned(16))) char smem[];
   E* act = reinterpret_cast<E*>(smem + L::act);
@@ -631,7 +637,9 @@ render_kernel(RenderArgs a) {
     // compositing state lives in thread 0; tokens: thread (v, c) = (tid / F, tid % F)
     float T = 1.0f, dsum = 0.0f, wsum = 0.0f, asum = 0.0f;
     bool done = false;
-    float tok = 0.0f;
+    float tok[TE];
+#pragma unroll
+    for (int j = 0; j < TE; ++j) tok[j] = 0.0f;
     if (tid == 0) s_flag[0] = 0;
 
     for (int c0 = 0; c0 < S; c0 += NS) {
@@ -656,7 +664,7 @@ render_kernel(RenderArgs a) {
       // 16 lanes of a quarter-wave take 16 columns of the same quarter) ------
       for (int item = tid; item < NC * 4; item += kThreads) {
         const int col = (item & 15) + 16 * (item >> 6), quarter = (item >> 4) & 3;
-        const int sl = col >> 2, v = col & 3;
+        const int sl = col / VP, v = col % VP;
         const float x = s_pos[4 * sl], y = s_pos[4 * sl + 1], z = s_pos[4 * sl + 2];
         float f[CPL];
         gather_quarter<F, PREC, HALF_MAP>(a, b, v, x, y, z, quarter, f);
@@ -701,8 +709,8 @@ render_kernel(RenderArgs a) {
           }
         }
 #pragma unroll
-        for (int v = 0; v < kVP; ++v) {
-          const int col = sl * kVP + v, k = F + q0;  // the pair never straddles a chunk
+        for (int v = 0; v < VP; ++v) {
+          const int col = sl * VP + v, k = F + q0;  // the pair never straddles a chunk
           E* d2 = act + act_chunk<H, PREC>(col, k / CE) + k % CE;
           if constexpr (PREC == kF32) {
             *reinterpret_cast<f32x2*>(d2) = f32x2{val[0], val[1]};
@@ -760,7 +768,7 @@ render_kernel(RenderArgs a) {
       if (tid < NS) {
         float sum = 0.0f;
         for (int v = 0; v < V; ++v)
-          for (int w = 0; w < kWaves; ++w) sum += sig[w * NC + tid * kVP + v];
+          for (int w = 0; w < kWaves; ++w) sum += sig[w * NC + tid * VP + v];
         const float al = sigmoidf(sum * inv_v + ws[so.sigma_b]);
         if constexpr (POINTS) {
           if (p0 + tid < a.n_points) a.out_occ[(long long)b * a.n_points + p0 + tid] = al;
@@ -790,16 +798,24 @@ render_kernel(RenderArgs a) {
       }
       __syncthreads();
       // ---- per-view tokens: sum_i w_i f_v(x_i) -----------------------------
-      if (tid < kVP * F) {
-        const int v = tid / F, c = tid % F;
+#pragma unroll
+      for (int j = 0; j < TE; ++j) {
+        const int e = tid + j * kThreads;
+        if (e < VP * F) {
+          const int v = e / F, c = e % F;
 #pragma unroll 4
-        for (int sl = 0; sl < NS; ++sl) tok += s_w[sl] * keep[(sl * kVP + v) * KF + c];
+          for (int sl = 0; sl < NS; ++sl) tok[j] += s_w[sl] * keep[(sl * VP + v) * KF + c];
+        }
       }
     }
     __syncthreads();
     if constexpr (!POINTS) {
       // ---- outputs: per-view tokens for the Hydra/RGB kernel, depth, sums --
-      if (tid < kVP * F) a.out_tok[item * kVP * F + tid] = (tid / F) < V ? tok : 0.0f;
+#pragma unroll
+      for (int j = 0; j < TE; ++j) {
+        const int e = tid + j * kThreads;
+        if (e < VP * F) a.out_tok[item * VP * F + e] = (e / F) < V ? tok[j] : 0.0f;
+      }
       if (tid == 0) {
         const float* zd = a.zdir + b * 3;
         a.out_depth[ray] = dsum * (zd[0] * dx + zd[1] * dy + zd[2] * dz);
@@ -818,7 +834,7 @@ render_kernel(RenderArgs a) {
 // the group, view t%4); LDS holds feature-major rows [row][token] padded to
 // kLd floats so both the row reads and the transposing token load are free of
 // bank conflicts.
-constexpr int kRaysPerGroup = 16;
+constexpr int kRaysPerGroup = 16;          // rays per group with 4 view slots
 constexpr int kTok = kRaysPerGroup * kVP;  // 64 = one token per lane
 constexpr int kLd = kTok + 1;
 
@@ -855,11 +871,13 @@ __device__ __forceinline__ void load_rows(const float* rows, int lane, float (&d
   for (int k = 0; k < N; ++k) dst[k] = rows[k * LD + lane];
 }
 
-template <int F, int H>
+// VP tokens per ray (view slots): a group is 64 / VP rays, lane = ray * VP + slot.
+template <int F, int H, int VP>
 __global__ void __launch_bounds__(kThreads)
 hydra_rgb_kernel(const float* __restrict__ tok, const int* __restrict__ list,
                  const unsigned* __restrict__ n_valid_ptr, const float* __restrict__ ws, int V,
                  float* __restrict__ out_rgb) {
+  constexpr int RPG = kTok / VP;  // rays per group
   typedef HydraRows<F> R;
   constexpr SmallOffsets so = small_offsets(F, H);
   constexpr int stack_stride = so.q[1] - so.q[0];
@@ -871,16 +889,16 @@ hydra_rgb_kernel(const float* __restrict__ tok, const int* __restrict__ list,
   // the group is list entries [item0, item0 + 16); the grid covers every ray
   // and groups past the device-side count leave at once
   const long long n_valid = *n_valid_ptr;
-  const long long item0 = (long long)blockIdx.x * kRaysPerGroup;
+  const long long item0 = (long long)blockIdx.x * RPG;
   if (item0 >= n_valid) return;
-  const int view = lane & 3;
-  const long long item = item0 + (lane >> 2);
+  const int view = lane % VP;
+  const long long item = item0 + lane / VP;
   const bool valid = item < n_valid;
   // tokens of the group are one contiguous block of 64 x F floats
-  const long long n_floats = (n_valid - item0) * kVP * F;
+  const long long n_floats = (n_valid - item0) * VP * F;
   for (int i = tid; i < kTok * F; i += kThreads) {
     const int tk = i / F, c = i % F;
-    sm[(R::X + c) * kLd + tk] = i < n_floats ? tok[item0 * kVP * F + i] : 0.0f;
+    sm[(R::X + c) * kLd + tk] = i < n_floats ? tok[item0 * VP * F + i] : 0.0f;
   }
   __syncthreads();
   const float vmask = view < V ? 1.0f : 0.0f;
@@ -913,8 +931,8 @@ hydra_rgb_kernel(const float* __restrict__ tok, const int* __restrict__ list,
         const float k = sm[(R::QKV + kHydraInner + e) * kLd + lane];
         const float v = sm[(R::QKV + 2 * kHydraInner + e) * kLd + lane];
         float kv = k * nk * v;
-        kv += __shfl_xor(kv, 1, 64);
-        kv += __shfl_xor(kv, 2, 64);
+#pragma unroll
+        for (int off = 1; off < VP; off <<= 1) kv += __shfl_xor(kv, off, 64);
         attn[e] = q * nq * kv;
       }
     }
@@ -2120,32 +2138,41 @@ compact_rays(const float* __restrict__ t, int S, long long n_rays, unsigned* cou
   }
 }
 
-template <int F, int H, int PREC, int NS, bool HALF_MAP, bool POINTS>
+template <int F, int H, int PREC, int NS, bool HALF_MAP, bool POINTS, int VP>
 int launch_kernels(const RenderArgs& a, hipStream_t st);
 
-template <int F, int H, int PREC, int NS>
+template <int F, int H, int PREC, int NS, int VP = kVP>
 int launch(const RenderArgs& a, hipStream_t st, bool dry) {
-  typedef Smem<F, H, PREC, NS> L;
+  typedef Smem<F, H, PREC, NS, VP> L;
   if constexpr (L::total > kLdsBytes) {
     return sailor::set_error(SAILOR_ERR_ARG, "render_rays: %d-sample chunks need %zu bytes of "
                              "LDS at this precision (limit %zu)", NS, L::total, kLdsBytes);
   } else {
   if (dry) return SAILOR_OK;  // configuration check only
+  if constexpr (VP != kVP) {  // more than 4 views: rays only (no point mode)
+    if (a.points)
+      return sailor::set_error(SAILOR_ERR_ARG, "density_points: at most %d views", kVP);
+    if constexpr (PREC != kF32) {
+      if (a.fmap_half) return launch_kernels<F, H, PREC, NS, true, false, VP>(a, st);
+    }
+    return launch_kernels<F, H, PREC, NS, false, false, VP>(a, st);
+  } else {
   if constexpr (PREC != kF32) {
     if (a.fmap_half) {
-      return a.points ? launch_kernels<F, H, PREC, NS, true, true>(a, st)
-                      : launch_kernels<F, H, PREC, NS, true, false>(a, st);
+      return a.points ? launch_kernels<F, H, PREC, NS, true, true, VP>(a, st)
+                      : launch_kernels<F, H, PREC, NS, true, false, VP>(a, st);
     }
   }
-  return a.points ? launch_kernels<F, H, PREC, NS, false, true>(a, st)
-                  : launch_kernels<F, H, PREC, NS, false, false>(a, st);
+  return a.points ? launch_kernels<F, H, PREC, NS, false, true, VP>(a, st)
+                  : launch_kernels<F, H, PREC, NS, false, false, VP>(a, st);
+  }
   }
 }
 
-template <int F, int H, int PREC, int NS, bool HALF_MAP, bool POINTS>
+template <int F, int H, int PREC, int NS, bool HALF_MAP, bool POINTS, int VP>
 int launch_kernels(const RenderArgs& a, hipStream_t st) {
-  typedef Smem<F, H, PREC, NS> L;
-  auto kern = render_kernel<F, H, PREC, NS, HALF_MAP, POINTS>;
+  typedef Smem<F, H, PREC, NS, VP> L;
+  auto kern = render_kernel<F, H, PREC, NS, HALF_MAP, POINTS, VP>;
   static int grid = 0;
   if (grid == 0) {
     SAILOR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -2164,9 +2191,10 @@ int launch_kernels(const RenderArgs& a, hipStream_t st) {
     if (int err = sailor::check_launch("render_rays")) return err;
   }
   if (POINTS || !(a.stages & 4)) return SAILOR_OK;
-  const long long groups = (a.n_rays + kRaysPerGroup - 1) / kRaysPerGroup;
-  hipLaunchKernelGGL((hydra_rgb_kernel<F, H>), dim3((unsigned)groups), dim3(kThreads), 0, st,
-                     a.out_tok, a.list, a.n_valid, a.wsmall, a.V, a.out_rgb);
+  constexpr int rpg = kTok / VP;
+  const long long groups = (a.n_rays + rpg - 1) / rpg;
+  hipLaunchKernelGGL((hydra_rgb_kernel<F, H, VP>), dim3((unsigned)groups), dim3(kThreads), 0,
+                     st, a.out_tok, a.list, a.n_valid, a.wsmall, a.V, a.out_rgb);
   return sailor::check_launch("render_rays (hydra)");
 }
 
@@ -2180,8 +2208,22 @@ int dispatch_prec(int prec, const RenderArgs& a, hipStream_t st, bool dry) {
   return sailor::set_error(SAILOR_ERR_ARG, "render_rays: precision %d unsupported", prec);
 }
 
+// 8 view slots: fp32 12-sample chunks (8 when S is not a multiple of 12),
+// 16-bit 8-sample chunks (the column counts of the 4-slot defaults).
+template <int F, int H>
+int dispatch_vp8(int prec, const RenderArgs& a, hipStream_t st, bool dry) {
+  switch (prec) {
+    case kF32: return a.S % 12 == 0 ? launch<F, H, kF32, 12, 8>(a, st, dry)
+                                    : launch<F, H, kF32, 8, 8>(a, st, dry);
+    case kBF16: return launch<F, H, kBF16, 8, 8>(a, st, dry);
+    case kF16: return launch<F, H, kF16, 8, 8>(a, st, dry);
+  }
+  return sailor::set_error(SAILOR_ERR_ARG, "render_rays: precision %d unsupported", prec);
+}
+
 template <int F, int H>
 int dispatch_ns(int ns, int prec, const RenderArgs& a, hipStream_t st, bool dry) {
+  if (a.V > kVP) return dispatch_vp8<F, H>(prec, a, st, dry);
   switch (ns) {
     case 8: return dispatch_prec<F, H, 8>(prec, a, st, dry);
     case 16: return dispatch_prec<F, H, 16>(prec, a, st, dry);
@@ -2198,10 +2240,15 @@ extern "C" size_t sailor_render_small_weights_floats(int F, int H) {
   return (size_t)small_offsets(F, H).total;
 }
 
-extern "C" size_t sailor_render_workspace_bytes(int B, int R, int F) {
-  if (B <= 0 || R < 0 || F <= 0) return 0;
+extern "C" size_t sailor_render_workspace_bytes_views(int B, int R, int F, int V) {
+  if (B <= 0 || R < 0 || F <= 0 || V < 1 || V > kMaxViews) return 0;
   const long long n = (long long)B * R;
-  return workspace_token_offset(n) + (size_t)n * kVP * F * sizeof(float);
+  const int vp = V <= kVP ? kVP : kMaxViews;
+  return workspace_token_offset(n) + (size_t)n * vp * F * sizeof(float);
+}
+
+extern "C" size_t sailor_render_workspace_bytes(int B, int R, int F) {
+  return sailor_render_workspace_bytes_views(B, R, F, kVP);
 }
 
 namespace {
@@ -2215,13 +2262,14 @@ int render_stages(int stages, const float* t, const float* rays, int B, int R, i
                   float* out_occ, void* workspace, size_t workspace_bytes, void* stream) {
   SAILOR_REQUIRE(B > 0 && R >= 0 && S > 0, "render_rays: bad sizes");
   SAILOR_REQUIRE((long long)B * R < (1ll << 31), "render_rays: at most 2^31-1 rays per call");
+  SAILOR_REQUIRE(V >= 1 && V <= kMaxViews, "render_rays: 1 to %d views supported", kMaxViews);
   SAILOR_REQUIRE(workspace != nullptr &&
-                     workspace_bytes >= sailor_render_workspace_bytes(B, R, F),
+                     workspace_bytes >= sailor_render_workspace_bytes_views(B, R, F, V),
                  "render_rays: workspace of %zu bytes is smaller than the %zu required",
-                 workspace_bytes, sailor_render_workspace_bytes(B, R, F));
-  SAILOR_REQUIRE(V >= 1 && V <= kVP, "render_rays: 1 to %d views supported", kVP);
+                 workspace_bytes, sailor_render_workspace_bytes_views(B, R, F, V));
   SAILOR_REQUIRE(Hf > 1 && Wf > 1, "render_rays: feature map too small");
   int ns = chunk_samples;
+  if (V > kVP) ns = (precision == kF32 && S % 12 == 0) ? 12 : 8;  // 8-slot instantiations
   if (ns <= 0) {
     // measured on MI355X (C2 frame): fp32 24-sample chunks, 16-bit 16-sample
     // chunks (two workgroups per CU)
@@ -2256,7 +2304,7 @@ int render_stages(int stages, const float* t, const float* rays, int B, int R, i
   }
   if (precision < kF32 || precision > kF16)
     return sailor::set_error(SAILOR_ERR_ARG, "render_rays: precision %d unsupported", precision);
-  if (ns != 8 && ns != 16 && ns != 24 && ns != 48)
+  if (V <= kVP && ns != 8 && ns != 16 && ns != 24 && ns != 48)
     return sailor::set_error(SAILOR_ERR_ARG, "render_rays: samples per chunk %d not "
                              "instantiated (8, 16, 24, 48)", ns);
   // every configuration error is reported before anything is launched

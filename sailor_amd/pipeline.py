@@ -119,8 +119,9 @@ class FramePipeline:
         self.rgb = torch.empty((1, nt, 3), dtype=_f32, **o)
         self.depth = torch.empty((1, nt), dtype=_f32, **o)
         self.aux = torch.empty((1, nt, 2), dtype=_f32, **o)
-        self.render_ws = torch.empty(_lib.lib().sailor_render_workspace_bytes(1, nt, self.mc.F),
-                                     dtype=torch.uint8, **o)
+        self.render_ws = torch.empty(
+            _lib.lib().sailor_render_workspace_bytes_views(1, nt, self.mc.F, V),
+            dtype=torch.uint8, **o)
         self.graph = None
 
     # ------------------------------------------------------------------

@@ -67,7 +67,7 @@ def render_rays(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, model: PackedMode
                    depth=torch.empty((B, R), dtype=_f32, device=dev),
                    aux=torch.empty((B, R, 2), dtype=_f32, device=dev),
                    occ=torch.zeros((B, R, S), dtype=_f32, device=dev) if want_occ else None)
-    ws_bytes = _lib.lib().sailor_render_workspace_bytes(B, R, F)
+    ws_bytes = _lib.lib().sailor_render_workspace_bytes_views(B, R, F, V)
     if out.get("workspace") is None or out["workspace"].numel() < ws_bytes:
         out["workspace"] = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     call("sailor_render_rays", ptr(t), ptr(rays), B, R, S, ptr(Kv), ptr(RTv), V, int(img_hw[0]),

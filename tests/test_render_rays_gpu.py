@@ -176,3 +176,26 @@ def test_density_query_matches_float64(prec, N, chunk):
                            (p.cfg.img, p.cfg.img)).numpy()
     err = np.max(np.abs(got.cpu().numpy() - ref))
     assert err <= (1e-5 if prec == "fp32" else 2e-2), err
+
+
+@pytest.mark.parametrize("views", [5, 6, 8])
+def test_render_more_views_matches_float64_reference(views):
+    """5 to 8 source views (8 view slots per sample): fp32 render vs the
+    float64 restatement (same tolerances as the 4-view test), and the bf16
+    form against it by PSNR."""
+    from sailor_amd.pipeline import FrameConfig
+    from sailor_amd.render import render_rays
+    cfg = FrameConfig(f"c1_v{views}", views, 512, 600.0, 128, 8, 40, 47.0, 48, "c1")
+    p = pipe("fp32", cfg)
+    out = render_rays(p.sorted, p.rays, p.K, p.RT, (cfg.img, cfg.img), p.fmap, p.bbox, p.zdir,
+                      p.model, want_occ=True)
+    torch.cuda.synchronize()
+    rgb, dep, ws, asum, occ = [r.numpy() for r in _reference(p)]
+    valid = p.sorted[0, :, 0].cpu().numpy() >= 0
+    assert valid.sum() > 50
+    np.testing.assert_allclose(out["occ"].cpu().numpy(), occ, atol=2e-5)
+    np.testing.assert_allclose(out["depth"].cpu().numpy(), dep, atol=5e-4)
+    np.testing.assert_allclose(out["rgb"].cpu().numpy(), rgb, atol=1e-4)
+    plo = pipe("bf16", cfg)
+    ps = psnr(p.rgb[0].cpu().numpy(), plo.rgb[0].cpu().numpy(), valid)
+    assert ps > 40.0, ps
