@@ -272,6 +272,46 @@ def fastnerf_infer(steps):
             "frac_of_fp32_vector_peak": flops / (ms / 1e3) / 1e12 / 157.3}
 
 
+def batching_form(steps, prec="fp32", n_frames=16):
+    """Config-5 block: 16 frames of uneven size (own seeded scene each,
+    performer scale U(0.6, 1.2), 2-6 source views of 1024^2, 256^3 grid,
+    512^2 render, 48 samples) in one hipGraph over 4 streams, each frame's
+    chunk size picked by the autotuner; against the same frames as 16
+    sequential single-frame graph launches (per-frame latency p50/p99)."""
+    import numpy as np
+    import torch
+    from sailor_amd.pipeline import CONFIGS, BatchPipeline
+    base = CONFIGS["c2"]
+    t0 = time.perf_counter()
+    bp = BatchPipeline(base, n_frames=n_frames, prec=prec)
+    setup_s = time.perf_counter() - t0
+    chunks = bp.autotune()
+    lat = []
+    for p in bp.pipes:
+        p.capture()
+        lat.append(_median_ms(p.replay, steps))
+        p.graph = None
+    bp.capture()
+    bp.replay()
+    torch.cuda.synchronize()
+    batch_ms = _median_ms(bp.replay, steps)
+    info = [p.check() for p in bp.pipes]
+    return {
+        "workload": (f"c5: {n_frames} frames, 2-6 views of {base.img}^2, scale U(0.6,1.2), "
+                     f"{base.dim}^3 grid, {base.target}^2 render, {base.S} samples, {prec}"),
+        "views": [c.n_views for c in bp.cfgs], "scale": [round(c.scale, 3) for c in bp.cfgs],
+        "occupied_voxels": [i["occupied"] for i in info],
+        "valid_rays": [i["valid_rays"] for i in info], "chunk_samples": chunks,
+        "batched": {"frames_per_s": n_frames / (batch_ms / 1e3), "ms_per_batch": batch_ms,
+                    "streams": 4},
+        "sequential": {"frames_per_s": n_frames / (sum(lat) / 1e3),
+                       "latency_ms_p50": float(np.percentile(lat, 50)),
+                       "latency_ms_p99": float(np.percentile(lat, 99)),
+                       "latency_ms": [round(x, 3) for x in lat]},
+        "setup_s": setup_s,
+    }
+
+
 def psnr(a, b, mask):
     import numpy as np
     m = mask[..., None].astype(np.float64)
@@ -349,6 +389,10 @@ def main():
                     help="skip the 1024^2 precision-form block")
     ap.add_argument("--no-c4", action="store_true",
                     help="skip the differentiation-form (fwd+bwd) block")
+    ap.add_argument("--no-c5", action="store_true",
+                    help="skip the 16-frame batching block")
+    ap.add_argument("--batching-only", action="store_true",
+                    help="run only the 16-frame batching block and print it")
     ap.add_argument("--no-infer", action="store_true",
                     help="skip the FastNerf inference-surface block")
     ap.add_argument("--infer-only", action="store_true",
@@ -370,6 +414,9 @@ def main():
     torch.cuda.set_device(0)
     if args.profile_kernels:
         profile_kernels(cfg)
+        return
+    if args.batching_only:
+        print(json.dumps(batching_form(args.steps)))
         return
     if args.infer_only:
         print(json.dumps(fastnerf_infer(args.steps)))
@@ -452,6 +499,8 @@ def main():
     if not args.no_c4 and args.workload == "c2":
         result["c4_training_form"] = training_form(max(3, args.steps // 4), args.warmup,
                                                    with_cpu=not args.no_cpu)
+    if not args.no_c5 and args.workload == "c2":
+        result["c5_batching_form"] = batching_form(max(3, args.steps // 4))
     if not args.no_infer and args.workload == "c2":
         result["fastnerf_infer"] = fastnerf_infer(max(3, args.steps // 4))
     if not args.no_cpu:

@@ -41,6 +41,7 @@ class FrameConfig:
     th_low: float = 13.5
     th_high: float = 13.5
     node_capacity: int = 4_000_000
+    scale: float = 1.0    # performer scale of the synthetic scene
 
 
 CONFIGS = {
@@ -147,10 +148,10 @@ class FramePipeline:
     def synthetic(cfg: FrameConfig, frame_id=0, **kw):
         """Pipeline loaded with the seeded capsule scene of `cfg`."""
         p = FramePipeline(cfg, **kw)
-        key = (frame_id, cfg.n_views, cfg.img, cfg.fx)
+        key = (frame_id, cfg.n_views, cfg.img, cfg.fx, cfg.scale)
         if key not in _SCENE_CACHE:
             _SCENE_CACHE[key] = scenes.make_scene(frame_id, cfg.n_views, cfg.img, cfg.img,
-                                                  cfg.fx)
+                                                  cfg.fx, scale=cfg.scale)
         sc = _SCENE_CACHE[key]
         mc = p.mc
         fm = scenes.feature_maps(frame_id, cfg.n_views, mc.F, mc.Hf, mc.Wf)
@@ -248,3 +249,123 @@ class FramePipeline:
 def _ptr_array(tensors):
     import ctypes
     return (ctypes.c_void_p * len(tensors))(*[t.data_ptr() for t in tensors])
+
+
+# ---------------------------------------------------------------------------
+def _scene_job(args):
+    frame_id, n_views, img, fx, scale = args
+    return (frame_id, n_views, img, fx, scale), scenes.make_scene(frame_id, n_views, img, img,
+                                                                  fx, scale=scale)
+
+
+def prefetch_scenes(cfgs, frame_ids, workers=8):
+    """Generates the synthetic scenes of several frames in worker processes
+    (spawned, so they hold no GPU state) into the scene cache."""
+    jobs = [(f, c.n_views, c.img, c.fx, c.scale) for c, f in zip(cfgs, frame_ids)
+            if (f, c.n_views, c.img, c.fx, c.scale) not in _SCENE_CACHE]
+    if not jobs:
+        return
+    if workers <= 1 or len(jobs) == 1:
+        for j in jobs:
+            k, sc = _scene_job(j)
+            _SCENE_CACHE[k] = sc
+        return
+    import multiprocessing as mp
+    with mp.get_context("spawn").Pool(min(workers, len(jobs))) as pool:
+        for k, sc in pool.imap_unordered(_scene_job, jobs):
+            _SCENE_CACHE[k] = sc
+
+
+class BatchPipeline:
+    """Several frames of uneven size (own scene, performer scale and view
+    count each) as one hipGraph: every frame is the fixed launch sequence of
+    FramePipeline on its own static buffers, frames are spread over a few
+    streams inside the capture so small frames overlap, and the graph is
+    replayed as one unit.  `autotune()` picks each frame's samples per chunk
+    by timing the instantiated sizes."""
+
+    def __init__(self, base: FrameConfig, n_frames=16, prec="fp32", seed=5000, views=(2, 6),
+                 scale=(0.6, 1.2), device="cuda", workers=8):
+        rng = np.random.default_rng(seed)
+        self.cfgs, self.frame_ids = [], []
+        for i in range(n_frames):
+            v = int(rng.integers(views[0], views[1] + 1))
+            sc = float(rng.uniform(*scale))
+            self.cfgs.append(FrameConfig(f"{base.name}_f{i}", v, base.img, base.fx, base.dim,
+                                         base.rate, base.target, base.target_fx, base.S,
+                                         base.model, scale=sc))
+            self.frame_ids.append(seed + i)
+        prefetch_scenes(self.cfgs, self.frame_ids, workers)
+        self.pipes = [FramePipeline.synthetic(c, frame_id=f, prec=prec, device=device)
+                      for c, f in zip(self.cfgs, self.frame_ids)]
+        self.dev = torch.device(device)
+        self.graph = None
+
+    def run(self):
+        for p in self.pipes:
+            p.run()
+
+    def autotune(self, candidates=(8, 16, 24, 48), reps=2):
+        """Per frame: the chunk size (samples per workgroup step) with the
+        shortest render, among the candidates that divide S and fit the LDS
+        (frames with more than 4 views use the fixed 8-slot instantiation)."""
+        chosen = []
+        for p in self.pipes:
+            p.run()
+            torch.cuda.synchronize()
+            if p.cfg.n_views > 4:
+                chosen.append(0)
+                continue
+            best = None
+            for ns in candidates:
+                if p.cfg.S % ns:
+                    continue
+                p.chunk_samples = ns
+                try:
+                    p.shade()
+                except _lib.SailorError:
+                    continue
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(reps):
+                    p.shade()
+                e1.record()
+                e1.synchronize()
+                ms = e0.elapsed_time(e1) / reps
+                if best is None or ms < best[0]:
+                    best = (ms, ns)
+            p.chunk_samples = best[1] if best else 0
+            chosen.append(p.chunk_samples)
+        return chosen
+
+    def capture(self, n_streams=4):
+        """One graph for all frames; frame k runs on stream k % n_streams."""
+        main = torch.cuda.current_stream(self.dev)
+        streams = [torch.cuda.Stream(self.dev) for _ in range(n_streams)]
+        for s in streams:
+            s.wait_stream(main)
+        for k, p in enumerate(self.pipes):  # warm-up outside capture
+            with torch.cuda.stream(streams[k % n_streams]):
+                p.run()
+        for s in streams:
+            main.wait_stream(s)
+        torch.cuda.synchronize(self.dev)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            cap = torch.cuda.current_stream(self.dev)
+            for s in streams:
+                s.wait_stream(cap)
+            for k, p in enumerate(self.pipes):
+                with torch.cuda.stream(streams[k % n_streams]):
+                    p.run()
+            for s in streams:
+                cap.wait_stream(s)
+        self.graph = g
+        return g
+
+    def replay(self):
+        if self.graph is None:
+            self.run()
+        else:
+            self.graph.replay()
