@@ -401,6 +401,38 @@ int sailor_freq_encode_backward(const float* grad_y, const float* y, int N,
                                 int D, int deg, float* grad_x, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Per-view sample features for the FastNerf surface                        */
+/* ------------------------------------------------------------------------ */
+
+/* [N][C][H][W] -> [N][H][W][C] (the channels-last maps sailor_sample_features
+ * reads). */
+int sailor_nchw_to_nhwc(const float* src, int N, int C, int H, int W, float* dst, void* stream);
+
+/* The chain the reference runs before InferNerf.infer_nerf
+ * (gui/gui_render.py:713-751; SRONet/BasicRenNet.py:245-259,296-311):
+ * points o + t d, proj_persp with the sample distances as p_depths
+ * (utils_render.py:380-410: grid -2 where the camera depth < 0.01 or t is
+ * outside [0.1, 4.0], projected depth -2 for the latter), z_normalizer
+ * (:656-663), grid_sample (align_corners=True, bilinear, zero padding) of the
+ * geometry features, depth map, colour features and normalised colours, and
+ * t-psdf = clamp(z - sampled depth, -0.01, 0.01) / 0.01; plus the ray
+ * direction in each camera (R d).
+ *  rays [B,R,6], dists [B,R,S], K [B,V,3,3], RT [B,V,3,4];
+ *  geo_cl [B,V,Hg,Wg,16], depth [B,V,Hd,Wd], rgbf_cl [B,V,Hr,Wr,16],
+ *  rgb_cl [B,V,Hc,Wc,3] (channels-last); depth_mid, depth_dist [B,V];
+ *  res: the image size of proj_persp's grid normalisation ((p - res/2) /
+ *  (res/2), integer halves).
+ *  Outputs in (batch, view, ray, sample) order: out_z, out_tpsdf [B,V,R,S],
+ *  out_geo, out_rgbf [B,V,R,S,16], out_rgb, out_dir [B,V,R,S,3]. */
+int sailor_sample_features(const float* rays, const float* dists, int B, int R, int S,
+                           const float* K, const float* RT, int V, const float* geo_cl, int Hg,
+                           int Wg, const float* depth, int Hd, int Wd, const float* rgbf_cl,
+                           int Hr, int Wr, const float* rgb_cl, int Hc, int Wc,
+                           const float* depth_mid, const float* depth_dist, int res,
+                           float* out_z, float* out_tpsdf, float* out_geo, float* out_rgbf,
+                           float* out_rgb, float* out_dir, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* FastNerf inference surface (InferNerf, c_lib/FastNerf/src/binding.cpp)   */
 /* ------------------------------------------------------------------------ */
 /* Parameters are the reference's packed half vectors (gui/gui_render.py:

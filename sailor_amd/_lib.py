@@ -52,6 +52,9 @@ _SIGS = {
     "sailor_density_points": [P, I, I, P, P, I, I, I, P, I, I, I, I, P, I, I, P, P, P, I, P, P],
     "sailor_render_rays_stages": [I, P, P, I, I, I, P, P, I, I, I, P, I, I, I, I, P, P, I, I, P,
                                   P, P, F, I, I, P, P, P, P, P, SZ, P],
+    "sailor_nchw_to_nhwc": [P, I, I, I, I, P, P],
+    "sailor_sample_features": [P, P, I, I, I, P, P, I, P, I, I, P, I, I, P, I, I, P, I, I, P, P, I,
+                               P, P, P, P, P, P, P],
     "sailor_infer_nerf_workspace_bytes": [I, I, I, I],
     "sailor_infer_nerf": [P, P, P, P, P, P, P, P, P, I, I, I, I, I, P, P, P, P, SZ, P],
     "sailor_infer_density": [P, P, P, P, I, I, I, P, P],
