@@ -3,8 +3,9 @@ of the GUI's FastNerf step built on them (sailor_amd.fastnerf.forward_nerf)
 against the float64 restatements oracle/gather_ref.py and
 oracle/fastnerf_ref.py, on ring cameras with maps of different resolutions,
 samples behind / outside the depth range and dropped rays.  Tolerance: fp32
-vs float64, 1e-4 absolute (1e-3 on t-psdf, which scales depth differences by
-100)."""
+vs float64, 1e-4 absolute; 5e-3 on t-psdf, which scales depth differences by
+100: the random depth maps change by up to 2 per pixel, so the ~1e-5 pixel
+difference of the fp32 projection moves it by up to ~3e-3."""
 import numpy as np
 import pytest
 import torch
@@ -58,7 +59,7 @@ def test_sample_features_match_float64(B):
                              maps["rgb"], mid, dist, res=512)
     names = ("z", "tpsdf", "geo", "rgbf", "rgb", "dir")
     for name, g in zip(names, got):
-        tol = 1e-3 if name == "tpsdf" else 1e-4
+        tol = 5e-3 if name == "tpsdf" else 1e-4
         np.testing.assert_allclose(g.double().cpu().numpy(), ref[name].numpy(), atol=tol,
                                    err_msg=name)
     # the case covers samples inside the maps and dropped ones
