@@ -401,6 +401,43 @@ int sailor_freq_encode_backward(const float* grad_y, const float* y, int N,
                                 int D, int deg, float* grad_x, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Preprocessing                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces depth_normalize (c_lib/VoxelEncoding/src/depth_normalizer.cu:
+ * 98-189), bound as VoxelEncoding.depth_normalize (binding.cpp:38).
+ *  depths f32 [B,V,H,W]; per view: min / max over depths in [valid_min,
+ *  valid_max] (99999 / -99999 when none), distance = max - min + 1e-10,
+ *  mid = (max + min) / 2.  user_dist < 0: out = clamp((d - mid) [* mask] /
+ *  distance [* 2 if divided2], -1, 1); else (d - mid) * mask / user_dist
+ *  with multi_mask, clamp((d - mid) / user_dist, -1, 1) without.
+ *  mask = 0 where d is outside the valid range (multi_mask), else 1.
+ *  Outputs out_normalized, out_mask [B,V,H,W], out_mid, out_dist [B,V];
+ *  workspace: 8 * B * V bytes. */
+int sailor_depth_normalize(const float* depths, int B, int V, int H, int W, float valid_min,
+                           float valid_max, float user_dist, int multi_mask, int divided2,
+                           float* out_normalized, float* out_mask, float* out_mid,
+                           float* out_dist, void* workspace, size_t workspace_bytes,
+                           void* stream);
+
+/* Replaces rays_selecting (sampling_kernel.cu:489-527,736-769): rows of
+ * ray_ori_dir [B,V,H,W,6] at sampled_xy int32 [B,V,N,2] (x, y) -> out
+ * [B,V,N,6].  An (x, y) outside the image selects zeros (the reference
+ * reads out of bounds). */
+int sailor_rays_selecting(const float* ray_ori_dir, int B, int V, int H, int W,
+                          const int* sampled_xy, int n_rays, float* out, void* stream);
+
+/* Replaces rgbdv_selecting (sampling_kernel.cu:530-607,771-825): rgbs
+ * [B,V,H,W,C], normals [B,V,H,W,3], depths [B,V,H,W] at sampled_xy
+ * [B,V,N,2] -> out_rgb [B,V,N,C], out_normals [B,V,N,3], out_depth
+ * [B,V,N], out_label [B,V,N] = 1 where the depth is > 0. */
+int sailor_rgbdv_selecting(const float* rgbs, int rgb_channels, const float* normals,
+                           const float* depths, int B, int V, int H, int W,
+                           const int* sampled_xy, int n_rays, float* out_rgb,
+                           float* out_normals, float* out_depth, float* out_label,
+                           void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Per-view sample features for the FastNerf surface                        */
 /* ------------------------------------------------------------------------ */
 

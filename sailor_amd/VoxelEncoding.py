@@ -399,3 +399,58 @@ def freq_encode_backward(grad_output, inputs, outputs, deg, device):
         call("sailor_freq_encode_backward", ptr(g), ptr(y), N, D, int(deg), ptr(gx),
              stream_of(dev))
     return gx
+
+
+# ---------------------------------------------------------------------------
+# preprocessing
+def depth_normalize(depths, valid_min_depth, valid_max_depth, user_dist, multi_mask, divided2,
+                    device):
+    """depth_normalize (depth_normalizer.cu:98-189): depths [B, N_v, H, W] ->
+    (normalized [B,N_v,H,W], mask [B,N_v,H,W], mid [B,N_v,1,1],
+    distance [B,N_v,1,1])."""
+    dev = _dev(device)
+    d = as_input(depths, "depths", _f32, dev)
+    B, V, H, W = d.shape
+    out = torch.empty_like(d)
+    mask = torch.empty_like(d)
+    mid = torch.empty((B, V, 1, 1), dtype=_f32, device=dev)
+    dist = torch.empty((B, V, 1, 1), dtype=_f32, device=dev)
+    ws = torch.empty(8 * B * V, dtype=torch.uint8, device=dev)
+    call("sailor_depth_normalize", ptr(d), B, V, H, W, float(valid_min_depth),
+         float(valid_max_depth), float(user_dist), int(bool(multi_mask)), int(bool(divided2)),
+         ptr(out), ptr(mask), ptr(mid), ptr(dist), ptr(ws), ws.numel(), stream_of(dev))
+    return out, mask, mid, dist
+
+
+def rays_selecting(ray_ori_dir, sampled_xy, device):
+    """rays_selecting (sampling_kernel.cu:736-769): [B,N_v,H,W,6] at
+    sampled_xy [B,N_v,N_rays,2] (x, y) -> [B,N_v,N_rays,6]."""
+    dev = _dev(device)
+    r = as_input(ray_ori_dir, "ray_ori_dir", _f32, dev)
+    xy = as_input(sampled_xy, "sampled_xy", torch.int32, dev)
+    B, V, H, W = r.shape[:4]
+    N = xy.shape[2]
+    out = torch.zeros((B, V, N, 6), dtype=_f32, device=dev)
+    call("sailor_rays_selecting", ptr(r), B, V, H, W, ptr(xy), N, ptr(out), stream_of(dev))
+    return out
+
+
+def rgbdv_selecting(target_rgbs, target_normals, target_depths, sampled_xy, device):
+    """rgbdv_selecting (sampling_kernel.cu:771-825): rgbs [B,N_v,H,W,C],
+    normals [B,N_v,H,W,3], depths [B,N_v,H,W,1] at sampled_xy [B,N_v,N,2] ->
+    (rgb [B,N_v,N,C], normals [B,N_v,N,3], depth [B,N_v,N,1], label [B,N_v,N,1])."""
+    dev = _dev(device)
+    rgb = as_input(target_rgbs, "target_rgbs", _f32, dev)
+    nml = as_input(target_normals, "target_normals", _f32, dev)
+    dep = as_input(target_depths, "target_depths", _f32, dev)
+    xy = as_input(sampled_xy, "sampled_xy", torch.int32, dev)
+    B, V, H, W, C = rgb.shape
+    N = xy.shape[2]
+    o = dict(dtype=_f32, device=dev)
+    out_rgb = torch.zeros((B, V, N, C), **o)
+    out_nml = torch.zeros((B, V, N, 3), **o)
+    out_dep = torch.zeros((B, V, N, 1), **o)
+    out_lab = torch.ones((B, V, N, 1), **o)
+    call("sailor_rgbdv_selecting", ptr(rgb), C, ptr(nml), ptr(dep), B, V, H, W, ptr(xy), N,
+         ptr(out_rgb), ptr(out_nml), ptr(out_dep), ptr(out_lab), stream_of(dev))
+    return out_rgb, out_nml, out_dep, out_lab

@@ -52,6 +52,9 @@ _SIGS = {
     "sailor_density_points": [P, I, I, P, P, I, I, I, P, I, I, I, I, P, I, I, P, P, P, I, P, P],
     "sailor_render_rays_stages": [I, P, P, I, I, I, P, P, I, I, I, P, I, I, I, I, P, P, I, I, P,
                                   P, P, F, I, I, P, P, P, P, P, SZ, P],
+    "sailor_depth_normalize": [P, I, I, I, I, F, F, F, I, I, P, P, P, P, P, SZ, P],
+    "sailor_rays_selecting": [P, I, I, I, I, P, I, P, P],
+    "sailor_rgbdv_selecting": [P, I, P, P, I, I, I, I, P, I, P, P, P, P, P],
     "sailor_nchw_to_nhwc": [P, I, I, I, I, P, P],
     "sailor_sample_features": [P, P, I, I, I, P, P, I, P, I, I, P, I, I, P, I, I, P, I, I, P, P, I,
                                P, P, P, P, P, P, P],

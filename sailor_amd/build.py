@@ -21,7 +21,7 @@ LIB = os.path.join(HERE, "libsailor_hip.so")
 ARCH = os.environ.get("SAILOR_OFFLOAD_ARCH", "gfx950")
 
 # files whose results are compared bit-exactly against the CPU oracle
-EXACT = {"fusion.hip", "octree.hip", "traversal.hip", "rays.hip", "render.hip",
+EXACT = {"fusion.hip", "octree.hip", "traversal.hip", "rays.hip", "render.hip", "preprocess.hip",
          "freq.hip", "trace.hip"}
 
 COMMON = [
