@@ -9,28 +9,9 @@ import numpy as np
 import pytest
 import torch
 
+from oracle.preprocess_ref import depth_normalize as _depth_normalize_ref
+
 pytestmark = pytest.mark.gpu
-
-
-def _depth_normalize_ref(d, lo, hi, user_dist, multi_mask, divided2):
-    """numpy float32 restatement of depth_normalize."""
-    d = d.astype(np.float32)
-    valid = (d >= np.float32(lo)) & (d <= np.float32(hi))
-    mn = np.where(valid, d, np.float32(99999.0)).min(axis=(2, 3), keepdims=True)
-    mx = np.where(valid, d, np.float32(-99999.0)).max(axis=(2, 3), keepdims=True)
-    dist = (mx - mn) + np.float32(1e-10)
-    mid = (mx + mn) / np.float32(2)
-    mask = np.where(valid, 1, 0).astype(np.float32) if multi_mask else np.ones_like(d)
-    if user_dist < 0:
-        r = ((d - mid) * mask) / dist if multi_mask else (d - mid) / dist
-        if divided2:
-            r = r * np.float32(2.0)
-        r = np.clip(r, -1, 1)
-    elif multi_mask:
-        r = ((d - mid) * mask) / np.float32(user_dist)
-    else:
-        r = np.clip((d - mid) / np.float32(user_dist), -1, 1)
-    return r.astype(np.float32), mask, mid, dist
 
 
 @pytest.mark.parametrize("user_dist,multi_mask,divided2",
