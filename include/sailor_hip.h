@@ -514,6 +514,34 @@ int sailor_infer_density(const float* points_z, const float* points_tpsdf,
 int sailor_infer_blending(const float* feats_neighbors, const float* ray_feats,
                           const void* params_blend, int N, float* out, void* stream);
 
+/* Replaces rays_interpolate (gui/gui_render.py:753-773,
+ * upsampling/BasicRenNet.py:348-367): per camera c, K[c], R[c] [3,3] and
+ * dirs [c][n_rays][3] -> out [c][n_rays*4][3], element ray*4+k the
+ * normalised R^-1 K^-1 [x + 0.5 (k&1), y + 0.5 (k>>1), 1] of the pixel
+ * (x, y) of K R dir. */
+int sailor_rays_interpolate(const float* K, const float* R, const float* dirs, int n_cams,
+                            int n_rays, float* out, void* stream);
+
+/* Replaces the body of gui_render.forward_upsampling + sample_neighbor
+ * (gui/gui_render.py:775-838) for one target camera: for every ray with
+ * valid_min < depth < valid_max, its 4 sub-pixel rays (rays_interpolate),
+ * surface points ray_ori + dir * depth projected into neighbour views
+ * nei0, nei1 (proj_persp, utils_render.py:380-410, grid scale res/2),
+ * bilinear colour features rgbf_cl [V,Hf,Wf,16], nearest colours rgb_cl
+ * [V,Hc,Wc,3] and depths [V,Hd,Wd], exp(-200 (z - d)^2), the blending MLP
+ * (15075 halves, as sailor_infer_blending) on [feats | R_v dir | weight]
+ * x 2 and ray_feats [N,30], and clamp(w0 c0 + w1 c1 + w2 coarse_rgb, 0, 1)
+ * -> out [N,4,3]; other rays get zeros.  K, R [V,3,3], T [V,3] are the
+ * source views; tar_K, tar_R [3,3] the target camera. */
+int sailor_forward_upsampling(const float* ray_ori, const float* ray_dir, const float* depth,
+                              const float* coarse_rgb, const float* ray_feats, int N,
+                              const float* tar_K, const float* tar_R, const float* K,
+                              const float* R, const float* T, int V, int nei0, int nei1,
+                              const float* rgbf_cl, int Hf, int Wf, const float* rgb_cl, int Hc,
+                              int Wc, const float* depths, int Hd, int Wd, int res,
+                              float valid_min, float valid_max, const void* params_blend,
+                              float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

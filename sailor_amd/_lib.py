@@ -62,6 +62,9 @@ _SIGS = {
     "sailor_infer_nerf": [P, P, P, P, P, P, P, P, P, I, I, I, I, I, P, P, P, P, SZ, P],
     "sailor_infer_density": [P, P, P, P, I, I, I, P, P],
     "sailor_infer_blending": [P, P, P, I, P, P],
+    "sailor_rays_interpolate": [P, P, P, I, I, P, P],
+    "sailor_forward_upsampling": [P, P, P, P, P, I, P, P, P, P, P, I, I, I, P, I, I, P, I, I, P,
+                                  I, I, I, F, F, P, P, P],
     "sailor_freq_encode_forward": [P, I, I, I, P, P],
     "sailor_freq_encode_backward": [P, P, I, I, I, P, P],
 }

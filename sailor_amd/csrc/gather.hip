@@ -6,46 +6,12 @@
 // one thread per (batch, view, ray, sample), channels-last maps so a bilinear
 // tap of the 16 feature channels is one 64-byte read, outputs written in the
 // (batch, view, ray, sample) order infer_nerf consumes.
-#include "common.h"
+#include "image_sample.h"
 
 namespace {
 
 constexpr int kGeoC = 16, kRgbC = 16;
-
-// Bilinear sample with align_corners=True and zero padding of a
-// channels-last map [H][W][C] at normalised grid (gx, gy) (F.grid_sample).
-template <int C>
-__device__ __forceinline__ void bilinear_cl(const float* __restrict__ map, int H, int W, float gx,
-                                            float gy, float (&out)[C]) {
-#pragma unroll
-  for (int c = 0; c < C; ++c) out[c] = 0.0f;
-  const float ix = (gx + 1.0f) * 0.5f * (float)(W - 1);
-  const float iy = (gy + 1.0f) * 0.5f * (float)(H - 1);
-  if (!(ix > -1.0f && ix < (float)W && iy > -1.0f && iy < (float)H)) return;
-  const float fx0 = floorf(ix), fy0 = floorf(iy);
-  const int x0 = (int)fx0, y0 = (int)fy0;
-  const float ax = ix - fx0, ay = iy - fy0;
-#pragma unroll
-  for (int tap = 0; tap < 4; ++tap) {
-    const int xx = x0 + (tap & 1), yy = y0 + (tap >> 1);
-    if (xx < 0 || xx >= W || yy < 0 || yy >= H) continue;
-    const float w = ((tap & 1) ? ax : 1.0f - ax) * ((tap >> 1) ? ay : 1.0f - ay);
-    const float* src = map + ((size_t)yy * W + xx) * C;
-    if constexpr (C % 4 == 0) {
-#pragma unroll
-      for (int c = 0; c < C; c += 4) {
-        const float4 v = *reinterpret_cast<const float4*>(src + c);
-        out[c] += w * v.x;
-        out[c + 1] += w * v.y;
-        out[c + 2] += w * v.z;
-        out[c + 3] += w * v.w;
-      }
-    } else {
-#pragma unroll
-      for (int c = 0; c < C; ++c) out[c] += w * src[c];
-    }
-  }
-}
+using sailor::bilinear_cl;
 
 struct GatherArgs {
   const float* rays;   // [B,R,6]

@@ -15,7 +15,7 @@
 //    features along the ray, then the two Hydra stacks with the cross-view
 //    sums as 2 lane shuffles (4 lanes per ray) and the RGB head on the lane
 //    of view 0; no host synchronisation, everything on the caller's stream.
-#include "common.h"
+#include "image_sample.h"
 
 namespace {
 
@@ -270,8 +270,36 @@ color_kernel(const half_t* __restrict__ params, const float* __restrict__ t,
   }
 }
 
-// upsampling_kernel (infer_mlp.cu:449-506): [N, 40] neighbour features and
-// [N, 30] ray features -> [N, 3] blend weights.
+// upsampling_kernel (infer_mlp.cu:449-506) for one point: [40] neighbour
+// features and [30] ray features (read from rf) -> 3 blend weights.
+__device__ __forceinline__ void blend_mlp(const half_t* __restrict__ w, const float (&x)[40],
+                                          const float* __restrict__ rf, float* row,
+                                          float (&o)[3]) {
+  float h[64], c[62], h2[32];
+  const half_t* wp = w;
+  dense_row<40, 64, 1>(wp, x, row);
+  from_row(row, h);
+  wp += dense_size<40, 64>();
+  dense_row<64, 64, 1>(wp, h, row);
+  from_row(row, h);
+  wp += dense_size<64, 64>();
+  dense_row<64, 32, 1>(wp, h, row);
+  from_row(row, h2);
+  wp += dense_size<64, 32>();
+#pragma unroll
+  for (int i = 0; i < 32; ++i) c[i] = h2[i];
+#pragma unroll
+  for (int i = 0; i < 30; ++i) c[32 + i] = rf[i];
+  dense_row<62, 64, 1>(wp, c, row);
+  from_row(row, h);
+  wp += dense_size<62, 64>();
+  dense_row<64, 32, 1>(wp, h, row);
+  from_row(row, h2);
+  wp += dense_size<64, 32>();
+  dense<32, 3, 2>(wp, h2, o);
+}
+
+// [N, 40] neighbour features and [N, 30] ray features -> [N, 3] blend weights.
 __global__ void __launch_bounds__(kBlock)
 blend_kernel(const half_t* __restrict__ params, const float* __restrict__ nf,
              const float* __restrict__ rf, long long n, float* __restrict__ out) {
@@ -281,32 +309,179 @@ blend_kernel(const half_t* __restrict__ params, const float* __restrict__ nf,
   stage(w, params, kBlendParams);
   for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n;
        p += (long long)gridDim.x * blockDim.x) {
-    float x[40], h[64], c[62], h2[32], o[3];
+    float x[40], o[3];
 #pragma unroll
     for (int i = 0; i < 40; ++i) x[i] = nf[p * 40 + i];
-    const half_t* wp = w;
-    dense_row<40, 64, 1>(wp, x, row);
-    from_row(row, h);
-    wp += dense_size<40, 64>();
-    dense_row<64, 64, 1>(wp, h, row);
-    from_row(row, h);
-    wp += dense_size<64, 64>();
-    dense_row<64, 32, 1>(wp, h, row);
-    from_row(row, h2);
-    wp += dense_size<64, 32>();
-#pragma unroll
-    for (int i = 0; i < 32; ++i) c[i] = h2[i];
-#pragma unroll
-    for (int i = 0; i < 30; ++i) c[32 + i] = rf[p * 30 + i];
-    dense_row<62, 64, 1>(wp, c, row);
-    from_row(row, h);
-    wp += dense_size<62, 64>();
-    dense_row<64, 32, 1>(wp, h, row);
-    from_row(row, h2);
-    wp += dense_size<64, 32>();
-    dense<32, 3, 2>(wp, h2, o);
+    blend_mlp(w, x, rf + p * 30, row, o);
 #pragma unroll
     for (int i = 0; i < 3; ++i) out[p * 3 + i] = o[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sub-pixel upsampling of the GUI (gui/gui_render.py:753-838): 4 rays per
+// pixel re-derived from the target camera, each blended from 2 neighbour
+// views and the coarse colour.
+
+// Inverse of a row-major 3x3 matrix by the adjugate.
+__device__ __forceinline__ void inv3(const float* __restrict__ m, float (&r)[9]) {
+  const float c00 = m[4] * m[8] - m[5] * m[7];
+  const float c01 = m[5] * m[6] - m[3] * m[8];
+  const float c02 = m[3] * m[7] - m[4] * m[6];
+  const float inv_det = 1.0f / (m[0] * c00 + m[1] * c01 + m[2] * c02);
+  r[0] = c00 * inv_det;
+  r[1] = (m[2] * m[7] - m[1] * m[8]) * inv_det;
+  r[2] = (m[1] * m[5] - m[2] * m[4]) * inv_det;
+  r[3] = c01 * inv_det;
+  r[4] = (m[0] * m[8] - m[2] * m[6]) * inv_det;
+  r[5] = (m[2] * m[3] - m[0] * m[5]) * inv_det;
+  r[6] = c02 * inv_det;
+  r[7] = (m[1] * m[6] - m[0] * m[7]) * inv_det;
+  r[8] = (m[0] * m[4] - m[1] * m[3]) * inv_det;
+}
+
+__device__ __forceinline__ void mat3_vec(const float* __restrict__ m, const float (&v)[3],
+                                         float (&o)[3]) {
+  o[0] = m[0] * v[0] + m[1] * v[1] + m[2] * v[2];
+  o[1] = m[3] * v[0] + m[4] * v[1] + m[5] * v[2];
+  o[2] = m[6] * v[0] + m[7] * v[1] + m[8] * v[2];
+}
+
+// rays_interpolate (gui_render.py:753-773) for sub-pixel k of direction d:
+// the pixel (x, y) of K R d, shifted by (+0.5 if k & 1, +0.5 if k & 2),
+// back-projected as R^-1 K^-1 [x, y, 1] and normalised (F.normalize, eps
+// 1e-12).  Kinv / Rinv are the inverses of K / R.
+__device__ __forceinline__ void interp_dir(const float* __restrict__ K,
+                                           const float* __restrict__ R, const float (&Kinv)[9],
+                                           const float (&Rinv)[9], const float (&d)[3], int k,
+                                           float (&out)[3]) {
+  float rd[3], q[3];
+  mat3_vec(R, d, rd);
+  mat3_vec(K, rd, q);
+  float px[3] = {q[0] / q[2], q[1] / q[2], 1.0f};
+  if (k & 1) px[0] += 0.5f;
+  if (k & 2) px[1] += 0.5f;
+  float u[3], v[3];
+  mat3_vec(Kinv, px, u);
+  mat3_vec(Rinv, u, v);
+  const float n = fmaxf(sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]), 1e-12f);
+  out[0] = v[0] / n;
+  out[1] = v[1] / n;
+  out[2] = v[2] / n;
+}
+
+// dirs [cams][n][3] -> out [cams][n * 4][3] (element (ray, k) at ray * 4 + k).
+__global__ void __launch_bounds__(kBlock)
+rays_interp_kernel(const float* __restrict__ K, const float* __restrict__ R,
+                   const float* __restrict__ dirs, int cams, int n, float* __restrict__ out) {
+  const long long total = (long long)cams * n * 4;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long ray = e >> 2;
+    const int cam = (int)(ray / n);
+    float Kinv[9], Rinv[9], d[3], o[3];
+    inv3(K + cam * 9, Kinv);
+    inv3(R + cam * 9, Rinv);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) d[i] = dirs[ray * 3 + i];
+    interp_dir(K + cam * 9, R + cam * 9, Kinv, Rinv, d, (int)(e & 3), o);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[e * 3 + i] = o[i];
+  }
+}
+
+struct UpsampleArgs {
+  const float* ray_ori;    // [N,3]
+  const float* ray_dir;    // [N,3]
+  const float* depth;      // [N]     rendered depth o_d
+  const float* coarse;     // [N,3]   rendered colour o_c
+  const float* feats;      // [N,30]  colour features o_c_feat
+  const float* tar_K;      // [3,3]
+  const float* tar_R;      // [3,3]
+  const float* K;          // [V,3,3] source views
+  const float* R;          // [V,3,3]
+  const float* T;          // [V,3]
+  const float* rgbf;       // [V,Hf,Wf,16] high-resolution colour features (channels last)
+  const float* rgb;        // [V,Hc,Wc,3]  colours (channels last)
+  const float* dep;        // [V,Hd,Wd]    depths
+  long long N;
+  int Hf, Wf, Hc, Wc, Hd, Wd, res, nei[2];
+  float vmin, vmax;        // rays with vmin < o_d < vmax are upsampled
+  float* out;              // [N,4,3]
+};
+
+// One thread per (ray, sub-pixel): gui_render.forward_upsampling +
+// sample_neighbor (:775-838) with the neighbour projection of proj_persp
+// (utils_render.py:380-410, depth range [0.1, 4]), F.grid_sample of the
+// features (bilinear) and of colour and depth (nearest), the Gaussian depth
+// agreement exp(-200 (z - d)^2), the blending MLP and the clamped blend
+// w0 c0 + w1 c1 + w2 c_coarse.  Rays outside the depth range get zeros.
+__global__ void __launch_bounds__(kBlock) upsample_kernel(const half_t* __restrict__ params,
+                                                          UpsampleArgs a) {
+  extern __shared__ __attribute__((aligned(16))) char smem_i[];
+  float* row = reinterpret_cast<float*>(smem_i) + threadIdx.x * kRow;
+  half_t* w = reinterpret_cast<half_t*>(smem_i + rows_bytes());
+  stage(w, params, kBlendParams);
+  float Kinv[9], Rinv[9];
+  inv3(a.tar_K, Kinv);
+  inv3(a.tar_R, Rinv);
+  const long long total = a.N * 4;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long p = e >> 2;
+    const float od = a.depth[p];
+    float res_c[3] = {0.0f, 0.0f, 0.0f};
+    if (od > a.vmin && od < a.vmax) {
+      float d[3], de[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) d[i] = a.ray_dir[p * 3 + i];
+      interp_dir(a.tar_K, a.tar_R, Kinv, Rinv, d, (int)(e & 3), de);
+      float pos[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) pos[i] = a.ray_ori[p * 3 + i] + de[i] * od;
+      const bool invalid_d = od < 0.1f || od > 4.0f;
+      const float half = (float)(a.res / 2);
+      float x[40], c_nei[2][3];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int v = a.nei[j];
+        const float* Rm = a.R + v * 9;
+        const float* Km = a.K + v * 9;
+        const float* Tv = a.T + v * 3;
+        float cam[3];
+        mat3_vec(Rm, pos, cam);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) cam[i] += Tv[i];
+        const bool invalid = cam[2] < 0.01f;
+        const float zd = invalid ? 1.0f : cam[2];
+        const float q[3] = {cam[0] / zd, cam[1] / zd, cam[2] / zd};
+        float gx = (Km[0] * q[0] + Km[1] * q[1] + Km[2] * q[2] - half) / half;
+        float gy = (Km[3] * q[0] + Km[4] * q[1] + Km[5] * q[2] - half) / half;
+        if (invalid || invalid_d) gx = gy = -2.0f;
+        const float pz = invalid_d ? -2.0f : cam[2];
+        float f[16], sd[1], sdir[3];
+        sailor::bilinear_cl<16>(a.rgbf + (size_t)v * a.Hf * a.Wf * 16, a.Hf, a.Wf, gx, gy, f);
+        sailor::nearest_cl<3>(a.rgb + (size_t)v * a.Hc * a.Wc * 3, a.Hc, a.Wc, gx, gy, c_nei[j]);
+        sailor::nearest_cl<1>(a.dep + (size_t)v * a.Hd * a.Wd, a.Hd, a.Wd, gx, gy, sd);
+        mat3_vec(Rm, de, sdir);
+        const float dz = pz - sd[0];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[20 * j + i] = f[i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) x[20 * j + 16 + i] = sdir[i];
+        x[20 * j + 19] = expf(-200.0f * (dz * dz));
+      }
+      float wts[3];
+      blend_mlp(w, x, a.feats + p * 30, row, wts);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const float c = wts[0] * c_nei[0][i] + wts[1] * c_nei[1][i] + wts[2] * a.coarse[p * 3 + i];
+        // torch.clamp: NaN stays NaN
+        res_c[i] = c != c ? c : fminf(fmaxf(c, 0.0f), 1.0f);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) a.out[e * 3 + i] = res_c[i];
   }
 }
 
@@ -327,6 +502,7 @@ int setup_all() {
   if (int e = lds_setup(density_kernel, kDensityLds)) return e;
   if (int e = lds_setup(color_kernel, kColorLds)) return e;
   if (int e = lds_setup(blend_kernel, kBlendLds)) return e;
+  if (int e = lds_setup(upsample_kernel, kBlendLds)) return e;
   done = SAILOR_OK;
   return SAILOR_OK;
 }
@@ -403,4 +579,36 @@ extern "C" int sailor_infer_blending(const float* feats_neighbors, const float* 
                      static_cast<const half_t*>(params_blend), feats_neighbors, ray_feats,
                      (long long)N, out);
   return sailor::check_launch("infer_blending");
+}
+
+extern "C" int sailor_rays_interpolate(const float* K, const float* R, const float* dirs,
+                                       int n_cams, int n_rays, float* out, void* stream) {
+  SAILOR_REQUIRE(n_cams > 0 && n_rays >= 0, "rays_interpolate: bad sizes cams=%d rays=%d",
+                 n_cams, n_rays);
+  if (n_rays == 0) return SAILOR_OK;
+  hipLaunchKernelGGL(rays_interp_kernel, dim3(grid_for((long long)n_cams * n_rays * 4)),
+                     dim3(kBlock), 0, sailor::as_stream(stream), K, R, dirs, n_cams, n_rays, out);
+  return sailor::check_launch("rays_interpolate");
+}
+
+extern "C" int sailor_forward_upsampling(
+    const float* ray_ori, const float* ray_dir, const float* depth, const float* coarse_rgb,
+    const float* ray_feats, int N, const float* tar_K, const float* tar_R, const float* K,
+    const float* R, const float* T, int V, int nei0, int nei1, const float* rgbf_cl, int Hf,
+    int Wf, const float* rgb_cl, int Hc, int Wc, const float* depths, int Hd, int Wd, int res,
+    float valid_min, float valid_max, const void* params_blend, float* out, void* stream) {
+  SAILOR_REQUIRE(N >= 0 && V > 0, "forward_upsampling: bad sizes N=%d V=%d", N, V);
+  SAILOR_REQUIRE(nei0 >= 0 && nei0 < V && nei1 >= 0 && nei1 < V,
+                 "forward_upsampling: neighbour views (%d, %d) outside [0, %d)", nei0, nei1, V);
+  SAILOR_REQUIRE(Hf > 1 && Wf > 1 && Hc > 1 && Wc > 1 && Hd > 1 && Wd > 1,
+                 "forward_upsampling: maps must be at least 2 x 2");
+  SAILOR_REQUIRE(res >= 2, "forward_upsampling: res must be at least 2");
+  if (N == 0) return SAILOR_OK;
+  if (int err = setup_all()) return err;
+  UpsampleArgs a{ray_ori, ray_dir, depth, coarse_rgb, ray_feats, tar_K, tar_R, K, R, T,
+                 rgbf_cl, rgb_cl, depths, (long long)N, Hf, Wf, Hc, Wc, Hd, Wd, res,
+                 {nei0, nei1}, valid_min, valid_max, out};
+  hipLaunchKernelGGL(upsample_kernel, dim3(grid_for((long long)N * 4)), dim3(kBlock), kBlendLds,
+                     sailor::as_stream(stream), static_cast<const half_t*>(params_blend), a);
+  return sailor::check_launch("forward_upsampling");
 }

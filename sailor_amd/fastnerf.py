@@ -1,8 +1,12 @@
-"""The reference GUI's FastNerf inference step on this library's kernels:
-per-view sample features (sailor_sample_features: proj_persp, z_normalizer,
-grid_sample x4, t-psdf, camera-space ray directions) feeding
-InferNerf.infer_nerf, for the rays whose samples are all valid
-(gui/gui_render.py:713-751, forward_nerf).
+"""The reference GUI's FastNerf inference steps on this library's kernels:
+  forward_nerf        per-view sample features (sailor_sample_features:
+                      proj_persp, z_normalizer, grid_sample x4, t-psdf,
+                      camera-space ray directions) feeding InferNerf.infer_nerf,
+                      for the rays whose samples are all valid
+                      (gui/gui_render.py:713-751);
+  rays_interpolate    the 4 sub-pixel rays of each ray (gui_render.py:753-773);
+  forward_upsampling  sub-pixel rays, neighbour-view sampling and the blending
+                      MLP in one kernel (gui_render.py:775-838).
 """
 from __future__ import annotations
 
@@ -88,3 +92,77 @@ def forward_nerf(ray_ori, ray_dir, sorted_dists, ks, rts, geo_cl, depths, rgbf_c
     o_d[:, valid] = d
     o_f[:, valid] = f
     return o_c, o_d, o_f
+
+
+def rays_interpolate(ray_dirs, num_rays, tar_K, tar_R):
+    """gui_render.rays_interpolate: ray_dirs [.., num_rays, 3] of the cameras
+    tar_K, tar_R ([.., 3, 3], one per camera) -> [1, cams * num_rays * 4, 3]
+    sub-pixel directions (ray-major, 4 per ray: +0 / +0.5 in x, then y)."""
+    dev = ray_dirs.device
+    K = as_input(tar_K, "tar_K", _f32, dev).reshape(-1, 3, 3)
+    R = as_input(tar_R, "tar_R", _f32, dev).reshape(-1, 3, 3)
+    n = int(num_rays)
+    d = as_input(ray_dirs, "ray_dirs", _f32, dev).reshape(-1, n, 3)
+    cams = d.shape[0]
+    if K.shape[0] != cams or R.shape[0] != cams:
+        raise RuntimeError(f"rays_interpolate: {cams} ray sets for {K.shape[0]} / {R.shape[0]} "
+                           "cameras")
+    out = torch.empty((1, cams * n * 4, 3), dtype=_f32, device=dev)
+    call("sailor_rays_interpolate", ptr(K), ptr(R), ptr(d), cams, n, ptr(out), stream_of(dev))
+    return out
+
+
+def _neighbours(nei_ids):
+    ids = nei_ids.reshape(-1)[:2].tolist() if isinstance(nei_ids, torch.Tensor) \
+        else list(nei_ids)[:2]
+    if len(ids) != 2:
+        raise RuntimeError("forward_upsampling blends exactly 2 neighbour views")
+    return int(ids[0]), int(ids[1])
+
+
+def forward_upsampling(ray_ori, ray_dir, o_d, o_c, o_c_feat, tar_K, tar_R, ks, rs, ts,
+                       rgb_feats_cl, rgbs_cl, depths, nei_ids, params_blend,
+                       valid_depth_min=0.2, valid_depth_max=4.0, res=512):
+    """gui_render.forward_upsampling + sample_neighbor for one target camera.
+    ray_ori, ray_dir [1,R,3]; rendered o_d [1,R,1], o_c [1,R,3], o_c_feat
+    [1,R,30]; tar_K, tar_R [3,3]; source views ks, rs [V,3,3], ts [V,3(,1)];
+    channels-last high-resolution colour features rgb_feats_cl [V,Hf,Wf,16]
+    and colours rgbs_cl [V,Hc,Wc,3]; depths [V,(1,)Hd,Wd]; nei_ids the 2
+    neighbour view indices; params_blend the 15075 blending halves.
+    Returns the sub-pixel colours [1,R,4,3] (zeros where o_d is outside
+    (valid_depth_min, valid_depth_max)).  Rays are not compacted: the depth
+    test runs in the kernel, so there is no host synchronisation."""
+    dev = o_d.device
+    R = o_d.reshape(-1).numel()
+    ori = as_input(ray_ori, "ray_ori", _f32, dev).reshape(R, 3)
+    dirs = as_input(ray_dir, "ray_dir", _f32, dev).reshape(R, 3)
+    od = as_input(o_d, "o_d", _f32, dev).reshape(R)
+    oc = as_input(o_c, "o_c", _f32, dev).reshape(R, 3)
+    of = as_input(o_c_feat, "o_c_feat", _f32, dev).reshape(R, 30)
+    tK = as_input(tar_K, "tar_K", _f32, dev).reshape(3, 3)
+    tR = as_input(tar_R, "tar_R", _f32, dev).reshape(3, 3)
+    K = as_input(ks, "ks", _f32, dev).reshape(-1, 3, 3)
+    V = K.shape[0]
+    Rm = as_input(rs, "rs", _f32, dev).reshape(V, 3, 3)
+    T = as_input(ts, "ts", _f32, dev).reshape(V, 3)
+    rf = as_input(rgb_feats_cl, "rgb_feats_cl", _f32, dev)
+    rc = as_input(rgbs_cl, "rgbs_cl", _f32, dev)
+    dp = as_input(depths, "depths", _f32, dev)
+    if rf.dim() != 4 or rf.shape[0] != V or rf.shape[-1] != 16:
+        raise RuntimeError("rgb_feats_cl must be [V, H, W, 16]")
+    if rc.dim() != 4 or rc.shape[0] != V or rc.shape[-1] != 3:
+        raise RuntimeError("rgbs_cl must be [V, H, W, 3]")
+    Hd, Wd = dp.shape[-2:]
+    if dp.numel() != V * Hd * Wd:
+        raise RuntimeError("depths must be [V, (1,) H, W]")
+    if not isinstance(params_blend, torch.Tensor) or not params_blend.is_cuda \
+            or params_blend.numel() < InferNerf.BLEND_PARAMS:
+        raise RuntimeError("params_blend must be a cuda tensor of 15075 halves")
+    pb = params_blend.to(device=dev, dtype=torch.float16).contiguous()
+    n0, n1 = _neighbours(nei_ids)
+    out = torch.empty((1, R, 4, 3), dtype=_f32, device=dev)
+    call("sailor_forward_upsampling", ptr(ori), ptr(dirs), ptr(od), ptr(oc), ptr(of), R,
+         ptr(tK), ptr(tR), ptr(K), ptr(Rm), ptr(T), V, n0, n1, ptr(rf), rf.shape[1], rf.shape[2],
+         ptr(rc), rc.shape[1], rc.shape[2], ptr(dp), Hd, Wd, int(res), float(valid_depth_min),
+         float(valid_depth_max), ptr(pb), ptr(out), stream_of(dev))
+    return out
