@@ -272,6 +272,59 @@ def fastnerf_infer(steps):
             "frac_of_fp32_vector_peak": flops / (ms / 1e3) / 1e12 / 157.3}
 
 
+def fastnerf_upsampling(steps):
+    """The GUI's sub-pixel upsampling (fastnerf.forward_upsampling) on one
+    512^2 target: 262,144 rays, ~60% on the surface (the rest outside the
+    depth range), 4 sub-pixel rays each blended from 2 of 4 views with
+    1024^2 x 16 colour features, 1024^2 colours and 512^2 depths; synthetic
+    inputs.  Algorithmic FLOPs per upsampled sub-pixel ray: the blending MLP
+    2*(40*64 + 64*64 + 64*32 + 62*64 + 64*32 + 32*3) = 29,632."""
+    import numpy as np
+    import torch
+    from sailor_amd import scenes
+    from sailor_amd.fastnerf import forward_upsampling
+    from sailor_amd.model import fastnerf_params
+    dev = "cuda"
+    side, V = 512, 4
+    R = side * side
+    g = torch.Generator(device=dev).manual_seed(21)
+    cams = scenes.ring_cameras(V, 600.0, 512, 512)
+    tgt = scenes.target_camera(512, 512, 600.0)
+    f = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(dev)
+    ks, rs = f(np.stack([c.K for c in cams])), f(np.stack([c.R for c in cams]))
+    ts = f(np.stack([c.T for c in cams]))
+    yy, xx = np.meshgrid(np.arange(side) + 0.5, np.arange(side) + 0.5, indexing="ij")
+    h = np.stack([xx.ravel(), yy.ravel(), np.ones(R)], 1)
+    d = (np.linalg.inv(tgt.K.astype(np.float64)) @ h.T).T @ tgt.R.astype(np.float64)
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    o = -tgt.R.T.astype(np.float64) @ tgt.T.astype(np.float64)
+    ray_ori = f(np.broadcast_to(o, (R, 3)))[None]
+    ray_dir = f(d)[None]
+    on = torch.rand((1, R, 1), device=dev, generator=g) < 0.6
+    o_d = torch.where(on, 2.0 + torch.rand((1, R, 1), device=dev, generator=g),
+                      torch.zeros((1, R, 1), device=dev))
+    o_c = torch.rand((1, R, 3), device=dev, generator=g)
+    o_f = torch.randn((1, R, 30), device=dev, generator=g)
+    rgbf = torch.randn((V, 1024, 1024, 16), device=dev, generator=g)
+    rgbs = torch.rand((V, 1024, 1024, 3), device=dev, generator=g)
+    deps = 2.0 + torch.rand((V, 512, 512), device=dev, generator=g)
+    pb = torch.from_numpy(fastnerf_params()[2]).to(dev)
+
+    def go():
+        return forward_upsampling(ray_ori, ray_dir, o_d, o_c, o_f, f(tgt.K), f(tgt.R), ks, rs, ts,
+                                  rgbf, rgbs, deps, (0, 1), pb)
+    go()
+    torch.cuda.synchronize()
+    ms = _median_ms(go, steps)
+    n_up = int(on.sum()) * 4
+    flops = n_up * 2 * (40 * 64 + 64 * 64 + 64 * 32 + 62 * 64 + 64 * 32 + 32 * 3)
+    return {"workload": f"forward_upsampling: {side}^2 rays ({n_up // 4} on the surface) x 4 "
+                        "sub-pixels, 2 of 4 views with 1024^2 features",
+            "ms": ms, "subpixel_rays_per_s": n_up / (ms / 1e3), "flops": flops,
+            "vector_tflops": flops / (ms / 1e3) / 1e12,
+            "frac_of_fp32_vector_peak": flops / (ms / 1e3) / 1e12 / 157.3}
+
+
 def batching_form(steps, n_frames=16, precs=("fp32", "bf16")):
     """Config-5 block: 16 frames of uneven size (own seeded scene each,
     performer scale U(0.6, 1.2), 2-6 source views of 1024^2, 256^3 grid,
@@ -431,6 +484,7 @@ def main():
         return
     if args.infer_only:
         print(json.dumps(fastnerf_infer(args.steps)))
+        print(json.dumps(fastnerf_upsampling(args.steps)))
         return
     if args.training_only:
         print(json.dumps(training_form(args.steps, args.warmup, with_cpu=not args.no_cpu)))
@@ -514,6 +568,7 @@ def main():
         result["c5_batching_form"] = batching_form(max(3, args.steps // 4))
     if not args.no_infer and args.workload == "c2":
         result["fastnerf_infer"] = fastnerf_infer(max(3, args.steps // 4))
+        result["fastnerf_upsampling"] = fastnerf_upsampling(max(3, args.steps // 4))
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(p32)
     else:

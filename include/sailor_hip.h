@@ -148,6 +148,63 @@ int sailor_octree_sampling_coarse(const int32_t* hit_idx, const float* tmin,
                                   int32_t* sampled_idx, float* sampled_t,
                                   void* stream);
 
+/* Replaces MultiLayerOctree::octree_traversal (intersection_kernel.cu:738-825;
+ * kernels ray_voxel_traversal with RayVoxelIntersectionNodeList :72-126,
+ * ray_octree_traversal :441-643), the recursive form used in training.
+ *  Coarse lists (hit_idx_coarse, tmin_coarse, tmax_coarse [B,R,n_max_hits_coarse])
+ *  hold the occupied coarse cells the DDA crosses; fine lists (hit_idx, tmin,
+ *  tmax [B,R,n_max_hits]) hold, per coarse hit in order, the coarse entry and
+ *  the children the ray crosses (last child first; a child whose interval
+ *  equals an earlier entry's replaces it when smaller); valid_len [B,R] =
+ *  sum of (tmax - tmin) * coarse size / entry size.  All lists -1 padded.
+ *  status int32 [3], zeroed by the caller: max coarse count, max fine count,
+ *  1 if a list needed more than its capacity (those rays are truncated).
+ *  The reference's entry order depends on atomics; this order is one of its
+ *  possible orders and is deterministic. */
+int sailor_octree_traversal(const int32_t* vol_coarse, const float* node_pos,
+                            const int32_t* node_info, int n_nodes, const float* vol_bbox,
+                            const float* voxel_size, int B, int dim_fine, int dim_coarse,
+                            int rate, const float* ray_ori, const float* ray_dir, int R,
+                            int n_max_hits_coarse, int n_max_hits, int th_early_stop,
+                            int opt_num_intersected, int32_t* hit_idx_coarse,
+                            float* tmin_coarse, float* tmax_coarse, int32_t* hit_idx,
+                            float* tmin, float* tmax, float* valid_len, int32_t* status,
+                            void* stream);
+
+/* Replaces MultiLayerOctree::ray_voxels_points_sampling (sampling_kernel.cu:
+ * 1178-1248; kernels ray_voxels_sampling_kernel :911-982 and
+ * ray_voxels_further_sampling_kernel :984-1068), the adaptive sampler:
+ * per hit max(1, (int)((max-min) * start_size/size * S / valid_len)) samples
+ * at min + off (i+1) - off/2 + off u, slots in hit order up to S, the rest
+ * in the smallest hit node.  noise f32 [B,R,S]; outputs sampled_idx int32,
+ * sampled_t f32 [B,R,S] (-1 for unused slots);
+ * start_scale = 2^(levels - start_level - 1). */
+int sailor_octree_sampling(const int32_t* hit_idx, const float* tmin, const float* tmax,
+                           const float* noise, const float* valid_len, const float* node_pos,
+                           int n_nodes, const float* voxel_size, int B, int R, int n_max_hits,
+                           int S, int start_scale, int32_t* sampled_idx, float* sampled_t,
+                           void* stream);
+
+/* Replaces MultiLayerOctree::trilinear_aggregate_features (sampling_kernel.cu:
+ * 1384-1421, kernel :162-234): feats f32 [B,C,R,S,8] (per sample the values
+ * at the 8 corners of its node, corner id 4x+2y+z) -> out f32 [B,C,R,S],
+ * trilinear in the point ray_ori + ray_dir * sampled_t; 0 where
+ * sampled_idx is -1 (or outside [0, n_nodes)).  feats 16-byte aligned. */
+int sailor_trilinear_aggregate(const float* node_pos, int n_nodes, const float* ray_ori,
+                               const float* ray_dir, const int32_t* sampled_idx,
+                               const float* sampled_t, const float* feats, int B, int R, int S,
+                               int C, float* out, void* stream);
+
+/* Replaces MultiLayerOctree::trilinear_aggregate_features_backward
+ * (sampling_kernel.cu:1423-1462, kernel :237-327): grad_out f32 [B,C,R,S]
+ * -> grad_feats f32 [B,C,R,S,8] (corner weight times grad; 0 where
+ * sampled_idx is -1).  grad_feats 16-byte aligned. */
+int sailor_trilinear_aggregate_backward(const float* node_pos, int n_nodes,
+                                        const float* ray_ori, const float* ray_dir,
+                                        const int32_t* sampled_idx, const float* sampled_t,
+                                        const float* grad_out, int B, int R, int S, int C,
+                                        float* grad_feats, void* stream);
+
 /* Replaces MultiLayerOctree::sort_samplings (sampling_kernel.cu:1308-1329):
  * ascending sort of each row of S values; ties keep their original order.
  *  t f32 [n_rows,S] -> sorted f32 [n_rows,S], order int32 [n_rows,S]       */

@@ -80,6 +80,14 @@ def lib():
                                                         f32p, f32p]
         L.orc_freq_fwd.argtypes = [f32p, c_ll, c_int, c_int, f32p]
         L.orc_freq_bwd.argtypes = [f32p, f32p, c_ll, c_int, c_int, f32p]
+        L.orc_octree_traversal.argtypes = [i32p, f32p, i32p, f32p, f32p] + [c_int] * 4 + \
+            [f32p, f32p] + [c_int] * 5 + [i32p, f32p, f32p, i32p, f32p, f32p, f32p, i32p]
+        L.orc_sampling_adaptive.argtypes = [i32p, f32p, f32p, f32p, f32p, f32p, f32p] + \
+            [c_int] * 5 + [i32p, f32p]
+        L.orc_trilinear_fwd.argtypes = [f32p, f32p, f32p, i32p, f32p, f32p] + [c_int] * 4 + \
+            [f32p]
+        L.orc_trilinear_bwd.argtypes = [f32p, f32p, f32p, i32p, f32p, f32p] + [c_int] * 4 + \
+            [f32p]
         _lib = L
     return _lib
 
@@ -209,6 +217,62 @@ def sampling(hit_idx, tmin, tmax, noise, valid_len, node_pos, voxel_size, S, sta
     lib().orc_sampling(hit_idx, _f(tmin), _f(tmax), _f(noise), _f(valid_len), _f(node_pos),
                        _f(voxel_size), B, R, n_max, S, start_scale, gmax, sidx, st)
     return sidx, st
+
+
+def octree_traversal(vol_coarse, node_pos, node_info, vol_bbox, voxel_size, dim_fine, rate,
+                     ray_ori, ray_dir, n_max_hits_coarse, n_max_hits, th_early_stop=-1,
+                     opt_num=-1):
+    """MultiLayerOctree::octree_traversal (recursive form), canonical order."""
+    vc = _i(vol_coarse)
+    B, dc = vc.shape[0], vc.shape[1]
+    ray_ori, ray_dir = _f(ray_ori), _f(ray_dir)
+    R = ray_ori.shape[1]
+    out = dict(idx_c=np.empty((B, R, n_max_hits_coarse), np.int32),
+               tmin_c=np.empty((B, R, n_max_hits_coarse), np.float32),
+               tmax_c=np.empty((B, R, n_max_hits_coarse), np.float32),
+               idx=np.empty((B, R, n_max_hits), np.int32),
+               tmin=np.empty((B, R, n_max_hits), np.float32),
+               tmax=np.empty((B, R, n_max_hits), np.float32),
+               valid_len=np.empty((B, R), np.float32), status=np.zeros(3, np.int32))
+    lib().orc_octree_traversal(vc, _f(node_pos), _i(node_info), _f(vol_bbox), _f(voxel_size), B,
+                               dim_fine, dc, rate, ray_ori, ray_dir, R, n_max_hits_coarse,
+                               n_max_hits, th_early_stop, opt_num, out["idx_c"], out["tmin_c"],
+                               out["tmax_c"], out["idx"], out["tmin"], out["tmax"],
+                               out["valid_len"], out["status"])
+    return out
+
+
+def sampling_adaptive(hit_idx, tmin, tmax, noise, valid_len, node_pos, voxel_size, S,
+                      start_scale):
+    """MultiLayerOctree::ray_voxels_points_sampling, canonical slot order."""
+    hit_idx = _i(hit_idx)
+    B, R, n_max = hit_idx.shape
+    sidx = np.empty((B, R, S), np.int32)
+    st = np.empty((B, R, S), np.float32)
+    lib().orc_sampling_adaptive(hit_idx, _f(tmin), _f(tmax), _f(noise), _f(valid_len),
+                                _f(node_pos), _f(voxel_size), B, R, n_max, S, start_scale,
+                                sidx, st)
+    return sidx, st
+
+
+def trilinear_fwd(node_pos, ray_ori, ray_dir, sidx, st, feats):
+    """trilinear_aggregate_features: feats [B,C,R,S,8] -> [B,C,R,S]."""
+    feats = _f(feats)
+    B, C, R, S, _ = feats.shape
+    out = np.empty((B, C, R, S), np.float32)
+    lib().orc_trilinear_fwd(_f(node_pos), _f(ray_ori), _f(ray_dir), _i(sidx), _f(st), feats,
+                            B, R, S, C, out)
+    return out
+
+
+def trilinear_bwd(node_pos, ray_ori, ray_dir, sidx, st, grad):
+    """trilinear_aggregate_features_backward: grad [B,C,R,S] -> [B,C,R,S,8]."""
+    grad = _f(grad)
+    B, C, R, S = grad.shape
+    out = np.empty((B, C, R, S, 8), np.float32)
+    lib().orc_trilinear_bwd(_f(node_pos), _f(ray_ori), _f(ray_dir), _i(sidx), _f(st), grad,
+                            B, R, S, C, out)
+    return out
 
 
 def sort_samplings(t):

@@ -699,3 +699,250 @@ void orc_freq_bwd(const float* gy, const float* y, long long N, int D, int deg, 
       gx[n * D + d] = r;
     }
 }
+
+/* ---------------------------------------------------------------------------
+ * Training-only kernels.
+ * ------------------------------------------------------------------------- */
+
+/* MultiLayerOctree::octree_traversal, intersection_kernel.cu:738-825:
+ * coarse DDA listing the coarse cells (ray_voxel_traversal with
+ * RayVoxelIntersectionNodeList :72-126, no valid length), then per coarse hit
+ * the DFS of ray_octree_traversal :441-643: the coarse entry with its coarse
+ * interval, then the children popped from the stack (last child first) with
+ * their slab intervals; an interval equal to an earlier entry of the same
+ * coarse hit replaces that entry's node when the child is smaller.  Entries
+ * are appended per ray in coarse-hit order (canonical order of the
+ * reference's atomicAdd queue) and valid_len accumulates
+ * (max - min) * (coarse size / entry size) in the same order.
+ * status[0] = max coarse count, status[1] = max fine count, status[2] = 1 if
+ * a fine list overflowed n_max (such rays are truncated). */
+void orc_octree_traversal(const int* vc, const float* node_pos, const int* node_info,
+                          const float* vol_bbox, const float* voxel_size, int B, int dim_fine,
+                          int dim_coarse, int rate, const float* ray_ori, const float* ray_dir,
+                          int R, int n_max_c, int n_max, int th_early_stop, int opt_num,
+                          int* idx_c, float* tmin_c, float* tmax_c, int* idx, float* tmin,
+                          float* tmax, float* valid_len, int* status) {
+  int ovf_c = 0;
+  float* vl_c = (float*)malloc(sizeof(float) * (size_t)B * R);
+  int gmax_c = orc_traverse(vc, node_pos, node_info, vol_bbox, voxel_size, B, dim_fine,
+                            dim_coarse, rate, ray_ori, ray_dir, R, n_max_c, th_early_stop, 1,
+                            opt_num, idx_c, tmin_c, tmax_c, vl_c, &ovf_c);
+  free(vl_c);
+  int gmax = 0, ovf = 0;
+#pragma omp parallel for reduction(max : gmax) reduction(| : ovf) schedule(dynamic, 256)
+  for (long long k = 0; k < (long long)B * R; ++k) {
+    const int* hc = idx_c + k * n_max_c;
+    const float* c0 = tmin_c + k * n_max_c;
+    const float* c1 = tmax_c + k * n_max_c;
+    int* hi = idx + k * n_max;
+    float* t0 = tmin + k * n_max;
+    float* t1 = tmax + k * n_max;
+    for (int i = 0; i < n_max; ++i) { hi[i] = -1; t0[i] = -1.0f; t1[i] = -1.0f; }
+    const float* o = ray_ori + 3 * k;
+    const float* d = ray_dir + 3 * k;
+    float vl = 0.0f;
+    int cnt = 0;
+    for (int h = 0; h < n_max_c; ++h) {
+      int node = hc[h];
+      if (node == -1) break;
+      float start_vs = node_pos[4 * node + 3];
+      int g0 = cnt;
+      if (cnt < n_max) { hi[cnt] = node; t0[cnt] = c0[h]; t1[cnt] = c1[h]; } else ovf = 1;
+      cnt += 1;
+      int first = node_info[4 * node + 2], nch = node_info[4 * node + 3];
+      if (first >= 0) {
+        for (int ci = nch - 1; ci >= 0; --ci) {
+          int ch = first + ci;
+          float cd[2];
+          ray_voxel(o, d, node_pos + 4 * ch, node_pos[4 * ch + 3], cd);
+          if (cd[0] == -1.0f || cd[1] == -1.0f) continue;
+          float vsz = node_pos[4 * ch + 3];
+          int found = 0, lim = cnt < n_max ? cnt : n_max;
+          for (int j = g0; j < lim; ++j) {
+            if (t0[j] == cd[0] && t1[j] == cd[1]) {
+              if (node_pos[4 * hi[j] + 3] > vsz) hi[j] = ch;
+              found = 1;
+              break;
+            }
+          }
+          if (!found) {
+            if (cnt < n_max) { hi[cnt] = ch; t0[cnt] = cd[0]; t1[cnt] = cd[1]; } else ovf = 1;
+            cnt += 1;
+          }
+        }
+      }
+      int lim = cnt < n_max ? cnt : n_max;
+      for (int j = g0; j < lim; ++j) {
+        float scale = start_vs / node_pos[4 * hi[j] + 3];
+        vl += (t1[j] - t0[j]) * scale;
+      }
+    }
+    valid_len[k] = vl;
+    if (cnt > gmax) gmax = cnt;
+  }
+  status[0] = gmax_c;
+  status[1] = gmax;
+  status[2] = ovf | ovf_c;
+}
+
+/* MultiLayerOctree::ray_voxels_points_sampling, sampling_kernel.cu:1178-1248:
+ * ray_voxels_sampling_kernel :911-982 places n = max(1, (int)((max - min) *
+ * (start_size / size) * S / valid_len)) samples per hit at
+ * min + off (i + 1) - off / 2 + off u (off = (max - min) / (n + 1), u the
+ * noise of the slot), slots taken in hit order (canonical order of the
+ * atomicAdd queue) until S are used; ray_voxels_further_sampling_kernel
+ * :984-1068 fills the remaining slots the same way in the smallest hit node
+ * (the one with the nearest interval mid-point among equal sizes).
+ * start_scale = 2^(levels - start_level - 1) (powf(2.0, ...) of :952). */
+void orc_sampling_adaptive(const int* hit_idx, const float* min_depths, const float* max_depths,
+                           const float* noise, const float* rays_valid_len, const float* node_pos,
+                           const float* voxel_size, int B, int R, int n_max, int S,
+                           int start_scale, int* sampled_idx, float* sampled_t) {
+#pragma omp parallel for schedule(static)
+  for (long long k = 0; k < (long long)B * R; ++k) {
+    int b = (int)(k / R);
+    const int* hi = hit_idx + k * n_max;
+    const float* mn_ = min_depths + k * n_max;
+    const float* mx_ = max_depths + k * n_max;
+    const float* rnd = noise + k * S;
+    int* oi = sampled_idx + k * S;
+    float* ot = sampled_t + k * S;
+    for (int j = 0; j < S; ++j) { oi[j] = -1; ot[j] = -1.0f; }
+    float valid_len = rays_valid_len[k];
+    float start_vs = voxel_size[b] * (float)start_scale;
+    int cnt = 0;
+    for (int i = 0; i < n_max; ++i) {
+      int vi = hi[i];
+      if (vi == -1) continue;
+      float mn = mn_[i], mx = mx_[i];
+      float cvs = node_pos[4 * vi + 3];
+      float rate = start_vs / cvs;
+      int n = sat_f2i((mx - mn) * rate * (float)S / valid_len);
+      n = n >= 1 ? n : 1;
+      float off = (mx - mn) / (float)(n + 1);
+      for (int j = 0; j < n; ++j) {
+        if (cnt >= S) break;
+        int rid = cnt++;
+        float u = rnd[rid];
+        float basic = mn + off * (float)(j + 1);
+        ot[rid] = (float)((double)basic - (double)off / 2.0 + (double)(off * u));
+        oi[rid] = vi;
+      }
+    }
+    float min_mid = 99999.0f, min_vs = 99999.0f, z0 = -1.0f, z1 = -1.0f;
+    int min_idx = -1;
+    for (int i = 0; i < n_max; ++i) {
+      int vi = hi[i];
+      if (vi == -1) break;
+      float vs = node_pos[4 * vi + 3];
+      if (vs <= min_vs) {
+        float a = mn_[i], c = mx_[i];
+        float mid = (float)((double)(a + c) / 2.0);
+        if (vs == min_vs) {
+          if (mid < min_mid) { min_mid = mid; z0 = a; z1 = c; min_idx = vi; }
+        } else {
+          min_mid = mid; z0 = a; z1 = c; min_idx = vi;
+        }
+        min_vs = vs;
+      }
+    }
+    if (min_idx < 0) continue;
+    int n_new = S - cnt;
+    if (n_new <= 0) continue;
+    float off = (z1 - z0) / (float)(n_new + 1);
+    int j = 0;
+    for (int i = cnt; i < S; ++i) {
+      float basic = z0 + off * (float)(j + 1);
+      float u = rnd[i];
+      ot[i] = (float)((double)basic - (double)off / 2.0 + (double)(off * u));
+      oi[i] = min_idx;
+      ++j;
+    }
+  }
+}
+
+/* Interpolation weights of trilinear_aggregate_features_kernel,
+ * sampling_kernel.cu:162-234: the point ori + dir * t inside node vi (corners
+ * centre +- size / 2, corner id 4 x + 2 y + z), tx along x, ty along y from
+ * the mean y of the corner pairs, tz along z from the mean z of 4 corners
+ * (double literals of :225-227). */
+static void tri_weights(const float* node_pos, int vi, const float* o, const float* d, float t,
+                        float* tx, float* ty, float* tz) {
+  float p[3];
+  for (int a = 0; a < 3; ++a) p[a] = o[a] + d[a] * t;
+  const float* c = node_pos + 4 * vi;
+  float vs = c[3];
+  float lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = c[a] + ((float)(-1) * vs) / 2.0f;
+    hi[a] = c[a] + ((float)(1) * vs) / 2.0f;
+  }
+  *tx = (p[0] - lo[0]) / (hi[0] - lo[0]);
+  double ym = (double)(lo[1] + lo[1]) / 2.0, yp = (double)(hi[1] + hi[1]) / 2.0;
+  *ty = (float)(((double)p[1] - ym) / (yp - ym));
+  double zm = (double)(((lo[2] + lo[2]) + lo[2]) + lo[2]) / 4.0;
+  double zp = (double)(((hi[2] + hi[2]) + hi[2]) + hi[2]) / 4.0;
+  *tz = (float)(((double)p[2] - zm) / (zp - zm));
+}
+
+static float lerpf_(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+/* trilinear_aggregate_features (sampling_kernel.cu:162-234, 1384-1421):
+ * feats [B,C,R,S,8] -> out [B,C,R,S]; samples with index -1 give 0. */
+void orc_trilinear_fwd(const float* node_pos, const float* ray_ori, const float* ray_dir,
+                       const int* sidx, const float* st, const float* feats, int B, int R,
+                       int S, int C, float* out) {
+#pragma omp parallel for schedule(static)
+  for (long long k = 0; k < (long long)B * R * S; ++k) {
+    long long b = k / ((long long)R * S), rs = k % ((long long)R * S), r = rs / S;
+    int vi = sidx[k];
+    float tx = 0, ty = 0, tz = 0;
+    if (vi != -1)
+      tri_weights(node_pos, vi, ray_ori + 3 * (b * R + r), ray_dir + 3 * (b * R + r), st[k],
+                  &tx, &ty, &tz);
+    for (int c = 0; c < C; ++c) {
+      long long oidx = (b * C + c) * (long long)R * S + rs;
+      if (vi == -1) { out[oidx] = 0.0f; continue; }
+      const float* f = feats + oidx * 8;
+      float x04 = lerpf_(f[0], f[4], tx), x15 = lerpf_(f[1], f[5], tx);
+      float x26 = lerpf_(f[2], f[6], tx), x37 = lerpf_(f[3], f[7], tx);
+      float y0 = lerpf_(x04, x26, ty), y1 = lerpf_(x15, x37, ty);
+      out[oidx] = lerpf_(y0, y1, tz);
+    }
+  }
+}
+
+/* trilinear_aggregate_features_backward (sampling_kernel.cu:237-327):
+ * grad [B,C,R,S] -> grad_feats [B,C,R,S,8] (corner weights times grad). */
+void orc_trilinear_bwd(const float* node_pos, const float* ray_ori, const float* ray_dir,
+                       const int* sidx, const float* st, const float* grad, int B, int R, int S,
+                       int C, float* grad_feats) {
+#pragma omp parallel for schedule(static)
+  for (long long k = 0; k < (long long)B * R * S; ++k) {
+    long long b = k / ((long long)R * S), rs = k % ((long long)R * S), r = rs / S;
+    int vi = sidx[k];
+    float w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (vi != -1) {
+      float tx, ty, tz;
+      tri_weights(node_pos, vi, ray_ori + 3 * (b * R + r), ray_dir + 3 * (b * R + r), st[k],
+                  &tx, &ty, &tz);
+      float z1 = tz, z0 = 1 - tz, y26 = ty, y04 = 1 - ty, y37 = ty, y15 = 1 - ty;
+      float x4 = tx, x0 = 1 - tx, x6 = tx, x2 = 1 - tx, x5 = tx, x1 = 1 - tx, x7 = tx,
+            x3 = 1 - tx;
+      w[0] = x0 * y04 * z0;
+      w[1] = x1 * y15 * z1;
+      w[2] = x2 * y26 * z0;
+      w[3] = x3 * y37 * z1;
+      w[4] = x4 * y04 * z0;
+      w[5] = x5 * y15 * z1;
+      w[6] = x6 * y26 * z0;
+      w[7] = x7 * y37 * z1;
+    }
+    for (int c = 0; c < C; ++c) {
+      long long gidx = (b * C + c) * (long long)R * S + rs;
+      float g = grad[gidx];
+      float* o = grad_feats + gidx * 8;
+      for (int i = 0; i < 8; ++i) o[i] = (vi == -1) ? 0.0f : g * w[i];
+    }
+  }
+}

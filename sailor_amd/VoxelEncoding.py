@@ -258,6 +258,109 @@ class MultiLayerOctree:
                  ptr(sidx), ptr(st), stream_of(dev))
         return sidx, st
 
+    # -- octree_traversal (intersection_kernel.cu:738-825) --------------------
+    def octree_traversal(self, batch_rays_ori, batch_rays_dir, n_max_hits_coarse, n_max_hits,
+                         start_level, th_early_stop, opt_num_intersected_voxels):
+        """Recursive traversal (training): returns (idx, min_depths, max_depths,
+        ray_total_valid_len [B,R,1], idx_c, min_depths_c, max_depths_c)."""
+        dev = self.node_pos.device
+        ori = as_input(batch_rays_ori, "batch_rays_ori", _f32, dev)
+        dirs = as_input(batch_rays_dir, "batch_rays_dir", _f32, dev)
+        B, R = ori.shape[0], ori.shape[1]
+        if B != self.batch_size:
+            raise RuntimeError(f"rays batch {B} != tree batch {self.batch_size}")
+        lvl = self._top(start_level)
+        nc, nf = int(n_max_hits_coarse), int(n_max_hits)
+        o = dict(device=dev)
+        idx_c = torch.empty((B, R, nc), dtype=_i32, **o)
+        t0_c = torch.empty((B, R, nc), dtype=_f32, **o)
+        t1_c = torch.empty((B, R, nc), dtype=_f32, **o)
+        idx = torch.empty((B, R, nf), dtype=_i32, **o)
+        t0 = torch.empty((B, R, nf), dtype=_f32, **o)
+        t1 = torch.empty((B, R, nf), dtype=_f32, **o)
+        vl = torch.empty((B, R, 1), dtype=_f32, **o)
+        status = torch.zeros((3,), dtype=_i32, **o)
+        with torch.cuda.device(dev):
+            call("sailor_octree_traversal", ptr(self._vols[lvl]), ptr(self.node_pos),
+                 ptr(self.node_info), self.node_pos.shape[0], ptr(self.volume_bbox),
+                 ptr(self.voxel_size), B, self.dims[0], self.dims[lvl], self.rate, ptr(ori),
+                 ptr(dirs), R, nc, nf, int(th_early_stop), int(opt_num_intersected_voxels),
+                 ptr(idx_c), ptr(t0_c), ptr(t1_c), ptr(idx), ptr(t0), ptr(t1), ptr(vl),
+                 ptr(status), stream_of(dev))
+        st = status.tolist()  # host read, as the reference's thrust reads
+        self.num_max_intersected_voxels = st[0]
+        self.num_max_intersected_voxels_fine = st[1]
+        if st[2]:
+            raise RuntimeError(f"octree_traversal: a ray needed more than n_max_hits_coarse="
+                               f"{nc} / n_max_hits={nf} entries")
+        return idx, t0, t1, vl, idx_c, t0_c, t1_c
+
+    # -- ray_voxels_points_sampling (sampling_kernel.cu:1178-1248) ------------
+    def ray_voxels_points_sampling(self, hits_idxs, min_depths, max_depths, rays_valid_len,
+                                   uniform_rand_values, start_level, num_sampled):
+        """Adaptive sampling (training): -> (sampled_idx [B,R,S], sampled_depths)."""
+        dev = self.node_pos.device
+        hi = as_input(hits_idxs, "hits_idxs", _i32, dev)
+        t0 = as_input(min_depths, "min_depths", _f32, dev)
+        t1 = as_input(max_depths, "max_depths", _f32, dev)
+        vl = as_input(rays_valid_len, "rays_valid_len", _f32, dev)
+        nz = as_input(uniform_rand_values, "uniform_rand_values", _f32, dev)
+        B, R, n_max = hi.shape
+        S = int(num_sampled)
+        if nz.numel() < B * R * S:
+            raise RuntimeError("uniform_rand_values must be [B, N_rays, num_sampled]")
+        self._top(start_level)
+        start_scale = 2 ** (self.num_levels - int(start_level) - 1)
+        sidx = torch.empty((B, R, S), dtype=_i32, device=dev)
+        st = torch.empty((B, R, S), dtype=_f32, device=dev)
+        with torch.cuda.device(dev):
+            call("sailor_octree_sampling", ptr(hi), ptr(t0), ptr(t1), ptr(nz), ptr(vl),
+                 ptr(self.node_pos), self.node_pos.shape[0], ptr(self.voxel_size), B, R, n_max,
+                 S, start_scale, ptr(sidx), ptr(st), stream_of(dev))
+        return sidx, st
+
+    # -- trilinear_aggregate_features[_backward] (sampling_kernel.cu:1384-1462)
+    def _tri_args(self, ray_ori, ray_dir, sampled_z, sampled_idx):
+        dev = self.node_pos.device
+        ori = as_input(ray_ori, "ray_ori", _f32, dev)
+        dirs = as_input(ray_dir, "ray_dir", _f32, dev)
+        t = as_input(sampled_z, "sampled_z", _f32, dev)
+        si = as_input(sampled_idx, "sampled_idx", _i32, dev)
+        B, R, S = t.shape
+        if si.shape != t.shape or ori.numel() != B * R * 3 or dirs.numel() != B * R * 3:
+            raise RuntimeError("trilinear_aggregate_features: shapes of rays / samples differ")
+        return dev, ori, dirs, t, si, B, R, S
+
+    def trilinear_aggregate_features(self, ray_ori, ray_dir, sampled_z, sampled_idx,
+                                     input_feats):
+        """input_feats [B,C,R,S,8] -> [B,C,R,S]."""
+        dev, ori, dirs, t, si, B, R, S = self._tri_args(ray_ori, ray_dir, sampled_z, sampled_idx)
+        f = as_input(input_feats, "input_feats", _f32, dev)
+        C = f.shape[1]
+        if tuple(f.shape) != (B, C, R, S, 8):
+            raise RuntimeError(f"input_feats must be [B, C, R, S, 8], got {tuple(f.shape)}")
+        out = torch.empty((B, C, R, S), dtype=_f32, device=dev)
+        with torch.cuda.device(dev):
+            call("sailor_trilinear_aggregate", ptr(self.node_pos), self.node_pos.shape[0],
+                 ptr(ori), ptr(dirs), ptr(si), ptr(t), ptr(f), B, R, S, C, ptr(out),
+                 stream_of(dev))
+        return out
+
+    def trilinear_aggregate_features_backward(self, ray_ori, ray_dir, sampled_z, sampled_idx,
+                                              input_feats, grad_output):
+        """grad_output [B,C,R,S] -> gradient of input_feats [B,C,R,S,8]."""
+        dev, ori, dirs, t, si, B, R, S = self._tri_args(ray_ori, ray_dir, sampled_z, sampled_idx)
+        C = int(input_feats.shape[1])
+        g = as_input(grad_output, "grad_output", _f32, dev)
+        if tuple(g.shape) != (B, C, R, S):
+            raise RuntimeError(f"grad_output must be [B, C, R, S], got {tuple(g.shape)}")
+        out = torch.empty((B, C, R, S, 8), dtype=_f32, device=dev)
+        with torch.cuda.device(dev):
+            call("sailor_trilinear_aggregate_backward", ptr(self.node_pos),
+                 self.node_pos.shape[0], ptr(ori), ptr(dirs), ptr(si), ptr(t), ptr(g), B, R, S,
+                 C, ptr(out), stream_of(dev))
+        return out
+
     def sort_samplings(self, sampled_depths):
         return sort_samplings(sampled_depths)
 

@@ -22,7 +22,7 @@ ARCH = os.environ.get("SAILOR_OFFLOAD_ARCH", "gfx950")
 
 # files whose results are compared bit-exactly against the CPU oracle
 EXACT = {"fusion.hip", "octree.hip", "traversal.hip", "rays.hip", "render.hip", "preprocess.hip",
-         "freq.hip", "trace.hip"}
+         "freq.hip", "trace.hip", "training.hip"}
 
 COMMON = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",

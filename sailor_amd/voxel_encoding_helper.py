@@ -8,6 +8,8 @@ Functions and stateful helpers that SRONet / SRONetUp call.
                        backward run the HIP compositing kernels
   volume_render_torch  _volume_render_torch (:130-189) -- the reference's own
                        PyTorch formulation, kept for training-time parity
+  voxel_trilinear_interpolation  _voxel_tri_interpolation (:12-57) -- forward
+                       and backward run the HIP trilinear kernels
 """
 from __future__ import annotations
 
@@ -22,6 +24,36 @@ MAX_VOL_DIST = 999999
 
 def _gpu_id(t):
     return int(t.device.index or 0)
+
+
+class _voxel_tri_interpolation(Function):
+    """Trilinear aggregation of per-corner sample features inside the
+    samples' octree nodes; only the features receive a gradient."""
+
+    @staticmethod
+    def forward(ctx, octree, ray_ori, ray_dir, sampled_depths, sampled_idx, sampled_feats):
+        assert (not ray_ori.requires_grad) and (not ray_dir.requires_grad) \
+            and (not sampled_depths.requires_grad) and (not sampled_idx.requires_grad), \
+            "Not supported"
+        ctx.octree = octree
+        ctx.ray_ori, ctx.ray_dir = ray_ori, ray_dir
+        ctx.sampled_depths, ctx.sampled_idx = sampled_depths, sampled_idx
+        ctx.save_for_backward(sampled_feats)
+        return octree.trilinear_aggregate_features(ray_ori, ray_dir, sampled_depths, sampled_idx,
+                                                   sampled_feats)
+
+    @staticmethod
+    def backward(ctx, grad_output_feats):
+        sampled_feats, = ctx.saved_tensors
+        grad = None
+        if ctx.needs_input_grad[5]:
+            grad = ctx.octree.trilinear_aggregate_features_backward(
+                ctx.ray_ori, ctx.ray_dir, ctx.sampled_depths, ctx.sampled_idx, sampled_feats,
+                grad_output_feats.contiguous())
+        return None, None, None, None, None, grad
+
+
+voxel_trilinear_interpolation = _voxel_tri_interpolation.apply
 
 
 class _volume_rendering_inference(Function):
@@ -196,8 +228,13 @@ class OcTreeOptions(nn.Module):
     def _ray_intersection(self, _ray_ori, _ray_dir, _instersect_start_level=1, th_early_stop=-1,
                           only_voxel_traversal=True, is_training=False,
                           opt_num_intersected_voxels=-1):
+        # e.g. 256 // 2 ** (levels - start - 1) coarse cells per axis, 3 x that
+        self._n_max_hits_coarse = self._vol_dim[0] * 3 // self._rate ** (
+            self._levels - _instersect_start_level - 1)
         if not only_voxel_traversal:
-            raise NotImplementedError("octree_traversal (recursive path) is not provided")
+            return self._octree.octree_traversal(_ray_ori, _ray_dir, self._n_max_hits_coarse,
+                                                 self._n_max_hits, _instersect_start_level,
+                                                 th_early_stop, opt_num_intersected_voxels)
         self._n_max_hits_coarse = 1024
         return self._octree.voxel_traversal(_ray_ori, _ray_dir, self._n_max_hits_coarse,
                                             _instersect_start_level, th_early_stop, is_training,
@@ -206,6 +243,14 @@ class OcTreeOptions(nn.Module):
     @torch.no_grad()
     def _sort_sampling(self, sampled_depths):
         return self._octree.sort_samplings(sampled_depths)
+
+    @torch.no_grad()
+    def _ray_voxels_sampling_points(self, _idx, _min_depths, _max_depths, _ray_total_valid_len,
+                                    _uniform_noise, _instersect_start_level, _n_sampled_points):
+        return self._octree.ray_voxels_points_sampling(_idx, _min_depths, _max_depths,
+                                                       _ray_total_valid_len, _uniform_noise,
+                                                       _instersect_start_level,
+                                                       _n_sampled_points)
 
     @torch.no_grad()
     def _ray_voxels_points_sampling_coarse(self, _idx, _min_depths, _max_depths, _uniform_noise,
