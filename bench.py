@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 METRIC = ("512² frames/s from 4 RGBD views; fwd+bwd Mrays/s, BF16 vs FP32 at "
           "ΔPSNR≤0.05")
 PEAKS = {"fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0}  # dense MFMA TFLOP/s (MI355X)
+HBM_GBS = 8000.0  # MI355X HBM3E peak, GB/s
 
 
 def mlp_flops_per_sample_view(F, H):
@@ -325,6 +326,86 @@ def fastnerf_upsampling(steps):
             "frac_of_fp32_vector_peak": flops / (ms / 1e3) / 1e12 / 157.3}
 
 
+def training_kernels(steps, C=8):
+    """Training-only octree kernels on the C2 scene (256^3 tree, 512^2 target
+    rays = 262,144): recursive octree_traversal, adaptive
+    ray_voxels_points_sampling (48 samples), trilinear_aggregate_features
+    forward and backward with C=8 channels.  Trilinear roofline: algorithmic
+    bytes per (valid sample, channel) 32 B corner values + 4 B output (fwd),
+    4 B gradient + 32 B corner gradients (bwd), plus 8 B (index, depth) per
+    sample; HBM-bound."""
+    import torch
+    from sailor_amd._lib import call, ptr, stream_of
+    from sailor_amd.pipeline import CONFIGS, FramePipeline
+    cfg = CONFIGS["c2"]
+    p = FramePipeline.synthetic(cfg, prec="fp32")
+    p.build()
+    p.trace()
+    torch.cuda.synchronize()
+    dev = p.dev
+    st = stream_of(dev)
+    R, S = cfg.target * cfg.target, cfg.S
+    ori, dirs = p.rays[..., :3].contiguous(), p.rays[..., 3:].contiguous()
+    nc, nf = 3 * p.dc, 512
+    o = dict(device=dev)
+    idx_c = torch.empty((1, R, nc), dtype=torch.int32, **o)
+    t0c, t1c = torch.empty((1, R, nc), **o), torch.empty((1, R, nc), **o)
+    idx = torch.empty((1, R, nf), dtype=torch.int32, **o)
+    t0, t1 = torch.empty((1, R, nf), **o), torch.empty((1, R, nf), **o)
+    vl = torch.empty((1, R), **o)
+    status = torch.zeros(3, dtype=torch.int32, **o)
+
+    def trav():
+        call("sailor_octree_traversal", ptr(p.occ1), ptr(p.node_pos), ptr(p.node_info), p.cap,
+             ptr(p.bbox), ptr(p.vs), 1, cfg.dim, p.dc, cfg.rate, ptr(ori), ptr(dirs), R, nc, nf,
+             -1, -1, ptr(idx_c), ptr(t0c), ptr(t1c), ptr(idx), ptr(t0), ptr(t1), ptr(vl),
+             ptr(status), st)
+    trav()
+    torch.cuda.synchronize()
+    stat = status.tolist()
+    if stat[2]:
+        raise RuntimeError("training_kernels: traversal lists overflowed")
+    noise = torch.rand((1, R, S), **o)
+    sidx = torch.empty((1, R, S), dtype=torch.int32, **o)
+    stt = torch.empty((1, R, S), **o)
+
+    def samp():
+        call("sailor_octree_sampling", ptr(idx), ptr(t0), ptr(t1), ptr(noise), ptr(vl),
+             ptr(p.node_pos), p.cap, ptr(p.vs), 1, R, nf, S, 2, ptr(sidx), ptr(stt), st)
+    samp()
+    feats = torch.randn((1, C, R, S, 8), **o)
+    out = torch.empty((1, C, R, S), **o)
+    grad = torch.randn((1, C, R, S), **o)
+    gfeats = torch.empty_like(feats)
+
+    def fwd():
+        call("sailor_trilinear_aggregate", ptr(p.node_pos), p.cap, ptr(ori), ptr(dirs),
+             ptr(sidx), ptr(stt), ptr(feats), 1, R, S, C, ptr(out), st)
+
+    def bwd():
+        call("sailor_trilinear_aggregate_backward", ptr(p.node_pos), p.cap, ptr(ori),
+             ptr(dirs), ptr(sidx), ptr(stt), ptr(grad), 1, R, S, C, ptr(gfeats), st)
+    res = {"workload": f"c2 tree, {R} rays, {S} samples, C={C}"}
+    res["octree_traversal_ms"] = _median_ms(trav, steps)
+    res["max_hits_coarse_fine"] = stat[:2]
+    res["adaptive_sampling_ms"] = _median_ms(samp, steps)
+    n_valid = int((sidx >= 0).sum())
+    res["valid_samples"] = n_valid
+    for name, fn in (("trilinear_fwd", fwd), ("trilinear_bwd", bwd)):
+        ms = _median_ms(fn, steps)
+        # valid samples read their C x 8 corner values (fwd) or C gradients
+        # (bwd); every slot is written (zeros for dropped samples) and reads
+        # its index and depth
+        if name == "trilinear_fwd":
+            nbytes = n_valid * C * 32 + R * S * C * 4 + R * S * 8
+        else:
+            nbytes = n_valid * C * 4 + R * S * C * 32 + R * S * 8
+        gbs = nbytes / (ms / 1e3) / 1e9
+        res[name] = {"ms": ms, "algorithmic_bytes": nbytes, "GB_per_s": gbs,
+                     "frac_of_hbm_peak": gbs / HBM_GBS}
+    return res
+
+
 def batching_form(steps, n_frames=16, precs=("fp32", "bf16")):
     """Config-5 block: 16 frames of uneven size (own seeded scene each,
     performer scale U(0.6, 1.2), 2-6 source views of 1024^2, 256^3 grid,
@@ -463,6 +544,8 @@ def main():
                     help="run only the FastNerf inference-surface block and print it")
     ap.add_argument("--training-only", action="store_true",
                     help="run only the differentiation-form block and print it")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="run only the training-only octree kernel block and print it")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="run the fixed launch sequence used for rocprofv3 counter passes")
     args = ap.parse_args()
@@ -488,6 +571,9 @@ def main():
         return
     if args.training_only:
         print(json.dumps(training_form(args.steps, args.warmup, with_cpu=not args.no_cpu)))
+        return
+    if args.kernels_only:
+        print(json.dumps(training_kernels(args.steps)))
         return
     p32 = FramePipeline.synthetic(cfg, prec="fp32")
     p32.run()

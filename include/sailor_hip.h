@@ -477,6 +477,12 @@ int sailor_depth_normalize(const float* depths, int B, int V, int H, int W, floa
                            float* out_dist, void* workspace, size_t workspace_bytes,
                            void* stream);
 
+/* Replaces rgb_normalizer (utils_render/utils_render.py:601-605):
+ * out = ((rgb - mid) / dist) * mask, rgb / out f32 [N,C,H,W], mask f32
+ * [N,1,H,W] broadcast over channels. */
+int sailor_rgb_normalize(const float* rgb, const float* mask, int N, int C, int H, int W,
+                         float mid, float dist, float* out, void* stream);
+
 /* Replaces rays_selecting (sampling_kernel.cu:489-527,736-769): rows of
  * ray_ori_dir [B,V,H,W,6] at sampled_xy int32 [B,V,N,2] (x, y) -> out
  * [B,V,N,6].  An (x, y) outside the image selects zeros (the reference

@@ -27,3 +27,12 @@ def depth_normalize(d, lo, hi, user_dist, multi_mask, divided2):
     else:
         r = np.clip((d - mid) / np.float32(user_dist), -1, 1)
     return r.astype(np.float32), mask, mid, dist
+
+
+def rgb_normalizer(rgb, mask, mid=0.5, dist=0.5):
+    """utils_render.py:601-605 in float32: ((rgb - mid) / dist) * mask, the
+    [N,1,H,W] mask broadcast over channels."""
+    rgb = np.asarray(rgb, np.float32)
+    m = np.asarray(mask, np.float32)
+    t = (rgb - np.float32(mid)) / np.float32(dist)
+    return (t * m).astype(np.float32)

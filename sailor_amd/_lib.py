@@ -63,6 +63,7 @@ _SIGS = {
     "sailor_infer_density": [P, P, P, P, I, I, I, P, P],
     "sailor_infer_blending": [P, P, P, I, P, P],
     "sailor_rays_interpolate": [P, P, P, I, I, P, P],
+    "sailor_rgb_normalize": [P, P, I, I, I, I, F, F, P, P],
     "sailor_octree_traversal": [P, P, P, I, P, P, I, I, I, I, P, P, I, I, I, I, I, P, P, P, P, P,
                                 P, P, P, P],
     "sailor_octree_sampling": [P, P, P, P, P, P, I, P, I, I, I, I, I, P, P, P],

@@ -1,7 +1,8 @@
 // Preprocessing either side of the frame build: per-view depth
-// normalisation (c_lib/VoxelEncoding/src/depth_normalizer.cu:98-189) and the
-// training-ray selectors rays_selecting / rgbdv_selecting
-// (sampling_kernel.cu:489-607,736-825).
+// normalisation (c_lib/VoxelEncoding/src/depth_normalizer.cu:98-189), colour
+// normalisation (utils_render/utils_render.py:601-605) and the training-ray
+// selectors rays_selecting / rgbdv_selecting (sampling_kernel.cu:489-607,
+// 736-825).
 //
 // Depth normalisation: the reference spins one float CAS loop per valid
 // pixel on the two per-view words; here each workgroup reduces its slice of
@@ -114,6 +115,19 @@ select_kernel(const float* __restrict__ src, int C, int H, int W, const int* __r
   }
 }
 
+// rgb_normalizer: ((rgb - mid) / dist) * mask, the mask [N,1,H,W]
+// broadcast over the C channels of rgb [N,C,H,W].
+__global__ void __launch_bounds__(kBlock)
+rgb_normalize_kernel(const float* __restrict__ rgb, const float* __restrict__ mask, int C,
+                     long long hw, long long total, float mid, float dist,
+                     float* __restrict__ out) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < total;
+       i += (long long)gridDim.x * kBlock) {
+    const long long n = i / (C * hw), p = i % hw;
+    out[i] = ((rgb[i] - mid) / dist) * mask[n * hw + p];
+  }
+}
+
 unsigned grid_of(long long work) {
   return (unsigned)std::max<long long>(1, std::min<long long>((work + kBlock - 1) / kBlock, 4096));
 }
@@ -179,4 +193,14 @@ extern "C" int sailor_rgbdv_selecting(const float* rgbs, int rgb_channels, const
   hipLaunchKernelGGL(select_kernel, dim3(g), dim3(kBlock), 0, st, depths, 1, H, W, sampled_xy,
                      nv, n_rays, out_depth, out_label);
   return sailor::check_launch("rgbdv_selecting (depth)");
+}
+
+extern "C" int sailor_rgb_normalize(const float* rgb, const float* mask, int N, int C, int H,
+                                    int W, float mid, float dist, float* out, void* stream) {
+  SAILOR_REQUIRE(N >= 0 && C > 0 && H > 0 && W > 0, "rgb_normalizer: bad sizes");
+  const long long hw = (long long)H * W, total = (long long)N * C * hw;
+  if (total == 0) return SAILOR_OK;
+  hipLaunchKernelGGL(rgb_normalize_kernel, dim3(grid_of(total)), dim3(kBlock), 0,
+                     sailor::as_stream(stream), rgb, mask, C, hw, total, mid, dist, out);
+  return sailor::check_launch("rgb_normalizer");
 }

@@ -323,9 +323,9 @@ trilinear_bwd_kernel(const float4* __restrict__ node_pos, int n_nodes,
     }
     for (int c = 0; c < C; ++c) {
       const long long gi = (b * C + c) * RS + rs;
-      const float g = grad[gi];
       float4 lo = make_float4(0.f, 0.f, 0.f, 0.f), hi = lo;
       if (ok) {
+        const float g = grad[gi];
         lo = make_float4(g * w[0], g * w[1], g * w[2], g * w[3]);
         hi = make_float4(g * w[4], g * w[5], g * w[6], g * w[7]);
       }

@@ -53,3 +53,15 @@ def test_rays_and_rgbdv_selecting_bit_exact():
     np.testing.assert_array_equal(n.cpu().numpy(), nml[bi, vi, Y, X])
     np.testing.assert_array_equal(d.cpu().numpy(), dep[bi, vi, Y, X])
     np.testing.assert_array_equal(lab.cpu().numpy(), (dep[bi, vi, Y, X] > 0).astype(np.float32))
+
+
+def test_rgb_normalizer_bit_exact():
+    from oracle.preprocess_ref import rgb_normalizer as ref
+    from sailor_amd.utils_render import rgb_normalizer
+    rng = np.random.default_rng(4)
+    rgb = rng.uniform(0, 1, (3, 3, 37, 53)).astype(np.float32)
+    mask = (rng.random((3, 1, 37, 53)) < 0.7).astype(np.float32)
+    got = rgb_normalizer(torch.from_numpy(rgb).cuda(), torch.from_numpy(mask).cuda())
+    np.testing.assert_array_equal(got.cpu().numpy(), ref(rgb, mask))
+    got = rgb_normalizer(torch.from_numpy(rgb).cuda(), torch.from_numpy(mask).cuda(), 0.3, 0.7)
+    np.testing.assert_array_equal(got.cpu().numpy(), ref(rgb, mask, 0.3, 0.7))

@@ -21,3 +21,16 @@ def test_range_closed_form():
     assert r2[0, 0, 0, 0] == -1.0 and r2[0, 0, 2, 3] == 1.0 and r2[0, 0, 1, 1] == 1.0  # clamped
     r3, *_ = depth_normalize(d, 0.2, 5.0, 4.0, True, False)
     assert r3[0, 0, 2, 3] == 0.25
+
+
+def test_rgb_normalizer_closed_form():
+    from oracle.preprocess_ref import rgb_normalizer
+    rgb = np.array([0.0, 0.25, 0.5, 1.0], np.float32).reshape(1, 1, 2, 2).repeat(3, 1)
+    mask = np.array([1, 1, 0, 1], np.float32).reshape(1, 1, 2, 2)
+    out = rgb_normalizer(rgb, mask)
+    np.testing.assert_array_equal(out[0, :, 0, 0], -1.0)
+    np.testing.assert_array_equal(out[0, :, 0, 1], -0.5)
+    np.testing.assert_array_equal(out[0, :, 1, 0], 0.0)      # masked
+    np.testing.assert_array_equal(out[0, :, 1, 1], 1.0)
+    out2 = rgb_normalizer(rgb, mask, mid=0.0, dist=1.0)
+    np.testing.assert_array_equal(out2, rgb * mask)
