@@ -371,7 +371,8 @@ def training_kernels(steps, C=8):
 
     def samp():
         call("sailor_octree_sampling", ptr(idx), ptr(t0), ptr(t1), ptr(noise), ptr(vl),
-             ptr(p.node_pos), p.cap, ptr(p.vs), 1, R, nf, S, 2, ptr(sidx), ptr(stt), st)
+             ptr(p.node_pos), p.cap, ptr(p.vs), 1, R, nf, stat[1], S, 2, ptr(sidx), ptr(stt),
+             st)
     samp()
     feats = torch.randn((1, C, R, S, 8), **o)
     out = torch.empty((1, C, R, S), **o)

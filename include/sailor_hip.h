@@ -178,12 +178,14 @@ int sailor_octree_traversal(const int32_t* vol_coarse, const float* node_pos,
  * at min + off (i+1) - off/2 + off u, slots in hit order up to S, the rest
  * in the smallest hit node.  noise f32 [B,R,S]; outputs sampled_idx int32,
  * sampled_t f32 [B,R,S] (-1 for unused slots);
- * start_scale = 2^(levels - start_level - 1). */
+ * start_scale = 2^(levels - start_level - 1); max_hits (<= n_max_hits) is
+ * the largest per-ray entry count of the traversal (status[1] of
+ * sailor_octree_traversal): the per-hit pass visits slots [0, max_hits). */
 int sailor_octree_sampling(const int32_t* hit_idx, const float* tmin, const float* tmax,
                            const float* noise, const float* valid_len, const float* node_pos,
                            int n_nodes, const float* voxel_size, int B, int R, int n_max_hits,
-                           int S, int start_scale, int32_t* sampled_idx, float* sampled_t,
-                           void* stream);
+                           int max_hits, int S, int start_scale, int32_t* sampled_idx,
+                           float* sampled_t, void* stream);
 
 /* Replaces MultiLayerOctree::trilinear_aggregate_features (sampling_kernel.cu:
  * 1384-1421, kernel :162-234): feats f32 [B,C,R,S,8] (per sample the values

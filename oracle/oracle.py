@@ -83,7 +83,7 @@ def lib():
         L.orc_octree_traversal.argtypes = [i32p, f32p, i32p, f32p, f32p] + [c_int] * 4 + \
             [f32p, f32p] + [c_int] * 5 + [i32p, f32p, f32p, i32p, f32p, f32p, f32p, i32p]
         L.orc_sampling_adaptive.argtypes = [i32p, f32p, f32p, f32p, f32p, f32p, f32p] + \
-            [c_int] * 5 + [i32p, f32p]
+            [c_int] * 6 + [i32p, f32p]
         L.orc_trilinear_fwd.argtypes = [f32p, f32p, f32p, i32p, f32p, f32p] + [c_int] * 4 + \
             [f32p]
         L.orc_trilinear_bwd.argtypes = [f32p, f32p, f32p, i32p, f32p, f32p] + [c_int] * 4 + \
@@ -243,15 +243,17 @@ def octree_traversal(vol_coarse, node_pos, node_info, vol_bbox, voxel_size, dim_
 
 
 def sampling_adaptive(hit_idx, tmin, tmax, noise, valid_len, node_pos, voxel_size, S,
-                      start_scale):
-    """MultiLayerOctree::ray_voxels_points_sampling, canonical slot order."""
+                      start_scale, max_hits=None):
+    """MultiLayerOctree::ray_voxels_points_sampling, canonical slot order;
+    max_hits = slots the per-hit pass visits (default: all)."""
     hit_idx = _i(hit_idx)
     B, R, n_max = hit_idx.shape
+    max_hits = n_max if max_hits is None else int(max_hits)
     sidx = np.empty((B, R, S), np.int32)
     st = np.empty((B, R, S), np.float32)
     lib().orc_sampling_adaptive(hit_idx, _f(tmin), _f(tmax), _f(noise), _f(valid_len),
-                                _f(node_pos), _f(voxel_size), B, R, n_max, S, start_scale,
-                                sidx, st)
+                                _f(node_pos), _f(voxel_size), B, R, n_max, max_hits, S,
+                                start_scale, sidx, st)
     return sidx, st
 
 

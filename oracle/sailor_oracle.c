@@ -790,14 +790,15 @@ void orc_octree_traversal(const int* vc, const float* node_pos, const int* node_
  * (start_size / size) * S / valid_len)) samples per hit at
  * min + off (i + 1) - off / 2 + off u (off = (max - min) / (n + 1), u the
  * noise of the slot), slots taken in hit order (canonical order of the
- * atomicAdd queue) until S are used; ray_voxels_further_sampling_kernel
+ * atomicAdd queue) until S are used, over the first max_hits slots
+ * (num_max_intersected_voxels_fine); ray_voxels_further_sampling_kernel
  * :984-1068 fills the remaining slots the same way in the smallest hit node
  * (the one with the nearest interval mid-point among equal sizes).
  * start_scale = 2^(levels - start_level - 1) (powf(2.0, ...) of :952). */
 void orc_sampling_adaptive(const int* hit_idx, const float* min_depths, const float* max_depths,
                            const float* noise, const float* rays_valid_len, const float* node_pos,
-                           const float* voxel_size, int B, int R, int n_max, int S,
-                           int start_scale, int* sampled_idx, float* sampled_t) {
+                           const float* voxel_size, int B, int R, int n_max, int max_hits,
+                           int S, int start_scale, int* sampled_idx, float* sampled_t) {
 #pragma omp parallel for schedule(static)
   for (long long k = 0; k < (long long)B * R; ++k) {
     int b = (int)(k / R);
@@ -811,7 +812,7 @@ void orc_sampling_adaptive(const int* hit_idx, const float* min_depths, const fl
     float valid_len = rays_valid_len[k];
     float start_vs = voxel_size[b] * (float)start_scale;
     int cnt = 0;
-    for (int i = 0; i < n_max; ++i) {
+    for (int i = 0; i < max_hits; ++i) {
       int vi = hi[i];
       if (vi == -1) continue;
       float mn = mn_[i], mx = mx_[i];

@@ -311,12 +311,16 @@ class MultiLayerOctree:
             raise RuntimeError("uniform_rand_values must be [B, N_rays, num_sampled]")
         self._top(start_level)
         start_scale = 2 ** (self.num_levels - int(start_level) - 1)
+        # slots the per-hit pass visits: the last traversal's largest count
+        # (num_max_intersected_voxels_fine), else every slot
+        max_hits = getattr(self, "num_max_intersected_voxels_fine", None)
+        max_hits = n_max if max_hits is None else min(int(max_hits), n_max)
         sidx = torch.empty((B, R, S), dtype=_i32, device=dev)
         st = torch.empty((B, R, S), dtype=_f32, device=dev)
         with torch.cuda.device(dev):
             call("sailor_octree_sampling", ptr(hi), ptr(t0), ptr(t1), ptr(nz), ptr(vl),
                  ptr(self.node_pos), self.node_pos.shape[0], ptr(self.voxel_size), B, R, n_max,
-                 S, start_scale, ptr(sidx), ptr(st), stream_of(dev))
+                 max_hits, S, start_scale, ptr(sidx), ptr(st), stream_of(dev))
         return sidx, st
 
     # -- trilinear_aggregate_features[_backward] (sampling_kernel.cu:1384-1462)

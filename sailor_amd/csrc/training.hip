@@ -160,8 +160,9 @@ adaptive_sampling_kernel(const int* __restrict__ hit_idx, const float* __restric
                          const float* __restrict__ tmax, const float* __restrict__ noise,
                          const float* __restrict__ valid_len,
                          const float4* __restrict__ node_pos, int n_nodes,
-                         const float* __restrict__ voxel_size, int B, int R, int n_max, int S,
-                         int start_scale, int* __restrict__ sidx, float* __restrict__ st) {
+                         const float* __restrict__ voxel_size, int B, int R, int n_max,
+                         int max_hits, int S, int start_scale, int* __restrict__ sidx,
+                         float* __restrict__ st) {
   const long long ray = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (ray >= (long long)B * R) return;
   const int b = (int)(ray / R);
@@ -178,7 +179,9 @@ adaptive_sampling_kernel(const int* __restrict__ hit_idx, const float* __restric
   const float vlen = valid_len[ray];
   const float start_vs = voxel_size[b] * (float)start_scale;
   int cnt = 0;
-  for (int i = 0; i < n_max; ++i) {
+  // ray_voxels_sampling_kernel visits the first max_hits slots of every ray
+  // (num_max_intersected_voxels_fine, :944)
+  for (int i = 0; i < max_hits; ++i) {
     const int vi = hi[i];
     if (vi == -1 || vi < 0 || vi >= n_nodes) continue;
     const float mn = mn_[i], mx = mx_[i];
@@ -374,15 +377,17 @@ extern "C" int sailor_octree_sampling(const int32_t* hit_idx, const float* tmin,
                                       const float* tmax, const float* noise,
                                       const float* valid_len, const float* node_pos,
                                       int n_nodes, const float* voxel_size, int B, int R,
-                                      int n_max_hits, int S, int start_scale,
+                                      int n_max_hits, int max_hits, int S, int start_scale,
                                       int32_t* sampled_idx, float* sampled_t, void* stream) {
   SAILOR_REQUIRE(B > 0 && R >= 0 && S > 0 && n_max_hits > 0 && n_nodes >= 0,
                  "ray_voxels_points_sampling: bad sizes");
+  SAILOR_REQUIRE(max_hits >= 0 && max_hits <= n_max_hits,
+                 "ray_voxels_points_sampling: max_hits %d outside [0, %d]", max_hits, n_max_hits);
   if (R == 0) return SAILOR_OK;
   hipLaunchKernelGGL(adaptive_sampling_kernel, dim3(sailor::ceil_div((long long)B * R, kThreads)),
                      dim3(kThreads), 0, sailor::as_stream(stream), hit_idx, tmin, tmax, noise,
                      valid_len, reinterpret_cast<const float4*>(node_pos), n_nodes, voxel_size,
-                     B, R, n_max_hits, S, start_scale, sampled_idx, sampled_t);
+                     B, R, n_max_hits, max_hits, S, start_scale, sampled_idx, sampled_t);
   return sailor::check_launch("ray_voxels_points_sampling");
 }
 

@@ -87,7 +87,8 @@ def test_adaptive_sampling_bitexact(case):
         sidx, st = tree.ray_voxels_points_sampling(idx, t0, t1, vl, nz, 0, S)
         ri, rt = orc.sampling_adaptive(idx.cpu().numpy(), t0.cpu().numpy(), t1.cpu().numpy(),
                                        nz.cpu().numpy(), vl.cpu().numpy().reshape(1, -1),
-                                       h["pos"], h["vs"], S, 2)  # 2^(levels - 0 - 1)
+                                       h["pos"], h["vs"], S, 2,  # 2^(levels - 0 - 1)
+                                       tree.num_max_intersected_voxels_fine)
         np.testing.assert_array_equal(sidx.cpu().numpy(), ri)
         np.testing.assert_array_equal(st.cpu().numpy(), rt)
         assert (ri >= 0).mean() > 0.05
