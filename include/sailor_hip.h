@@ -207,6 +207,18 @@ int sailor_trilinear_aggregate_backward(const float* node_pos, int n_nodes,
                                         const float* grad_out, int B, int R, int S, int C,
                                         float* grad_feats, void* stream);
 
+/* Replaces MultiLayerOctree::project_sampled_xyz (sampling_kernel.cu:1333-1381,
+ * kernel project_xyz :55-157): per (batch, view, ray, sample) the sample
+ * point (and with record_corners its node's 8 corners, corner id 4x+2y+z)
+ * projected by RTs [B,V,3,4] and Ks [B,V,3,3] to (p/z - w/2) / (w/2) ->
+ * proj_xy f32 [B,V,R,S,(9,)2], camera z -> proj_z [B,V,R,S,(9,)1]; -1 where
+ * sampled_idx is -1 (corners also -1 for an index outside [0, n_nodes)). */
+int sailor_project_sampled_xyz(const float* node_pos, int n_nodes, const float* ray_ori,
+                               const float* ray_dir, const int32_t* sampled_idx,
+                               const float* sampled_t, const float* Ks, const float* RTs, int B,
+                               int V, int R, int S, int ori_w, int ori_h, int record_corners,
+                               float* proj_xy, float* proj_z, void* stream);
+
 /* Replaces MultiLayerOctree::sort_samplings (sampling_kernel.cu:1308-1329):
  * ascending sort of each row of S values; ties keep their original order.
  *  t f32 [n_rows,S] -> sorted f32 [n_rows,S], order int32 [n_rows,S]       */

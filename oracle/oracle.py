@@ -88,6 +88,8 @@ def lib():
             [f32p]
         L.orc_trilinear_bwd.argtypes = [f32p, f32p, f32p, i32p, f32p, f32p] + [c_int] * 4 + \
             [f32p]
+        L.orc_project.argtypes = [f32p, f32p, f32p, i32p, f32p, f32p, f32p] + [c_int] * 7 + \
+            [f32p, f32p]
         _lib = L
     return _lib
 
@@ -275,6 +277,21 @@ def trilinear_bwd(node_pos, ray_ori, ray_dir, sidx, st, grad):
     lib().orc_trilinear_bwd(_f(node_pos), _f(ray_ori), _f(ray_dir), _i(sidx), _f(st), grad,
                             B, R, S, C, out)
     return out
+
+
+def project(node_pos, ray_ori, ray_dir, sidx, st, Ks, RTs, ori_w, ori_h, corners=False):
+    """project_xyz: -> (xy [B,V,R,S,(9,)2], z [B,V,R,S,(9,)1])."""
+    sidx = _i(sidx)
+    B, R, S = sidx.shape
+    V = np.asarray(Ks).shape[1]
+    np_ = 9 if corners else 1
+    xy = np.empty((B, V, R, S, np_, 2), np.float32)
+    z = np.empty((B, V, R, S, np_, 1), np.float32)
+    lib().orc_project(_f(node_pos), _f(ray_ori), _f(ray_dir), sidx, _f(st), _f(Ks), _f(RTs), B,
+                      V, R, S, ori_w, ori_h, int(corners), xy, z)
+    if not corners:
+        xy, z = xy[..., 0, :], z[..., 0, :]
+    return xy, z
 
 
 def sort_samplings(t):

@@ -947,3 +947,52 @@ void orc_trilinear_bwd(const float* node_pos, const float* ray_ori, const float*
     }
   }
 }
+
+/* project_xyz, sampling_kernel.cu:55-157 (MultiLayerOctree::project_sampled_xyz
+ * :1333-1381): xy [B,V,R,S,np,2], z [B,V,R,S,np] with np = 9 (point + 8
+ * corners) or 1; -1 where the sample index is -1. */
+void orc_project(const float* node_pos, const float* ray_ori, const float* ray_dir,
+                 const int* sidx, const float* st, const float* Ks, const float* RTs, int B,
+                 int V, int R, int S, int ori_w, int ori_h, int corners, float* xy, float* z) {
+  int np_ = corners ? 9 : 1;
+  long long total = (long long)B * V * R * S;
+#pragma omp parallel for schedule(static)
+  for (long long k = 0; k < total; ++k) {
+    for (int c = 0; c < np_; ++c) {
+      xy[(k * np_ + c) * 2] = -1.0f;
+      xy[(k * np_ + c) * 2 + 1] = -1.0f;
+      z[k * np_ + c] = -1.0f;
+    }
+    long long rs = k % ((long long)R * S), bv = k / ((long long)R * S), b = bv / V, r = rs / S;
+    long long si = b * R * S + rs;
+    int vi = sidx[si];
+    if (vi == -1) continue;
+    const float* RT = RTs + bv * 12;
+    const float* K = Ks + bv * 9;
+    const float* o = ray_ori + (b * R + r) * 3;
+    const float* d = ray_dir + (b * R + r) * 3;
+    float t = st[si];
+    for (int c = 0; c < np_; ++c) {
+      float q[3];
+      if (c == 0) {
+        for (int a = 0; a < 3; ++a) q[a] = o[a] + d[a] * t;
+      } else {
+        const float* nd = node_pos + 4 * vi;
+        int id = c - 1;
+        int sg[3] = {(id >> 2) & 1 ? 1 : -1, (id >> 1) & 1 ? 1 : -1, id & 1 ? 1 : -1};
+        for (int a = 0; a < 3; ++a) q[a] = nd[a] + ((float)sg[a] * nd[3]) / 2.0f;
+      }
+      float cx = RT[0] * q[0] + RT[1] * q[1] + RT[2] * q[2] + RT[3];
+      float cy = RT[4] * q[0] + RT[5] * q[1] + RT[6] * q[2] + RT[7];
+      float cz = RT[8] * q[0] + RT[9] * q[1] + RT[10] * q[2] + RT[11];
+      float px = K[0] * cx + K[1] * cy + K[2] * cz;
+      float py = K[3] * cx + K[4] * cy + K[5] * cz;
+      float pz = K[6] * cx + K[7] * cy + K[8] * cz;
+      px = px / pz;
+      py = py / pz;
+      xy[(k * np_ + c) * 2] = (float)(((double)px - ori_w / 2.0) / (ori_w / 2.0));
+      xy[(k * np_ + c) * 2 + 1] = (float)(((double)py - ori_h / 2.0) / (ori_h / 2.0));
+      z[k * np_ + c] = cz;
+    }
+  }
+}

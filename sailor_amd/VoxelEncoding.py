@@ -365,6 +365,26 @@ class MultiLayerOctree:
                  C, ptr(out), stream_of(dev))
         return out
 
+    # -- project_sampled_xyz (sampling_kernel.cu:1333-1381) --------------------
+    def project_sampled_xyz(self, ray_ori, ray_dir, sampled_z, sampled_idx, Ks, RTs, ori_w,
+                            ori_h, num_views, record_corners):
+        """-> (proj_xy [B,V,R,S,(9,)2], proj_z [B,V,R,S,(9,)1]); -1 for dropped
+        samples."""
+        dev, ori, dirs, t, si, B, R, S = self._tri_args(ray_ori, ray_dir, sampled_z, sampled_idx)
+        V = int(num_views)
+        K = as_input(Ks, "Ks", _f32, dev)
+        RT = as_input(RTs, "RTs", _f32, dev)
+        if K.numel() != B * V * 9 or RT.numel() != B * V * 12:
+            raise RuntimeError("Ks must be [B, num_views, 3, 3] and RTs [B, num_views, 3, 4]")
+        shape = (B, V, R, S) + ((9,) if record_corners else ())
+        xy = torch.empty(shape + (2,), dtype=_f32, device=dev)
+        z = torch.empty(shape + (1,), dtype=_f32, device=dev)
+        with torch.cuda.device(dev):
+            call("sailor_project_sampled_xyz", ptr(self.node_pos), self.node_pos.shape[0],
+                 ptr(ori), ptr(dirs), ptr(si), ptr(t), ptr(K), ptr(RT), B, V, R, S, int(ori_w),
+                 int(ori_h), int(bool(record_corners)), ptr(xy), ptr(z), stream_of(dev))
+        return xy, z
+
     def sort_samplings(self, sampled_depths):
         return sort_samplings(sampled_depths)
 

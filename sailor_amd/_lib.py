@@ -68,6 +68,7 @@ _SIGS = {
                                 P, P, P, P],
     "sailor_octree_sampling": [P, P, P, P, P, P, I, P, I, I, I, I, I, I, P, P, P],
     "sailor_trilinear_aggregate": [P, I, P, P, P, P, P, I, I, I, I, P, P],
+    "sailor_project_sampled_xyz": [P, I, P, P, P, P, P, P, I, I, I, I, I, I, I, P, P, P],
     "sailor_trilinear_aggregate_backward": [P, I, P, P, P, P, P, I, I, I, I, P, P],
     "sailor_forward_upsampling": [P, P, P, P, P, I, P, P, P, P, P, I, I, I, P, I, I, P, I, I, P,
                                   I, I, I, F, F, P, P, P],

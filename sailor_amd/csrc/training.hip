@@ -338,6 +338,73 @@ trilinear_bwd_kernel(const float4* __restrict__ node_pos, int n_nodes,
   }
 }
 
+// project_xyz (sampling_kernel.cu:55-157): one thread per (batch, view,
+// ray, sample); the point ori + dir * t (and with corners, the node's 8
+// corners, id 4 x + 2 y + z) through [R | T] and K, divided by the third
+// coordinate and mapped to [-1, 1] as (p - w / 2.0) / (w / 2.0) in double.
+// Samples with index -1 keep the -1 fill.
+__global__ void __launch_bounds__(kThreads)
+project_kernel(const float4* __restrict__ node_pos, int n_nodes,
+               const float* __restrict__ ray_ori, const float* __restrict__ ray_dir,
+               const float* __restrict__ Ks, const float* __restrict__ RTs,
+               const int* __restrict__ sidx, const float* __restrict__ st, int B, int V, int R,
+               int S, int ori_w, int ori_h, int corners, float* __restrict__ xy,
+               float* __restrict__ z) {
+  const long long total = (long long)B * V * R * S;
+  const int np_ = corners ? 9 : 1;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (long long)gridDim.x * blockDim.x) {
+    const long long rs = k % ((long long)R * S);
+    const long long bv = k / ((long long)R * S);
+    const long long b = bv / V;
+    const long long r = rs / S;
+    const long long si = b * R * S + rs;
+    const int vi = sidx[si];
+    if (vi == -1) continue;
+    const float* RT = RTs + bv * 12;
+    const float* K = Ks + bv * 9;
+    const float* o = ray_ori + (b * R + r) * 3;
+    const float* d = ray_dir + (b * R + r) * 3;
+    const float t = st[si];
+    const float pt[3] = {o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t};
+    const float4 nd = (vi >= 0 && vi < n_nodes) ? node_pos[vi] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const bool corner_ok = corners && vi >= 0 && vi < n_nodes;
+    const double hw = (double)ori_w / 2.0, hh = (double)ori_h / 2.0;
+    for (int c = 0; c < np_; ++c) {
+      float q[3];
+      if (c == 0) {
+        q[0] = pt[0];
+        q[1] = pt[1];
+        q[2] = pt[2];
+      } else {
+        if (!corner_ok) break;
+        const int id = c - 1;
+        q[0] = nd.x + ((float)((id >> 2) & 1 ? 1 : -1) * nd.w) / 2.0f;
+        q[1] = nd.y + ((float)((id >> 1) & 1 ? 1 : -1) * nd.w) / 2.0f;
+        q[2] = nd.z + ((float)(id & 1 ? 1 : -1) * nd.w) / 2.0f;
+      }
+      const float cx = RT[0] * q[0] + RT[1] * q[1] + RT[2] * q[2] + RT[3];
+      const float cy = RT[4] * q[0] + RT[5] * q[1] + RT[6] * q[2] + RT[7];
+      const float cz = RT[8] * q[0] + RT[9] * q[1] + RT[10] * q[2] + RT[11];
+      float px = K[0] * cx + K[1] * cy + K[2] * cz;
+      float py = K[3] * cx + K[4] * cy + K[5] * cz;
+      const float pz = K[6] * cx + K[7] * cy + K[8] * cz;
+      px = px / pz;
+      py = py / pz;
+      const long long o2 = (k * np_ + c) * 2;
+      xy[o2] = (float)(((double)px - hw) / hw);
+      xy[o2 + 1] = (float)(((double)py - hh) / hh);
+      z[k * np_ + c] = cz;
+    }
+  }
+}
+
+__global__ void fill_value(float* __restrict__ a, long long n, float v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    a[i] = v;
+}
+
 unsigned grid_of(long long work) {
   return (unsigned)std::max<long long>(
       1, std::min<long long>((work + kThreads - 1) / kThreads, 256 * 32));
@@ -424,4 +491,26 @@ extern "C" int sailor_trilinear_aggregate_backward(const float* node_pos, int n_
                      n_nodes, ray_ori, ray_dir, sampled_idx, sampled_t, grad_out, B, R, S, C,
                      reinterpret_cast<float4*>(grad_feats));
   return sailor::check_launch("trilinear_aggregate_features_backward");
+}
+
+extern "C" int sailor_project_sampled_xyz(const float* node_pos, int n_nodes, const float* ray_ori,
+                                          const float* ray_dir, const int32_t* sampled_idx,
+                                          const float* sampled_t, const float* Ks,
+                                          const float* RTs, int B, int V, int R, int S,
+                                          int ori_w, int ori_h, int record_corners,
+                                          float* proj_xy, float* proj_z, void* stream) {
+  SAILOR_REQUIRE(B > 0 && V > 0 && R >= 0 && S > 0 && n_nodes >= 0 && ori_w > 0 && ori_h > 0,
+                 "project_sampled_xyz: bad sizes");
+  if (R == 0) return SAILOR_OK;
+  hipStream_t st = sailor::as_stream(stream);
+  const long long n = (long long)B * V * R * S * (record_corners ? 9 : 1);
+  hipLaunchKernelGGL(fill_value, dim3(grid_of(2 * n)), dim3(kThreads), 0, st, proj_xy, 2 * n,
+                     -1.0f);
+  hipLaunchKernelGGL(fill_value, dim3(grid_of(n)), dim3(kThreads), 0, st, proj_z, n, -1.0f);
+  if (int e = sailor::check_launch("project_sampled_xyz (fill)")) return e;
+  hipLaunchKernelGGL(project_kernel, dim3(grid_of((long long)B * V * R * S)), dim3(kThreads), 0,
+                     st, reinterpret_cast<const float4*>(node_pos), n_nodes, ray_ori, ray_dir,
+                     Ks, RTs, sampled_idx, sampled_t, B, V, R, S, ori_w, ori_h,
+                     record_corners ? 1 : 0, proj_xy, proj_z);
+  return sailor::check_launch("project_sampled_xyz");
 }

@@ -253,6 +253,17 @@ class OcTreeOptions(nn.Module):
                                                        _n_sampled_points)
 
     @torch.no_grad()
+    def _project_sampled_xyz(self, _ray_ori, _ray_dir, _sampled_depths, _sampled_idx, Ks, RTs,
+                             _ori_w, _ori_h, _num_views, _record_corners=False):
+        if len(Ks.shape) == 3:
+            Ks = Ks.view(self._batch_size, -1, 3, 3).contiguous()
+        if len(RTs.shape) == 3:
+            RTs = RTs.view(self._batch_size, -1, 3, 4).contiguous()
+        return self._octree.project_sampled_xyz(_ray_ori, _ray_dir, _sampled_depths,
+                                                _sampled_idx, Ks, RTs, _ori_w, _ori_h,
+                                                _num_views, _record_corners)
+
+    @torch.no_grad()
     def _ray_voxels_points_sampling_coarse(self, _idx, _min_depths, _max_depths, _uniform_noise,
                                            _ray_total_valid_len, _instersect_start_level,
                                            _n_sampled_points):

@@ -138,3 +138,18 @@ def test_trilinear_adjoint_at_size(case):
                                                       torch.ones_like(G[:, :1]))
     ok = (sidx >= 0)[:, None]
     np.testing.assert_allclose(ones.sum(-1)[ok].cpu().numpy(), 1.0, atol=1e-6)
+
+
+@pytest.mark.parametrize("corners", [False, True])
+def test_project_sampled_xyz_bitexact(case, corners):
+    from sailor_amd import scenes
+    dim, rate, tree, ori, dirs, h = case
+    sidx, st = _samples(tree, ori, dirs, S=12)
+    cams = scenes.ring_cameras(3, 150.0, 128, 128)
+    K = np.stack([c.K for c in cams])[None].astype(np.float32)
+    RT = np.stack([c.RT for c in cams])[None].astype(np.float32)
+    xy, z = tree.project_sampled_xyz(ori, dirs, st, sidx, _t(K), _t(RT), 128, 128, 3, corners)
+    rxy, rz = orc.project(h["pos"], h["ori"], h["dirs"], sidx.cpu().numpy(), st.cpu().numpy(),
+                          K, RT, 128, 128, corners)
+    np.testing.assert_array_equal(xy.cpu().numpy(), rxy)
+    np.testing.assert_array_equal(z.cpu().numpy(), rz)

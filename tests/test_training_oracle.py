@@ -219,3 +219,29 @@ def test_trilinear_backward_is_adjoint():
     ok = sidx >= 0
     np.testing.assert_allclose(ones.sum(-1)[ok], 1.0, atol=1e-6)
     assert np.all(ones[~ok] == 0)
+
+
+def test_project_closed_form():
+    """K = [[w/2, 0, w/2], [0, h/2, h/2], [0, 0, 1]], RT = [I | (0, 0, 3)]:
+    the grid value of a point is X / (Z + 3) and Y / (Z + 3); corners are the
+    node centre +- size / 2 in corner order 4x + 2y + z; -1 for dropped
+    samples."""
+    pos, o, d, sidx, st, _ = _tri_case(B=1, R=20, S=6)
+    w, h = 64, 48
+    K = np.array([[w / 2, 0, w / 2], [0, h / 2, h / 2], [0, 0, 1]], np.float32)[None, None]
+    RT = np.concatenate([np.eye(3), [[0], [0], [3.0]]], 1).astype(np.float32)[None, None]
+    xy, z = O.project(pos, o, d, sidx, st, K, RT, w, h, corners=True)
+    p = o[:, :, None].astype(np.float64) + d[:, :, None].astype(np.float64) * st[..., None]
+    node = pos[np.maximum(sidx, 0)].astype(np.float64)
+    corner = np.array([[(k >> 2) & 1, (k >> 1) & 1, k & 1] for k in range(8)], np.float64)
+    pts = np.concatenate([p[..., None, :], node[..., None, :3] + (corner - 0.5) *
+                          node[..., None, 3:4]], -2)                        # [B,R,S,9,3]
+    ok = sidx >= 0
+    zz = pts[..., 2] + 3.0
+    np.testing.assert_allclose(xy[:, 0, ..., 0][ok], (pts[..., 0] / zz)[ok], atol=2e-6)
+    np.testing.assert_allclose(xy[:, 0, ..., 1][ok], (pts[..., 1] / zz)[ok], atol=2e-6)
+    np.testing.assert_allclose(z[:, 0, ..., 0][ok], zz[ok], atol=1e-6)
+    assert np.all(xy[:, 0][~ok] == -1) and np.all(z[:, 0][~ok] == -1)
+    xy1, z1 = O.project(pos, o, d, sidx, st, K, RT, w, h, corners=False)
+    np.testing.assert_array_equal(xy1, xy[..., 0, :])
+    np.testing.assert_array_equal(z1, z[..., 0, :])
