@@ -259,18 +259,33 @@ def fastnerf_infer(steps):
     pdt = torch.from_numpy(pd).to(dev)
     pct = torch.from_numpy(pc).to(dev)
 
-    def go():
-        return InferNerf.infer_nerf(t, z, ps, geo, rgb, frgb, dirs, pdt, pct, B, R, S, Vg, Vt, 0)
-    go()
-    torch.cuda.synchronize()
-    ms = _median_ms(go, steps)
     flops = (Vg * R * S * 2 * (18 * 64 + 3 * 64 * 64) + Vt * R * S * 2 * (18 * 16 + 16 * 8)
              + R * S * 2 * (2 * 64 * 64 + 64) + Vt * R * 2 * 2 * (3 * 30 * 32 + 3 * 32 * 30)
              + R * 2 * (30 * 64 + 64 * 32 + 32 * 3))
-    return {"workload": f"infer_nerf: {R} rays x {S} samples, {Vg} geometry / {Vt} texture views",
-            "ms": ms, "rays_per_s": R / (ms / 1e3), "flops": flops,
-            "vector_tflops": flops / (ms / 1e3) / 1e12,
-            "frac_of_fp32_vector_peak": flops / (ms / 1e3) / 1e12 / 157.3}
+    out = {"workload": f"infer_nerf: {R} rays x {S} samples, {Vg} geometry / {Vt} texture views",
+           "flops": flops}
+    for prec in ("fp16", "fp32"):
+        def go():
+            return InferNerf.infer_nerf(t, z, ps, geo, rgb, frgb, dirs, pdt, pct, B, R, S, Vg,
+                                        Vt, 0, precision=prec)
+        go()
+        torch.cuda.synchronize()
+        ms = _median_ms(go, steps)
+        out[prec] = {"ms": ms, "rays_per_s": R / (ms / 1e3),
+                     "tflops": flops / (ms / 1e3) / 1e12}
+    # density pass alone (dominant: ~97 % of the FLOPs), fp16 MFMA form
+    pd16 = pdt.half().contiguous()
+
+    def dens():
+        InferNerf.infer_density(z, ps, geo, pd16, B, R * S, Vg, 0)
+    dens()
+    torch.cuda.synchronize()
+    dms = _median_ms(dens, steps)
+    dflops = Vg * R * S * 2 * (18 * 64 + 3 * 64 * 64) + R * S * 2 * (2 * 64 * 64 + 64)
+    out["density_fp16"] = {"ms": dms, "flops": dflops,
+                           "tflops": dflops / (dms / 1e3) / 1e12,
+                           "frac_of_fp16_mfma_peak": dflops / (dms / 1e3) / 1e12 / PEAKS["fp16"]}
+    return out
 
 
 def fastnerf_upsampling(steps):

@@ -578,12 +578,31 @@ int sailor_infer_nerf(const float* sorted_dists, const float* points_z,
                       int tex_views, float* out_depth, float* out_rgb, float* out_feats,
                       void* workspace, size_t workspace_bytes, void* stream);
 
+/* sailor_infer_nerf computes the density MLPs (the reference's half
+ * precision) on fp16 MFMA: layer inputs rounded to half, fp32 accumulation,
+ * for up to 8 geometry views (more views use the fp32 kernels).  The _fp32
+ * variant keeps every activation in fp32 on the vector ALUs (no counterpart
+ * in the reference; for callers that want the exact value of the half
+ * parameters). */
+int sailor_infer_nerf_fp32(const float* sorted_dists, const float* points_z,
+                           const float* points_tpsdf, const float* feats_geo, const float* rgbs,
+                           const float* feats_rgb, const float* dirs, const void* params_density,
+                           const void* params_color, int B, int R, int S, int geo_views,
+                           int tex_views, float* out_depth, float* out_rgb, float* out_feats,
+                           void* workspace, size_t workspace_bytes, void* stream);
+
 /* Replaces infer_density_mlp (infer_mlp.cu:583-634), bound as
  * InferNerf.infer_density: points_z, points_tpsdf [B*V*N], feats_geo
  * [B*V*N,16] in (batch, view, point) order -> out [B,N] occupancy. */
 int sailor_infer_density(const float* points_z, const float* points_tpsdf,
                          const float* feats_geo, const void* params_density, int B, int N,
                          int V, float* out, void* stream);
+
+/* fp32 vector-ALU form of sailor_infer_density (no counterpart in the
+ * reference; as sailor_infer_nerf_fp32). */
+int sailor_infer_density_fp32(const float* points_z, const float* points_tpsdf,
+                              const float* feats_geo, const void* params_density, int B, int N,
+                              int V, float* out, void* stream);
 
 /* Replaces infer_upsampling (infer_mlp.cu:793-821), bound as
  * InferNerf.infer_blending: feats_neighbors [N,40], ray_feats [N,30] ->

@@ -23,7 +23,11 @@ at 11644); blending 15075 (40->64->64->32, then [32 | ray feats 30] ->
 
 The kernels compute in half precision (__hfma accumulation); this
 restatement computes in `dtype` (float64 for the tests) from the same half
-parameters, so it is the exact value the reference approximates.  Kept
+parameters, so it is the exact value the reference approximates.  With
+act_dtype=torch.float16 the inputs of the density MLPs' matrix products
+(feature MLP layers, the two hidden occupancy layers, both geometry-feature
+layers) are rounded to half first: the numerics of a half-input,
+wide-accumulation evaluation.  Kept
 quirks: q and k are divided by their norm with no epsilon; compositing skips
 samples with negative depth, returns zeros for a ray whose last sample depth
 is negative, and stops after the sample that takes T below 1e-4; the depth
@@ -120,33 +124,38 @@ def _params(p, dtype):
     return _t(np.asarray(p, np.float16).astype(np.float64), dtype)
 
 
-def density_feats(z, tpsdf, geo, pd, dtype=torch.float64):
+def _round(x, act_dtype):
+    return x if act_dtype is None else x.to(act_dtype).to(x.dtype)
+
+
+def density_feats(z, tpsdf, geo, pd, dtype=torch.float64, act_dtype=None):
     """Per (view, point): 64 density features and 8 geometry features
     (density_feat_kernel).  z, tpsdf [N], geo [N, 16]."""
     feats, _, geo_l = unpack_density(_params(pd, dtype))
     x = torch.cat([_t(z, dtype)[:, None], _t(tpsdf, dtype)[:, None], _t(geo, dtype)], 1)
     h = x
     for wb in feats:
-        h = _relu(_lin(h, wb))
-    g = _relu(_lin(_relu(_lin(x, geo_l[0])), geo_l[1]))
+        h = _relu(_lin(_round(h, act_dtype), wb))
+    g = _relu(_lin(_round(_relu(_lin(_round(x, act_dtype), geo_l[0])), act_dtype), geo_l[1]))
     return h, g
 
 
-def occupancy(mean_feats, pd, dtype=torch.float64):
+def occupancy(mean_feats, pd, dtype=torch.float64, act_dtype=None):
     """density_sigma_kernel on [N, 64] view-averaged features -> [N]."""
     _, sigma, _ = unpack_density(_params(pd, dtype))
-    h = _relu(_lin(mean_feats, sigma[0]))
-    h = _relu(_lin(h, sigma[1]))
+    h = _relu(_lin(_round(mean_feats, act_dtype), sigma[0]))
+    h = _relu(_lin(_round(h, act_dtype), sigma[1]))
     return torch.sigmoid(_lin(h, sigma[2]))[:, 0]
 
 
-def infer_density(points_z, points_tpsdf, feats_geo, pd, B, N, V, dtype=torch.float64):
+def infer_density(points_z, points_tpsdf, feats_geo, pd, B, N, V, dtype=torch.float64,
+                  act_dtype=None):
     """infer_density_mlp: inputs [B*V*N] in (batch, view, point) order ->
     occupancy [B, 1, N]."""
     f, _ = density_feats(np.reshape(points_z, -1), np.reshape(points_tpsdf, -1),
-                         np.reshape(feats_geo, (-1, 16)), pd, dtype)
+                         np.reshape(feats_geo, (-1, 16)), pd, dtype, act_dtype)
     m = f.reshape(B, V, N, 64).mean(1).reshape(B * N, 64)
-    return occupancy(m, pd, dtype).reshape(B, 1, N)
+    return occupancy(m, pd, dtype, act_dtype).reshape(B, 1, N)
 
 
 def composite(t, alpha, feats, t_thres=1e-4):
@@ -199,15 +208,15 @@ def hydra_rgb(tokens, pc, dtype=torch.float64):
 
 
 def infer_nerf(sorted_dists, points_z, points_tpsdf, feats_geo, rgbs, feats_rgb, dirs, pd, pc,
-               B, R, S, Vg, Vt, dtype=torch.float64):
+               B, R, S, Vg, Vt, dtype=torch.float64, act_dtype=None):
     """infer_fast_rendering.  sorted_dists [B, R, S]; per-view inputs flattened
     in (batch, view, ray, sample) order: z, tpsdf [B*Vg*R*S], feats_geo
     [.., 16]; rgbs, dirs [B*Vt*R*S, 3], feats_rgb [.., 16].
     Returns (depth [B, R, 1], rgb [B, R, 3], colour features [B, R, 30])."""
     f, g = density_feats(np.reshape(points_z, -1), np.reshape(points_tpsdf, -1),
-                         np.reshape(feats_geo, (-1, 16)), pd, dtype)
+                         np.reshape(feats_geo, (-1, 16)), pd, dtype, act_dtype)
     m = f.reshape(B, Vg, R * S, 64).mean(1).reshape(B * R * S, 64)
-    alpha = occupancy(m, pd, dtype).reshape(B * R, S)
+    alpha = occupancy(m, pd, dtype, act_dtype).reshape(B * R, S)
     g = g.reshape(B, Vg, R, S, 8)[:, :Vt]
     t = _t(sorted_dists, dtype).reshape(B * R, S)
     fr = _t(feats_rgb, dtype).reshape(B, Vt, R, S, 16)

@@ -10,8 +10,12 @@ torch's current stream with no host synchronisation (the reference creates
 and destroys streams and calls cudaDeviceSynchronize per call,
 infer_mlp.cu:677-680,771,783-785).
 
-Numerics: fp32 arithmetic on the given half parameters, where the reference
-accumulates in half; outputs agree with it to half-precision rounding.
+Numerics (`precision`, a keyword the reference does not have):
+  "fp16" (default)  the density MLPs on fp16 MFMA -- layer inputs rounded to
+                    half as in the reference's half arithmetic, products
+                    accumulated in fp32 -- everything else in fp32;
+  "fp32"            every activation in fp32 on the vector ALUs: the exact
+                    value of the half parameters to fp32 rounding.
 """
 from __future__ import annotations
 
@@ -28,6 +32,12 @@ def _dev(device):
     return torch.device("cuda", int(device))
 
 
+def _entry(name, precision):
+    if precision not in ("fp16", "fp32"):
+        raise RuntimeError(f"precision must be 'fp16' or 'fp32', got {precision!r}")
+    return name if precision == "fp16" else name + "_fp32"
+
+
 def _params(p, n, name, dev):
     if not isinstance(p, torch.Tensor) or not p.is_cuda:
         raise RuntimeError(f"{name} must be a cuda tensor")
@@ -38,7 +48,7 @@ def _params(p, n, name, dev):
 
 def infer_nerf(sorted_dists, points_z, points_tpsdf, sampled_feats_geo, sampled_rgbs,
                sampled_feats_rgb, rays_dir, net_params_density, net_params_color, batch_size,
-               n_rays, n_sampled, geo_num_views, tex_num_views, device):
+               n_rays, n_sampled, geo_num_views, tex_num_views, device, precision="fp16"):
     """infer_fast_rendering (infer_mlp.cu:638-789).
     sorted_dists [B, n_rays, n_sampled]; per-view inputs flattened in (batch,
     view, ray, sample) order: points_z, points_tpsdf [B*Vg*R*S(, 1)],
@@ -70,14 +80,15 @@ def infer_nerf(sorted_dists, points_z, points_tpsdf, sampled_feats_geo, sampled_
     feats = torch.empty((B, R, 30), dtype=_f32, device=dev)
     wsb = _lib.lib().sailor_infer_nerf_workspace_bytes(B, R, S, Vt)
     ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=dev)
-    call("sailor_infer_nerf", ptr(t), ptr(z), ptr(ps), ptr(geo), ptr(rgb), ptr(frgb), ptr(dirs),
+    call(_entry("sailor_infer_nerf", precision), ptr(t), ptr(z), ptr(ps), ptr(geo), ptr(rgb),
+         ptr(frgb), ptr(dirs),
          ptr(pd), ptr(pc), B, R, S, Vg, Vt, ptr(depth), ptr(color), ptr(feats), ptr(ws),
          ws.numel(), stream_of(dev))
     return depth, color, feats
 
 
 def infer_density(points_z, points_tpsdf, sampled_feats_geo, net_params_density, batch_size,
-                  num_points, num_views, device):
+                  num_points, num_views, device, precision="fp16"):
     """infer_density_mlp (infer_mlp.cu:583-634): occupancy [B, 1, N] of points
     seen from `num_views` views (inputs in (batch, view, point) order)."""
     dev = _dev(device)
@@ -90,7 +101,8 @@ def infer_density(points_z, points_tpsdf, sampled_feats_geo, net_params_density,
             raise RuntimeError(f"{name} has {x.numel()} values, expected {B * V * N * c}")
     pd = _params(net_params_density, DENSITY_PARAMS, "net_params_density", dev)
     out = torch.empty((B, 1, N), dtype=_f32, device=dev)
-    call("sailor_infer_density", ptr(z), ptr(ps), ptr(geo), ptr(pd), B, N, V, ptr(out),
+    call(_entry("sailor_infer_density", precision), ptr(z), ptr(ps), ptr(geo), ptr(pd), B, N, V,
+         ptr(out),
          stream_of(dev))
     return out
 

@@ -61,6 +61,8 @@ _SIGS = {
     "sailor_infer_nerf_workspace_bytes": [I, I, I, I],
     "sailor_infer_nerf": [P, P, P, P, P, P, P, P, P, I, I, I, I, I, P, P, P, P, SZ, P],
     "sailor_infer_density": [P, P, P, P, I, I, I, P, P],
+    "sailor_infer_nerf_fp32": [P, P, P, P, P, P, P, P, P, I, I, I, I, I, P, P, P, P, SZ, P],
+    "sailor_infer_density_fp32": [P, P, P, P, I, I, I, P, P],
     "sailor_infer_blending": [P, P, P, I, P, P],
     "sailor_rays_interpolate": [P, P, P, I, I, P, P],
     "sailor_rgb_normalize": [P, P, I, I, I, I, F, F, P, P],

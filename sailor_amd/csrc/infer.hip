@@ -151,6 +151,241 @@ density_kernel(const half_t* __restrict__ params, const float* __restrict__ z,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Density MLPs on MFMA (v_mfma_f32_32x32x16_f16): the reference evaluates
+// them in half precision; here the layer inputs are rounded to half, the
+// products are accumulated in fp32 (the half weights are exact).
+//
+// A wave takes 32 columns = 32 / VS points x VS view slots (VS = V rounded up
+// to a power of two <= 8).  Weight rows are permuted so that, with the 32x32
+// accumulator layout (lane l, accumulator r -> row 8 (r/4) + 4 (l/32) + r%4,
+// column l%32), accumulator r of M-tile mt in lane half h holds output unit
+// 32 mt + 16 (r/8) + 8 h + r%8: exactly the 8 inputs k = 16 s + 8 h + j that
+// the same lane supplies as the B operand of K-step s = 2 mt + r/8 of the
+// next layer.  Activations therefore stay in registers from layer to layer;
+// the view mean is 1-3 lane shuffles; only the packed A fragments (45 KB) and
+// biases live in LDS.
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// 4 feature layers, 2 hidden occupancy layers, the 2 geometry-feature layers
+constexpr int kMfmaLayers = 8;
+__host__ __device__ constexpr int mfma_steps(int l) { return l == 0 || l == 6 ? 2 : l == 7 ? 1 : 4; }
+__host__ __device__ constexpr int mfma_mts(int l) { return l >= 6 ? 1 : 2; }
+__host__ __device__ constexpr int mfma_frag_base(int l) {  // in f16x8 units
+  return l <= 6 ? (l == 0 ? 0 : 256 + (l - 1) * 512) : 2816 + 128;
+}
+constexpr int kMfmaFrags = 2816 + 128 + 64;  // 3008 f16x8 = 48,128 B
+// parameter offsets (halves) of the layers' weights, input and output widths
+__host__ __device__ constexpr int mfma_w_off(int l) {
+  return l == 0 ? 0
+       : l < 4  ? 1216 + (l - 1) * 4160
+       : l < 6  ? kFeatParams + (l - 4) * 4160
+       : l == 6 ? kGeoOffset
+                : kGeoOffset + 18 * 16 + 16;
+}
+__host__ __device__ constexpr int mfma_k_in(int l) { return l == 0 || l == 6 ? 18 : l == 7 ? 16 : 64; }
+__host__ __device__ constexpr int mfma_n_out(int l) { return l == 6 ? 16 : l == 7 ? 8 : 64; }
+
+__host__ __device__ constexpr int perm_unit(int mt, int m) {
+  return 32 * mt + 16 * (m / 16) + 8 * ((m / 4) % 2) + 4 * ((m / 8) % 2) + m % 4;
+}
+__device__ __forceinline__ constexpr int acc_unit(int mt, int r, int h) {
+  return 32 * mt + 16 * (r / 8) + 8 * h + r % 8;
+}
+
+struct DensityMfmaLds {
+  f16x8 frag[kMfmaFrags];
+  float bias[kMfmaLayers][64];
+  float w_out[64];
+  float b_out;
+};
+
+// acc = bias + W x for one layer (STEPS K-steps of 16) on this wave's 32 columns
+// `zo` is an opaque zero redefined every tile: it keeps the bias reads in
+// the tile loop, so the registers go to the weight fragments, which the
+// compiler keeps resident across tiles
+template <int STEPS>
+__device__ __forceinline__ void mfma_layer(const DensityMfmaLds& L, int layer,
+                                           const f16x8 (&bin)[4], f32x16 (&acc)[2], int lane,
+                                           int zo) {
+  const int h = lane >> 5;
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[mt][r] = L.bias[layer][acc_unit(mt, r, h) + zo];
+  const f16x8* fr = L.frag + mfma_frag_base(layer);
+#pragma unroll
+  for (int s = 0; s < STEPS; ++s)
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+      acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fr[(mt * STEPS + s) * 64 + lane], bin[s],
+                                                       acc[mt], 0, 0, 0);
+}
+
+// B operands of the next layer: relu(acc), rounded to half
+__device__ __forceinline__ void relu_to_b(const f32x16 (&acc)[2], f16x8 (&b)[4]) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) b[s][j] = (half_t)fmaxf(acc[s / 2][8 * (s % 2) + j], 0.0f);
+}
+
+// one M-tile (the geometry layers: <= 32 outputs)
+template <int STEPS>
+__device__ __forceinline__ void mfma_layer1(const DensityMfmaLds& L, int layer,
+                                            const f16x8 (&bin)[4], f32x16& acc, int lane,
+                                            int zo) {
+  const int h = lane >> 5;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = L.bias[layer][acc_unit(0, r, h) + zo];
+  const f16x8* fr = L.frag + mfma_frag_base(layer);
+#pragma unroll
+  for (int s = 0; s < STEPS; ++s)
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(fr[s * 64 + lane], bin[s], acc, 0, 0, 0);
+}
+
+template <int VS>
+__global__ void __launch_bounds__(kBlock)
+density_mfma_kernel(const half_t* __restrict__ params, const float* __restrict__ z,
+                    const float* __restrict__ tpsdf, const float* __restrict__ geo, int B,
+                    long long n_pts, int V, float* __restrict__ occ,
+                    float* __restrict__ geo_out, int n_geo_out) {
+  __shared__ DensityMfmaLds L;
+  // ---- stage the permuted A fragments, biases and the small weights
+  for (int i = threadIdx.x; i < kMfmaFrags * 8; i += kBlock) {
+    const int f = i >> 3, j = i & 7;
+    int layer = kMfmaLayers - 1;
+    while (f < mfma_frag_base(layer)) --layer;
+    const int steps = mfma_steps(layer), local = f - mfma_frag_base(layer);
+    const int ln = local & 63, s = (local >> 6) % steps, mt = (local >> 6) / steps;
+    const int u = perm_unit(mt, ln & 31), k = 16 * s + 8 * (ln >> 5) + j;
+    const int kin = mfma_k_in(layer);
+    L.frag[f][j] = (k < kin && u < mfma_n_out(layer)) ? params[mfma_w_off(layer) + u * kin + k]
+                                                      : (half_t)0.0f;
+  }
+  for (int i = threadIdx.x; i < kMfmaLayers * 64; i += kBlock) {
+    const int layer = i / 64, u = i % 64, no = mfma_n_out(layer);
+    L.bias[layer][u] =
+        u < no ? (float)params[mfma_w_off(layer) + no * mfma_k_in(layer) + u] : 0.0f;
+  }
+  for (int i = threadIdx.x; i < 64; i += kBlock) L.w_out[i] = (float)params[kGeoOffset - 65 + i];
+  if (threadIdx.x == 0) L.b_out = (float)params[kGeoOffset - 1];
+  __syncthreads();
+
+  constexpr int P = 32 / VS;
+  const int lane = threadIdx.x & 63, h = lane >> 5, n = lane & 31;
+  const int v = n % VS;
+  const long long total = (long long)B * n_pts;
+  const long long tiles = (total + P - 1) / P;
+  const long long wave0 = (long long)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const long long n_waves = (long long)gridDim.x * (kBlock / 64);
+  const float inv = 1.0f / (float)V;
+  // this lane's share of a column's inputs x = [z, tpsdf, geo 0..15]:
+  // half 0 holds x 0..7 and x 16..17, half 1 holds x 8..15 (its layer-0 B
+  // operand); the next tile's share is loaded before the current tile's
+  // MFMA chain runs (one wave per SIMD: nothing else hides the latency)
+  auto load_share = [&](long long tile, float (&x)[10]) {
+#pragma unroll
+    for (int i = 0; i < 10; ++i) x[i] = 0.0f;
+    const long long p = tile * P + n / VS;
+    if (tile < tiles && p < total && v < V) {
+      const long long b = p / n_pts, q = p - b * n_pts;
+      const long long e = (b * V + v) * n_pts + q;
+      const float* g = geo + e * 16;
+      if (h == 0) {
+        x[0] = z[e];
+        x[1] = tpsdf[e];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) x[2 + i] = g[i];
+        x[8] = g[14];
+        x[9] = g[15];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = g[6 + i];
+      }
+    }
+  };
+  float xc[10];
+  load_share(wave0, xc);
+  for (long long tile = wave0; tile < tiles; tile += n_waves) {
+    float xn[10];
+    load_share(tile + n_waves, xn);
+    const long long p = tile * P + n / VS;
+    const bool col_ok = p < total && v < V;
+    const long long b = p / n_pts, q = p - b * n_pts;
+    f16x8 bin[4];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      bin[0][j] = (half_t)xc[j];
+      bin[1][j] = (half_t)(h == 0 && j < 2 ? xc[8 + j] : 0.0f);
+      bin[2][j] = (half_t)0.0f;
+      bin[3][j] = (half_t)0.0f;
+    }
+    int zo;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(zo));
+    f16x8 bin0[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) bin0[t] = bin[t];
+    f32x16 acc[2];
+    mfma_layer<2>(L, 0, bin, acc, lane, zo);
+#pragma unroll
+    for (int layer = 1; layer < 4; ++layer) {
+      relu_to_b(acc, bin);
+      mfma_layer<4>(L, layer, bin, acc, lane, zo);
+    }
+    // mean of the ReLU outputs over the point's views (fp32)
+    const float cm = col_ok ? 1.0f : 0.0f;
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        float t = fmaxf(acc[mt][r], 0.0f) * cm;
+#pragma unroll
+        for (int off = 1; off < VS; off <<= 1) t += __shfl_xor(t, off, 64);
+        acc[mt][r] = t * inv;
+      }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) bin[s][j] = (half_t)acc[s / 2][8 * (s % 2) + j];
+    mfma_layer<4>(L, 4, bin, acc, lane, zo);
+    relu_to_b(acc, bin);
+    mfma_layer<4>(L, 5, bin, acc, lane, zo);
+    float part = 0.0f;
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) part += L.w_out[acc_unit(mt, r, h)] * fmaxf(acc[mt][r], 0.0f);
+    part += __shfl_xor(part, 32, 64);
+    if (h == 0 && v == 0 && p < total) occ[p] = 1.0f / (1.0f + expf(-(part + L.b_out)));
+    // geometry features of the texture views: 18 -> 16 -> 8 on MFMA from the
+    // layer-0 operand; hidden unit 8 h + r sits in accumulator r < 8 of half
+    // h, which is the next layer's B operand, and output r < 8 lands in
+    // accumulator r of half 0.  The whole wave runs the MFMAs (an MFMA reads
+    // the A and B registers of every lane, so a lane-dependent branch around
+    // it would feed it stale rows); only the store is per lane.
+    if (geo_out) {
+      f32x16 g;
+      mfma_layer1<2>(L, 6, bin0, g, lane, zo);
+      f16x8 gb[4];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) gb[0][j] = (half_t)fmaxf(g[j], 0.0f);
+      mfma_layer1<1>(L, 7, gb, g, lane, zo);
+      if (h == 0 && col_ok && v < n_geo_out) {
+        float4* o4 =
+            reinterpret_cast<float4*>(geo_out + ((b * n_geo_out + v) * n_pts + q) * 8);
+        o4[0] = make_float4(fmaxf(g[0], 0.f), fmaxf(g[1], 0.f), fmaxf(g[2], 0.f),
+                            fmaxf(g[3], 0.f));
+        o4[1] = make_float4(fmaxf(g[4], 0.f), fmaxf(g[5], 0.f), fmaxf(g[6], 0.f),
+                            fmaxf(g[7], 0.f));
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 10; ++i) xc[i] = xn[i];
+  }
+}
+
 // One lane per (ray, texture view): composite the view's 30 features
 // [rgb feats 16 | rgb 3 | geometry features 8 | direction 3] with the
 // occupancies (volume_rendering_occ_kernel, infer_mlp.cu:508-581), then the
@@ -524,13 +759,45 @@ extern "C" size_t sailor_infer_nerf_workspace_bytes(int B, int R, int S, int tex
   return (n * sizeof(float) + 255) / 256 * 256 + n * tex_views * 8 * sizeof(float);
 }
 
-extern "C" int sailor_infer_nerf(const float* sorted_dists, const float* points_z,
-                                 const float* points_tpsdf, const float* feats_geo,
-                                 const float* rgbs, const float* feats_rgb, const float* dirs,
-                                 const void* params_density, const void* params_color, int B,
-                                 int R, int S, int geo_views, int tex_views, float* out_depth,
-                                 float* out_rgb, float* out_feats, void* workspace,
-                                 size_t workspace_bytes, void* stream) {
+namespace {
+
+// Density pass: the MFMA kernel for up to 8 geometry views (half inputs,
+// fp32 accumulation), or the fp32 vector kernel.
+int launch_density(bool mfma, const half_t* pd, const float* z, const float* tpsdf,
+                   const float* geo, int B, long long n_pts, int V, float* occ, float* geo_out,
+                   int n_geo_out, hipStream_t st) {
+  if (mfma && V <= 8) {
+    const int VS = V <= 1 ? 1 : V <= 2 ? 2 : V <= 4 ? 4 : 8;
+    const long long tiles = ((long long)B * n_pts + 32 / VS - 1) / (32 / VS);
+    // persistent workgroups, 3 per CU (50 KB of LDS and ~150 registers per
+    // lane each): the weights are staged once per workgroup
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 256;
+    const unsigned grid = (unsigned)std::max<long long>(
+        1, std::min<long long>((tiles + kBlock / 64 - 1) / (kBlock / 64), 3LL * cus));
+#define SAILOR_DENSITY_MFMA(N)                                                            \
+  hipLaunchKernelGGL(density_mfma_kernel<N>, dim3(grid), dim3(kBlock), 0, st, pd, z, tpsdf, \
+                     geo, B, n_pts, V, occ, geo_out, n_geo_out)
+    if (VS == 1) SAILOR_DENSITY_MFMA(1);
+    else if (VS == 2) SAILOR_DENSITY_MFMA(2);
+    else if (VS == 4) SAILOR_DENSITY_MFMA(4);
+    else SAILOR_DENSITY_MFMA(8);
+#undef SAILOR_DENSITY_MFMA
+    return sailor::check_launch("infer (density, mfma)");
+  }
+  hipLaunchKernelGGL(density_kernel, dim3(grid_for((long long)B * n_pts)), dim3(kBlock),
+                     kDensityLds, st, pd, z, tpsdf, geo, B, n_pts, V, occ, geo_out, n_geo_out);
+  return sailor::check_launch("infer (density)");
+}
+
+int infer_nerf_impl(bool mfma, const float* sorted_dists, const float* points_z,
+                    const float* points_tpsdf, const float* feats_geo, const float* rgbs,
+                    const float* feats_rgb, const float* dirs, const void* params_density,
+                    const void* params_color, int B, int R, int S, int geo_views, int tex_views,
+                    float* out_depth, float* out_rgb, float* out_feats, void* workspace,
+                    size_t workspace_bytes, void* stream) {
   SAILOR_REQUIRE(B > 0 && R >= 0 && S > 0, "infer_nerf: bad sizes B=%d R=%d S=%d", B, R, S);
   SAILOR_REQUIRE(geo_views >= 1 && tex_views >= 1 && tex_views <= kTexSlots &&
                      tex_views <= geo_views,
@@ -546,27 +813,67 @@ extern "C" int sailor_infer_nerf(const float* sorted_dists, const float* points_
   float* geo8 = reinterpret_cast<float*>(static_cast<char*>(workspace) +
                                          ((size_t)B * n_pts * sizeof(float) + 255) / 256 * 256);
   if (int err = setup_all()) return err;
-  hipLaunchKernelGGL(density_kernel, dim3(grid_for((long long)B * n_pts)), dim3(kBlock),
-                     kDensityLds, st,
-                     static_cast<const half_t*>(params_density), points_z, points_tpsdf,
-                     feats_geo, B, n_pts, geo_views, occ, geo8, tex_views);
-  if (int err = sailor::check_launch("infer_nerf (density)")) return err;
+  if (int err = launch_density(mfma, static_cast<const half_t*>(params_density), points_z,
+                               points_tpsdf, feats_geo, B, n_pts, geo_views, occ, geo8,
+                               tex_views, st))
+    return err;
   hipLaunchKernelGGL(color_kernel, dim3(grid_for((long long)B * R * kTexSlots)), dim3(kBlock),
-                     kColorLds, st, static_cast<const half_t*>(params_color), sorted_dists, occ, feats_rgb,
-                     rgbs, geo8, dirs, B, R, S, tex_views, 1e-4f, out_depth, out_rgb, out_feats);
+                     kColorLds, st, static_cast<const half_t*>(params_color), sorted_dists, occ,
+                     feats_rgb, rgbs, geo8, dirs, B, R, S, tex_views, 1e-4f, out_depth, out_rgb,
+                     out_feats);
   return sailor::check_launch("infer_nerf (colour)");
+}
+
+int infer_density_impl(bool mfma, const float* points_z, const float* points_tpsdf,
+                       const float* feats_geo, const void* params_density, int B, int N, int V,
+                       float* out, void* stream) {
+  SAILOR_REQUIRE(B > 0 && N >= 0 && V >= 1, "infer_density: bad sizes B=%d N=%d V=%d", B, N, V);
+  if (N == 0) return SAILOR_OK;
+  if (int err = setup_all()) return err;
+  return launch_density(mfma, static_cast<const half_t*>(params_density), points_z, points_tpsdf,
+                        feats_geo, B, (long long)N, V, out, nullptr, 0,
+                        sailor::as_stream(stream));
+}
+
+}  // namespace
+
+extern "C" int sailor_infer_nerf(const float* sorted_dists, const float* points_z,
+                                 const float* points_tpsdf, const float* feats_geo,
+                                 const float* rgbs, const float* feats_rgb, const float* dirs,
+                                 const void* params_density, const void* params_color, int B,
+                                 int R, int S, int geo_views, int tex_views, float* out_depth,
+                                 float* out_rgb, float* out_feats, void* workspace,
+                                 size_t workspace_bytes, void* stream) {
+  return infer_nerf_impl(true, sorted_dists, points_z, points_tpsdf, feats_geo, rgbs, feats_rgb,
+                         dirs, params_density, params_color, B, R, S, geo_views, tex_views,
+                         out_depth, out_rgb, out_feats, workspace, workspace_bytes, stream);
+}
+
+extern "C" int sailor_infer_nerf_fp32(const float* sorted_dists, const float* points_z,
+                                      const float* points_tpsdf, const float* feats_geo,
+                                      const float* rgbs, const float* feats_rgb,
+                                      const float* dirs, const void* params_density,
+                                      const void* params_color, int B, int R, int S,
+                                      int geo_views, int tex_views, float* out_depth,
+                                      float* out_rgb, float* out_feats, void* workspace,
+                                      size_t workspace_bytes, void* stream) {
+  return infer_nerf_impl(false, sorted_dists, points_z, points_tpsdf, feats_geo, rgbs, feats_rgb,
+                         dirs, params_density, params_color, B, R, S, geo_views, tex_views,
+                         out_depth, out_rgb, out_feats, workspace, workspace_bytes, stream);
 }
 
 extern "C" int sailor_infer_density(const float* points_z, const float* points_tpsdf,
                                     const float* feats_geo, const void* params_density, int B,
                                     int N, int V, float* out, void* stream) {
-  SAILOR_REQUIRE(B > 0 && N >= 0 && V >= 1, "infer_density: bad sizes B=%d N=%d V=%d", B, N, V);
-  if (N == 0) return SAILOR_OK;
-  if (int err = setup_all()) return err;
-  hipLaunchKernelGGL(density_kernel, dim3(grid_for((long long)B * N)), dim3(kBlock), kDensityLds,
-                     sailor::as_stream(stream), static_cast<const half_t*>(params_density),
-                     points_z, points_tpsdf, feats_geo, B, (long long)N, V, out, nullptr, 0);
-  return sailor::check_launch("infer_density");
+  return infer_density_impl(true, points_z, points_tpsdf, feats_geo, params_density, B, N, V,
+                            out, stream);
+}
+
+extern "C" int sailor_infer_density_fp32(const float* points_z, const float* points_tpsdf,
+                                         const float* feats_geo, const void* params_density,
+                                         int B, int N, int V, float* out, void* stream) {
+  return infer_density_impl(false, points_z, points_tpsdf, feats_geo, params_density, B, N, V,
+                            out, stream);
 }
 
 extern "C" int sailor_infer_blending(const float* feats_neighbors, const float* ray_feats,

@@ -68,9 +68,11 @@ def sample_features(rays, dists, ks, rts, geo_cl, depths, rgbf_cl, rgb_cl, depth
 
 
 def forward_nerf(ray_ori, ray_dir, sorted_dists, ks, rts, geo_cl, depths, rgbf_cl, rgb_cl,
-                 depths_mid, depths_dist, params_density, params_color, res=512):
+                 depths_mid, depths_dist, params_density, params_color, res=512,
+                 precision="fp16"):
     """gui_render.forward_nerf for one batch: keeps the rays whose samples
-    are all valid, gathers their features and runs InferNerf.infer_nerf.
+    are all valid, gathers their features and runs InferNerf.infer_nerf
+    (`precision` as there).
     Returns (colour [1,R,3], depth [1,R,1], colour features [1,R,30]) with
     zeros for the other rays."""
     dev = sorted_dists.device
@@ -87,7 +89,8 @@ def forward_nerf(ray_ori, ray_dir, sorted_dists, ks, rts, geo_cl, depths, rgbf_c
     z, tps, geo, rgbf, rgb, dirs = sample_features(rays, dists, ks, rts, geo_cl, depths, rgbf_cl,
                                                    rgb_cl, depths_mid, depths_dist, res)
     d, c, f = InferNerf.infer_nerf(dists, z, tps, geo, rgb, rgbf, dirs, params_density,
-                                   params_color, 1, valid.numel(), S, V, V, dev.index or 0)
+                                   params_color, 1, valid.numel(), S, V, V, dev.index or 0,
+                                   precision=precision)
     o_c[:, valid] = c
     o_d[:, valid] = d
     o_f[:, valid] = f

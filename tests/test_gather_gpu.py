@@ -75,7 +75,8 @@ def test_forward_nerf_chain_matches_float64():
     cu = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
     c, d, f = forward_nerf(cu(rays[..., :3]), cu(rays[..., 3:]), cu(dists), cu(K), cu(RT),
                            _cl(maps["geo"]), cu(maps["dep"][:, :, 0]), _cl(maps["rgbf"]),
-                           _cl(maps["rgb"]), cu(mid), cu(dist), cu(pd).half(), cu(pc).half())
+                           _cl(maps["rgb"]), cu(mid), cu(dist), cu(pd).half(), cu(pc).half(),
+                           precision="fp32")
     torch.cuda.synchronize()
     valid = np.nonzero(np.all(dists[0] != -1, -1))[0]
     sf = gr.sample_features(rays[:, valid], dists[:, valid], K, RT, maps["geo"], maps["dep"],
