@@ -326,19 +326,21 @@ def fastnerf_upsampling(steps):
     deps = 2.0 + torch.rand((V, 512, 512), device=dev, generator=g)
     pb = torch.from_numpy(fastnerf_params()[2]).to(dev)
 
-    def go():
-        return forward_upsampling(ray_ori, ray_dir, o_d, o_c, o_f, f(tgt.K), f(tgt.R), ks, rs, ts,
-                                  rgbf, rgbs, deps, (0, 1), pb)
-    go()
-    torch.cuda.synchronize()
-    ms = _median_ms(go, steps)
     n_up = int(on.sum()) * 4
     flops = n_up * 2 * (40 * 64 + 64 * 64 + 64 * 32 + 62 * 64 + 64 * 32 + 32 * 3)
-    return {"workload": f"forward_upsampling: {side}^2 rays ({n_up // 4} on the surface) x 4 "
-                        "sub-pixels, 2 of 4 views with 1024^2 features",
-            "ms": ms, "subpixel_rays_per_s": n_up / (ms / 1e3), "flops": flops,
-            "vector_tflops": flops / (ms / 1e3) / 1e12,
-            "frac_of_fp32_vector_peak": flops / (ms / 1e3) / 1e12 / 157.3}
+    out = {"workload": f"forward_upsampling: {side}^2 rays ({n_up // 4} on the surface) x 4 "
+                       "sub-pixels, 2 of 4 views with 1024^2 features", "flops": flops}
+    tK, tR = f(tgt.K), f(tgt.R)
+    for prec in ("fp16", "fp32"):
+        def go():
+            return forward_upsampling(ray_ori, ray_dir, o_d, o_c, o_f, tK, tR, ks, rs, ts, rgbf,
+                                      rgbs, deps, (0, 1), pb, precision=prec)
+        go()
+        torch.cuda.synchronize()
+        ms = _median_ms(go, steps)
+        out[prec] = {"ms": ms, "subpixel_rays_per_s": n_up / (ms / 1e3),
+                     "tflops": flops / (ms / 1e3) / 1e12}
+    return out
 
 
 def training_kernels(steps, C=8):

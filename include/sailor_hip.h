@@ -638,6 +638,20 @@ int sailor_forward_upsampling(const float* ray_ori, const float* ray_dir, const 
                               float valid_min, float valid_max, const void* params_blend,
                               float* out, void* stream);
 
+/* sailor_forward_upsampling runs the blending MLP (the reference's half
+ * precision) on fp16 MFMA: layer inputs rounded to half, fp32 accumulation.
+ * The _fp32 variant keeps it in fp32 on the vector ALUs (no counterpart in
+ * the reference). */
+int sailor_forward_upsampling_fp32(const float* ray_ori, const float* ray_dir,
+                                   const float* depth, const float* coarse_rgb,
+                                   const float* ray_feats, int N, const float* tar_K,
+                                   const float* tar_R, const float* K, const float* R,
+                                   const float* T, int V, int nei0, int nei1,
+                                   const float* rgbf_cl, int Hf, int Wf, const float* rgb_cl,
+                                   int Hc, int Wc, const float* depths, int Hd, int Wd, int res,
+                                   float valid_min, float valid_max, const void* params_blend,
+                                   float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -232,14 +232,16 @@ def infer_nerf(sorted_dists, points_z, points_tpsdf, feats_geo, rgbs, feats_rgb,
     return dep.reshape(B, R, 1), rgb.reshape(B, R, 3), feats.reshape(B, R, 30)
 
 
-def infer_blending(feats_neighbors, ray_feats, pb, dtype=torch.float64):
-    """upsampling_kernel: [N, 40], [N, 30] -> blend weights [N, 3]."""
+def infer_blending(feats_neighbors, ray_feats, pb, dtype=torch.float64, act_dtype=None):
+    """upsampling_kernel: [N, 40], [N, 30] -> blend weights [N, 3]; with
+    act_dtype the inputs of the first five layers are rounded to it."""
     L = unpack_blend(_params(pb, dtype))
+    r = lambda a: _round(a, act_dtype)
     x = _t(feats_neighbors, dtype)
-    h = _relu(_lin(x, L[0]))
-    h = _relu(_lin(h, L[1]))
-    h = _relu(_lin(h, L[2]))
+    h = _relu(_lin(r(x), L[0]))
+    h = _relu(_lin(r(h), L[1]))
+    h = _relu(_lin(r(h), L[2]))
     h = torch.cat([h, _t(ray_feats, dtype)], 1)
-    h = _relu(_lin(h, L[3]))
-    h = _relu(_lin(h, L[4]))
+    h = _relu(_lin(r(h), L[3]))
+    h = _relu(_lin(r(h), L[4]))
     return torch.sigmoid(_lin(h, L[5]))

@@ -46,7 +46,7 @@ def rays_interpolate(ray_dirs, K, R):
 
 def forward_upsampling(ray_ori, ray_dir, o_d, o_c, o_c_feat, tar_K, tar_R, ks, rs, ts,
                        rgb_feats, rgbs, depths, nei, pb, valid_min=0.2, valid_max=4.0,
-                       res=512, trace=None):
+                       res=512, trace=None, act_dtype=None):
     """ray_ori, ray_dir [R,3]; o_d [R]; o_c [R,3]; o_c_feat [R,30]; tar_K,
     tar_R [3,3]; ks, rs [V,3,3]; ts [V,3]; channel-first maps rgb_feats
     [V,16,Hf,Wf], rgbs [V,3,Hc,Wc], depths [V,1,Hd,Wd]; nei (n0, n1); pb the
@@ -54,7 +54,8 @@ def forward_upsampling(ray_ori, ray_dir, o_d, o_c, o_c_feat, tar_K, tar_R, ks, r
     valid ray indices and, per valid ray and sub-pixel, the pixel
     coordinates of the nearest-mode taps (for tie detection), the 40 blend
     inputs, the neighbour colours, the blend weights, the directions and the
-    neighbour-camera depths."""
+    neighbour-camera depths.  act_dtype rounds the blending MLP's layer
+    inputs (fastnerf_ref.infer_blending)."""
     o_d = _t(o_d).reshape(-1)
     Rn = o_d.numel()
     out = torch.zeros((Rn, 4, 3), dtype=torch.float64)
@@ -89,7 +90,7 @@ def forward_upsampling(ray_ori, ray_dir, o_d, o_c, o_c_feat, tar_K, tar_R, ks, r
             H, W = _t(m).shape[-2:]
             pix.append(torch.stack([(grids[v][:, 0] + 1) / 2 * (W - 1),
                                     (grids[v][:, 1] + 1) / 2 * (H - 1)], -1))
-    w = fr.infer_blending(torch.cat(feats, -1), of, pb)
+    w = fr.infer_blending(torch.cat(feats, -1), of, pb, act_dtype=act_dtype)
     c = torch.clamp(w[:, 0:1] * cols[0] + w[:, 1:2] * cols[1] + w[:, 2:3] * oc, 0, 1)
     out[valid] = c.reshape(n, 4, 3)
     if trace is not None:

@@ -74,6 +74,8 @@ _SIGS = {
     "sailor_trilinear_aggregate_backward": [P, I, P, P, P, P, P, I, I, I, I, P, P],
     "sailor_forward_upsampling": [P, P, P, P, P, I, P, P, P, P, P, I, I, I, P, I, I, P, I, I, P,
                                   I, I, I, F, F, P, P, P],
+    "sailor_forward_upsampling_fp32": [P, P, P, P, P, I, P, P, P, P, P, I, I, I, P, I, I, P, I, I,
+                                       P, I, I, I, F, F, P, P, P],
     "sailor_freq_encode_forward": [P, I, I, I, P, P],
     "sailor_freq_encode_backward": [P, P, I, I, I, P, P],
 }

@@ -720,6 +720,223 @@ __global__ void __launch_bounds__(kBlock) upsample_kernel(const half_t* __restri
   }
 }
 
+// ---------------------------------------------------------------------------
+// The same sub-pixel upsampling with the blending MLP on fp16 MFMA (layer
+// inputs rounded to half, fp32 accumulation).  A wave takes 32 sub-pixel
+// rays (8 rays x 4 sub-pixels); lane half h gathers neighbour view nei[h]
+// (bilinear features, nearest colour and depth, the Gaussian weight), the
+// halves exchange their shares with 32-lane shuffles to form the layer-0
+// operand, and the layers 40 -> 64 -> 64 -> 32, [32 | ray features 30] ->
+// 64 -> 32 run as a register chain with the row permutation of the density
+// kernel; 32 -> 3 is a dot product and one shuffle.
+struct BlendLayer {
+  int w_off, k_in, n_out, steps, mts, frag_base;
+};
+__host__ __device__ constexpr BlendLayer blend_layer(int l) {
+  return l == 0   ? BlendLayer{0, 40, 64, 3, 2, 0}
+         : l == 1 ? BlendLayer{2624, 64, 64, 4, 2, 384}
+         : l == 2 ? BlendLayer{6784, 64, 32, 4, 1, 896}
+         : l == 3 ? BlendLayer{8864, 62, 64, 4, 2, 1152}
+                  : BlendLayer{12896, 64, 32, 4, 1, 1664};
+}
+constexpr int kBlendMfmaLayers = 5, kBlendFrags = 1920, kBlendOutOff = 14976;
+
+struct BlendMfmaLds {
+  f16x8 frag[kBlendFrags];
+  float bias[kBlendMfmaLayers][64];
+  float w_out[3][32];
+  float b_out[4];
+};
+
+template <int STEPS, int MTS>
+__device__ __forceinline__ void blend_layer_mfma(const BlendMfmaLds& L, int layer,
+                                                 const f16x8 (&bin)[4], f32x16 (&acc)[2],
+                                                 int lane, int zo) {
+  const int h = lane >> 5;
+#pragma unroll
+  for (int mt = 0; mt < MTS; ++mt)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[mt][r] = L.bias[layer][acc_unit(mt, r, h) + zo];
+  const f16x8* fr = L.frag + blend_layer(layer).frag_base;
+#pragma unroll
+  for (int s = 0; s < STEPS; ++s)
+#pragma unroll
+    for (int mt = 0; mt < MTS; ++mt)
+      acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fr[(mt * STEPS + s) * 64 + lane], bin[s],
+                                                       acc[mt], 0, 0, 0);
+}
+
+__global__ void __launch_bounds__(kBlock) upsample_mfma_kernel(const half_t* __restrict__ params,
+                                                               UpsampleArgs a) {
+  __shared__ BlendMfmaLds L;
+  for (int i = threadIdx.x; i < kBlendFrags * 8; i += kBlock) {
+    const int f = i >> 3, j = i & 7;
+    int layer = kBlendMfmaLayers - 1;
+    while (f < blend_layer(layer).frag_base) --layer;
+    const BlendLayer d = blend_layer(layer);
+    const int local = f - d.frag_base;
+    const int ln = local & 63, s = (local >> 6) % d.steps, mt = (local >> 6) / d.steps;
+    const int u = perm_unit(mt, ln & 31), k = 16 * s + 8 * (ln >> 5) + j;
+    L.frag[f][j] = (k < d.k_in && u < d.n_out) ? params[d.w_off + u * d.k_in + k]
+                                               : (half_t)0.0f;
+  }
+  for (int i = threadIdx.x; i < kBlendMfmaLayers * 64; i += kBlock) {
+    const int layer = i / 64, u = i % 64;
+    const BlendLayer d = blend_layer(layer);
+    L.bias[layer][u] = u < d.n_out ? (float)params[d.w_off + d.n_out * d.k_in + u] : 0.0f;
+  }
+  for (int i = threadIdx.x; i < 96; i += kBlock) L.w_out[i / 32][i % 32] = (float)params[kBlendOutOff + i];
+  if (threadIdx.x < 3) L.b_out[threadIdx.x] = (float)params[kBlendOutOff + 96 + threadIdx.x];
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, h = lane >> 5, n = lane & 31;
+  float Kinv[9], Rinv[9];
+  inv3(a.tar_K, Kinv);
+  inv3(a.tar_R, Rinv);
+  const int vview = a.nei[h];
+  const float* Rm = a.R + vview * 9;
+  const float* Km = a.K + vview * 9;
+  const float* Tv = a.T + vview * 3;
+  const float* rgbf_v = a.rgbf + (size_t)vview * a.Hf * a.Wf * 16;
+  const float* rgb_v = a.rgb + (size_t)vview * a.Hc * a.Wc * 3;
+  const float* dep_v = a.dep + (size_t)vview * a.Hd * a.Wd;
+  const float half_res = (float)(a.res / 2);
+  const long long tiles = (a.N * 4 + 31) / 32;
+  const long long wave0 = (long long)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const long long n_waves = (long long)gridDim.x * (kBlock / 64);
+  for (long long tile = wave0; tile < tiles; tile += n_waves) {
+    const long long e = tile * 32 + n;  // sub-pixel ray (ray e / 4, sub-pixel e % 4)
+    const long long p = e >> 2;
+    const bool in = p < a.N;
+    const float od = in ? a.depth[p] : 0.0f;
+    const bool ok = in && od > a.vmin && od < a.vmax;
+    if (!__any(ok)) {
+      if (h == 0 && in) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) a.out[e * 3 + i] = 0.0f;
+      }
+      continue;
+    }
+    // this half's neighbour view: 16 features, direction, Gaussian weight
+    float f[16], dv[3], coff = 0.0f, col[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int i = 0; i < 16; ++i) f[i] = 0.0f;
+    dv[0] = dv[1] = dv[2] = 0.0f;
+    if (ok) {
+      float d[3], de[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) d[i] = a.ray_dir[p * 3 + i];
+      interp_dir(a.tar_K, a.tar_R, Kinv, Rinv, d, (int)(e & 3), de);
+      float pos[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) pos[i] = a.ray_ori[p * 3 + i] + de[i] * od;
+      const bool invalid_d = od < 0.1f || od > 4.0f;
+      float cam[3];
+      mat3_vec(Rm, pos, cam);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) cam[i] += Tv[i];
+      const bool invalid = cam[2] < 0.01f;
+      const float zd = invalid ? 1.0f : cam[2];
+      const float q[3] = {cam[0] / zd, cam[1] / zd, cam[2] / zd};
+      float gx = (Km[0] * q[0] + Km[1] * q[1] + Km[2] * q[2] - half_res) / half_res;
+      float gy = (Km[3] * q[0] + Km[4] * q[1] + Km[5] * q[2] - half_res) / half_res;
+      if (invalid || invalid_d) gx = gy = -2.0f;
+      const float pz = invalid_d ? -2.0f : cam[2];
+      float sd[1];
+      sailor::bilinear_cl<16>(rgbf_v, a.Hf, a.Wf, gx, gy, f);
+      sailor::nearest_cl<3>(rgb_v, a.Hc, a.Wc, gx, gy, col);
+      sailor::nearest_cl<1>(dep_v, a.Hd, a.Wd, gx, gy, sd);
+      mat3_vec(Rm, de, dv);
+      const float dz = pz - sd[0];
+      coff = expf(-200.0f * (dz * dz));
+    }
+    // layer-0 operand x = [f(nei0) 16 | dir 3 | w | f(nei1) 16 | dir 3 | w],
+    // k = 16 s + 8 h + j: half 0 supplies x 0..7, 16..23, 32..39 and half 1
+    // x 8..15, 24..31
+    float of[16], od3[3], ocf;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) of[i] = __shfl_xor(f[i], 32, 64);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) od3[i] = __shfl_xor(dv[i], 32, 64);
+    ocf = __shfl_xor(coff, 32, 64);
+    f16x8 bin[4];
+    if (h == 0) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) bin[0][j] = (half_t)f[j];
+      bin[1][0] = (half_t)dv[0];
+      bin[1][1] = (half_t)dv[1];
+      bin[1][2] = (half_t)dv[2];
+      bin[1][3] = (half_t)coff;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bin[1][4 + j] = (half_t)of[j];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bin[2][j] = (half_t)of[12 + j];
+      bin[2][4] = (half_t)od3[0];
+      bin[2][5] = (half_t)od3[1];
+      bin[2][6] = (half_t)od3[2];
+      bin[2][7] = (half_t)ocf;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        bin[0][j] = (half_t)of[8 + j];
+        bin[1][j] = (half_t)f[4 + j];
+        bin[2][j] = (half_t)0.0f;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) bin[3][j] = (half_t)0.0f;
+    int zo;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(zo));
+    f32x16 acc[2];
+    blend_layer_mfma<3, 2>(L, 0, bin, acc, lane, zo);
+    relu_to_b(acc, bin);
+    blend_layer_mfma<4, 2>(L, 1, bin, acc, lane, zo);
+    relu_to_b(acc, bin);
+    blend_layer_mfma<4, 1>(L, 2, bin, acc, lane, zo);
+    // [relu(layer 2) 32 | ray colour features 30 | 0 0]
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) bin[s][j] = (half_t)fmaxf(acc[0][8 * s + j], 0.0f);
+    {
+      const float* rf = a.feats + (in ? p : 0) * 30;
+#pragma unroll
+      for (int s = 2; s < 4; ++s)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int k = 16 * (s - 2) + 8 * h + j;
+          bin[s][j] = (half_t)(k < 30 && in ? rf[k] : 0.0f);
+        }
+    }
+    blend_layer_mfma<4, 2>(L, 3, bin, acc, lane, zo);
+    relu_to_b(acc, bin);
+    blend_layer_mfma<4, 1>(L, 4, bin, acc, lane, zo);
+    float w[3];
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+      float part = 0.0f;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) part += L.w_out[o][acc_unit(0, r, h)] * fmaxf(acc[0][r], 0.0f);
+      part += __shfl_xor(part, 32, 64);
+      w[o] = 1.0f / (1.0f + expf(-(part + L.b_out[o])));
+    }
+    float ocol[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ocol[i] = __shfl_xor(col[i], 32, 64);
+    if (h == 0 && in) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        float c = 0.0f;
+        if (ok) {
+          c = w[0] * col[i] + w[1] * ocol[i] + w[2] * a.coarse[p * 3 + i];
+          c = c != c ? c : fminf(fmaxf(c, 0.0f), 1.0f);
+        }
+        a.out[e * 3 + i] = c;
+      }
+    }
+  }
+}
+
 // Dynamic LDS of a kernel: per-thread rows + the parameter group (half).
 template <typename K>
 int lds_setup(K kern, size_t bytes) {
@@ -898,12 +1115,16 @@ extern "C" int sailor_rays_interpolate(const float* K, const float* R, const flo
   return sailor::check_launch("rays_interpolate");
 }
 
-extern "C" int sailor_forward_upsampling(
-    const float* ray_ori, const float* ray_dir, const float* depth, const float* coarse_rgb,
-    const float* ray_feats, int N, const float* tar_K, const float* tar_R, const float* K,
-    const float* R, const float* T, int V, int nei0, int nei1, const float* rgbf_cl, int Hf,
-    int Wf, const float* rgb_cl, int Hc, int Wc, const float* depths, int Hd, int Wd, int res,
-    float valid_min, float valid_max, const void* params_blend, float* out, void* stream) {
+namespace {
+
+int forward_upsampling_impl(bool mfma, const float* ray_ori, const float* ray_dir,
+                            const float* depth, const float* coarse_rgb, const float* ray_feats,
+                            int N, const float* tar_K, const float* tar_R, const float* K,
+                            const float* R, const float* T, int V, int nei0, int nei1,
+                            const float* rgbf_cl, int Hf, int Wf, const float* rgb_cl, int Hc,
+                            int Wc, const float* depths, int Hd, int Wd, int res,
+                            float valid_min, float valid_max, const void* params_blend,
+                            float* out, void* stream) {
   SAILOR_REQUIRE(N >= 0 && V > 0, "forward_upsampling: bad sizes N=%d V=%d", N, V);
   SAILOR_REQUIRE(nei0 >= 0 && nei0 < V && nei1 >= 0 && nei1 < V,
                  "forward_upsampling: neighbour views (%d, %d) outside [0, %d)", nei0, nei1, V);
@@ -915,7 +1136,46 @@ extern "C" int sailor_forward_upsampling(
   UpsampleArgs a{ray_ori, ray_dir, depth, coarse_rgb, ray_feats, tar_K, tar_R, K, R, T,
                  rgbf_cl, rgb_cl, depths, (long long)N, Hf, Wf, Hc, Wc, Hd, Wd, res,
                  {nei0, nei1}, valid_min, valid_max, out};
-  hipLaunchKernelGGL(upsample_kernel, dim3(grid_for((long long)N * 4)), dim3(kBlock), kBlendLds,
-                     sailor::as_stream(stream), static_cast<const half_t*>(params_blend), a);
+  hipStream_t st = sailor::as_stream(stream);
+  if (mfma) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 256;
+    const long long tiles = ((long long)N * 4 + 31) / 32;
+    const unsigned grid = (unsigned)std::max<long long>(
+        1, std::min<long long>((tiles + kBlock / 64 - 1) / (kBlock / 64), 3LL * cus));
+    hipLaunchKernelGGL(upsample_mfma_kernel, dim3(grid), dim3(kBlock), 0, st,
+                       static_cast<const half_t*>(params_blend), a);
+  } else {
+    hipLaunchKernelGGL(upsample_kernel, dim3(grid_for((long long)N * 4)), dim3(kBlock),
+                       kBlendLds, st, static_cast<const half_t*>(params_blend), a);
+  }
   return sailor::check_launch("forward_upsampling");
+}
+
+}  // namespace
+
+extern "C" int sailor_forward_upsampling(
+    const float* ray_ori, const float* ray_dir, const float* depth, const float* coarse_rgb,
+    const float* ray_feats, int N, const float* tar_K, const float* tar_R, const float* K,
+    const float* R, const float* T, int V, int nei0, int nei1, const float* rgbf_cl, int Hf,
+    int Wf, const float* rgb_cl, int Hc, int Wc, const float* depths, int Hd, int Wd, int res,
+    float valid_min, float valid_max, const void* params_blend, float* out, void* stream) {
+  return forward_upsampling_impl(true, ray_ori, ray_dir, depth, coarse_rgb, ray_feats, N, tar_K,
+                                 tar_R, K, R, T, V, nei0, nei1, rgbf_cl, Hf, Wf, rgb_cl, Hc, Wc,
+                                 depths, Hd, Wd, res, valid_min, valid_max, params_blend, out,
+                                 stream);
+}
+
+extern "C" int sailor_forward_upsampling_fp32(
+    const float* ray_ori, const float* ray_dir, const float* depth, const float* coarse_rgb,
+    const float* ray_feats, int N, const float* tar_K, const float* tar_R, const float* K,
+    const float* R, const float* T, int V, int nei0, int nei1, const float* rgbf_cl, int Hf,
+    int Wf, const float* rgb_cl, int Hc, int Wc, const float* depths, int Hd, int Wd, int res,
+    float valid_min, float valid_max, const void* params_blend, float* out, void* stream) {
+  return forward_upsampling_impl(false, ray_ori, ray_dir, depth, coarse_rgb, ray_feats, N, tar_K,
+                                 tar_R, K, R, T, V, nei0, nei1, rgbf_cl, Hf, Wf, rgb_cl, Hc, Wc,
+                                 depths, Hd, Wd, res, valid_min, valid_max, params_blend, out,
+                                 stream);
 }

@@ -125,7 +125,7 @@ def _neighbours(nei_ids):
 
 def forward_upsampling(ray_ori, ray_dir, o_d, o_c, o_c_feat, tar_K, tar_R, ks, rs, ts,
                        rgb_feats_cl, rgbs_cl, depths, nei_ids, params_blend,
-                       valid_depth_min=0.2, valid_depth_max=4.0, res=512):
+                       valid_depth_min=0.2, valid_depth_max=4.0, res=512, precision="fp16"):
     """gui_render.forward_upsampling + sample_neighbor for one target camera.
     ray_ori, ray_dir [1,R,3]; rendered o_d [1,R,1], o_c [1,R,3], o_c_feat
     [1,R,30]; tar_K, tar_R [3,3]; source views ks, rs [V,3,3], ts [V,3(,1)];
@@ -134,7 +134,10 @@ def forward_upsampling(ray_ori, ray_dir, o_d, o_c, o_c_feat, tar_K, tar_R, ks, r
     neighbour view indices; params_blend the 15075 blending halves.
     Returns the sub-pixel colours [1,R,4,3] (zeros where o_d is outside
     (valid_depth_min, valid_depth_max)).  Rays are not compacted: the depth
-    test runs in the kernel, so there is no host synchronisation."""
+    test runs in the kernel, so there is no host synchronisation.
+    precision "fp16" (default): the blending MLP on fp16 MFMA (half inputs,
+    fp32 accumulation; the reference evaluates it in half); "fp32": on the
+    vector ALUs in fp32."""
     dev = o_d.device
     R = o_d.reshape(-1).numel()
     ori = as_input(ray_ori, "ray_ori", _f32, dev).reshape(R, 3)
@@ -164,7 +167,10 @@ def forward_upsampling(ray_ori, ray_dir, o_d, o_c, o_c_feat, tar_K, tar_R, ks, r
     pb = params_blend.to(device=dev, dtype=torch.float16).contiguous()
     n0, n1 = _neighbours(nei_ids)
     out = torch.empty((1, R, 4, 3), dtype=_f32, device=dev)
-    call("sailor_forward_upsampling", ptr(ori), ptr(dirs), ptr(od), ptr(oc), ptr(of), R,
+    if precision not in ("fp16", "fp32"):
+        raise RuntimeError(f"precision must be 'fp16' or 'fp32', got {precision!r}")
+    name = "sailor_forward_upsampling" + ("" if precision == "fp16" else "_fp32")
+    call(name, ptr(ori), ptr(dirs), ptr(od), ptr(oc), ptr(of), R,
          ptr(tK), ptr(tR), ptr(K), ptr(Rm), ptr(T), V, n0, n1, ptr(rf), rf.shape[1], rf.shape[2],
          ptr(rc), rc.shape[1], rc.shape[2], ptr(dp), Hd, Wd, int(res), float(valid_depth_min),
          float(valid_depth_max), ptr(pb), ptr(out), stream_of(dev))
