@@ -424,6 +424,28 @@ def training_kernels(steps, C=8):
     return res
 
 
+def gui_frame(steps):
+    """The GUI's per-frame render on the FastNerf path (gui/gui_render.py:
+    921-934) at its own settings (4 x 512^2 source views, 128^3 grid / 4,
+    512^2 target, 32 samples, upsampled 4 sub-pixels per ray): TSDF + tree,
+    trace + sort, per-view features + infer_nerf, upsampling, as one frame;
+    the valid-ray compaction syncs the host as in the reference."""
+    import torch
+    from sailor_amd.gui import GuiFrame
+    out = {"workload": "gui frame: 4 x 512^2 views, 128^3 grid / 4, 512^2 target, 32 samples, "
+                       "x4 sub-pixel upsampling, FastNerf (16-ch 128^2 features, 16-ch 1024^2 "
+                       "high-resolution features)"}
+    for prec in ("fp16", "fp32"):
+        g = GuiFrame(precision=prec)
+        g.render()
+        torch.cuda.synchronize()
+        ms = _median_ms(g.render, steps)
+        out[prec] = {"ms_per_frame": ms, "frames_per_s": 1e3 / ms}
+        out["valid_rays"] = int(torch.all(g.pipe.sorted[0] != -1, dim=-1).sum())
+        del g
+    return out
+
+
 def batching_form(steps, n_frames=16, precs=("fp32", "bf16")):
     """Config-5 block: 16 frames of uneven size (own seeded scene each,
     performer scale U(0.6, 1.2), 2-6 source views of 1024^2, 256^3 grid,
@@ -586,6 +608,7 @@ def main():
     if args.infer_only:
         print(json.dumps(fastnerf_infer(args.steps)))
         print(json.dumps(fastnerf_upsampling(args.steps)))
+        print(json.dumps(gui_frame(args.steps)))
         return
     if args.training_only:
         print(json.dumps(training_form(args.steps, args.warmup, with_cpu=not args.no_cpu)))
@@ -673,6 +696,7 @@ def main():
     if not args.no_infer and args.workload == "c2":
         result["fastnerf_infer"] = fastnerf_infer(max(3, args.steps // 4))
         result["fastnerf_upsampling"] = fastnerf_upsampling(max(3, args.steps // 4))
+        result["gui_frame"] = gui_frame(max(3, args.steps // 4))
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(p32)
     else:
