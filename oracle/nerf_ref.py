@@ -54,11 +54,13 @@ def gather_features(x, Kv, RTv, fmaps, img_hw):
     return feats
 
 
-def hydra_rgb(hydra, rgbw, toks):
-    """Hydra encoder stacks over the view tokens toks [N, V, F] and the RGB
-    head on the first view's token (aggregation.py:101-128,164-171,299-307;
-    nerf.py:117-124).  hydra: list of dicts of tensors (q, k, v, fc, w1, b1,
-    w2, b2); rgbw: [(w1, b1), (w2, b2), (w3, b3)].  Returns rgb [N, 3]."""
+def hydra_encode(hydra, toks):
+    """Hydra encoder stacks over the view tokens toks [N, V, F]
+    (aggregation.py:101-128 HydraAttention, :164-171 HydraForward,
+    :299-307 HydraEncoder): q, k, v projections, q and k L2-normalised over
+    features, kv = sum over views of k * v, out = fc(q * kv) + x, then
+    x' = w2 relu(w1 out + b1) + b2 + out.  hydra: list of dicts of tensors
+    (q, k, v, fc, w1, b1, w2, b2).  Returns the tokens [N, V, F]."""
     x_tok = toks
     for st in hydra:
         q = Fn.normalize(x_tok @ st["q"].T, dim=-1)
@@ -67,6 +69,14 @@ def hydra_rgb(hydra, rgbw, toks):
         kv = (k * vv).sum(dim=-2, keepdim=True)
         out = (q * kv) @ st["fc"].T + x_tok
         x_tok = torch.relu(out @ st["w1"].T + st["b1"]) @ st["w2"].T + st["b2"] + out
+    return x_tok
+
+
+def hydra_rgb(hydra, rgbw, toks):
+    """hydra_encode, then the RGB head on the first view's token
+    (nerf.py:117-124).  rgbw: [(w1, b1), (w2, b2), (w3, b3)].  Returns rgb
+    [N, 3]."""
+    x_tok = hydra_encode(hydra, toks)
     h = x_tok[:, 0]
     (w1, b1), (w2, b2), (w3, b3) = rgbw
     h = torch.relu(h @ w1.T + b1)
@@ -115,7 +125,8 @@ def render_rays(weights, t, rays, Kv, RTv, fmaps, bbox, zdir, img_hw, t_thres=1e
     asum [B,R], occ [B,R,S] (zeros for dropped rays).  With a dict `trace`
     (B == 1), trace["x0"][v] holds view v's first-layer input [N,S,63+F]
     and trace["pre"][v][l] its layer-l pre-activation [N,S,H] (gradients
-    retained), for the valid rays in order.  With mm_dtype (e.g. bfloat16)
+    retained), trace["toks"] the per-view tokens [N,V,F] and trace["occ"]
+    the occupancies [N,S], for the valid rays in order.  With mm_dtype (e.g. bfloat16)
     the dense layers' inputs and weights are rounded to that type and the
     products accumulated in `dtype` (mixed-precision matrix units; gradients
     are rounded where they pass the same casts)."""
@@ -182,6 +193,8 @@ def render_rays(weights, t, rays, Kv, RTv, fmaps, bbox, zdir, img_hw, t_thres=1e
                 toks[:, v] += w[:, None] * feats[v][:, s]
             Tt = torch.where(alive, Tt * (1 - a), Tt)
             alive = alive & ~(Tt < t_thres)
+        if trace is not None:
+            trace["toks"], trace["occ"] = toks, alpha
         rgb = hydra_rgb(P["hydra"], P["rgb"], toks)
         rgb_out[b, idx] = rgb
         dep_out[b, idx] = dsum * (d @ zdir[b])
