@@ -446,6 +446,65 @@ def gui_frame(steps):
     return out
 
 
+def standalone_kernels(steps):
+    """The stand-alone HBM-bound kernels of the path at their configuration
+    sizes, each against the 8 TB/s HBM roofline with the per-unit
+    algorithmic bytes of the path's measurement table:
+      volume_rendering_occ_forward   C2: 262,144 rays x 48 samples, RGB;
+                                     20 B per sample read, 24 B per ray written
+      volume_rendering_training_forward / _backward  C4: 524,288 rays x 48,
+                                     RGB; fwd 24 B per sample (t, sort index,
+                                     sigma, ...) + 24 B per ray; bwd fwd bytes
+                                     + 16 B per sample written + 24 B per ray
+      freq_encode_forward            4,194,304 points, degree 10: 12 B read +
+                                     252 B written per point
+      depth_normalize                C2: 4 x 1024^2 depths: 4 B read (twice:
+                                     min/max, then normalise) + 8 B written
+                                     per pixel
+    Inputs are seeded synthetic tensors resident in HBM."""
+    import torch
+    from sailor_amd import VoxelEncoding as VE
+    dev = "cuda"
+    g = torch.Generator(device=dev).manual_seed(9)
+    out = {}
+
+    def rec(name, fn, nbytes, unit):
+        fn()
+        torch.cuda.synchronize()
+        ms = _median_ms(fn, steps)
+        gbs = nbytes / (ms / 1e3) / 1e9
+        out[name] = {"ms": ms, "algorithmic_bytes": nbytes, "GB_per_s": gbs,
+                     "frac_of_hbm_peak": gbs / HBM_GBS, "unit": unit}
+
+    R, S = 262144, 48
+    t = torch.sort(2.0 + torch.rand((1, R, S), device=dev, generator=g), -1)[0]
+    occ = torch.rand((1, R, S, 1), device=dev, generator=g)
+    rgb = torch.rand((1, R, S, 3), device=dev, generator=g)
+    rec("volume_rendering_occ_forward", lambda: VE.volume_rendering_occ_forward(t, occ, rgb, 0, 0.0),
+        R * S * 20 + R * 24, "C2 rays x 48 samples")
+    R4 = 524288
+    t4 = torch.sort(2.0 + torch.rand((1, R4, S), device=dev, generator=g), -1)[0]
+    si = torch.arange(S, device=dev, dtype=torch.int32).expand(1, R4, S).contiguous()
+    sig = torch.rand((1, R4, S, 1), device=dev, generator=g)
+    f4 = torch.rand((1, R4, S, 3), device=dev, generator=g)
+    fo = VE.volume_rendering_training_forward(t4, si, sig, f4, 0, 0.0, True)
+    grads = [torch.rand_like(x) for x in fo]
+    rec("volume_rendering_training_forward",
+        lambda: VE.volume_rendering_training_forward(t4, si, sig, f4, 0, 0.0, True),
+        R4 * S * 24 + R4 * 24, "C4 rays x 48 samples")
+    rec("volume_rendering_training_backward",
+        lambda: VE.volume_rendering_training_backward(*grads, *fo, t4, si, sig, f4, 0, 0.0, True),
+        R4 * S * 40 + R4 * 48, "C4 rays x 48 samples")
+    N = 4194304
+    x = torch.rand((N, 3), device=dev, generator=g) * 2 - 1
+    rec("freq_encode_forward", lambda: VE.freq_encode_forward(x, 10, 0), N * (12 + 252),
+        "points")
+    d = 1.5 + torch.rand((1, 4, 1024, 1024), device=dev, generator=g) * 2
+    rec("depth_normalize", lambda: VE.depth_normalize(d, 0.2, 4.0, -1, False, True, 0),
+        4 * 1024 * 1024 * (4 + 4 + 8), "pixels")
+    return out
+
+
 def batching_form(steps, n_frames=16, precs=("fp32", "bf16")):
     """Config-5 block: 16 frames of uneven size (own seeded scene each,
     performer scale U(0.6, 1.2), 2-6 source views of 1024^2, 256^3 grid,
@@ -615,6 +674,7 @@ def main():
         return
     if args.kernels_only:
         print(json.dumps(training_kernels(args.steps)))
+        print(json.dumps(standalone_kernels(args.steps)))
         return
     p32 = FramePipeline.synthetic(cfg, prec="fp32")
     p32.run()
@@ -697,6 +757,7 @@ def main():
         result["fastnerf_infer"] = fastnerf_infer(max(3, args.steps // 4))
         result["fastnerf_upsampling"] = fastnerf_upsampling(max(3, args.steps // 4))
         result["gui_frame"] = gui_frame(max(3, args.steps // 4))
+        result["standalone_kernels"] = standalone_kernels(max(3, args.steps // 4))
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(p32)
     else:

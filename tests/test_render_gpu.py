@@ -23,10 +23,10 @@ def _rand_rays(B=2, R=513, S=48, F=3, seed=0, invalid_frac=0.1):
     return t, occ, feats
 
 
-@pytest.mark.parametrize("F", [3, 8, 33])
-def test_occ_forward_bitexact(F):
+@pytest.mark.parametrize("F,S", [(3, 48), (8, 48), (33, 48), (3, 37), (5, 21), (1, 7)])
+def test_occ_forward_bitexact(F, S):
     from sailor_amd import VoxelEncoding as VE
-    t, occ, feats = _rand_rays(F=F)
+    t, occ, feats = _rand_rays(F=F, S=S)
     of, od, ow, oa = VE.volume_rendering_occ_forward(
         torch.from_numpy(t).cuda(), torch.from_numpy(occ).cuda(),
         torch.from_numpy(feats).cuda(), 0, 1e-4)
