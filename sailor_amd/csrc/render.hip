@@ -14,6 +14,8 @@
 // One lane per ray with the samples walked in order keeps the reference's
 // exact floating-point recurrence (T *= 1 - alpha, running sums); the
 // per-ray rows are contiguous so consecutive lanes stream consecutive rows.
+#include <type_traits>
+
 #include "common.h"
 
 namespace {
@@ -369,6 +371,82 @@ __global__ void zero2(float* __restrict__ a, long long na, float* __restrict__ b
   }
 }
 
+// backward_walk with a ray's channels split over a group of LPR lanes (the
+// lane at `base` carries channel 0): every lane of the group runs the same
+// recurrence on the same samples, so T, w and the running sums are
+// identical in all of them; lane j keeps channel j's prefix and gradient
+// (lanes j >= F repeat channel F-1 and write nothing).  The channel sum of
+// the sigma gradient is taken in channel order from the group's lanes, as
+// the reference's loop adds it; lane 0 of the group writes it.  All lanes of
+// a group take the same branches, so the shuffles always find their source
+// lane active.
+template <int LPR>
+__device__ __forceinline__ void backward_walk_split(
+    const float* tr, const int* ir, const float* sr, const float* frow, int j, int base,
+    const float* gf, const float* of, float gd, float gw, float ga, float d_final,
+    float ws_final, float a_final, int S, int F, float t_thres, int occ_mode, float* gfo,
+    float* gso) {
+  const int last = ir[S - 1];
+  if (last < 0 || last >= S || tr[last] < 0) return;
+  const int jc = min(j, F - 1);
+  const bool own = j < F;
+  const float gfj = gf[jc], ofj = of[jc];
+  float pf = 0.0f;
+  float T = 1.0f, ws = 0.0f, d = 0.0f, alphas = 0.0f;
+  WalkSample<1> nxt = read_sample<1>(tr, sr, frow, ir[0], S, F, jc, 1);
+  int si2 = S > 1 ? ir[1] : 0;
+  for (int i = 0; i < S; ++i) {
+    const WalkSample<1> c = nxt;
+    if (i + 1 < S) nxt = read_sample<1>(tr, sr, frow, si2, S, F, jc, 1);
+    si2 = i + 2 < S ? ir[i + 2] : 0;
+    if (!c.ok) continue;  // malformed permutation entry
+    const int si = c.si;
+    const float z = c.z;
+    if (z < 0) continue;
+    float alpha, delta = 0.0f;
+    if (occ_mode) {
+      alpha = c.sg;
+    } else {
+      delta = (i < S - 1) ? next_depth(tr, ir, i, S) - z : 1e10f;
+      alpha = 1.0f - expf(-c.sg * delta);
+    }
+    const float w = alpha * T;
+    pf += w * c.f[0];
+    d += w * z;
+    ws += w;
+    alphas += w * alpha;
+    T *= 1.0f - alpha;
+    if (T < t_thres) break;
+    if (own) gfo[(size_t)si * F + j] = gfj * w;
+    const float term = gfj * (T * c.f[0] - (ofj - pf));
+    float gsf = 0.0f;
+#pragma unroll
+    for (int q = 0; q < LPR; ++q) {
+      const float tq = __shfl(term, base + q, 64);
+      if (q < F) gsf += tq;
+    }
+    float g;
+    if (occ_mode) {
+      // clipped 1/(1-alpha) (:497); the test is evaluated in double there
+      const float tmp = 1.0 / ((double)(1.0f - alpha) + 1e-7) <= 15.0
+                            ? 1.0f / (1.0f - alpha)
+                            : 15.0f;
+      const float g_ws_ = gw * (T * 1.0f - (ws_final - ws)) * tmp;
+      const float g_d = gd * (T * z - (d_final - d)) * tmp;
+      const float g_a = ga * (2.0f * T * alpha - (a_final - alphas)) * tmp;
+      g = g_ws_ + g_d + gsf * tmp + g_a;
+    } else {
+      const float tmp = gw * (1.0f - ws_final);
+      const float g_ws_ = delta * (gw * (T * 1.0f - (ws_final - ws)) + tmp);
+      const float g_d = delta * (gd * (T * z - (d_final - d)) + tmp);
+      const float g_a = delta * (ga * (2.0f * T * alpha - (a_final - alphas)) + tmp);
+      const float g_f = delta * (gsf + tmp);
+      g = g_ws_ + g_d + g_f + g_a;
+    }
+    if (j == 0) gso[si] = g;
+  }
+}
+
 // ---- Streamed LDS forms (F <= 8) -------------------------------------------
 // The walk visits a ray's samples in sort-index order, so a row cannot be
 // consumed in chunks.  A persistent one-wave workgroup owns tiles of RB rays:
@@ -376,7 +454,9 @@ __global__ void zero2(float* __restrict__ a, long long na, float* __restrict__ b
 // the features, S*F per ray) are read with coalesced loads into registers
 // (a few words per lane and array, each array unrolled on its own so the
 // loop body stays small), stored into odd-stride LDS rows (lanes reading the same
-// sample slot hit distinct banks), and lane r < RB walks ray r there.  The
+// sample slot hit distinct banks), and each ray is walked there by a group
+// of 4 (F <= 4) or 8 lanes, one channel per lane, so a 16-ray tile fills the
+// wave.  The
 // loads of the wave's next tile are issued before the walk of the current
 // one, so memory latency overlaps the walk.  The gradient form stages the
 // per-ray inputs as well (its walk touches no global memory) and builds its
@@ -503,7 +583,7 @@ size_t train_backward_tile_lds(int RB, int S, int F) {
 // leave two or three per SIMD; it is compiled for two.
 constexpr int kStreamWaves(int RB) { return RB >= 16 ? 2 : 4; }
 
-template <int RB, int FM, int ETS, int ETF>
+template <int RB, int LPR, int ETS, int ETF>
 __global__ void __launch_bounds__(kTileThreads)
 __attribute__((amdgpu_waves_per_eu(kStreamWaves(RB))))
 train_forward_stream(const float* __restrict__ t, const int* __restrict__ sidx,
@@ -532,23 +612,26 @@ train_forward_stream(const float* __restrict__ t, const int* __restrict__ sidx,
   for (;;) {
     tile_store<ETS, ETF>(g, RB, nr, lane, v, lds_u);
     __syncthreads();
-    const long long ray = tile * RB + lane;
     const int nr_cur = nr;
     const long long next = tile + gridDim.x;
     if (next < n_tiles) {
       nr = rays_in(next);
       tile_load<ETS, ETF>(g, ut, ui, us, uf, next * RB, nr, lane, v);
     }
-    if (lane < nr_cur) {
-      float acc[FM], d, ws, alphas;
-      forward_walk<FM>(lt + lane * g.TS, li + lane * g.TS, ls + lane * g.TS, lf + lane * g.FS,
-                       S, F, 0, F, t_thres, occ_mode, acc, d, ws, alphas);
-#pragma unroll
-      for (int j = 0; j < FM; ++j)
-        if (j < F) out_feats[ray * F + j] = acc[j];
-      out_depth[ray] = d;
-      out_ws[ray] = ws;
-      out_alpha[ray] = alphas;
+    // LPR lanes per ray, lane j of a group accumulating channel j
+    const int j = lane % LPR;
+    for (int r = lane / LPR; r - lane / LPR < nr_cur; r += kTileThreads / LPR) {
+      if (r >= nr_cur) continue;
+      float acc[1], d, ws, alphas;
+      forward_walk<1>(lt + r * g.TS, li + r * g.TS, ls + r * g.TS, lf + r * g.FS, S, F,
+                      min(j, F - 1), 1, t_thres, occ_mode, acc, d, ws, alphas);
+      const long long ray = tile * RB + r;
+      if (j < F) out_feats[ray * F + j] = acc[0];
+      if (j == 0) {
+        out_depth[ray] = d;
+        out_ws[ray] = ws;
+        out_alpha[ray] = alphas;
+      }
     }
     if (next >= n_tiles) break;
     __syncthreads();  // the walk's LDS reads are done before the next stores
@@ -556,7 +639,7 @@ train_forward_stream(const float* __restrict__ t, const int* __restrict__ sidx,
   }
 }
 
-template <int RB, int FM, int ETS, int ETF>
+template <int RB, int LPR, int ETS, int ETF>
 __global__ void __launch_bounds__(kTileThreads)
 __attribute__((amdgpu_waves_per_eu(2)))
 train_backward_stream(const float* __restrict__ g_feats, const float* __restrict__ g_depth,
@@ -630,12 +713,15 @@ train_backward_stream(const float* __restrict__ g_feats, const float* __restrict
       tile_load<ETS, ETF>(g, ut, ui, us, uf, next * RB, nr, lane, v);
       load_ray(next, nr);
     }
-    if (lane < nr_cur) {
-      const float* p = lpr + lane * PR;
-      backward_walk<FM>(lt + lane * g.TS, li + lane * g.TS, ls + lane * g.TS, lf + lane * g.FS,
-                       p, p + F, p[2 * F], p[2 * F + 1], p[2 * F + 2], p[2 * F + 3],
-                       p[2 * F + 4], p[2 * F + 5], S, F, t_thres, occ_mode, lgf + lane * g.FS,
-                       lgs + lane * g.TS);
+    // LPR lanes per ray, lane j of a group carrying channel j
+    const int j = lane % LPR;
+    for (int r = lane / LPR; r - lane / LPR < nr_cur; r += kTileThreads / LPR) {
+      if (r >= nr_cur) continue;
+      const float* p = lpr + r * PR;
+      backward_walk_split<LPR>(lt + r * g.TS, li + r * g.TS, ls + r * g.TS, lf + r * g.FS, j,
+                               lane - j, p, p + F, p[2 * F], p[2 * F + 1], p[2 * F + 2],
+                               p[2 * F + 3], p[2 * F + 4], p[2 * F + 5], S, F, t_thres,
+                               occ_mode, lgf + r * g.FS, lgs + r * g.TS);
     }
     __syncthreads();
     // write the gradient rows out and leave them zeroed for the next tile
@@ -675,6 +761,25 @@ int pick_stream(size_t (*lds_of)(int, int, int), int S, int F) {
       return i;
   }
   return -1;
+}
+
+// Calls launch(RB, ETS, ETF) with table entry i's shape as compile-time
+// constants.
+template <int I, typename L>
+int dispatch_stream_from(int i, L& launch) {
+  if constexpr (I < (int)(sizeof(kStreamShapes) / sizeof(kStreamShapes[0]))) {
+    if (i == I)
+      return launch(std::integral_constant<int, kStreamShapes[I].rb>{},
+                    std::integral_constant<int, kStreamShapes[I].ets>{},
+                    std::integral_constant<int, kStreamShapes[I].etf>{});
+    return dispatch_stream_from<I + 1>(i, launch);
+  }
+  return SAILOR_ERR_ARG;
+}
+
+template <typename L>
+int dispatch_stream(int i, L& launch) {
+  return dispatch_stream_from<0>(i, launch);
 }
 
 // Persistent grid of a streamed kernel: its resident workgroups per CU at
@@ -766,21 +871,15 @@ extern "C" int sailor_volume_rendering_training_forward(
   const long long n = (long long)B * R;
   if (n == 0) return SAILOR_OK;
   hipStream_t st = sailor::as_stream(stream);
-  switch (pick_stream(train_forward_tile_lds, S, F)) {
-    case 0:
-      return launch_forward_stream<16, 12, 36>(t, sort_idx, sigma, feats, n, S, F, t_thresh,
-                                           support_occ, out_feats, out_depth, out_ws, out_alpha,
-                                           st);
-    case 1:
-      return launch_forward_stream<8, 12, 36>(t, sort_idx, sigma, feats, n, S, F, t_thresh,
-                                          support_occ, out_feats, out_depth, out_ws, out_alpha,
-                                          st);
-    case 2:
-      return launch_forward_stream<4, 12, 36>(t, sort_idx, sigma, feats, n, S, F, t_thresh,
-                                          support_occ, out_feats, out_depth, out_ws, out_alpha,
-                                          st);
-    default:
-      break;
+  const int sh = pick_stream(train_forward_tile_lds, S, F);
+  if (sh >= 0) {
+    auto launch = [&](auto rb, auto ets, auto etf) {
+      return launch_forward_stream<decltype(rb)::value, decltype(ets)::value,
+                                   decltype(etf)::value>(t, sort_idx, sigma, feats, n, S, F,
+                                                         t_thresh, support_occ, out_feats,
+                                                         out_depth, out_ws, out_alpha, st);
+    };
+    return dispatch_stream(sh, launch);
   }
   for (int f0 = 0; f0 < F; f0 += 8) {
     hipLaunchKernelGGL(train_forward_kernel<8>, dim3(sailor::ceil_div(n, kThreads)),
@@ -801,24 +900,15 @@ extern "C" int sailor_volume_rendering_training_backward(
   const long long n = (long long)B * R;
   if (n == 0) return SAILOR_OK;
   hipStream_t st = sailor::as_stream(stream);
-  switch (pick_stream(train_backward_tile_lds, S, F)) {
-    case 0:
-      return launch_backward_stream<16, 12, 36>(g_feats, g_depth, g_ws, g_alpha, out_feats,
-                                            out_depth, out_ws, out_alpha, t, sort_idx, sigma,
-                                            feats, n, S, F, t_thresh, support_occ, grad_feats,
-                                            grad_sigma, st);
-    case 1:
-      return launch_backward_stream<8, 12, 36>(g_feats, g_depth, g_ws, g_alpha, out_feats,
-                                           out_depth, out_ws, out_alpha, t, sort_idx, sigma,
-                                           feats, n, S, F, t_thresh, support_occ, grad_feats,
-                                           grad_sigma, st);
-    case 2:
-      return launch_backward_stream<4, 12, 36>(g_feats, g_depth, g_ws, g_alpha, out_feats,
-                                           out_depth, out_ws, out_alpha, t, sort_idx, sigma,
-                                           feats, n, S, F, t_thresh, support_occ, grad_feats,
-                                           grad_sigma, st);
-    default:
-      break;
+  const int sh = pick_stream(train_backward_tile_lds, S, F);
+  if (sh >= 0) {
+    auto launch = [&](auto rb, auto ets, auto etf) {
+      return launch_backward_stream<decltype(rb)::value, decltype(ets)::value,
+                                    decltype(etf)::value>(
+          g_feats, g_depth, g_ws, g_alpha, out_feats, out_depth, out_ws, out_alpha, t, sort_idx,
+          sigma, feats, n, S, F, t_thresh, support_occ, grad_feats, grad_sigma, st);
+    };
+    return dispatch_stream(sh, launch);
   }
   const long long na = n * S * F, nb = n * S;
   int zb = (int)std::min<long long>(4096, (na + nb + 255) / 256);
