@@ -255,6 +255,19 @@ train_forward_kernel(const float* __restrict__ t, const int* __restrict__ sidx,
   }
 }
 
+// The reference's clip test 1.0 / ((double)(1 - alpha) + 1e-7) <= 15.0
+// (:497) decided without the double division away from its boundary:
+// x >= 0.0666667 gives 1/x < 14.9999993 and 0 < x <= 0.0666666 gives
+// 1/x > 15.0000015, margins far above the division's rounding; the band
+// between, x <= 0 and NaN take the division itself.  Agrees with the direct
+// form for every float alpha (checked exhaustively on the CPU).
+__device__ __forceinline__ bool clip_le15(float alpha) {
+  const double x = (double)(1.0f - alpha) + 1e-7;
+  if (x >= 0.0666667) return true;
+  if (x > 0.0 && x <= 0.0666666) return false;
+  return 1.0 / x <= 15.0;
+}
+
 // Gradient walk of one ray (reference :244-381, :384-522): re-walks the
 // prefix and uses the stored totals for the suffix sums.  Only samples
 // before early termination receive a gradient (the sample that ends the walk
@@ -323,9 +336,7 @@ __device__ __forceinline__ void backward_walk(const float* tr, const int* ir, co
     float g;
     if (occ_mode) {
       // clipped 1/(1-alpha) (:497); the test is evaluated in double there
-      const float tmp = 1.0 / ((double)(1.0f - alpha) + 1e-7) <= 15.0
-                            ? 1.0f / (1.0f - alpha)
-                            : 15.0f;
+      const float tmp = clip_le15(alpha) ? 1.0f / (1.0f - alpha) : 15.0f;
       const float g_ws_ = gw * (T * 1.0f - (ws_final - ws)) * tmp;
       const float g_d = gd * (T * z - (d_final - d)) * tmp;
       const float g_a = ga * (2.0f * T * alpha - (a_final - alphas)) * tmp;
@@ -369,6 +380,12 @@ __global__ void zero2(float* __restrict__ a, long long na, float* __restrict__ b
     if (i < na) a[i] = 0.0f;
     else b[i - na] = 0.0f;
   }
+}
+
+// Value of lane Q of each aligned group of 4 lanes (DPP quad_perm).
+template <int Q>
+__device__ __forceinline__ float quad_lane(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), Q * 0x55, 0xf, 0xf, false));
 }
 
 // backward_walk with a ray's channels split over a group of LPR lanes (the
@@ -420,17 +437,25 @@ __device__ __forceinline__ void backward_walk_split(
     if (own) gfo[(size_t)si * F + j] = gfj * w;
     const float term = gfj * (T * c.f[0] - (ofj - pf));
     float gsf = 0.0f;
+    if constexpr (LPR == 4) {
+      // groups of 4 are the hardware's lane quads: broadcast by DPP
+      const float t0 = quad_lane<0>(term), t1 = quad_lane<1>(term);
+      const float t2 = quad_lane<2>(term), t3 = quad_lane<3>(term);
+      gsf += t0;
+      if (1 < F) gsf += t1;
+      if (2 < F) gsf += t2;
+      if (3 < F) gsf += t3;
+    } else {
 #pragma unroll
-    for (int q = 0; q < LPR; ++q) {
-      const float tq = __shfl(term, base + q, 64);
-      if (q < F) gsf += tq;
+      for (int q = 0; q < LPR; ++q) {
+        const float tq = __shfl(term, base + q, 64);
+        if (q < F) gsf += tq;
+      }
     }
     float g;
     if (occ_mode) {
       // clipped 1/(1-alpha) (:497); the test is evaluated in double there
-      const float tmp = 1.0 / ((double)(1.0f - alpha) + 1e-7) <= 15.0
-                            ? 1.0f / (1.0f - alpha)
-                            : 15.0f;
+      const float tmp = clip_le15(alpha) ? 1.0f / (1.0f - alpha) : 15.0f;
       const float g_ws_ = gw * (T * 1.0f - (ws_final - ws)) * tmp;
       const float g_d = gd * (T * z - (d_final - d)) * tmp;
       const float g_a = ga * (2.0f * T * alpha - (a_final - alphas)) * tmp;
