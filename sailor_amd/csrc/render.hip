@@ -193,7 +193,10 @@ __device__ __forceinline__ WalkSample<FMAX> read_sample(const float* tr, const f
 // The reads run ahead of the recurrence (a step's data one step early, the
 // sort index two), so their latency overlaps the previous step's
 // arithmetic; the arithmetic and its order are those of the reference.
-template <int FMAX>
+// Without IDX the samples are in order already (step i is sample i) and
+// alpha is the occupancy: that is the inference compositing (reference
+// :666-745), whose early-out test on t[S-1] is the same one.
+template <int FMAX, bool IDX = true>
 __device__ __forceinline__ void forward_walk(const float* tr, const int* ir, const float* sr,
                                              const float* frow, int S, int F, int f0, int nf,
                                              float t_thres, int occ_mode, float (&acc)[FMAX],
@@ -202,19 +205,20 @@ __device__ __forceinline__ void forward_walk(const float* tr, const int* ir, con
   for (int j = 0; j < FMAX; ++j) acc[j] = 0.0f;
   float T = 1.0f;
   d = ws = alphas = 0.0f;
-  const int last = ir[S - 1];
+  auto idx = [&](int i) { if constexpr (IDX) return ir[i]; else return i; };
+  const int last = idx(S - 1);
   if (last < 0 || last >= S || tr[last] < 0) return;
-  WalkSample<FMAX> nxt = read_sample<FMAX>(tr, sr, frow, ir[0], S, F, f0, nf);
-  int si2 = S > 1 ? ir[1] : 0;
+  WalkSample<FMAX> nxt = read_sample<FMAX>(tr, sr, frow, idx(0), S, F, f0, nf);
+  int si2 = S > 1 ? idx(1) : 0;
   for (int i = 0; i < S; ++i) {
     const WalkSample<FMAX> c = nxt;
     if (i + 1 < S) nxt = read_sample<FMAX>(tr, sr, frow, si2, S, F, f0, nf);
-    si2 = i + 2 < S ? ir[i + 2] : 0;
+    si2 = i + 2 < S ? idx(i + 2) : 0;
     if (!c.ok) continue;  // malformed permutation entry
     const float z = c.z;
     if (z < 0) continue;
     float alpha;
-    if (occ_mode) {
+    if (!IDX || occ_mode) {
       alpha = c.sg;
     } else {
       const float delta = (i < S - 1) ? next_depth(tr, ir, i, S) - z : 1e10f;
@@ -535,14 +539,14 @@ struct TileRegs {
 
 // Issue the loads of the tile starting at ray ray0 with nr rays.  Words past
 // the tile read its first word instead (always in bounds) and are not stored.
-template <int ETS, int ETF>
+template <int ETS, int ETF, bool IDX>
 __device__ __forceinline__ void tile_load(const TileShape& g, const unsigned* __restrict__ t,
                                           const unsigned* __restrict__ sidx,
                                           const unsigned* __restrict__ sigma,
                                           const unsigned* __restrict__ feats, long long ray0,
                                           int nr, int lane, TileRegs<ETS, ETF>& v) {
   const unsigned* bt = t + ray0 * g.S;
-  const unsigned* bi = sidx + ray0 * g.S;
+  const unsigned* bi = IDX ? sidx + ray0 * g.S : t;
   const unsigned* bs = sigma + ray0 * g.S;
   const unsigned* bf = feats + ray0 * g.SF;
   const int nrs = nr * g.S, nrf = nr * g.SF;
@@ -552,7 +556,7 @@ __device__ __forceinline__ void tile_load(const TileShape& g, const unsigned* __
     const int e = ln + u * kTileThreads;
     const int k = e < nrs ? e : 0;
     v.t[u] = bt[k];
-    v.i[u] = bi[k];
+    if constexpr (IDX) v.i[u] = bi[k];
     v.s[u] = bs[k];
   }
 #pragma unroll
@@ -564,7 +568,7 @@ __device__ __forceinline__ void tile_load(const TileShape& g, const unsigned* __
 
 // Store a loaded tile into the LDS rows: t | sort index | sigma (RB rows of
 // stride TS each), then the features (RB rows of stride FS).
-template <int ETS, int ETF>
+template <int ETS, int ETF, bool IDX>
 __device__ __forceinline__ void tile_store(const TileShape& g, int RB, int nr, int lane,
                                            const TileRegs<ETS, ETF>& v, unsigned* lds) {
   const int nrs = nr * g.S, nrf = nr * g.SF;
@@ -580,7 +584,7 @@ __device__ __forceinline__ void tile_store(const TileShape& g, int RB, int nr, i
       const int r = div_small(e, g.S, g.inv_s);
       const int w = r * g.TS + (e - r * g.S);
       lt[w] = v.t[u];
-      li[w] = v.i[u];
+      if constexpr (IDX) li[w] = v.i[u];
       ls[w] = v.s[u];
     }
   }
@@ -608,7 +612,7 @@ size_t train_backward_tile_lds(int RB, int S, int F) {
 // leave two or three per SIMD; it is compiled for two.
 constexpr int kStreamWaves(int RB) { return RB >= 16 ? 2 : 4; }
 
-template <int RB, int LPR, int ETS, int ETF>
+template <int RB, int LPR, int ETS, int ETF, bool IDX>
 __global__ void __launch_bounds__(kTileThreads)
 __attribute__((amdgpu_waves_per_eu(kStreamWaves(RB))))
 train_forward_stream(const float* __restrict__ t, const int* __restrict__ sidx,
@@ -633,22 +637,22 @@ train_forward_stream(const float* __restrict__ t, const int* __restrict__ sidx,
   const unsigned* uf = reinterpret_cast<const unsigned*>(feats);
   TileRegs<ETS, ETF> v;
   int nr = rays_in(tile);
-  tile_load<ETS, ETF>(g, ut, ui, us, uf, tile * RB, nr, lane, v);
+  tile_load<ETS, ETF, IDX>(g, ut, ui, us, uf, tile * RB, nr, lane, v);
   for (;;) {
-    tile_store<ETS, ETF>(g, RB, nr, lane, v, lds_u);
+    tile_store<ETS, ETF, IDX>(g, RB, nr, lane, v, lds_u);
     __syncthreads();
     const int nr_cur = nr;
     const long long next = tile + gridDim.x;
     if (next < n_tiles) {
       nr = rays_in(next);
-      tile_load<ETS, ETF>(g, ut, ui, us, uf, next * RB, nr, lane, v);
+      tile_load<ETS, ETF, IDX>(g, ut, ui, us, uf, next * RB, nr, lane, v);
     }
     // LPR lanes per ray, lane j of a group accumulating channel j
     const int j = lane % LPR;
     for (int r = lane / LPR; r - lane / LPR < nr_cur; r += kTileThreads / LPR) {
       if (r >= nr_cur) continue;
       float acc[1], d, ws, alphas;
-      forward_walk<1>(lt + r * g.TS, li + r * g.TS, ls + r * g.TS, lf + r * g.FS, S, F,
+      forward_walk<1, IDX>(lt + r * g.TS, li + r * g.TS, ls + r * g.TS, lf + r * g.FS, S, F,
                       min(j, F - 1), 1, t_thres, occ_mode, acc, d, ws, alphas);
       const long long ray = tile * RB + r;
       if (j < F) out_feats[ray * F + j] = acc[0];
@@ -714,10 +718,10 @@ train_backward_stream(const float* __restrict__ g_feats, const float* __restrict
     pr[21] = o_alpha[ray];
   };
   int nr = rays_in(tile);
-  tile_load<ETS, ETF>(g, ut, ui, us, uf, tile * RB, nr, lane, v);
+  tile_load<ETS, ETF, true>(g, ut, ui, us, uf, tile * RB, nr, lane, v);
   load_ray(tile, nr);
   for (;;) {
-    tile_store<ETS, ETF>(g, RB, nr, lane, v, lds_u);
+    tile_store<ETS, ETF, true>(g, RB, nr, lane, v, lds_u);
     if (lane < nr) {
       float* p = lpr + lane * PR;
 #pragma unroll
@@ -735,7 +739,7 @@ train_backward_stream(const float* __restrict__ g_feats, const float* __restrict
     const long long next = tile + gridDim.x;
     if (next < n_tiles) {
       nr = rays_in(next);
-      tile_load<ETS, ETF>(g, ut, ui, us, uf, next * RB, nr, lane, v);
+      tile_load<ETS, ETF, true>(g, ut, ui, us, uf, next * RB, nr, lane, v);
       load_ray(next, nr);
     }
     // LPR lanes per ray, lane j of a group carrying channel j
@@ -822,19 +826,20 @@ int stream_grid(K kern, size_t lds, long long n_tiles, int* grid) {
   return SAILOR_OK;
 }
 
-template <int RB, int ETS, int ETF>
+// IDX false: samples already in order, no sort index (inference compositing).
+template <int RB, int ETS, int ETF, bool IDX>
 int launch_forward_stream(const float* t, const int32_t* sort_idx, const float* sigma,
                           const float* feats, long long n, int S, int F, float t_thresh,
                           int support_occ, float* out_feats, float* out_depth, float* out_ws,
-                          float* out_alpha, hipStream_t st) {
-  auto kern = F <= 4 ? train_forward_stream<RB, 4, ETS, ETF>
-                     : train_forward_stream<RB, 8, ETS, ETF>;
+                          float* out_alpha, hipStream_t st, const char* what) {
+  auto kern = F <= 4 ? train_forward_stream<RB, 4, ETS, ETF, IDX>
+                     : train_forward_stream<RB, 8, ETS, ETF, IDX>;
   const size_t lds = train_forward_tile_lds(RB, S, F);
   int grid = 0;
   if (int e = stream_grid(kern, lds, (n + RB - 1) / RB, &grid)) return e;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kTileThreads), lds, st, t, sort_idx, sigma, feats, n,
                      S, F, t_thresh, support_occ, out_feats, out_depth, out_ws, out_alpha);
-  return sailor::check_launch("volume_rendering_training_forward");
+  return sailor::check_launch(what);
 }
 
 template <int RB, int ETS, int ETF>
@@ -866,18 +871,25 @@ extern "C" int sailor_volume_rendering_occ_forward(const float* t, const float* 
   const long long n = (long long)B * R;
   if (n == 0) return SAILOR_OK;
   hipStream_t st = sailor::as_stream(stream);
+  // rows that fit a tile: the streamed walk without a sort index
+  const int sh = pick_stream(train_forward_tile_lds, S, F);
+  if (sh >= 0) {
+    auto launch = [&](auto rb, auto ets, auto etf) {
+      return launch_forward_stream<decltype(rb)::value, decltype(ets)::value,
+                                   decltype(etf)::value, false>(
+          t, nullptr, occ, feats, n, S, F, t_thresh, 1, out_feats, out_depth, out_ws, out_alpha,
+          st, "volume_rendering_occ_forward");
+    };
+    return dispatch_stream(sh, launch);
+  }
   if (F <= 8) {
-    // 8-sample chunks (measured: 0.21 ms vs 0.29 ms with 16-sample chunks at
-    // 262,144 rays x 48 samples x 3 channels; equal at 8 channels)
-    const int cs = 8;
+    // longer rows: chunks of 8 samples staged in LDS (0.21 ms vs 0.29 ms with
+    // 16-sample chunks at 262,144 rays x 48 samples x 3 channels)
+    constexpr int cs = 8;
     const size_t lds = (size_t)(kThreads / 64) * 64 * (2 * (cs + 1) + cs * F + 1) * sizeof(float);
     const dim3 grid((unsigned)sailor::ceil_div(n, kThreads));
-    if (cs == 16)
-      hipLaunchKernelGGL(occ_forward_tiled<16>, grid, dim3(kThreads), lds, st, t, occ, feats, n,
-                         S, F, t_thresh, out_feats, out_depth, out_ws, out_alpha);
-    else
-      hipLaunchKernelGGL(occ_forward_tiled<8>, grid, dim3(kThreads), lds, st, t, occ, feats, n,
-                         S, F, t_thresh, out_feats, out_depth, out_ws, out_alpha);
+    hipLaunchKernelGGL(occ_forward_tiled<cs>, grid, dim3(kThreads), lds, st, t, occ, feats, n, S,
+                       F, t_thresh, out_feats, out_depth, out_ws, out_alpha);
     return sailor::check_launch("volume_rendering_occ_forward");
   }
   for (int f0 = 0; f0 < F; f0 += 8) {
@@ -900,9 +912,9 @@ extern "C" int sailor_volume_rendering_training_forward(
   if (sh >= 0) {
     auto launch = [&](auto rb, auto ets, auto etf) {
       return launch_forward_stream<decltype(rb)::value, decltype(ets)::value,
-                                   decltype(etf)::value>(t, sort_idx, sigma, feats, n, S, F,
-                                                         t_thresh, support_occ, out_feats,
-                                                         out_depth, out_ws, out_alpha, st);
+                                   decltype(etf)::value, true>(
+          t, sort_idx, sigma, feats, n, S, F, t_thresh, support_occ, out_feats, out_depth,
+          out_ws, out_alpha, st, "volume_rendering_training_forward");
     };
     return dispatch_stream(sh, launch);
   }

@@ -23,10 +23,15 @@ def _rand_rays(B=2, R=513, S=48, F=3, seed=0, invalid_frac=0.1):
     return t, occ, feats
 
 
-@pytest.mark.parametrize("F,S", [(3, 48), (8, 48), (33, 48), (3, 37), (5, 21), (1, 7)])
-def test_occ_forward_bitexact(F, S):
+# streamed LDS tiles (S <= 48 at 16 rays, F = 8 at 4 rays with 8 lanes per
+# ray, F = 5 in two passes of 8-lane groups), the chunked LDS form for rows
+# longer than a tile (S = 200), the per-lane form for F > 8, and a ray count
+# that gives each persistent workgroup many tiles
+@pytest.mark.parametrize("F,S,R", [(3, 48, 513), (8, 48, 513), (33, 48, 513), (3, 37, 513),
+                                   (5, 21, 513), (1, 7, 513), (3, 200, 513), (3, 48, 70001)])
+def test_occ_forward_bitexact(F, S, R):
     from sailor_amd import VoxelEncoding as VE
-    t, occ, feats = _rand_rays(F=F, S=S)
+    t, occ, feats = _rand_rays(R=R, F=F, S=S)
     of, od, ow, oa = VE.volume_rendering_occ_forward(
         torch.from_numpy(t).cuda(), torch.from_numpy(occ).cuda(),
         torch.from_numpy(feats).cuda(), 0, 1e-4)
