@@ -5,10 +5,11 @@
 // voxel_traversal (intersection_kernel.cu:827-895) ->
 // ray_voxels_points_sampling_coarse (sampling_kernel.cu:1070-1177).  The
 // sampler needs the total weighted valid length of a ray before it can place
-// the first sample, so instead of storing every hit the walk is done twice:
-// pass 1 accumulates the valid length, pass 2 re-walks and emits samples
-// (stopping once all S slots are used), and a third, usually very short walk
-// places the leftover midpoint samples.  Every expression is the one of the
+// the first sample, so instead of storing every hit in HBM the walk is done
+// in passes: pass 1 accumulates the valid length, pass 2 emits samples
+// (stopping once all S slots are used), and a third pass places the leftover
+// midpoint samples.  Pass 1 keeps up to kHitCache hits per ray in LDS, and
+// rays within that replay them for passes 2 and 3; longer rays walk again.  Every expression is the one of the
 // separate kernels, so the samples are bit-identical to the unfused path.
 // Compiled with -ffp-contract=off.
 #include "common.h"
@@ -35,6 +36,33 @@ struct SampleSink {
   }
   __device__ __forceinline__ bool stop() const { return has >= ss.S; }
 };
+
+// Hits of walk 1 kept in LDS (up to kHitCache per ray, one column per
+// lane): rays with at most that many hits replay them to the sample and
+// leftover sinks instead of walking the DDA again.  A sink receives the same
+// (pos, node, t0, t1) sequence either way; stop() only cuts off puts that
+// would change nothing, so the samples are identical.
+constexpr int kHitCache = 32;
+
+struct CacheSink {
+  int* node;
+  float* t0;
+  float* t1;
+  __device__ __forceinline__ void put(int pos, int nd, float a, float b) {
+    if (pos < kHitCache) {
+      node[pos * kThreads] = nd;
+      t0[pos * kThreads] = a;
+      t1[pos * kThreads] = b;
+    }
+  }
+  __device__ __forceinline__ bool stop() const { return false; }
+};
+
+template <class Sink>
+__device__ __forceinline__ void replay(const CacheSink& c, int count, Sink& sink) {
+  for (int k = 0; k < count && !sink.stop(); ++k)
+    sink.put(k, c.node[k * kThreads], c.t0[k * kThreads], c.t1[k * kThreads]);
+}
 
 struct LeftoverSink {
   const float* nz;
@@ -75,6 +103,8 @@ trace_kernel(const float* __restrict__ K, const float* __restrict__ Rm,
              const float* __restrict__ noise, float* __restrict__ rays_out,
              int* __restrict__ sidx, float* __restrict__ st, float* __restrict__ vl_out) {
   __shared__ float s_m[18];
+  __shared__ int s_node[kHitCache * kThreads];
+  __shared__ float s_t0[kHitCache * kThreads], s_t1[kHitCache * kThreads];
   const int b = blockIdx.y;
   if (threadIdx.x == 0) {
     inv3(K + (size_t)b * 9, s_m);
@@ -112,9 +142,10 @@ trace_kernel(const float* __restrict__ K, const float* __restrict__ Rm,
   sailor::RayCtx ctx;
   ctx.init(vc, node_pos, node_info, n_nodes, vol_bbox, voxel_size, b, dim_fine, dim_coarse,
            rate, o, d);
-  NullSink ns;
+  CacheSink cs{s_node + threadIdx.x, s_t0 + threadIdx.x, s_t1 + threadIdx.x};
   float vl = 0.0f;
-  const int count = sailor::traverse(ctx, -1, false, -1, ns, &vl);
+  const int count = sailor::traverse(ctx, -1, false, -1, cs, &vl);
+  const bool cached = count <= kHitCache;
   if (vl_out) vl_out[ray] = vl;
   int* oi = sidx + ray * S;
   float* ot = st + ray * S;
@@ -126,12 +157,18 @@ trace_kernel(const float* __restrict__ K, const float* __restrict__ Rm,
     ss.ss.init(vl, voxel_size[b] * (float)start_scale, S);
     ss.nz = nz; ss.oi = oi; ss.ot = ot; ss.np = node_pos; ss.has = 0;
     float dummy;
-    sailor::traverse(ctx, -1, false, -1, ss, &dummy);
+    if (cached)
+      replay(cs, count, ss);
+    else
+      sailor::traverse(ctx, -1, false, -1, ss, &dummy);
     has = ss.has;
     // ---- walk 3: leftover midpoints (:1156-1174)
     if (has < S) {
       LeftoverSink ls{nz, oi, ot, has, S - has, S};
-      sailor::traverse(ctx, -1, false, -1, ls, &dummy);
+      if (cached)
+        replay(cs, count, ls);
+      else
+        sailor::traverse(ctx, -1, false, -1, ls, &dummy);
       has = ls.has;
     }
   }
