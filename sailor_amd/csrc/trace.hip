@@ -8,8 +8,9 @@
 // the first sample, so instead of storing every hit in HBM the walk is done
 // in passes: pass 1 accumulates the valid length, pass 2 emits samples
 // (stopping once all S slots are used), and a third pass places the leftover
-// midpoint samples.  Pass 1 keeps up to kHitCache hits per ray in LDS, and
-// rays within that replay them for passes 2 and 3; longer rays walk again.  Every expression is the one of the
+// midpoint samples.  Pass 1 keeps up to kHitCache hit node ids per ray in
+// LDS, and rays within that replay them for passes 2 and 3; longer rays walk
+// again.  Every expression is the one of the
 // separate kernels, so the samples are bit-identical to the unfused path.
 // Compiled with -ffp-contract=off.
 #include "common.h"
@@ -37,31 +38,32 @@ struct SampleSink {
   __device__ __forceinline__ bool stop() const { return has >= ss.S; }
 };
 
-// Hits of walk 1 kept in LDS (up to kHitCache per ray, one column per
-// lane): rays with at most that many hits replay them to the sample and
-// leftover sinks instead of walking the DDA again.  A sink receives the same
-// (pos, node, t0, t1) sequence either way; stop() only cuts off puts that
-// would change nothing, so the samples are identical.
-constexpr int kHitCache = 32;
+// Hits of walk 1 kept in LDS (up to kHitCache node ids per ray, one column
+// per lane): rays with at most that many hits replay them to the sample and
+// leftover sinks instead of walking the DDA again.  A hit's interval is the
+// slab test of the ray against that node, recomputed from the same inputs by
+// the same code as in the walk, so only the node id is kept.  A sink receives
+// the same (pos, node, t0, t1) sequence either way; stop() only cuts off puts
+// that would change nothing, so the samples are identical.
+constexpr int kHitCache = 44;
 
 struct CacheSink {
   int* node;
-  float* t0;
-  float* t1;
-  __device__ __forceinline__ void put(int pos, int nd, float a, float b) {
-    if (pos < kHitCache) {
-      node[pos * kThreads] = nd;
-      t0[pos * kThreads] = a;
-      t1[pos * kThreads] = b;
-    }
+  __device__ __forceinline__ void put(int pos, int nd, float, float) {
+    if (pos < kHitCache) node[pos * kThreads] = nd;
   }
   __device__ __forceinline__ bool stop() const { return false; }
 };
 
 template <class Sink>
-__device__ __forceinline__ void replay(const CacheSink& c, int count, Sink& sink) {
-  for (int k = 0; k < count && !sink.stop(); ++k)
-    sink.put(k, c.node[k * kThreads], c.t0[k * kThreads], c.t1[k * kThreads]);
+__device__ __forceinline__ void replay(const sailor::RayCtx& ctx, const CacheSink& c, int count,
+                                       Sink& sink) {
+  for (int k = 0; k < count && !sink.stop(); ++k) {
+    const int nd = c.node[k * kThreads];
+    const float4 q = ctx.np[nd];
+    const float2 iv = sailor::slab_test(ctx.o, ctx.inv, make_float3(q.x, q.y, q.z), q.w);
+    sink.put(k, nd, iv.x, iv.y);
+  }
 }
 
 struct LeftoverSink {
@@ -104,7 +106,6 @@ trace_kernel(const float* __restrict__ K, const float* __restrict__ Rm,
              int* __restrict__ sidx, float* __restrict__ st, float* __restrict__ vl_out) {
   __shared__ float s_m[18];
   __shared__ int s_node[kHitCache * kThreads];
-  __shared__ float s_t0[kHitCache * kThreads], s_t1[kHitCache * kThreads];
   const int b = blockIdx.y;
   if (threadIdx.x == 0) {
     inv3(K + (size_t)b * 9, s_m);
@@ -142,7 +143,7 @@ trace_kernel(const float* __restrict__ K, const float* __restrict__ Rm,
   sailor::RayCtx ctx;
   ctx.init(vc, node_pos, node_info, n_nodes, vol_bbox, voxel_size, b, dim_fine, dim_coarse,
            rate, o, d);
-  CacheSink cs{s_node + threadIdx.x, s_t0 + threadIdx.x, s_t1 + threadIdx.x};
+  CacheSink cs{s_node + threadIdx.x};
   float vl = 0.0f;
   const int count = sailor::traverse(ctx, -1, false, -1, cs, &vl);
   const bool cached = count <= kHitCache;
@@ -158,7 +159,7 @@ trace_kernel(const float* __restrict__ K, const float* __restrict__ Rm,
     ss.nz = nz; ss.oi = oi; ss.ot = ot; ss.np = node_pos; ss.has = 0;
     float dummy;
     if (cached)
-      replay(cs, count, ss);
+      replay(ctx, cs, count, ss);
     else
       sailor::traverse(ctx, -1, false, -1, ss, &dummy);
     has = ss.has;
@@ -166,7 +167,7 @@ trace_kernel(const float* __restrict__ K, const float* __restrict__ Rm,
     if (has < S) {
       LeftoverSink ls{nz, oi, ot, has, S - has, S};
       if (cached)
-        replay(cs, count, ls);
+        replay(ctx, cs, count, ls);
       else
         sailor::traverse(ctx, -1, false, -1, ls, &dummy);
       has = ls.has;

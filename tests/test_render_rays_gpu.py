@@ -54,10 +54,10 @@ def test_fused_trace_matches_oracle_chain():
                   cfg.target)[0, 0].reshape(1, -1, 6)
     np.testing.assert_array_equal(p.rays.cpu().numpy(), r6)
     trv = orc.traverse(vc, pos, nfo, bbox, vs, cfg.dim, cfg.rate, r6[..., :3], r6[..., 3:], 1024)
-    # the trace kernel replays up to 32 hits per ray from LDS and walks the
+    # the trace kernel replays up to 44 hits per ray from LDS and walks the
     # DDA again for longer rays: the scene must exercise both
     hits = (trv["idx"] >= 0).sum(-1)
-    assert ((hits > 0) & (hits <= 32)).any() and (hits > 32).any(), np.bincount(hits.ravel())
+    assert ((hits > 0) & (hits <= 44)).any() and (hits > 44).any(), np.bincount(hits.ravel())
     noise = np.full((1, r6.shape[1], cfg.S), 0.5, np.float32)
     sidx, st = orc.sampling(trv["idx"], trv["tmin"], trv["tmax"], noise, trv["valid_len"], pos,
                             vs, cfg.S, cfg.rate, trv["gmax"])
