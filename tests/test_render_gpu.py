@@ -25,10 +25,12 @@ def _rand_rays(B=2, R=513, S=48, F=3, seed=0, invalid_frac=0.1):
 
 # streamed LDS tiles (S <= 48 at 16 rays, F = 8 at 4 rays with 8 lanes per
 # ray, F = 5 in two passes of 8-lane groups), the chunked LDS form for rows
-# longer than a tile (S = 200), the per-lane form for F > 8, and a ray count
-# that gives each persistent workgroup many tiles
+# longer than a tile (S = 200), the per-lane form for F > 8, a ray count
+# that gives each persistent workgroup many tiles, and tiles holding only
+# two rays (B = 2, R = 1)
 @pytest.mark.parametrize("F,S,R", [(3, 48, 513), (8, 48, 513), (33, 48, 513), (3, 37, 513),
-                                   (5, 21, 513), (1, 7, 513), (3, 200, 513), (3, 48, 70001)])
+                                   (5, 21, 513), (1, 7, 513), (3, 200, 513), (3, 48, 70001),
+                                   (3, 48, 1), (8, 48, 1)])
 def test_occ_forward_bitexact(F, S, R):
     from sailor_amd import VoxelEncoding as VE
     t, occ, feats = _rand_rays(R=R, F=F, S=S)
@@ -46,10 +48,11 @@ def test_occ_forward_bitexact(F, S, R):
 # (S <= 48, S*F <= 144), 8 rays (S = 52) and 4 rays (F = 8), the per-lane global
 # form for rows too long for a tile (S = 200) and for more than 8 channels
 # (F = 12).  602 rays leave a partial last tile for every tile size; the
-# R = 70001 case gives each persistent workgroup many tiles.
+# R = 70001 case gives each persistent workgroup many tiles; R = 1 leaves a
+# single tile of two rays.
 @pytest.mark.parametrize("F,S,R", [(3, 32, 301), (3, 48, 301), (3, 52, 301), (8, 48, 301),
                                    (5, 21, 301), (1, 7, 301), (3, 200, 301), (12, 48, 301),
-                                   (3, 48, 70001)])
+                                   (3, 48, 70001), (3, 48, 1), (5, 21, 1)])
 @pytest.mark.parametrize("occ_mode", [True, False])
 def test_training_forward_backward(occ_mode, F, S, R):
     from sailor_amd import VoxelEncoding as VE
