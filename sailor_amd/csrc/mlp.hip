@@ -1716,16 +1716,29 @@ render_backward_kernel(BackArgs ba) {
 // partials of all workgroups.
 constexpr int kLdB = 68;  // 16-byte aligned rows of 64 tokens
 
+// Row blocks are shared by values that are never live together, so that two
+// workgroups fit in a CU's LDS (F = 32: 288 rows of 272 B):
+//  * X0 / X2 / DHID: the input tokens (stack 0's input, dead once stack 0
+//    has run), then stack 1's output (the final tokens, dead after the RGB
+//    head's backward), then dL/d hidden and d attention while the stacks are
+//    walked back;
+//  * X1: stack 1's input, then the input tokens again (reloaded) for stack
+//    0's backward;
+//  * HID also holds the RGB head's dz3 [3] (the hidden rows are recomputed
+//    before the stacks need them).
 template <int F>
 struct HydraBwdRows {
-  static constexpr int X0 = 0, X1 = F, X2 = 2 * F;    // stack inputs, final tokens
-  static constexpr int QKV = 3 * F;      // q, k, v then dq, dk, dv [96]; RGB r1 [64] + r2 [32]
+  static_assert(F >= kHydraInner, "DHID shares the X0 rows");
+  static_assert(kHydraStacks == 2, "X0 / X1 reuse assumes two stacks");
+  static constexpr int X0 = 0, X1 = F, X2 = 0;
+  static constexpr int QKV = 2 * F;      // q, k, v then dq, dk, dv [96]; RGB r1 [64] + r2 [32]
   static constexpr int OUT = QKV + 3 * kHydraInner;   // attention output [F]
   static constexpr int HID = OUT + F;    // FFN hidden, then attention values [32]
+  static constexpr int DZ3 = HID;        // dL/dz of the RGB output [3]
   static constexpr int DX = HID + kHydraInner;        // dL/d stack output, then input [F]
   static constexpr int DOUT = DX + F;                 // [F]
-  static constexpr int DHID = DOUT + F;               // dL/d hidden, then d attention [32]
-  static constexpr int ROWS = DHID + kHydraInner;
+  static constexpr int DHID = X2;                     // dL/d hidden, then d attention [32]
+  static constexpr int ROWS = DOUT + F;
   static constexpr size_t bytes = (size_t)ROWS * kLdB * sizeof(float);
 };
 
@@ -1862,6 +1875,7 @@ __device__ __forceinline__ void hydra_stack_fwd(const float* ws, int st, float* 
 
 template <int F, int H>
 __global__ void __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(2)))
 hydra_rgb_backward_kernel(const float* __restrict__ tok, const int* __restrict__ list,
                           const unsigned* __restrict__ n_valid_ptr, const float* __restrict__ ws,
                           int V, const float* __restrict__ grad_rgb, float* __restrict__ dtok,
@@ -1916,15 +1930,15 @@ hydra_rgb_backward_kernel(const float* __restrict__ tok, const int* __restrict__
         for (int k = 0; k < kRgbH2; ++k) z += ws[so.r3w + j * kRgbH2 + k] * r2[k];
         const float rgb = sigmoidf(z);
         const float gr = on ? grad_rgb[(long long)list[item] * 3 + j] : 0.0f;
-        sm[(R::DHID + j) * kLdB + lane] = gr * rgb * (1.0f - rgb);
+        sm[(R::DZ3 + j) * kLdB + lane] = gr * rgb * (1.0f - rgb);
       }
     }
     __syncthreads();
-    accum_outer<3, kRgbH2>(sm + R::DHID * kLdB, sm + (R::QKV + kRgbH1) * kLdB, g + so.r3w,
+    accum_outer<3, kRgbH2>(sm + R::DZ3 * kLdB, sm + (R::QKV + kRgbH1) * kLdB, g + so.r3w,
                            g + so.r3b, tid);
     {  // dr2 = (Wr3^T dz3) . [r2 > 0]  -> DOUT rows [0, 32)
       float dz[3];
-      load_rows<3, kLdB>(sm + R::DHID * kLdB, lane, dz);
+      load_rows<3, kLdB>(sm + R::DZ3 * kLdB, lane, dz);
       constexpr int NK = kRgbH2 / kWaves;
       token_dense_t<3, kRgbH2, NK>(ws + so.r3w, dz, wave * NK, sm + R::DOUT * kLdB, nullptr,
                                    sm + (R::QKV + kRgbH1) * kLdB, lane);
@@ -1952,7 +1966,16 @@ hydra_rgb_backward_kernel(const float* __restrict__ tok, const int* __restrict__
     __syncthreads();
     // ---- stacks, last to first ----------------------------------------------
     for (int st = kHydraStacks - 1; st >= 0; --st) {
-      const int xin = st == 0 ? R::X0 : R::X1;
+      const int xin = R::X1;
+      if (st == 0) {
+        // stack 0's input: the tokens again, into the rows that held stack
+        // 1's input (its backward has finished with them)
+        for (int i = tid; i < kTok * F; i += kThreads) {
+          const int tk = i / F, c = i % F;
+          sm[(R::X1 + c) * kLdB + tk] = i < n_floats ? tok[item0 * kVP * F + i] : 0.0f;
+        }
+        __syncthreads();
+      }
       hydra_stack_fwd<F, H>(ws, st, sm, xin, -1, lane, wave, V);
       const float* w1 = ws + so.w1[0] + st * stride;
       const float* w2 = ws + so.w2[0] + st * stride;
