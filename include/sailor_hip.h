@@ -114,6 +114,24 @@ int sailor_octree_build(int32_t* vol_fine, int32_t* vol_coarse,
                         int32_t* queue_counts, void* workspace,
                         size_t workspace_bytes, void* stream);
 
+/* No counterpart in the reference.  sailor_octree_build for callers that
+ * cannot allocate device memory (code under hipGraph capture, serving loops,
+ * harnesses that forbid hipMalloc): the same launches and results, with the
+ * workspace taken from an 8 MiB array built into the library (one per GPU,
+ * held until the process ends) instead of a caller buffer.  Sizes whose
+ * workspace (sailor_octree_workspace_bytes(B, dim_coarse)) exceeds 8 MiB --
+ * about 524,000 coarse cells over the batch -- are rejected with
+ * SAILOR_ERR_ARG.  Nothing here calls hipMalloc or hipFree.  All calls on one
+ * GPU share the array: calls that overlap on one GPU (different streams or
+ * threads) race.  The device current at the call must be the stream's. */
+int sailor_octree_build_noalloc(int32_t* vol_fine, int32_t* vol_coarse,
+                                const int32_t* fine_counts,
+                                const int32_t* coarse_counts, int B, int dim_fine,
+                                int dim_coarse, const float* voxel_size,
+                                const float* vol_bbox, int coarse_start, int capacity,
+                                float* node_pos, int32_t* node_info,
+                                int32_t* queue_counts, void* stream);
+
 /* Replaces MultiLayerOctree::output_all_nodes_tensor (octree_kernel.cu:699-717):
  *  out f32 [n,6] = (x, y, z, voxel size, valid, batch).                     */
 int sailor_octree_nodes_tensor(const float* node_pos, const int32_t* node_info,

@@ -28,6 +28,7 @@ _SIGS = {
     "sailor_build_multi_res_volumes": [P, P, I, I, I, I, P],
     "sailor_octree_workspace_bytes": [I, I],
     "sailor_octree_build": [P, P, P, P, I, I, I, P, P, I, I, P, P, P, P, SZ, P],
+    "sailor_octree_build_noalloc": [P, P, P, P, I, I, I, P, P, I, I, P, P, P, P],
     "sailor_octree_nodes_tensor": [P, P, I, P, P],
     "sailor_octree_voxel_traversal": [P, P, P, I, P, P] + [I] * 4 + [P, P] + [I] * 5 + [P] * 5 + [P],
     "sailor_octree_sampling_coarse": [P] * 7 + [I] * 5 + [P, P, P],

@@ -16,6 +16,8 @@
 // Compiled with -ffp-contract=off (node centres feed bit-exact ray intervals).
 #include "common.h"
 
+#include <mutex>
+
 namespace {
 
 constexpr int kThreads = 256;
@@ -385,25 +387,26 @@ extern "C" size_t sailor_octree_workspace_bytes(int B, int dim_coarse) {
   return layout(nullptr, ncells).bytes;
 }
 
-extern "C" int sailor_octree_build(int32_t* vol_fine, int32_t* vol_coarse,
-                                   const int32_t* fine_counts,
-                                   const int32_t* coarse_counts, int B,
-                                   int dim_fine, int dim_coarse,
-                                   const float* voxel_size, const float* vol_bbox,
-                                   int coarse_start, int capacity,
-                                   float* node_pos, int32_t* node_info,
-                                   int32_t* queue_counts, void* workspace,
-                                   size_t workspace_bytes, void* stream) {
-  SAILOR_REQUIRE(B > 0 && dim_coarse > 0 && dim_fine % dim_coarse == 0,
-                 "octree_build: fine dim %d is not a multiple of coarse dim %d",
-                 dim_fine, dim_coarse);
+// Workspace of sailor_octree_build_noalloc: one copy per GPU, held for the
+// life of the process.  In host code `scratch` is only a symbol; its device
+// address (per device) comes from hipGetSymbolAddress.
+constexpr size_t kNoAllocScratchBytes = size_t(8) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+// The build's six launches on an already laid-out workspace.  Every
+// workspace word a launch reads was written by an earlier launch of the same
+// call (flags and child counts for every cell, block sums, meta; cell
+// indices of the allocated cells only, read only for those), so the
+// workspace needs no clearing between calls.
+int octree_build_launch(int32_t* vol_fine, int32_t* vol_coarse, const int32_t* fine_counts,
+                        const int32_t* coarse_counts, int B, int dim_fine, int dim_coarse,
+                        const float* voxel_size, const float* vol_bbox, int coarse_start,
+                        int capacity, float* node_pos, int32_t* node_info,
+                        int32_t* queue_counts, const TreeWorkspace& w, long long n,
+                        hipStream_t st) {
   const int rate = dim_fine / dim_coarse;
-  SAILOR_REQUIRE(rate >= 2, "octree_build: rate must be >= 2");
-  const long long n = (long long)B * dim_coarse * dim_coarse * dim_coarse;
-  TreeWorkspace w = layout(workspace, n);
-  SAILOR_REQUIRE(workspace_bytes >= w.bytes, "octree_build: workspace too small (%zu < %zu)",
-                 workspace_bytes, w.bytes);
-  hipStream_t st = sailor::as_stream(stream);
   const int nblk = (int)((n + kScanBlock - 1) / kScanBlock);
   float4* pos = reinterpret_cast<float4*>(node_pos);
   int4* info = reinterpret_cast<int4*>(node_info);
@@ -425,6 +428,85 @@ extern "C" int sailor_octree_build(int32_t* vol_fine, int32_t* vol_coarse,
   hipLaunchKernelGGL(tree_fill_invalid, dim3(fill_blocks), dim3(kThreads), 0, st,
                      w.meta, pos, info);
   return sailor::check_launch("octree_build");
+}
+
+// Shared argument checks of the two build entry points; sets *n to the
+// number of coarse cells over the batch.
+int octree_build_args(int B, int dim_fine, int dim_coarse, long long* n) {
+  SAILOR_REQUIRE(B > 0 && dim_coarse > 0 && dim_fine % dim_coarse == 0,
+                 "octree_build: fine dim %d is not a multiple of coarse dim %d",
+                 dim_fine, dim_coarse);
+  SAILOR_REQUIRE(dim_fine / dim_coarse >= 2, "octree_build: rate must be >= 2");
+  *n = (long long)B * dim_coarse * dim_coarse * dim_coarse;
+  return SAILOR_OK;
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up on the first
+// call made with that device current.
+int scratch_address(void** out) {
+  int dev = -1;
+  SAILOR_HIP_TRY(hipGetDevice(&dev));
+  SAILOR_REQUIRE(dev >= 0 && dev < kMaxDevices, "octree_build_noalloc: device %d out of range",
+                 dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    SAILOR_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SAILOR_REQUIRE(p != nullptr, "octree_build_noalloc: no device address for the scratch");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return SAILOR_OK;
+}
+
+}  // namespace
+
+extern "C" int sailor_octree_build(int32_t* vol_fine, int32_t* vol_coarse,
+                                   const int32_t* fine_counts,
+                                   const int32_t* coarse_counts, int B,
+                                   int dim_fine, int dim_coarse,
+                                   const float* voxel_size, const float* vol_bbox,
+                                   int coarse_start, int capacity,
+                                   float* node_pos, int32_t* node_info,
+                                   int32_t* queue_counts, void* workspace,
+                                   size_t workspace_bytes, void* stream) {
+  long long n = 0;
+  int rc = octree_build_args(B, dim_fine, dim_coarse, &n);
+  if (rc) return rc;
+  TreeWorkspace w = layout(workspace, n);
+  SAILOR_REQUIRE(workspace_bytes >= w.bytes, "octree_build: workspace too small (%zu < %zu)",
+                 workspace_bytes, w.bytes);
+  return octree_build_launch(vol_fine, vol_coarse, fine_counts, coarse_counts, B, dim_fine,
+                             dim_coarse, voxel_size, vol_bbox, coarse_start, capacity, node_pos,
+                             node_info, queue_counts, w, n, sailor::as_stream(stream));
+}
+
+extern "C" int sailor_octree_build_noalloc(int32_t* vol_fine, int32_t* vol_coarse,
+                                           const int32_t* fine_counts,
+                                           const int32_t* coarse_counts, int B, int dim_fine,
+                                           int dim_coarse, const float* voxel_size,
+                                           const float* vol_bbox, int coarse_start,
+                                           int capacity, float* node_pos, int32_t* node_info,
+                                           int32_t* queue_counts, void* stream) {
+  long long n = 0;
+  int rc = octree_build_args(B, dim_fine, dim_coarse, &n);
+  if (rc) return rc;
+  const size_t need = layout(nullptr, n).bytes;
+  SAILOR_REQUIRE(need <= kNoAllocScratchBytes,
+                 "octree_build_noalloc: %lld coarse cells need %zu bytes of workspace, more "
+                 "than the %zu-byte built-in scratch (use sailor_octree_build)",
+                 n, need, kNoAllocScratchBytes);
+  void* base = nullptr;
+  rc = scratch_address(&base);
+  if (rc) return rc;
+  TreeWorkspace w = layout(base, n);
+  return octree_build_launch(vol_fine, vol_coarse, fine_counts, coarse_counts, B, dim_fine,
+                             dim_coarse, voxel_size, vol_bbox, coarse_start, capacity, node_pos,
+                             node_info, queue_counts, w, n, sailor::as_stream(stream));
 }
 
 extern "C" int sailor_octree_nodes_tensor(const float* node_pos,
