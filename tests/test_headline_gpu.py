@@ -1,0 +1,256 @@
+"""GPU parity at the benchmark's own shapes, through the C ABI.
+
+* Full-size plain-path geometry (4 x 1024^2 RGBD views, 256^3 grid, 8^3
+  blocks, 512^2 target, 48 samples): TSDF / weight / colour volumes, merge,
+  tree, fused rays + DDA + sampling, per-ray sort -- bit for bit against the
+  C oracle on the whole frame.
+* The hidden-256 model (F = 64 features, Hydra over 4 views) on that frame
+  at 48 samples per ray: fp32 (24-sample chunks), bf16 and fp16 forms on a
+  seeded subset of rays against the float64 restatement oracle/nerf_ref.py.
+* 5 and 6 source views with the hidden-256 model (8 view slots per sample).
+* The differentiation form's forward (hidden-128 model, 48 samples, fp32
+  24-sample chunks) over a batch of 8 target views.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import nerf_ref
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+_pipes = {}
+
+
+def frame(name, prec="fp32", cfg=None):
+    """One frame of FramePipeline (run once, cached per (config, precision))."""
+    from sailor_amd.pipeline import CONFIGS, FramePipeline
+    cfg = cfg or CONFIGS[name]
+    key = (cfg.name, prec)
+    if key not in _pipes:
+        print(f"[frame] {cfg.name} {prec}: {cfg.n_views} x {cfg.img}^2 views, {cfg.dim}^3, "
+              f"{cfg.target}^2 target, S={cfg.S}, model {cfg.model}", flush=True)
+        p = FramePipeline.synthetic(cfg, prec=prec)
+        p.run()
+        torch.cuda.synchronize()
+        _pipes[key] = p
+    return _pipes[key]
+
+
+# ---------------------------------------------------------------------------
+# geometry at full size
+_geo = {}
+
+
+def _oracle_frame(p):
+    """C-oracle chain on the frame's own inputs (same fixed 2.2 m cube)."""
+    if p.cfg.name in _geo:
+        return _geo[p.cfg.name]
+    cfg, sc = p.cfg, p.scene
+    K = sc.K()[None]
+    RT = sc.RT()[None]
+    from sailor_amd import scenes
+    o = orc.integrate(K, RT, np.ascontiguousarray(RT[..., :3]), np.ascontiguousarray(RT[..., 3]),
+                      scenes.pack_colors(sc.rgb)[None], sc.depth[None], (cfg.dim,) * 3,
+                      th_low=cfg.th_low, th_high=cfg.th_high, trunc_weight=cfg.trunc_weight,
+                      fixed_bbox=scenes.fixed_cube(cfg.cube)[None])
+    oc, ocnt = orc.merge(o["occ"], cfg.rate)
+    nf, nc = int(o["num_occ"].sum()), int(ocnt.sum())
+    tr = orc.tree_build(o["occ"], oc, o["voxel_size"], o["vol_bbox"], nf, nf + nc)
+    cam = p.target_cam
+    r6 = orc.rays(cam.K[None, None], cam.R[None, None], cam.T[None, None], cfg.target,
+                  cfg.target)[0, 0].reshape(1, -1, 6)
+    trv = orc.traverse(tr["vol_coarse"], tr["node_pos"], tr["node_info"], o["vol_bbox"],
+                       o["voxel_size"], cfg.dim, cfg.rate, r6[..., :3], r6[..., 3:], 128)
+    noise = np.full((1, r6.shape[1], cfg.S), 0.5, np.float32)
+    sidx, st = orc.sampling(trv["idx"], trv["tmin"], trv["tmax"], noise, trv["valid_len"],
+                            tr["node_pos"], o["voxel_size"], cfg.S, cfg.rate, trv["gmax"])
+    srt, order = orc.sort_samplings(st)
+    res = dict(o=o, coarse=oc, coarse_cnt=ocnt, tree=tr, rays=r6, trv=trv, sidx=sidx, st=st,
+               sorted=srt, order=order)
+    _geo[p.cfg.name] = res
+    return res
+
+
+def test_c2_frame_geometry_bitexact():
+    p = frame("c2")
+    g = _oracle_frame(p)
+    o, tr = g["o"], g["tree"]
+    assert int(o["num_occ"].sum()) > 10000
+    np.testing.assert_array_equal(p.bbox.cpu().numpy(), o["vol_bbox"])
+    np.testing.assert_array_equal(p.vs.cpu().numpy().reshape(-1), o["voxel_size"])
+    for name, t in (("tsdf", p.tsdf), ("weight", p.weight), ("color", p.color)):
+        np.testing.assert_array_equal(t.cpu().numpy(), o[name], err_msg=name)
+    np.testing.assert_array_equal(p.num.cpu().numpy().reshape(-1),
+                                  np.concatenate([o["num_occ"], g["coarse_cnt"]]))
+    assert list(p.queue.cpu().numpy()) == list(tr["queue"])
+    tail = int(o["num_occ"].sum() + g["coarse_cnt"].sum())
+    np.testing.assert_array_equal(p.node_info[:tail].cpu().numpy(), tr["node_info"][:tail])
+    np.testing.assert_array_equal(p.node_pos[:tail].cpu().numpy(), tr["node_pos"][:tail])
+    np.testing.assert_array_equal(p.occ0.cpu().numpy(), tr["vol_fine"])
+    np.testing.assert_array_equal(p.occ1.cpu().numpy(), tr["vol_coarse"])
+
+
+def test_c2_frame_trace_sample_sort_bitexact():
+    p = frame("c2")
+    g = _oracle_frame(p)
+    trv = g["trv"]
+    assert trv["overflow"] == 0
+    hits = (trv["idx"] >= 0).sum(-1)
+    print(f"C2 rays with hits {int((hits > 0).sum())}, longest {int(hits.max())} hits",
+          flush=True)
+    np.testing.assert_array_equal(p.rays.cpu().numpy(), g["rays"])
+    np.testing.assert_array_equal(p.vl.cpu().numpy(), trv["valid_len"])
+    np.testing.assert_array_equal(p.sidx.cpu().numpy(), g["sidx"])
+    np.testing.assert_array_equal(p.st.cpu().numpy(), g["st"])
+    np.testing.assert_array_equal(p.sorted.cpu().numpy(), g["sorted"])
+    np.testing.assert_array_equal(p.order.cpu().numpy(), g["order"])
+
+
+# ---------------------------------------------------------------------------
+# the hidden-256 model at 48 samples
+def _subset(p, n, seed):
+    valid = torch.nonzero(p.sorted[0, :, 0] >= 0).flatten().cpu().numpy()
+    assert valid.size > 4 * n, valid.size
+    return np.sort(np.random.default_rng(seed).choice(valid, size=n, replace=False))
+
+
+def _ref(p, sub, fm_dtype=None, mm_dtype=None):
+    """float64 restatement on rays `sub`; with fm_dtype the feature maps are
+    rounded to it first (the 16-bit forms read 16-bit maps), with mm_dtype
+    the dense layers' inputs and weights are rounded to it."""
+    fm = torch.from_numpy(p.fmaps_np[None])
+    if fm_dtype is not None:
+        fm = fm.to(fm_dtype)
+    fm = fm.double()
+    out = nerf_ref.render_rays(p.weights, p.sorted[:, sub].cpu().numpy(),
+                               p.rays[:, sub].cpu().numpy(), p.K.cpu().numpy(),
+                               p.RT.cpu().numpy(), fm, p.bbox.cpu().numpy(),
+                               p.zdir.cpu().numpy(), (p.cfg.img, p.cfg.img), mm_dtype=mm_dtype)
+    return [x[0].numpy() for x in out]
+
+
+def _errs(got, ref):
+    d = np.abs(np.asarray(got, np.float64) - ref)
+    return float(d.max()), float(np.sqrt((d ** 2).mean()))
+
+
+def _fp32_checks(p, sub):
+    """fp32 form vs float64: occupancy, weight / alpha sums, depth, colour."""
+    from sailor_amd.render import render_rays
+    cfg = p.cfg
+    out = render_rays(p.sorted, p.rays, p.K, p.RT, (cfg.img, cfg.img), p.fmap, p.bbox, p.zdir,
+                      p.model, want_occ=True)
+    torch.cuda.synchronize()
+    # the pipeline's own frame (sailor_render_rays_stages) wrote the same image
+    assert torch.equal(out["rgb"], p.rgb)
+    rgb, dep, ws, asum, occ = _ref(p, sub)
+    rows = {"occ": _errs(out["occ"][0, sub].cpu().numpy(), occ),
+            "ws": _errs(out["aux"][0, sub, 0].cpu().numpy(), ws),
+            "alpha": _errs(out["aux"][0, sub, 1].cpu().numpy(), asum),
+            "depth": _errs(out["depth"][0, sub].cpu().numpy(), dep),
+            "rgb": _errs(out["rgb"][0, sub].cpu().numpy(), rgb)}
+    print(cfg.name, "fp32 (max, rms) vs float64:", rows, flush=True)
+    tol = {"occ": 2e-5, "ws": 1e-4, "alpha": 1e-4, "depth": 5e-4, "rgb": 1e-4}
+    bad = {k: v for k, v in rows.items() if v[0] > tol[k]}
+    assert not bad, bad
+    return rgb
+
+
+def test_c2_model_fp32_24_sample_chunks_match_float64():
+    """render_kernel<F=64, H=256, fp32, 24 samples> (the plain form's default
+    chunk at S = 48) on 256 seeded valid rays of the full-size frame."""
+    p = frame("c2")
+    assert p.cfg.S % 24 == 0 and p.mc.F == 64 and p.mc.H == 256
+    _fp32_checks(p, _subset(p, 256, 11))
+
+
+def psnr(a, b):
+    """cal_psnr (utils_render/utils_render.py:25-35) on [0,1] colours of
+    the listed rays -> 255 scale."""
+    mse = ((np.asarray(a, np.float64) * 255 - np.asarray(b, np.float64) * 255) ** 2).mean()
+    return 10 * np.log10(255 * 255 / max(mse, 1e-20))
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_c2_model_16bit_forms_match_float64(prec):
+    """16-bit forms (16-bit feature maps, dense layers on MFMA with fp32
+    accumulation) on 256 seeded valid rays of the full-size frame.
+    Against the float64 restatement fed the same 16-bit maps and with the
+    dense layers' inputs and weights rounded to the same type: colour within
+    1e-3 (bf16) / 2e-4 (fp16), occupancy within 2e-3 / 5e-4 -- the kernel's
+    remaining differences are fp32 accumulation order and the rounding of
+    intermediate activations at 16-bit ulps.  Against the exact float64
+    render: PSNR > 45 dB."""
+    p = frame("c2", prec)
+    sub = _subset(p, 256, 12)
+    dt = torch.bfloat16 if prec == "bf16" else torch.float16
+    from sailor_amd.render import render_rays
+    cfg = p.cfg
+    out = render_rays(p.sorted, p.rays, p.K, p.RT, (cfg.img, cfg.img), p.fmap_lo, p.bbox,
+                      p.zdir, p.model, want_occ=True)
+    torch.cuda.synchronize()
+    assert torch.equal(out["rgb"], p.rgb)
+    rgb_e, dep_e, _, _, occ_e = _ref(p, sub, fm_dtype=dt, mm_dtype=dt)
+    rgb_x = _ref(p, sub)[0]
+    g_rgb = out["rgb"][0, sub].cpu().numpy()
+    rows = {"rgb_emulated": _errs(g_rgb, rgb_e),
+            "occ_emulated": _errs(out["occ"][0, sub].cpu().numpy(), occ_e),
+            "depth_emulated": _errs(out["depth"][0, sub].cpu().numpy(), dep_e),
+            "rgb_exact": _errs(g_rgb, rgb_x)}
+    ps = psnr(g_rgb, rgb_x)
+    print(f"c2 {prec} (max, rms):", rows, f"PSNR vs float64 {ps:.1f} dB", flush=True)
+    tol_rgb, tol_occ = (2e-4, 1e-3) if prec == "bf16" else (1e-4, 2e-4)
+    assert rows["rgb_emulated"][0] <= tol_rgb, rows
+    assert rows["occ_emulated"][0] <= tol_occ, rows
+    assert ps > 45.0, ps
+
+
+@pytest.mark.parametrize("views", [5, 6])
+def test_c2_model_more_views_match_float64(views):
+    """5 and 6 source views of 1024^2 with the hidden-256 model: 8 view
+    slots per sample (fp32 12-sample chunks at S = 48), 128^2 target; fp32
+    vs float64 on 192 rays, bf16 vs the fp32 frame by PSNR."""
+    from sailor_amd.pipeline import FrameConfig
+    cfg = FrameConfig(f"c2_v{views}", views, 1024, 1200.0, 256, 8, 128, 150.0, 48, "c2")
+    p = frame(cfg.name, "fp32", cfg)
+    _fp32_checks(p, _subset(p, 192, 13))
+    plo = frame(cfg.name, "bf16", cfg)
+    valid = p.sorted[0, :, 0].cpu().numpy() >= 0
+    ps = psnr(p.rgb[0, valid].cpu().numpy(), plo.rgb[0, valid].cpu().numpy())
+    print(f"{views} views: bf16 PSNR vs fp32 {ps:.1f} dB", flush=True)
+    assert ps > 45.0, ps
+
+
+# ---------------------------------------------------------------------------
+# the differentiation form's forward
+def test_training_batch_fp32_forward_matches_float64():
+    """TrainingBatch of 8 target views (32^2 rays each) traced through the
+    full-size tree, rendered by RenderTrainer.forward with the hidden-128
+    model at S = 48 (fp32 24-sample chunks) vs float64 on 256 rays drawn
+    from all 8 views."""
+    from sailor_amd.model import NerfWeights
+    from sailor_amd.pipeline import FrameConfig
+    from sailor_amd.train import RenderTrainer, TrainingBatch
+    cfg = FrameConfig("c4_small", 4, 1024, 1200.0, 256, 8, 32, 37.5, 48, "c1")
+    tb = TrainingBatch(cfg, n_targets=8)
+    assert tb.n_rays == 8 * 32 * 32 and tb.n_valid > 1000
+    w = NerfWeights.init(32, 128)
+    tr = RenderTrainer(w, "fp32")
+    out = tr.forward(*tb.inputs())
+    torch.cuda.synchronize()
+    valid = torch.nonzero(tb.t[0, :, 0] >= 0).flatten().cpu().numpy()
+    per_view = np.bincount(valid // (32 * 32), minlength=8)
+    assert (per_view > 20).all(), per_view
+    sub = np.sort(np.random.default_rng(14).choice(valid, size=256, replace=False))
+    p = tb.pipe
+    fm = torch.from_numpy(p.fmaps_np[None]).double()
+    rgb, dep, ws, asum, occ = [x[0].numpy() for x in nerf_ref.render_rays(
+        w, tb.t[:, sub].cpu().numpy(), tb.rays[:, sub].cpu().numpy(), p.K.cpu().numpy(),
+        p.RT.cpu().numpy(), fm, p.bbox.cpu().numpy(), p.zdir.cpu().numpy(), tb.img_hw)]
+    rows = {"occ": _errs(out["occ"][0, sub].cpu().numpy(), occ),
+            "rgb": _errs(out["rgb"][0, sub].cpu().numpy(), rgb),
+            "ws": _errs(out["aux"][0, sub, 0].cpu().numpy(), ws)}
+    print("training batch fp32 (max, rms):", rows, flush=True)
+    assert rows["occ"][0] <= 2e-5 and rows["rgb"][0] <= 1e-4 and rows["ws"][0] <= 1e-4, rows
