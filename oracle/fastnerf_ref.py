@@ -34,6 +34,15 @@ is negative, and stops after the sample that takes T below 1e-4; the depth
 output is sum_i w_i t_i; ReLU is max(x, 0) with NaN -> 0, so a ray without
 samples (0/0 in the normalisation) gets NaN colour features but a finite
 colour.
+
+How it is pinned: tests/test_fastnerf_oracle.py (layout offsets, compositing
+vs the C oracle, Hydra view permutation and an explicit per-view loop,
+density vs view mean) and tests/test_fastnerf_pins.py (every MLP against
+torch.nn.Sequential modules loaded at the kernels' offsets, in the kernels'
+input order [z, tpsdf, geo]; the occupancy and RGB heads' sigmoids; Hydra
+stacks against the identity-projection and zero-value closed forms; copies
+of this file with a ReLU, the sigmoid, either residual dropped or z / tpsdf
+swapped fail those checks).
 """
 from __future__ import annotations
 
