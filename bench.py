@@ -7,7 +7,9 @@ multi-resolution merge, two-layer tree (32^3 coarse cells of 8^3 voxels),
 512^2 target rays, DDA + 48 adaptive samples per ray, per-ray sort, fused
 per-view MLP (hidden 256 x 4, 64-channel features from [4,64,256,256] maps)
 + occupancy compositing + Hydra attention over the 4 views + RGB head.
-The frame is captured once into a hipGraph and replayed.
+The frame is captured once into a hipGraph and replayed: `warmup` untimed
+replays, then `steps` timed ones with a HIP event pair around each;
+frames/s = 1 / median (p99 and mean reported beside it).
 
 `value` is the FP32 form's frames/s.  The BF16 form (MFMA bf16, fp32
 accumulate) is timed the same way and its PSNR against the FP32 render is
@@ -39,17 +41,33 @@ def mlp_flops_per_sample_view(F, H):
     return 2 * ((63 + F) * H + 3 * H * H)
 
 
-def time_frames(pipe, steps, warmup):
+def time_frames(pipe, steps, warmup, stats=None):
+    """Seconds per frame: the frame's hipGraph is replayed `warmup` times
+    untimed, then exactly `steps` times between two device synchronisations,
+    with a HIP event pair around every replay on the replay stream.  Returns
+    the median event time; `stats` (a dict) receives median / p99 / mean of
+    the per-frame times and the wall-clock mean of the bracketed loop."""
+    import numpy as np
     import torch
     pipe.capture()
     for _ in range(warmup):
         pipe.replay()
+    st = torch.cuda.current_stream()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for e0, e1 in evs:
+        e0.record(st)
         pipe.replay()
+        e1.record(st)
     torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / steps
+    wall = (time.perf_counter() - t0) / steps
+    ms = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
+    if stats is not None:
+        stats.update(median_ms=float(np.median(ms)), p99_ms=float(np.percentile(ms, 99)),
+                     mean_ms=float(ms.mean()), wall_mean_ms=wall * 1e3)
+    return float(np.median(ms)) / 1e3
 
 
 def stage_times(pipe, reps=5):
@@ -67,6 +85,52 @@ def stage_times(pipe, reps=5):
         torch.cuda.synchronize()
         res[name] = ev[0].elapsed_time(ev[1]) / reps
     return res
+
+
+def hydra_flops_per_ray(F, V, inner=32):
+    """Algorithmic FLOPs of the Hydra encoder (2 stacks: q, k, v, fc, w1, w2
+    per view token) and the RGB head (F -> 64 -> 32 -> 3) for one ray."""
+    stack = 2 * V * (3 * F * inner + inner * F + F * inner + inner * F)
+    return 2 * stack + 2 * (F * 64 + 64 * 32 + 32 * 3)
+
+
+def frame_roofline(pipe, stages, prec, n_valid):
+    """Frame-level roofline: each stage's fraction of its own roofline,
+    weighted by its share of the frame's GPU time.
+      build  (TSDF + merge + tree)  HBM: 16 B written per voxel + the V maps
+             (depth, colour) read once + 4 B per fine voxel read by the merge
+             and again by the tree + 32 B per node written
+      trace  (rays + DDA + samples, then the per-ray sort)  HBM: 24 B rays +
+             4 B valid length + 8 S B samples written per ray, 8 S B read and
+             8 S B written by the sort
+      sample kernel  MFMA: 2 sum(in * out) per (valid sample, view)
+      Hydra + RGB head  vector FP32 (157.3 TFLOP/s): hydra_flops_per_ray per
+             valid ray
+      the rest of the shade stage (ray listing) counts as time at 0."""
+    cfg, mc = pipe.cfg, pipe.mc
+    N = cfg.dim ** 3
+    R = cfg.target * cfg.target
+    nodes = sum(pipe.check()[k] for k in ("n_coarse", "n_fine"))
+    b_build = N * 16 + cfg.n_views * cfg.img * cfg.img * 8 + 2 * N * 4 + nodes * 32
+    b_trace = R * (24 + 4 + 8 * cfg.S) + R * 16 * cfg.S
+    f_mlp = n_valid * cfg.S * cfg.n_views * mlp_flops_per_sample_view(mc.F, mc.H)
+    f_hyd = n_valid * hydra_flops_per_ray(mc.F, cfg.n_views)
+    rows = {
+        "build": (stages["build"], b_build / (stages["build"] / 1e3) / 1e9 / HBM_GBS),
+        "trace": (stages["trace"], b_trace / (stages["trace"] / 1e3) / 1e9 / HBM_GBS),
+        "sample_kernel": (stages["sample_kernel"],
+                          f_mlp / (stages["sample_kernel"] / 1e3) / 1e12 / PEAKS[prec]),
+        "hydra_kernel": (stages["hydra_kernel"],
+                         f_hyd / (stages["hydra_kernel"] / 1e3) / 1e12 / PEAKS["fp32"]),
+        "shade_rest": (max(stages["shade"] - stages["sample_kernel"] - stages["hydra_kernel"],
+                           0.0), 0.0),
+    }
+    total = sum(t for t, _ in rows.values())
+    frac = sum(t / total * f for t, f in rows.values())
+    return {"frac": frac, "stage_ms_and_frac": {k: [round(t, 4), round(f, 4)]
+                                                 for k, (t, f) in rows.items()},
+            "algorithmic": {"build_bytes": b_build, "trace_bytes": b_trace,
+                            "sample_flops": f_mlp, "hydra_flops": f_hyd}}
 
 
 def kernel_ms(pipe, stage, reps=10):
@@ -623,8 +687,8 @@ def cpu_baseline(pipe, n_rays_sample=1024):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bf16", action="store_true")
@@ -680,7 +744,8 @@ def main():
     p32.run()
     torch.cuda.synchronize()
     info = p32.check()
-    sec = time_frames(p32, args.steps, args.warmup)
+    timing = {}
+    sec = time_frames(p32, args.steps, args.warmup, timing)
     stages = stage_times(p32)
     # roofline of the dominant kernel, the fused sample kernel (stage 2 of
     # sailor_render_rays): algorithmic FLOPs count evaluated (valid) samples
@@ -692,6 +757,7 @@ def main():
     stages["sample_kernel"] = k32
     stages["hydra_kernel"] = kernel_ms(p32, 4)
     achieved = flops / (k32 / 1e3) / 1e12
+    frame_rl = frame_roofline(p32, stages, "fp32", n_valid)
     result = {
         "metric": METRIC, "value": 1.0 / sec, "unit": "frames/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
@@ -705,12 +771,14 @@ def main():
                          f"[{cfg.n_views},{mc.F},{mc.Hf},{mc.Wf}]"),
             "occupied_voxels": info["occupied"], "tree_nodes": [info["n_coarse"], info["n_fine"]],
             "valid_rays": n_valid, "stage_ms": stages, "graph": True,
+            "frame_timing": timing,
         },
         "roofline": {
             "bound": "mfma", "achieved": achieved, "peak": PEAKS["fp32"], "unit": "TFLOP/s",
             "frac": achieved / PEAKS["fp32"], "traffic": measured_traffic("fp32"),
             "kernel": "render_kernel<64,256,fp32,24> (sailor_render_rays stage 2)",
             "flops_per_launch": flops, "avg_launch_ms": k32,
+            "frame": frame_rl,
         },
     }
     rgb32 = p32.rgb[0].cpu().numpy()
@@ -720,7 +788,8 @@ def main():
             plo = FramePipeline.synthetic(cfg, prec=prec)
             plo.run()
             torch.cuda.synchronize()
-            s_lo = time_frames(plo, args.steps, args.warmup)
+            t_lo = {}
+            s_lo = time_frames(plo, args.steps, args.warmup, t_lo)
             st_lo = stage_times(plo)
             k_lo = kernel_ms(plo, 2)
             st_lo["sample_kernel"] = k_lo
@@ -734,6 +803,8 @@ def main():
                 "sample_kernel_tflops": flops / (k_lo / 1e3) / 1e12,
                 "sample_kernel_frac_of_peak": flops / (k_lo / 1e3) / 1e12 / PEAKS[prec],
                 "traffic": measured_traffic(prec),
+                "frame_timing": t_lo,
+                "frame_roofline": frame_roofline(plo, st_lo, prec, n_valid),
             }
             del plo
     if not args.no_early_exit:

@@ -67,6 +67,19 @@ int sailor_get_origin_bbox(const float* depths, const float* K, const float* R,
                            const float* T, int B, int V, int H, int W,
                            float th_bbox, float* vol_origin, void* stream);
 
+/* Replaces get_center_xyz (fusion_kernel.cu:536-598; kernel
+ * obtain_center_points :186-233), reached from the GUI through
+ * _integrate_depths.get_target_center (voxel_encoding_helper.py:256-264):
+ * per batch, the sum of the world points back-projected from the pixels with
+ * 0.1 m <= depth <= 4 m and their count.
+ *  depths f32 [B,V,H,W], K/R f32 [B,V,3,3], T f32 [B,V,3] ->
+ *  vol_center f32 [B,3] (sum of x, y, z), vol_num_pts f32 [B] (count).
+ *  The sums are formed in double in a fixed order (the reference adds float
+ *  atomics in scheduling order).  workspace: at least 8192 * B bytes. */
+int sailor_get_center_xyz(const float* depths, const float* K, const float* R, const float* T,
+                          int B, int V, int H, int W, float* vol_center, float* vol_num_pts,
+                          void* workspace, size_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Two-layer occupancy tree                                                  */
 /* ------------------------------------------------------------------------ */
@@ -306,6 +319,38 @@ int sailor_volume_rendering_training_backward(
     const int32_t* sort_idx, const float* sigma, const float* feats, int B,
     int R, int S, int F, float t_thresh, int support_occ, float* grad_feats,
     float* grad_sigma, void* stream);
+
+/* Replaces _volume_render_torch.forward / .forward_occ
+ * (c_lib/VoxelEncoding/voxel_encoding_helper.py:132-164 / :166-186), the
+ * PyTorch compositing the reference trains with: samples in the order given,
+ * no sort index and no early termination.  occ_mode 0 (density form):
+ * delta_i = t_{i+1} - t_i with delta_{S-1} = 0, alpha_i = 1 - exp(-delta_i
+ * sigma_i), T_i = exp(-sum_{j<i} delta_j sigma_j); occ_mode 1 (occupancy
+ * form): alpha_i = sigma_i, T_i = prod_{j<i} (1 - alpha_j + 1e-6).  w_i =
+ * alpha_i T_i; out_feats [n_rays,F] = sum w_i f_i, out_depth / out_ws /
+ * out_alpha [n_rays] = sum w_i t_i / sum w_i / sum w_i alpha_i.
+ *  t, sigma f32 [n_rays,S]; feats f32 [n_rays,S,F]; valid uint8 [n_rays] or
+ *  NULL (rays with 0 output zeros, the reference's valid_rays mask).
+ *  compensated != 0: Kahan-compensated exponent prefix sum and output sums
+ *  (no counterpart in the reference). */
+int sailor_volume_render_torch_forward(const float* t, const float* sigma, const float* feats,
+                                       const unsigned char* valid, long long n_rays, int S,
+                                       int F, int occ_mode, int compensated, float* out_feats,
+                                       float* out_depth, float* out_ws, float* out_alpha,
+                                       void* stream);
+
+/* Gradient of sailor_volume_render_torch_forward (what torch autograd gives
+ * the reference through the same tensor ops) with respect to sigma and
+ * feats, from the gradients of its four outputs: grad_sigma [n_rays,S],
+ * grad_feats [n_rays,S,F] (fully overwritten; zero for invalid rays).
+ * Sample depths are constants, as in the reference's training step.
+ * F <= 64. */
+int sailor_volume_render_torch_backward(const float* g_feats, const float* g_depth,
+                                        const float* g_ws, const float* g_alpha, const float* t,
+                                        const float* sigma, const float* feats,
+                                        const unsigned char* valid, long long n_rays, int S,
+                                        int F, int occ_mode, float* grad_sigma,
+                                        float* grad_feats, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Fused radiance evaluation                                                 */

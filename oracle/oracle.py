@@ -7,6 +7,8 @@ adds float64 NumPy / torch restatements used as accuracy oracles.
 
 How each part is pinned (tests/test_oracle.py; test names in brackets):
   * inverse3x3: numpy.linalg.inv  [test_inverse3x3_matches_linalg]
+  * center_xyz: fronto-parallel plane closed form, float64 numpy
+    back-projection of the capsule scene  [test_center_xyz_*]
   * bbox / resized bbox: closed form on a fronto-parallel plane, float64
     brute force over posed views, centred-cube closed form  [test_bbox_*,
     test_resized_bbox_is_centred_cube]
@@ -57,6 +59,11 @@ def lib():
         L.orc_inverse3x3.argtypes = [f32p, c_int, f32p]
         L.orc_origin_bbox.argtypes = [f32p, f32p, f32p, f32p, c_int, c_int, c_int, c_int,
                                       c_float, f32p]
+        L.orc_center_xyz.argtypes = [f32p, f32p, f32p, f32p, c_int, c_int, c_int, c_int,
+                                     np.ctypeslib.ndpointer(dtype=np.float64,
+                                                            flags="C_CONTIGUOUS"),
+                                     np.ctypeslib.ndpointer(dtype=np.float64,
+                                                            flags="C_CONTIGUOUS")]
         L.orc_resized_bbox.argtypes = [f32p, c_int, c_int, f32p, f32p]
         L.orc_integrate.argtypes = [i32p, i32p, f32p, f32p, f32p, f32p] + [c_int] * 7 + \
             [c_float] * 4 + [f32p, f32p, f32p, f32p, f32p]
@@ -117,6 +124,17 @@ def origin_bbox(depths, K, R, T, th_bbox):
     out = np.empty((B, 3, 2), np.float32)
     lib().orc_origin_bbox(depths, K, R, T, B, V, H, W, th_bbox, out)
     return out
+
+
+def center_xyz(depths, K, R, T):
+    """get_center_xyz: (sum of back-projected valid points [B,3], count [B])
+    in float64 (float32 point arithmetic)."""
+    depths, K, R, T = _f(depths), _f(K), _f(R), _f(T)
+    B, V, H, W = depths.shape
+    c = np.zeros((B, 3), np.float64)
+    n = np.zeros((B,), np.float64)
+    lib().orc_center_xyz(depths, K, R, T, B, V, H, W, c, n)
+    return c, n
 
 
 def resized_bbox(origin, dim):

@@ -89,6 +89,44 @@ void orc_origin_bbox(const float* depths, const float* K, const float* R, const 
   free(Rinv);
 }
 
+/* get_center_xyz / obtain_center_points, fusion_kernel.cu:186-233,536-598:
+ * per batch the sum of the back-projected points of the pixels with
+ * 0.1 <= d <= 4 (float32 point arithmetic as the bbox pass) accumulated in
+ * double in pixel order, and their count. */
+void orc_center_xyz(const float* depths, const float* K, const float* R, const float* T,
+                    int B, int V, int H, int W, double* center, double* num) {
+  float* Kinv = (float*)malloc(sizeof(float) * 9 * B * V);
+  float* Rinv = (float*)malloc(sizeof(float) * 9 * B * V);
+  orc_inverse3x3(K, B * V, Kinv);
+  orc_inverse3x3(R, B * V, Rinv);
+  for (int b = 0; b < B; ++b) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int v = 0; v < V; ++v) {
+      const float* ki = Kinv + 9 * (b * V + v);
+      const float* ri = Rinv + 9 * (b * V + v);
+      const float* t = T + 3 * (b * V + v);
+      for (int py = 0; py < H; ++py)
+        for (int px = 0; px < W; ++px) {
+          float d = depths[(((size_t)b * V + v) * H + py) * W + px];
+          if (d < 0.1f || d > 4.0f) continue;
+          float fx = (float)px, fy = (float)py;
+          float cx = (ki[0] * fx + ki[1] * fy + ki[2]) * d;
+          float cy = (ki[3] * fx + ki[4] * fy + ki[5]) * d;
+          float cz = (ki[6] * fx + ki[7] * fy + ki[8]) * d;
+          cx -= t[0]; cy -= t[1]; cz -= t[2];
+          acc[0] += (double)(ri[0] * cx + ri[1] * cy + ri[2] * cz);
+          acc[1] += (double)(ri[3] * cx + ri[4] * cy + ri[5] * cz);
+          acc[2] += (double)(ri[6] * cx + ri[7] * cy + ri[8] * cz);
+          acc[3] += 1.0;
+        }
+    }
+    for (int a = 0; a < 3; ++a) center[3 * b + a] = acc[a];
+    num[b] = acc[3];
+  }
+  free(Kinv);
+  free(Rinv);
+}
+
 /* obtain_resized_bbox, fusion_kernel.cu:51-103 */
 void orc_resized_bbox(const float* origin, int B, int dim, float* bbox, float* vs) {
   for (int i = 0; i < B; ++i) {

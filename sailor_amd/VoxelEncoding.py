@@ -19,7 +19,9 @@ Deliberate differences (all documented in the kernels):
 from __future__ import annotations
 
 import ctypes
+from collections import namedtuple
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -94,6 +96,25 @@ def get_origin_bbox(depth_ims, cam_intr, cam_R, cam_T, th_bbox, device):
     return out
 
 
+def get_center_xyz(depth_ims, cam_intr, cam_R, cam_T, th_bbox, device):
+    """get_center_xyz (fusion_kernel.cu:536-598) -> (vol_center [B,3] = sum
+    of the back-projected valid points, vol_num_pts [B,1] = their count);
+    th_bbox is unused, as in the reference."""
+    dev = depth_ims.device
+    depths = as_input(depth_ims, "depth_ims", _f32)
+    B, V, H, W = depths.shape
+    K = as_input(cam_intr, "cam_intr", _f32, dev).reshape(B, V, 3, 3)
+    R = as_input(cam_R, "cam_R", _f32, dev).reshape(B, V, 3, 3)
+    T = as_input(cam_T, "cam_T", _f32, dev).reshape(B, V, 3)
+    center = torch.empty((B, 3), dtype=_f32, device=dev)
+    num = torch.empty((B, 1), dtype=_f32, device=dev)
+    ws = torch.empty(8192 * B, dtype=torch.uint8, device=dev)
+    with torch.cuda.device(dev):
+        call("sailor_get_center_xyz", ptr(depths), ptr(K), ptr(R), ptr(T), B, V, H, W,
+             ptr(center), ptr(num), ptr(ws), ws.numel(), stream_of(dev))
+    return center, num
+
+
 # ---------------------------------------------------------------------------
 # multi-resolution volumes and the tree
 def build_multi_res_volumes(o_volumes, num_occupied_voxels, level, rate, device):
@@ -117,6 +138,11 @@ def build_multi_res_volumes(o_volumes, num_occupied_voxels, level, rate, device)
     with torch.cuda.device(dev):
         call("sailor_build_multi_res_volumes", vols, cnts, level, B, dim, int(rate),
              stream_of(dev))
+
+
+OcTreeNode = namedtuple("OcTreeNode", "valid depth gindex vindex valid_num_leaves size vol_dim "
+                                     "batch_id voxel_size xyz_center xyz_center_ father_gindex "
+                                     "children_gindex")
 
 
 class MultiLayerOctree:
@@ -186,6 +212,44 @@ class MultiLayerOctree:
         if q[0] < 0:
             raise RuntimeError("octree build failed: node capacity exceeded")
         return q
+
+    def get_nodes(self):
+        """get_octree_nodes (octree.h:250-254, bound as get_nodes,
+        binding.cpp:59): one OcTreeNode per node slot, in gindex order, with
+        the fields of the reference's OcTree record (octree.h:29-55) that
+        describe the node: valid, depth (log2 of the level's dim), gindex,
+        vindex (raster index in its level's volume), valid_num_leaves,
+        size (fine voxels per edge), vol_dim, batch_id, voxel_size (edge
+        length), xyz_center (integer cell), xyz_center_ (centre),
+        father_gindex and children_gindex (rate^3 slots: the occupied
+        children in raster order, then -1).  Host copy; synchronises."""
+        pos = self.node_pos.cpu().numpy()
+        info = self.node_info.cpu().numpy()
+        bbox = self.volume_bbox.reshape(-1, 3, 2).cpu().numpy()
+        vs = self.voxel_size.reshape(-1).cpu().numpy()
+        dims = self.dims
+        rate = dims[0] // dims[1]
+        out = []
+        for g in range(self.num_octree_nodes):
+            b, parent, child0, nchild = (int(x) for x in info[g])
+            if b < 0:
+                out.append(OcTreeNode(0, -1, g, -1, 0, 0, 0, -1, 0.0, (-1, -1, -1),
+                                      (0.0, 0.0, 0.0), -1, [-1] * rate ** 3))
+                continue
+            size = float(pos[g, 3])
+            coarse = parent < 0
+            lvl_dim = dims[1] if coarse else dims[0]
+            cell = size / vs[b]
+            # centre = box min + (cell - 1) / 2 * vs + cell * vs * i (octree.h:110-151)
+            idx = tuple(int(round((pos[g, a] - bbox[b, a, 0] - (cell - 1) / 2 * vs[b]) / size))
+                        for a in range(3))
+            vindex = (idx[0] * lvl_dim + idx[1]) * lvl_dim + idx[2]
+            kids = list(range(child0, child0 + nchild)) if child0 >= 0 else []
+            out.append(OcTreeNode(1, int(np.log2(lvl_dim)), g, vindex, nchild,
+                                  int(round(cell)), lvl_dim, b, size, idx,
+                                  tuple(float(x) for x in pos[g, :3]), parent,
+                                  kids + [-1] * (rate ** 3 - len(kids))))
+        return out
 
     def get_nodes_tensor(self):
         """output_all_nodes_tensor: [N,6] (x, y, z, size, valid, batch)."""
@@ -501,6 +565,61 @@ def volume_rendering_training_backward(grad_output_feats, grad_output_depths, gr
              ptr(si), ptr(s), ptr(f), B, R, S, F, float(t_thresh), int(bool(support_occ)),
              ptr(gf), ptr(gs), stream_of(dev))
     return gf, gs
+
+
+def volume_render_torch_forward(t, sigma, feats, valid, occ_mode, compensated=False):
+    """Compositing with the tensor formulation of _volume_render_torch
+    (voxel_encoding_helper.py:132-186): t, sigma [B,R,S], feats [B,R,S,F],
+    valid uint8 [B,R] or None -> (feats [B,R,F], depth, ws, alpha [B,R,1]).
+    `compensated` selects Kahan-compensated sums (not in the reference)."""
+    dev = t.device
+    t = check_input(t, "sampled_depths", _f32)
+    s = check_input(sigma, "sigma", _f32, dev)
+    f = check_input(feats, "feats", _f32, dev)
+    B, R, S = t.shape
+    F = f.shape[-1]
+    if s.numel() != B * R * S or f.numel() != B * R * S * F:
+        raise RuntimeError("volume_render_torch: sigma must be [B, R, S(, 1)] and feats "
+                           "[B, R, S, F] for sampled_depths [B, R, S]")
+    if valid is not None:
+        check_input(valid, "valid_rays", torch.uint8, dev)
+        if valid.numel() != B * R:
+            raise RuntimeError("volume_render_torch: valid_rays must be [B, R]")
+    o = dict(dtype=_f32, device=dev)
+    of, od = torch.empty((B, R, F), **o), torch.empty((B, R, 1), **o)
+    ow, oa = torch.empty((B, R, 1), **o), torch.empty((B, R, 1), **o)
+    with torch.cuda.device(dev):
+        call("sailor_volume_render_torch_forward", ptr(t), ptr(s), ptr(f), ptr(valid), B * R, S,
+             F, int(bool(occ_mode)), int(bool(compensated)), ptr(of), ptr(od), ptr(ow), ptr(oa),
+             stream_of(dev))
+    return of, od, ow, oa
+
+
+def volume_render_torch_backward(g_feats, g_depth, g_ws, g_alpha, t, sigma, feats, valid,
+                                 occ_mode):
+    """Gradient of volume_render_torch_forward with respect to sigma and
+    feats from the four output gradients (None = zero) ->
+    (grad_sigma [B,R,S], grad_feats [B,R,S,F])."""
+    dev = t.device
+    B, R, S = t.shape
+    F = feats.shape[-1]
+    shapes = ((B, R, F), (B, R, 1), (B, R, 1), (B, R, 1))
+    gs = []
+    for g, shp, name in zip((g_feats, g_depth, g_ws, g_alpha), shapes,
+                            ("feats", "depth", "ws", "alpha")):
+        if g is None:
+            g = torch.zeros(shp, dtype=_f32, device=dev)
+        g = as_input(g, f"grad_{name}", _f32, dev)
+        if g.numel() != int(torch.Size(shp).numel()):
+            raise RuntimeError(f"volume_render_torch: grad_{name} must be {list(shp)}")
+        gs.append(g)
+    grad_sigma = torch.empty((B, R, S), dtype=_f32, device=dev)
+    grad_feats = torch.empty((B, R, S, F), dtype=_f32, device=dev)
+    with torch.cuda.device(dev):
+        call("sailor_volume_render_torch_backward", *[ptr(g) for g in gs], ptr(t), ptr(sigma),
+             ptr(feats), ptr(valid), B * R, S, F, int(bool(occ_mode)), ptr(grad_sigma),
+             ptr(grad_feats), stream_of(dev))
+    return grad_sigma, grad_feats
 
 
 # ---------------------------------------------------------------------------

@@ -25,6 +25,7 @@ _SIGS = {
     "sailor_abi_version": [],
     "sailor_integrate": [P] * 9 + [I] * 7 + [F] * 5 + [P] * 8,
     "sailor_get_origin_bbox": [P] * 4 + [I] * 4 + [F, P, P],
+    "sailor_get_center_xyz": [P, P, P, P, I, I, I, I, P, P, P, SZ, P],
     "sailor_build_multi_res_volumes": [P, P, I, I, I, I, P],
     "sailor_octree_workspace_bytes": [I, I],
     "sailor_octree_build": [P, P, P, P, I, I, I, P, P, I, I, P, P, P, P, SZ, P],
@@ -38,6 +39,8 @@ _SIGS = {
     "sailor_volume_rendering_occ_forward": [P, P, P, I, I, I, I, F, P, P, P, P, P],
     "sailor_volume_rendering_training_forward": [P, P, P, P, I, I, I, I, F, I, P, P, P, P, P],
     "sailor_volume_rendering_training_backward": [P] * 12 + [I, I, I, I, F, I, P, P, P],
+    "sailor_volume_render_torch_forward": [P, P, P, P, LL, I, I, I, I, P, P, P, P, P],
+    "sailor_volume_render_torch_backward": [P] * 8 + [LL, I, I, I, P, P, P],
     "sailor_render_small_weights_floats": [I, I],
     "sailor_render_rays": [P, P, I, I, I, P, P, I, I, I, P, I, I, I, I, P, P, I, I, P, P, P, F,
                            I, I, P, P, P, P, P, SZ, P],

@@ -259,6 +259,83 @@ integrate_kernel(float* __restrict__ tsdf_vol, float* __restrict__ weight_vol,
   }
 }
 
+// Sum of the back-projected world points of the valid pixels (0.1 m <= d
+// <= 4 m) and their count per batch (obtain_center_points,
+// fusion_kernel.cu:186-233), with the same per-pixel arithmetic as the bbox
+// pass.  The reference adds every point with float atomics (order-dependent
+// low bits); here each workgroup sums its pixels in double and writes one
+// partial [x, y, z, n]; center_finalize adds the partials in block order, so
+// the result is reproducible and exact to float32 rounding of the double
+// sum.  grid = (kCenterBlocks, B).
+constexpr int kCenterBlocks = 256;
+
+__global__ void __launch_bounds__(kThreads)
+center_partials(const float* __restrict__ depths, const float* __restrict__ K,
+                const float* __restrict__ R, const float* __restrict__ T, int V, int H, int W,
+                double* __restrict__ partials) {
+  __shared__ float s_inv[kMaxViews * 18];
+  __shared__ double s_red[4][kThreads / 64];
+  const int b = blockIdx.y;
+  for (int v = threadIdx.x; v < V; v += blockDim.x) {
+    inverse3x3(K + ((size_t)b * V + v) * 9, s_inv + v * 18);
+    inverse3x3(R + ((size_t)b * V + v) * 9, s_inv + v * 18 + 9);
+  }
+  __syncthreads();
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const long long HW = (long long)H * W;
+  const long long n = (long long)V * HW;
+  const float* dep = depths + (size_t)b * n;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    float d = dep[i];
+    if (d < 0.1f || d > 4.0f) continue;
+    int v = (int)(i / HW);
+    long long r = i - (long long)v * HW;
+    int py = (int)(r / W);
+    int px = (int)(r - (long long)py * W);
+    const float* ki = s_inv + v * 18;
+    const float* ri = ki + 9;
+    const float* t = T + ((size_t)b * V + v) * 3;
+    float fx = (float)px, fy = (float)py;
+    float cx = (ki[0] * fx + ki[1] * fy + ki[2]) * d;
+    float cy = (ki[3] * fx + ki[4] * fy + ki[5]) * d;
+    float cz = (ki[6] * fx + ki[7] * fy + ki[8]) * d;
+    cx -= t[0];
+    cy -= t[1];
+    cz -= t[2];
+    acc[0] += (double)(ri[0] * cx + ri[1] * cy + ri[2] * cz);
+    acc[1] += (double)(ri[3] * cx + ri[4] * cy + ri[5] * cz);
+    acc[2] += (double)(ri[6] * cx + ri[7] * cy + ri[8] * cz);
+    acc[3] += 1.0;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    double x = acc[a];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (lane == 0) s_red[a][wid] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double x = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) x += s_red[threadIdx.x][w];
+    partials[((size_t)b * gridDim.x + blockIdx.x) * 4 + threadIdx.x] = x;
+  }
+}
+
+__global__ void center_finalize(const double* __restrict__ partials, int B, int blocks,
+                                float* __restrict__ center, float* __restrict__ num_pts) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int k = 0; k < blocks; ++k)
+    for (int a = 0; a < 4; ++a) acc[a] += partials[((size_t)b * blocks + k) * 4 + a];
+  center[b * 3 + 0] = (float)acc[0];
+  center[b * 3 + 1] = (float)acc[1];
+  center[b * 3 + 2] = (float)acc[2];
+  num_pts[b] = (float)acc[3];
+}
+
 int launch_bbox(const float* depths, const float* K, const float* R,
                 const float* T, int B, int V, int H, int W, float th,
                 int* keys, hipStream_t st) {
@@ -316,4 +393,22 @@ extern "C" int sailor_get_origin_bbox(const float* depths, const float* K,
   hipLaunchKernelGGL(keys_to_float, dim3(sailor::ceil_div(B * 6, 64)), dim3(64),
                      0, st, vol_origin, B * 6);
   return sailor::check_launch("get_origin_bbox");
+}
+
+extern "C" int sailor_get_center_xyz(const float* depths, const float* K, const float* R,
+                                     const float* T, int B, int V, int H, int W,
+                                     float* vol_center, float* vol_num_pts, void* workspace,
+                                     size_t workspace_bytes, void* stream) {
+  SAILOR_REQUIRE(B > 0 && V > 0 && V <= kMaxViews && H > 0 && W > 0,
+                 "get_center_xyz: bad sizes (B=%d, V=%d, H=%d, W=%d)", B, V, H, W);
+  const size_t need = (size_t)B * kCenterBlocks * 4 * sizeof(double);
+  SAILOR_REQUIRE(workspace_bytes >= need, "get_center_xyz: workspace too small (%zu < %zu)",
+                 workspace_bytes, need);
+  hipStream_t st = sailor::as_stream(stream);
+  double* part = reinterpret_cast<double*>(workspace);
+  hipLaunchKernelGGL(center_partials, dim3(kCenterBlocks, B), dim3(kThreads), 0, st, depths,
+                     K, R, T, V, H, W, part);
+  hipLaunchKernelGGL(center_finalize, dim3(sailor::ceil_div(B, 64)), dim3(64), 0, st, part, B,
+                     kCenterBlocks, vol_center, vol_num_pts);
+  return sailor::check_launch("get_center_xyz");
 }

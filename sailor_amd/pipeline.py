@@ -271,9 +271,16 @@ def prefetch_scenes(cfgs, frame_ids, workers=8):
             _SCENE_CACHE[k] = sc
         return
     import multiprocessing as mp
-    with mp.get_context("spawn").Pool(min(workers, len(jobs))) as pool:
+    pool = mp.get_context("spawn").Pool(min(workers, len(jobs)))
+    try:
         for k, sc in pool.imap_unordered(_scene_job, jobs):
             _SCENE_CACHE[k] = sc
+        pool.close()        # workers finish and exit on their own
+    except BaseException:
+        pool.terminate()
+        raise
+    finally:
+        pool.join()
 
 
 class BatchPipeline:

@@ -5,10 +5,32 @@ from __future__ import annotations
 import torch
 
 from . import _lib
-from ._lib import as_input, call, ptr, stream_of
+from ._lib import call, check_input, ptr, stream_of
 from .model import PREC_IDS
 
 _f32, _i32 = torch.float32, torch.int32
+
+
+def _check(t, name, dtype, shape, device):
+    """Input validation before any launch: a CUDA tensor of `dtype` on
+    `device`, contiguous, of exactly `shape`."""
+    check_input(t, name, dtype, device)
+    if tuple(t.shape) != tuple(shape):
+        raise RuntimeError(f"{name} must have shape {list(shape)}, got {list(t.shape)}")
+    return t
+
+
+def _check_out(out, shapes, device):
+    """A reusable output dict matches the call: every listed entry present
+    (or None where allowed) with the expected shape, float32/int32 dtype and
+    device."""
+    for key, (shape, dtype, required) in shapes.items():
+        v = out.get(key)
+        if v is None:
+            if required:
+                raise RuntimeError(f"out[{key!r}] is missing")
+            continue
+        _check(v, f"out[{key!r}]", dtype, shape, device)
 
 
 def trace_sample(K, R, T, img_hw, vol_coarse, node_pos, node_info, vol_bbox, voxel_size,
@@ -22,6 +44,22 @@ def trace_sample(K, R, T, img_hw, vol_coarse, node_pos, node_info, vol_bbox, vox
     dc = vol_coarse.shape[1]
     S = noise.shape[-1]
     n = Hh * Ww
+    _check(vol_coarse, "vol_coarse", torch.int32, (B, dc, dc, dc), dev)
+    _check(K, "K", _f32, (B, 3, 3), dev)
+    _check(R, "R", _f32, (B, 3, 3), dev)
+    _check(T, "T", _f32, (B, 3), dev)
+    _check(noise, "noise", _f32, (B, n, S), dev)
+    _check(vol_bbox, "vol_bbox", _f32, (B, 3, 2), dev)
+    check_input(voxel_size, "voxel_size", _f32, dev)
+    if voxel_size.numel() != B:
+        raise RuntimeError(f"voxel_size must hold {B} values")
+    check_input(node_pos, "node_pos", _f32, dev)
+    if node_pos.dim() != 2 or node_pos.shape[1] != 4:
+        raise RuntimeError("node_pos must be [n_nodes, 4]")
+    _check(node_info, "node_info", torch.int32, tuple(node_pos.shape), dev)
+    if out is not None:
+        _check_out(out, dict(rays=((B, n, 6), _f32, False), sidx=((B, n, S), torch.int32, True),
+                             st=((B, n, S), _f32, True), vl=((B, n), _f32, True)), dev)
     if out is None:
         out = dict(rays=torch.empty((B, n, 6), dtype=_f32, device=dev) if want_rays else None,
                    sidx=torch.empty((B, n, S), dtype=_i32, device=dev),
@@ -53,8 +91,20 @@ def render_rays(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, model: PackedMode
     bbox [B,3,2]; zdir [B,3].
     Returns dict(rgb [B,R,3], depth [B,R], aux [B,R,2], occ [B,R,S] or None)."""
     dev = t.device
+    check_input(t, "t", _f32)
+    if t.dim() != 3:
+        raise RuntimeError(f"t must be [B, R, S], got {list(t.shape)}")
     B, R, S = t.shape
+    check_input(Kv, "Kv", _f32, dev)
+    if Kv.dim() != 4 or Kv.shape[0] != B or tuple(Kv.shape[2:]) != (3, 3):
+        raise RuntimeError(f"Kv must be [{B}, V, 3, 3], got {list(Kv.shape)}")
     V = Kv.shape[1]
+    _check(rays, "rays", _f32, (B, R, 6), dev)
+    _check(RTv, "RTv", _f32, (B, V, 3, 4), dev)
+    _check(bbox, "bbox", _f32, (B, 3, 2), dev)
+    _check(zdir, "zdir", _f32, (B, 3), dev)
+    if fmap_cl.dim() != 5 or fmap_cl.shape[0] != B or fmap_cl.shape[1] != V:
+        raise RuntimeError(f"fmap_cl must be [{B}, {V}, Hf, Wf, F], got {list(fmap_cl.shape)}")
     Hf, Wf, F = fmap_cl.shape[2], fmap_cl.shape[3], fmap_cl.shape[4]
     if F != model.F:
         raise RuntimeError(f"feature maps have {F} channels, model expects {model.F}")
@@ -62,6 +112,15 @@ def render_rays(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, model: PackedMode
                 torch.float16: PREC_IDS["fp16"]}.get(fmap_cl.dtype)
     if map_prec is None:
         raise RuntimeError(f"feature maps of dtype {fmap_cl.dtype} are not supported")
+    if map_prec not in (0, PREC_IDS[model.prec]):
+        raise RuntimeError(f"{fmap_cl.dtype} feature maps need a {fmap_cl.dtype} model, "
+                           f"got {model.prec}")
+    check_input(fmap_cl, "fmap_cl", fmap_cl.dtype, dev)
+    if out is not None:
+        _check_out(out, dict(rgb=((B, R, 3), _f32, True), depth=((B, R), _f32, True),
+                             aux=((B, R, 2), _f32, False), occ=((B, R, S), _f32, False)), dev)
+        if want_occ and out.get("occ") is None:
+            raise RuntimeError("want_occ with an out dict that has no 'occ' buffer")
     if out is None:
         out = dict(rgb=torch.empty((B, R, 3), dtype=_f32, device=dev),
                    depth=torch.empty((B, R), dtype=_f32, device=dev),
@@ -85,9 +144,25 @@ def density_points(points, Kv, RTv, img_hw, fmap_cl, bbox, model: PackedModel, o
     """Occupancy of the fused model at points [B,N,3] (infer_density path):
     per-view MLP on gathered features + encoding, view mean, occupancy head.
     Returns [B,N] float32."""
+    dev = points.device
+    check_input(points, "points", _f32)
+    if points.dim() != 3 or points.shape[2] != 3:
+        raise RuntimeError(f"points must be [B, N, 3], got {list(points.shape)}")
     B, N = points.shape[0], points.shape[1]
+    check_input(Kv, "Kv", _f32, dev)
+    if Kv.dim() != 4 or Kv.shape[0] != B or tuple(Kv.shape[2:]) != (3, 3):
+        raise RuntimeError(f"Kv must be [{B}, V, 3, 3], got {list(Kv.shape)}")
     V = Kv.shape[1]
+    _check(RTv, "RTv", _f32, (B, V, 3, 4), dev)
+    _check(bbox, "bbox", _f32, (B, 3, 2), dev)
+    if fmap_cl.dim() != 5 or fmap_cl.shape[0] != B or fmap_cl.shape[1] != V:
+        raise RuntimeError(f"fmap_cl must be [{B}, {V}, Hf, Wf, F], got {list(fmap_cl.shape)}")
+    check_input(fmap_cl, "fmap_cl", fmap_cl.dtype, dev)
     Hf, Wf, F = fmap_cl.shape[2], fmap_cl.shape[3], fmap_cl.shape[4]
+    if F != model.F:
+        raise RuntimeError(f"feature maps have {F} channels, model expects {model.F}")
+    if out is not None:
+        _check(out, "out", _f32, (B, N), dev)
     map_prec = {torch.float32: 0, torch.bfloat16: PREC_IDS["bf16"],
                 torch.float16: PREC_IDS["fp16"]}.get(fmap_cl.dtype)
     if map_prec is None:

@@ -186,8 +186,11 @@ class RenderTrainer:
 
     def forward(self, t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, t_thres=1e-4):
         from . import render
-        B, R, _ = t.shape
-        if self._out is not None and self._out["rgb"].shape[:2] != (B, R):
+        B, R, S = t.shape
+        # the output buffers are reused only for a batch of the same shape on
+        # the same device (the kernel writes occ[ray * S + sample])
+        o = self._out
+        if o is not None and (tuple(o["occ"].shape) != (B, R, S) or o["occ"].device != t.device):
             self._out = None
         self._out = render.render_rays(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir,
                                        self.model, t_thres, early_exit=False, want_occ=True,
@@ -230,10 +233,12 @@ class RenderTrainer:
                                      sc.grad_small, sc.partials, n_valid, t_thres,
                                      self.batch_items, columns=cols)
             self.columns = cols
-        small = self.weights.split_small(self._scratch.grad_small)
-        return dict(dense_w=[natural_first_layer_grad(dW[0], F)] + dW[1:], dense_b=db,
-                    sigma_w=small["sigma_w"], sigma_b=small["sigma_b"], hydra=small["hydra"],
-                    rgb=small["rgb"])
+        # the scratch buffers are overwritten by the next backward(): hand out
+        # copies, so gradients of several micro-batches can be accumulated
+        small = self.weights.split_small(self._scratch.grad_small.clone())
+        return dict(dense_w=[natural_first_layer_grad(dW[0], F)] + [w.clone() for w in dW[1:]],
+                    dense_b=[b.clone() for b in db], sigma_w=small["sigma_w"],
+                    sigma_b=small["sigma_b"], hydra=small["hydra"], rgb=small["rgb"])
 
 
 class TrainingBatch:
@@ -281,3 +286,122 @@ def mse_grad(rgb, target, n_valid):
     """dL/d rgb of L = mean over the valid rays' channels of (rgb - target)^2
     (dropped rays render 0 in both)."""
     return (rgb - target) * (2.0 / (3.0 * max(n_valid, 1)))
+
+
+# ---------------------------------------------------------------------------
+# autograd form: parameters as torch leaves, packed on the device each step
+def _pack_index(prec, H, K):
+    """Flat gather index (into a natural-order [H, K] layer) of every slot of
+    pack_layer's output, so that packing is one device gather."""
+    import numpy as np
+    from .model import kmap, unit_rows
+    rows, cols = np.broadcast_arrays(unit_rows(prec, H), kmap(prec, K)[None, :, :, None, :])
+    return torch.from_numpy((rows.astype(np.int64) * K + cols).reshape(-1))
+
+
+class RadianceField(torch.nn.Module):
+    """The fused render's parameters (NerfWeights layout) as nn.Parameters.
+
+    forward() renders through FusedRender, a torch.autograd.Function whose
+    backward is the HIP differentiation of the fused render (Hydra + RGB head
+    backward, compositing reverse scan, per-view MLP with per-ray recompute;
+    RenderTrainer), so a training step is
+        rgb = field(*batch.inputs()); loss(rgb).backward(); optimizer.step()
+    as in the reference's SRONet training step (SRONet/SRONet.py:238-258),
+    where torch autograd differentiates nerf.py:139-186 and
+    _volume_render_torch.forward_occ (voxel_encoding_helper.py:166-186).
+    The packed device buffers are rebuilt (device gathers, no host copies)
+    whenever a parameter changed since the last render."""
+
+    def __init__(self, weights, prec="fp32", device="cuda", batch_items=16384, fused=True):
+        super().__init__()
+        import numpy as np
+        dev = torch.device(device)
+        P = lambda a: torch.nn.Parameter(torch.as_tensor(np.asarray(a, np.float32)).to(dev))
+        self.weights, self.prec = weights, prec
+        self.F, self.H = weights.F, weights.H
+        self.dense_w = torch.nn.ParameterList([P(w) for w in weights.dense_w])
+        self.dense_b = torch.nn.ParameterList([P(b) for b in weights.dense_b])
+        self.sigma_w = P(weights.sigma_w)
+        self.sigma_b = P(np.array([weights.sigma_b], np.float32))
+        self.hydra = torch.nn.ParameterList(
+            [P(st[k]) for st in weights.hydra for k in HYDRA_KEYS])
+        self.rgb = torch.nn.ParameterList([P(x) for wb in weights.rgb for x in wb])
+        K0 = self.F + 64
+        self._idx = dict(first=_pack_index(prec, self.H, K0).to(dev),
+                         hidden=_pack_index(prec, self.H, self.H).to(dev))
+        self.trainer = RenderTrainer(weights, prec, dev, batch_items=batch_items, fused=fused)
+        self._versions = None
+
+    def flat_params(self):
+        return [*self.dense_w, *self.dense_b, self.sigma_w, self.sigma_b, *self.hydra,
+                *self.rgb]
+
+    def _pack(self):
+        """Repacks the device buffers the kernels read from the parameters."""
+        vers = tuple(p._version for p in self.flat_params())
+        if vers == self._versions:
+            return
+        dt = {"fp32": torch.float32, "bf16": torch.bfloat16, "fp16": torch.float16}[self.prec]
+        F, H = self.F, self.H
+        with torch.no_grad():
+            W0 = self.dense_w[0]
+            first = torch.cat([W0[:, 63:63 + F], W0[:, :63], torch.zeros_like(W0[:, :1])], 1)
+            parts = [first.reshape(-1)[self._idx["first"]]]
+            parts += [self.dense_w[i].reshape(-1)[self._idx["hidden"]] for i in (1, 2, 3)]
+            m = self.trainer.model
+            m.w_dense.copy_(torch.cat(parts).to(dt))
+            m.b_dense.copy_(torch.stack(list(self.dense_b)))
+            wt = [self.dense_w[i].t().contiguous().reshape(-1)[self._idx["hidden"]]
+                  for i in (3, 2, 1)]
+            self.trainer.wt.copy_(torch.cat(wt).to(dt))
+            small = [self.sigma_w.reshape(-1),
+                     torch.cat([self.sigma_b.reshape(1), self.sigma_b.new_zeros(3)])]
+            small += [p.reshape(-1) for p in self.hydra]
+            r = list(self.rgb)
+            small += [r[0].reshape(-1), r[1], r[2].reshape(-1), r[3], r[4].reshape(-1),
+                      torch.cat([r[5], r[5].new_zeros(1)])]
+            m.w_small.copy_(torch.cat(small))
+        self._versions = vers
+
+    def forward(self, t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, t_thres=1e-4):
+        """-> (rgb [B,R,3] differentiable, depth [B,R], aux [B,R,2])."""
+        self._pack()
+        return FusedRender.apply(self, t, rays, Kv, RTv, tuple(img_hw), fmap_cl, bbox, zdir,
+                                 float(t_thres), *self.flat_params())
+
+
+HYDRA_KEYS = ("q", "k", "v", "fc", "w1", "b1", "w2", "b2")
+
+
+class FusedRender(torch.autograd.Function):
+    """rgb = fused render(parameters); backward = the HIP kernels' gradient
+    (no activations stored: the sample kernel's backward recomputes each
+    ray's layers from its inputs)."""
+
+    @staticmethod
+    def forward(ctx, field, t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, t_thres, *params):
+        tr = field.trainer
+        out = tr.forward(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, t_thres)
+        # this render's occupancies and workspace belong to its backward: the
+        # next forward allocates its own
+        ctx.field, ctx.state = field, (tr._ctx, out)
+        tr._out = None
+        depth, aux = out["depth"].clone(), out["aux"].clone()
+        ctx.mark_non_differentiable(depth, aux)
+        return out["rgb"].clone(), depth, aux
+
+    @staticmethod
+    def backward(ctx, g_rgb, g_depth, g_aux):
+        field = ctx.field
+        if g_rgb is None:
+            return (None,) * (10 + len(field.flat_params()))
+        tr = field.trainer
+        tr._ctx, tr._out = ctx.state
+        gr = tr.backward(g_rgb.contiguous())
+        tr._out = None
+        grads = [*gr["dense_w"], *gr["dense_b"], gr["sigma_w"], gr["sigma_b"].reshape(1)]
+        grads += [st[k] for st in gr["hydra"] for k in HYDRA_KEYS]
+        grads += [x for wb in gr["rgb"] for x in wb]
+        grads = [g.reshape(p.shape) for g, p in zip(grads, field.flat_params())]
+        return (None,) * 10 + tuple(grads)

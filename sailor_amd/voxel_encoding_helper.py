@@ -6,8 +6,9 @@ Functions and stateful helpers that SRONet / SRONetUp call.
   volume_render_infer  _volume_rendering_inference (:62-80)
   volume_render        _volume_render_training (:83-127) -- forward and
                        backward run the HIP compositing kernels
-  volume_render_torch  _volume_render_torch (:130-189) -- the reference's own
-                       PyTorch formulation, kept for training-time parity
+  volume_render_torch  _volume_render_torch (:130-189) -- the compositing the
+                       reference trains with (tensor-op semantics), forward
+                       and backward on HIP kernels
   voxel_trilinear_interpolation  _voxel_tri_interpolation (:12-57) -- forward
                        and backward run the HIP trilinear kernels
 """
@@ -97,51 +98,52 @@ class _volume_render_training(Function):
 volume_render = _volume_render_training.apply
 
 
+class _volume_render_scan(Function):
+    """Compositing of the samples in the order given with the tensor
+    formulation's semantics (last density interval 0, occupancy transmittance
+    prod(1 - alpha + 1e-6), no early stop), forward and gradient on the HIP
+    kernels sailor_volume_render_torch_forward / _backward.  Sample depths
+    are constants (the reference's sampler output carries no gradient)."""
+
+    @staticmethod
+    def forward(ctx, depths, sigmas, feats, valid_rays, occ_mode):
+        if depths.requires_grad:
+            raise RuntimeError("volume_render_torch: sampled depths must not require grad")
+        t = depths.float().contiguous()
+        s = sigmas.float().reshape(t.shape).contiguous()
+        f = feats.float().contiguous()
+        vr = None
+        if valid_rays is not None:
+            vr = valid_rays.reshape(t.shape[:-1]).to(torch.uint8).contiguous()
+        out = VoxelEncoding.volume_render_torch_forward(t, s, f, vr, occ_mode)
+        ctx.occ_mode = occ_mode
+        ctx.sigma_shape = sigmas.shape
+        ctx.save_for_backward(t, s, f, vr)
+        return out
+
+    @staticmethod
+    def backward(ctx, g_feats, g_depths, g_ws, g_alphas):
+        t, s, f, vr = ctx.saved_tensors
+        gs, gf = VoxelEncoding.volume_render_torch_backward(
+            g_feats, g_depths, g_ws, g_alphas, t, s, f, vr, ctx.occ_mode)
+        gs = gs.view(ctx.sigma_shape) if ctx.needs_input_grad[1] else None
+        gf = gf if ctx.needs_input_grad[2] else None
+        return None, gs, gf, None, None
+
+
 class _volume_render_torch(nn.Module):
-    """The reference's PyTorch compositing (autograd through torch ops)."""
+    """_volume_render_torch (voxel_encoding_helper.py:130-189): same methods
+    and arguments, running the HIP compositing kernels with torch autograd
+    through _volume_render_scan.  sampled_depths [B, R, S], sigma [B, R, S, 1],
+    feats [B, R, S, F], valid_rays [B, R] or None -> (feats [B, R, F],
+    depths, ws, alphas [B, R, 1]).  max_depths is unused, as in the
+    reference."""
 
     def forward(self, sampled_depths, sigma, feats, max_depths, valid_rays=None):
-        batch_size, num_rays, num_sampled = sampled_depths.shape
-        _sd = sampled_depths.view(-1, num_sampled)
-        density = sigma.view(-1, num_sampled)
-        dists = _sd[:, 1:] - _sd[:, :-1]
-        dists = torch.cat([dists, torch.full([dists.shape[0], 1], 0.0, device=dists.device)], -1)
-        free_energy = dists * density
-        shifted = torch.cat([torch.full([dists.shape[0], 1], 0.0, device=dists.device),
-                             free_energy], dim=-1)
-        alpha = 1 - torch.exp(-free_energy)
-        transmittance = torch.exp(-torch.cumsum(shifted, dim=-1))
-        weights = (alpha * transmittance[:, :-1]).view(batch_size, num_rays, num_sampled, 1)
-        depths = torch.sum(weights * sampled_depths[..., None], -2)
-        alphas = torch.sum(weights * alpha.view(batch_size, num_rays, num_sampled, 1), -2)
-        feats = torch.sum(weights * feats, -2)
-        ws = torch.sum(weights, -2)
-        if valid_rays is not None:
-            inv = ~valid_rays[..., None].bool()
-            depths = depths.masked_fill(inv, 0)
-            feats = feats.masked_fill(inv, 0)
-            alphas = alphas.masked_fill(inv, 0)
-            ws = ws.masked_fill(inv, 0)
-        return feats, depths, ws, alphas
+        return _volume_render_scan.apply(sampled_depths, sigma, feats, valid_rays, False)
 
     def forward_occ(self, sampled_depths, sigma, feats, max_depths, valid_rays=None):
-        batch_size, num_rays, num_sampled = sampled_depths.shape
-        alpha = sigma.view(-1, num_sampled)
-        eps = 1e-6
-        trans = torch.cumprod(torch.cat([torch.ones((alpha.shape[0], 1), device=alpha.device),
-                                         1. - alpha + eps], -1), -1)[:, :-1]
-        weights = (alpha * trans).view(batch_size, num_rays, num_sampled, 1)
-        depths = torch.sum(weights * sampled_depths[..., None], -2)
-        alphas = torch.sum(weights * alpha.view(batch_size, num_rays, num_sampled, 1), -2)
-        feats = torch.sum(weights * feats, -2)
-        ws = torch.sum(weights, -2)
-        if valid_rays is not None:
-            inv = ~valid_rays[..., None].bool()
-            depths = depths.masked_fill(inv, 0)
-            feats = feats.masked_fill(inv, 0)
-            alphas = alphas.masked_fill(inv, 0)
-            ws = ws.masked_fill(inv, 0)
-        return feats, depths, ws, alphas
+        return _volume_render_scan.apply(sampled_depths, sigma, feats, valid_rays, True)
 
 
 volume_render_torch = _volume_render_torch()
@@ -174,6 +176,15 @@ class _integrate_depths(nn.Module):
             occ[i].fill_(-1)
             num[i].fill_(0)
         return occ, num
+
+    @torch.no_grad()
+    def get_target_center(self, inputs_data, batch_size=1, _th_boox=0.025):
+        """Mean of the back-projected valid depth points per batch [B, 3]
+        (voxel_encoding_helper.py:256-264)."""
+        _, depths, _, Ks, _, Rs, Ts = self.preprocess_data(batch_size, inputs_data)
+        center, num = VoxelEncoding.get_center_xyz(depths, Ks, Rs, Ts, _th_boox,
+                                                   _gpu_id(depths))
+        return center / num
 
     @torch.no_grad()
     def get_origin_bbox(self, inputs_data, batch_size=1, _th_bbox=0.025):

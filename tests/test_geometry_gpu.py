@@ -191,3 +191,58 @@ def test_run_twice_identical():
     b = run_case("small_r4")[0]
     for k in ("node_info", "node_pos", "idx", "tmin", "st", "order"):
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+def test_get_center_xyz_matches_oracle():
+    """get_center_xyz (sum of the back-projected valid points and their
+    count per batch) vs the oracle's float64 sum of the same float32 points;
+    both sums are double, so they agree to float32 rounding."""
+    from sailor_amd import VoxelEncoding as VE
+    inp = scene_inputs(0, 4, 512, 512, 600.0)
+    dep = np.concatenate([inp["depths"], inp["depths"][:, ::-1]], 0)   # B = 2
+    K = np.concatenate([inp["K"]] * 2)
+    R = np.concatenate([inp["R"]] * 2)
+    T = np.concatenate([inp["T"]] * 2)
+    c, n = VE.get_center_xyz(_t(dep), _t(K), _t(R), _t(T), 0.025, 0)
+    rc, rn = orc.center_xyz(dep, K, R, T)
+    np.testing.assert_array_equal(n.cpu().numpy().reshape(-1), rn.astype(np.float32))
+    np.testing.assert_allclose(c.cpu().numpy(), rc, rtol=2e-7, atol=1e-6)
+    assert rn.min() > 1e5
+
+
+def test_get_nodes_records():
+    """MultiLayerOctree.get_nodes: every valid record's gindex sits at its
+    vindex in its level's rewritten volume, its centre equals node_pos,
+    children point back at their parent, counts match queue_control."""
+    from sailor_amd import VoxelEncoding as VE
+    inp = scene_inputs(0, 4, 128, 128, 150.0)
+    dim, rate = 32, 4
+    occ0 = torch.full((1, dim, dim, dim), -1, dtype=torch.int32, device="cuda")
+    dc = dim // rate
+    occ1 = torch.full((1, dc, dc, dc), -1, dtype=torch.int32, device="cuda")
+    num0 = torch.zeros((1, 1), dtype=torch.int32, device="cuda")
+    num1 = torch.zeros((1, 1), dtype=torch.int32, device="cuda")
+    *_, bbox, vs = VE.integrate(occ0, num0, _t(inp["K"]), _t(inp["RT"]), _t(inp["R"]),
+                                _t(inp["T"]), _t(inp["colors"]), _t(inp["depths"]),
+                                _t(inp["masks"]), 1.0, 0.025, 3.0, 13.5, 13.5, 0)
+    VE.build_multi_res_volumes([occ0, occ1], [num0, num1], 2, rate, 0)
+    totals = [int(num0.sum()), int(num1.sum())]
+    tree = VE.MultiLayerOctree(2, 1, rate, 0)
+    tree.init_octree([occ0, occ1], totals, [num0, num1], [0, totals[0]], vs, bbox)
+    tree.build_octree()
+    nodes = tree.get_nodes()
+    assert len(nodes) == sum(totals)
+    vols = {dim: occ0.cpu().numpy().reshape(-1), dim // rate: occ1.cpu().numpy().reshape(-1)}
+    pos = tree.node_pos.cpu().numpy()
+    valid = [nd for nd in nodes if nd.valid]
+    n_coarse = sum(1 for nd in valid if nd.father_gindex < 0)
+    assert [n_coarse, len(valid) - n_coarse] == tree.get_num_nodes_valid()
+    for nd in valid:
+        assert vols[nd.vol_dim][nd.vindex] == nd.gindex
+        np.testing.assert_array_equal(np.array(nd.xyz_center_, np.float32), pos[nd.gindex, :3])
+        kids = [k for k in nd.children_gindex if k >= 0]
+        assert len(kids) == nd.valid_num_leaves
+        for k in kids:
+            assert nodes[k].father_gindex == nd.gindex and nodes[k].size == 1
+        if nd.father_gindex < 0:
+            assert nd.size == rate and nd.depth == 3

@@ -149,3 +149,22 @@ def test_split_small_inverts_small_buffer():
     for (a, b), (c, d) in zip(sp["rgb"], w.rgb):
         np.testing.assert_array_equal(a, c)
         np.testing.assert_array_equal(b, d)
+
+
+def test_device_gather_packing_matches_pack_layer():
+    """RadianceField repacks its parameters with one gather per layer
+    (train._pack_index): identical to model.pack_layer for every precision
+    and layer shape of the benchmark models."""
+    import numpy as np
+    import torch
+    from sailor_amd.model import NerfWeights, natural_first_layer, pack_layer
+    from sailor_amd.train import _pack_index
+    for F, H in ((32, 128), (64, 256)):
+        w = NerfWeights.init(F, H)
+        for prec in ("fp32", "bf16"):
+            first = natural_first_layer(w.dense_w[0], F)
+            got = torch.from_numpy(first).reshape(-1)[_pack_index(prec, H, F + 64)].numpy()
+            np.testing.assert_array_equal(got, pack_layer(first, prec).reshape(-1))
+            Wt = np.ascontiguousarray(w.dense_w[2].T)
+            got = torch.from_numpy(Wt).reshape(-1)[_pack_index(prec, H, H)].numpy()
+            np.testing.assert_array_equal(got, pack_layer(Wt, prec).reshape(-1))

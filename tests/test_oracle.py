@@ -647,3 +647,45 @@ def test_nerf_ref_encoding_and_compositing_agree_with_oracle():
     np.testing.assert_allclose(asum.numpy(), oa, rtol=1e-5, atol=1e-6)
     # depth is t composited times (d . zdir) = -1 for these rays
     np.testing.assert_allclose(dep.numpy(), -od, rtol=1e-5, atol=1e-6)
+
+
+def test_center_xyz_plane_closed_form():
+    """A fronto-parallel plane at depth d filling a W x H image seen by a
+    camera at the origin: the points' x sum is d * sum_px (px - cx) / fx
+    (closed form), y likewise, z = d per pixel; pixels outside [0.1, 4] m
+    are dropped."""
+    from oracle import oracle as orc
+    H, W, f, c, d = 6, 8, 5.0, 3.5, 2.0
+    K = np.array([[f, 0, c], [0, f, 2.5], [0, 0, 1]], np.float32).reshape(1, 1, 3, 3)
+    R = np.eye(3, dtype=np.float32).reshape(1, 1, 3, 3)
+    T = np.zeros((1, 1, 3), np.float32)
+    dep = np.full((1, 1, H, W), d, np.float32)
+    dep[0, 0, 0, 0] = 4.5                                  # too far: dropped
+    cen, n = orc.center_xyz(dep, K, R, T)
+    px, py = np.meshgrid(np.arange(W), np.arange(H))
+    keep = np.ones((H, W), bool)
+    keep[0, 0] = False
+    assert n[0] == keep.sum()
+    # float32 point arithmetic: a few ulp per point
+    np.testing.assert_allclose(cen[0, 0], d * ((px - c) / f)[keep].sum(), rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(cen[0, 1], d * ((py - 2.5) / f)[keep].sum(), rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(cen[0, 2], d * keep.sum(), rtol=1e-6)
+
+
+def test_center_xyz_matches_float64_backprojection():
+    from oracle import oracle as orc
+    from tests.helpers import scene_inputs
+    inp = scene_inputs(0, 4, 64, 64, 75.0)
+    cen, n = orc.center_xyz(inp["depths"], inp["K"], inp["R"], inp["T"])
+    d = inp["depths"][0].astype(np.float64)
+    tot, cnt = np.zeros(3), 0
+    for v in range(d.shape[0]):
+        py, px = np.nonzero((d[v] >= 0.1) & (d[v] <= 4.0))
+        z = d[v][py, px]
+        cam = np.linalg.inv(inp["K"][0, v].astype(np.float64)) @ np.stack([px, py, np.ones_like(px)])
+        cam = cam * z - inp["T"][0, v].astype(np.float64)[:, None]
+        world = np.linalg.inv(inp["R"][0, v].astype(np.float64)) @ cam
+        tot += world.sum(1)
+        cnt += z.size
+    assert n[0] == cnt > 100
+    np.testing.assert_allclose(cen[0], tot, rtol=1e-5)

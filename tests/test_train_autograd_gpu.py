@@ -1,0 +1,285 @@
+"""The differentiation form behind torch.autograd (RadianceField /
+FusedRender) on the configuration's loss: MSE between the render of the
+Xavier-initialised model and an fp32 render made with perturbed weights
+(Xavier + N(0, 0.01)), on rays of an 8-target-view training batch traced
+through the full-size tree (4 x 1024^2 views, 256^3 grid, 48 samples).
+
+Checks:
+  * autograd gradients equal RenderTrainer.backward's (same kernels) and an
+    optimizer step repacks the parameters the kernels read;
+  * every parameter tensor's gradient vs torch autograd of the float64
+    restatement oracle/nerf_ref.py with the same loss: relative L2 error
+    <= 1e-4 for fp32 on the rays where no float32 pre-activation takes the
+    other ReLU branch than in float64 (found from the kernel's per-column
+    deltas); the bf16 form's errors are measured against the 1e-2 bar and
+    reported, and must stay within 2x of float64 autograd with the same
+    bf16 rounding of the dense layers;
+  * central finite differences (h = 1e-3, float64 restatement) on 256
+    random parameters against the fp32 gradient;
+  * gradients of two backward passes are independent tensors.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import nerf_ref
+
+pytestmark = pytest.mark.gpu
+
+_cache = {}
+
+
+def batch():
+    if "tb" not in _cache:
+        from sailor_amd.pipeline import FrameConfig
+        from sailor_amd.train import TrainingBatch
+        cfg = FrameConfig("c4_auto", 4, 1024, 1200.0, 256, 8, 32, 37.5, 48, "c1")
+        print("[batch] 8 x 32^2 rays through the 256^3 tree", flush=True)
+        _cache["tb"] = TrainingBatch(cfg, n_targets=8)
+    return _cache["tb"]
+
+
+def subset_inputs(n, seed):
+    """Inputs of RadianceField for n seeded valid rays of the batch."""
+    tb = batch()
+    valid = torch.nonzero(tb.t[0, :, 0] >= 0).flatten().cpu().numpy()
+    sub = np.sort(np.random.default_rng(seed).choice(valid, size=n, replace=False))
+    idx = torch.from_numpy(sub).cuda()
+    t = tb.t[:, idx].contiguous()
+    rays = tb.rays[:, idx].contiguous()
+    p = tb.pipe
+    return sub, (t, rays, p.K, p.RT, tb.img_hw, p.fmap, p.bbox, p.zdir)
+
+
+def target_rgb(inputs):
+    """fp32 render with the perturbed weights (the configuration's target)."""
+    from sailor_amd.model import NerfWeights
+    from sailor_amd.train import RenderTrainer
+    wp = NerfWeights.init(32, 128, perturb_std=0.01)
+    out = RenderTrainer(wp, "fp32").forward(*inputs)
+    return out["rgb"].clone()
+
+
+def mse(rgb, tgt):
+    return ((rgb - tgt) ** 2).mean()
+
+
+def ref_grads(w, inputs, tgt, dtype=torch.float64):
+    """Gradients of the same MSE by torch autograd through nerf_ref."""
+    t, rays, K, RT, img_hw, fmap, bbox, zdir = inputs
+    P = nerf_ref.params_of(w, dtype=dtype, requires_grad=True)
+    fm = torch.from_numpy(batch().pipe.fmaps_np[None]).to(dtype)
+    trace = {}
+    rgb = nerf_ref.render_rays(w, t.cpu().numpy(), rays.cpu().numpy(), K.cpu().numpy(),
+                               RT.cpu().numpy(), fm, bbox.cpu().numpy(), zdir.cpu().numpy(),
+                               img_hw, dtype=dtype, params=P, trace=trace)[0]
+    mse(rgb, tgt.cpu().to(dtype)).backward()
+    flat = [*P["dense_w"], *P["dense_b"], P["sigma_w"], P["sigma_b"]]
+    flat += [st[k] for st in P["hydra"] for k in ("q", "k", "v", "fc", "w1", "b1", "w2", "b2")]
+    flat += [x for wb in P["rgb"] for x in wb]
+    return [x.grad.reshape(-1).numpy() for x in flat], trace, P
+
+
+NAMES = [f"dense_w{i}" for i in range(4)] + [f"dense_b{i}" for i in range(4)] + \
+    ["sigma_w", "sigma_b"] + [f"hydra{s}.{k}" for s in range(2)
+                              for k in ("q", "k", "v", "fc", "w1", "b1", "w2", "b2")] + \
+    [f"rgb{i}.{k}" for i in range(3) for k in ("w", "b")]
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_autograd_step_matches_trainer_and_repacks():
+    from sailor_amd.model import NerfWeights
+    from sailor_amd.train import RadianceField, RenderTrainer
+    _, inputs = subset_inputs(512, 1)
+    tgt = target_rgb(inputs)
+    w = NerfWeights.init(32, 128)
+    field = RadianceField(w, "fp32")
+    rgb, _, _ = field(*inputs)
+    mse(rgb, tgt).backward()
+    got = [p.grad.detach().reshape(-1).cpu().numpy() for p in field.flat_params()]
+    tr = RenderTrainer(w, "fp32")
+    out = tr.forward(*inputs)
+    assert torch.equal(out["rgb"], rgb.detach())
+    g = (2.0 / rgb.numel()) * (out["rgb"] - tgt)
+    gr = tr.backward(g)
+    ref = [*gr["dense_w"], *gr["dense_b"], gr["sigma_w"], gr["sigma_b"].reshape(1)]
+    ref += [st[k] for st in gr["hydra"] for k in ("q", "k", "v", "fc", "w1", "b1", "w2", "b2")]
+    ref += [x for wb in gr["rgb"] for x in wb]
+    for name, a, b in zip(NAMES, got, ref):
+        np.testing.assert_allclose(a, b.reshape(-1).cpu().numpy(), rtol=1e-5, atol=1e-9,
+                                   err_msg=name)
+    # an optimizer step changes the parameters; the next render uses them
+    opt = torch.optim.SGD(field.parameters(), lr=0.5)
+    opt.step()
+    rgb2, _, _ = field(*inputs)
+    w2 = NerfWeights.init(32, 128)
+    ps = [p.detach().cpu().numpy() for p in field.flat_params()]
+    w2.dense_w, w2.dense_b = ps[0:4], ps[4:8]
+    w2.sigma_w, w2.sigma_b = ps[8], float(ps[9][0])
+    keys = ("q", "k", "v", "fc", "w1", "b1", "w2", "b2")
+    w2.hydra = [dict(zip(keys, ps[10 + 8 * s:18 + 8 * s])) for s in range(2)]
+    w2.rgb = [(ps[26], ps[27]), (ps[28], ps[29]), (ps[30], ps[31])]
+    ref2 = RenderTrainer(w2, "fp32").forward(*inputs)["rgb"]
+    assert not torch.equal(rgb2, rgb)
+    torch.testing.assert_close(rgb2.detach(), ref2, rtol=0, atol=0)
+
+
+def test_two_backwards_give_independent_gradients():
+    from sailor_amd.model import NerfWeights
+    from sailor_amd.train import RenderTrainer
+    _, inputs = subset_inputs(256, 2)
+    tr = RenderTrainer(NerfWeights.init(32, 128), "fp32")
+    out = tr.forward(*inputs)
+    g1 = tr.backward(torch.ones_like(out["rgb"]))
+    keep = [g1["dense_w"][1].clone(), g1["hydra"][0]["q"].clone(), g1["sigma_w"].clone()]
+    g2 = tr.backward(-3.0 * torch.ones_like(out["rgb"]))
+    for a, b in zip(keep, [g1["dense_w"][1], g1["hydra"][0]["q"], g1["sigma_w"]]):
+        assert torch.equal(a, b)
+    # linear in dL/drgb up to float32 summation order (atomic accumulation)
+    for a, b in ((g2["dense_w"][1], g1["dense_w"][1]), (g2["hydra"][0]["q"], g1["hydra"][0]["q"])):
+        assert float((a + 3.0 * b).norm() / (3.0 * b).norm()) <= 1e-5
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_gradients_match_float64_autograd(prec):
+    from sailor_amd.model import NerfWeights
+    from sailor_amd.train import RadianceField
+    sub, inputs = subset_inputs(192, 3)
+    tgt = target_rgb(inputs)
+    w = NerfWeights.init(32, 128)
+    field = RadianceField(w, prec)
+    rgb, _, _ = field(*inputs)
+    mse(rgb, tgt).backward()
+    got = [p.grad.detach().reshape(-1).double().cpu().numpy() for p in field.flat_params()]
+    ref, trace, _ = ref_grads(w, inputs, tgt)
+    errs = {n: _rel(a, b) for n, a, b in zip(NAMES, got, ref)}
+    print(prec, "rel L2 vs float64 autograd:",
+          {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
+    if prec == "bf16":
+        over = {k: v for k, v in errs.items() if v > 1e-2}
+        print(f"bf16: {len(over)} of {len(errs)} tensors above the 1e-2 bar: {over}")
+        # bf16 rounding of the forward's layer inputs alone moves gradients
+        # of this loss by a few percent; the kernel has to stay within 2x of
+        # mixed-precision autograd with the same rounding
+        _, emu = _emulated(w, inputs, tgt)
+        for n, a, e, b in zip(NAMES, got, emu, ref):
+            base = _rel(e, b)
+            assert _rel(a, b) <= max(1e-2, 2 * base), (n, _rel(a, b), base)
+        return
+    # fp32: exclude the rays on which a float32 pre-activation has the
+    # other sign than the float64 one (that column takes the other ReLU
+    # branch; no precision of the sums can match it), found from the
+    # kernel's own per-column deltas (column form of the backward)
+    from sailor_amd.train import RenderTrainer, workspace_views
+    ref32, _, _ = ref_grads(w, inputs, tgt, dtype=torch.float32)
+    base = {n: _rel(a.astype(np.float64), b) for n, a, b in zip(NAMES, ref32, ref)}
+    print("float32 torch autograd rel L2 vs float64:", {k: f"{v:.1e}" for k, v in base.items()})
+    tr = RenderTrainer(w, "fp32", batch_items=1 << 20, fused=False)
+    out = tr.forward(*inputs)
+    g = (2.0 / out["rgb"].numel()) * (out["rgb"] - tgt)
+    tr.backward(g, keep_columns=True)
+    A, D = tr.columns
+    n_valid, lst, _ = workspace_views(out["workspace"], len(sub), 32)
+    assert n_valid == len(sub)
+    S = inputs[0].shape[-1]
+    pos = np.argsort(lst.cpu().numpy())             # ray -> list position
+    dpre = [torch.stack([trace["pre"][v][l].grad for v in range(4)], 2).numpy() for l in range(4)]
+    flips = np.zeros(len(sub), bool)
+    for l in range(4):
+        d = D[l].double().cpu().numpy().reshape(n_valid, S, 4, -1)[pos]
+        off = np.abs(d - dpre[l]).max(-1) > 1e-4 * np.abs(dpre[l]).max()
+        flips |= off.any((1, 2))
+    clean = ~flips
+    print(f"fp32: {flips.sum()} of {len(sub)} rays with a ReLU flip")
+    assert clean.sum() >= 0.8 * len(sub)
+    keep = torch.from_numpy(np.nonzero(clean)[0]).cuda()
+    inputs2 = (inputs[0][:, keep].contiguous(), inputs[1][:, keep].contiguous()) + inputs[2:]
+    tgt2 = tgt[:, keep].contiguous()
+    field2 = RadianceField(w, "fp32")
+    rgb2, _, _ = field2(*inputs2)
+    mse(rgb2, tgt2).backward()
+    got2 = [p.grad.detach().reshape(-1).double().cpu().numpy() for p in field2.flat_params()]
+    ref2, _, _ = ref_grads(w, inputs2, tgt2)
+    errs2 = {n: _rel(a, b) for n, a, b in zip(NAMES, got2, ref2)}
+    print("fp32 flip-free rays rel L2:", {k: f"{v:.1e}" for k, v in errs2.items()}, flush=True)
+    bad = {k: v for k, v in errs2.items() if v > 1e-4}
+    assert not bad, bad
+
+
+def _emulated(w, inputs, tgt):
+    """float64 autograd with bf16-rounded dense-layer inputs and weights."""
+    t, rays, K, RT, img_hw, fmap, bbox, zdir = inputs
+    P = nerf_ref.params_of(w, requires_grad=True)
+    fm = torch.from_numpy(batch().pipe.fmaps_np[None]).double()
+    rgb = nerf_ref.render_rays(w, t.cpu().numpy(), rays.cpu().numpy(), K.cpu().numpy(),
+                               RT.cpu().numpy(), fm, bbox.cpu().numpy(), zdir.cpu().numpy(),
+                               img_hw, params=P, mm_dtype=torch.bfloat16)[0]
+    mse(rgb, tgt.cpu().double()).backward()
+    flat = [*P["dense_w"], *P["dense_b"], P["sigma_w"], P["sigma_b"]]
+    flat += [st[k] for st in P["hydra"] for k in ("q", "k", "v", "fc", "w1", "b1", "w2", "b2")]
+    flat += [x for wb in P["rgb"] for x in wb]
+    return rgb, [x.grad.reshape(-1).numpy() for x in flat]
+
+
+def test_central_differences_on_256_parameters():
+    """(L(p + h) - L(p - h)) / 2h of the float64 restatement at h = 1e-3 on
+    256 parameters drawn across all tensors vs the fp32 gradient:
+    |fd - g| <= 1e-3 max|g| of the tensor + 1e-8.  A parameter whose +-h
+    step moves a pre-activation across zero makes the difference quotient
+    average two ReLU branches; each such parameter must agree with the
+    difference quotient at h = 1e-6 instead, and they must be few (<= 5%)."""
+    from sailor_amd.model import NerfWeights
+    from sailor_amd.train import RadianceField
+    _, inputs = subset_inputs(64, 4)
+    tgt = target_rgb(inputs)
+    w = NerfWeights.init(32, 128)
+    field = RadianceField(w, "fp32")
+    rgb, _, _ = field(*inputs)
+    mse(rgb, tgt).backward()
+    got = [p.grad.detach().reshape(-1).double().cpu().numpy() for p in field.flat_params()]
+    t, rays, K, RT, img_hw, fmap, bbox, zdir = inputs
+    args = (t.cpu().numpy(), rays.cpu().numpy(), K.cpu().numpy(), RT.cpu().numpy(),
+            torch.from_numpy(batch().pipe.fmaps_np[None]).double(), bbox.cpu().numpy(),
+            zdir.cpu().numpy(), img_hw)
+    tg = tgt.cpu().double()
+    P = nerf_ref.params_of(w)
+    # sigma_b is a 0-d tensor: perturbed through a 1-element view
+    flat = [*P["dense_w"], *P["dense_b"], P["sigma_w"], P["sigma_b"].view(1)]
+    flat += [st[k] for st in P["hydra"] for k in ("q", "k", "v", "fc", "w1", "b1", "w2", "b2")]
+    flat += [x for wb in P["rgb"] for x in wb]
+
+    def loss():
+        with torch.no_grad():
+            return float(mse(nerf_ref.render_rays(w, *args, params=P)[0], tg))
+    rng = np.random.default_rng(17)
+    sizes = np.array([x.numel() for x in flat])
+    picks = []
+    for i in range(256):          # every tensor first, then by size
+        ti = i if i < len(flat) else int(rng.choice(len(flat), p=sizes / sizes.sum()))
+        picks.append((ti, int(rng.integers(0, sizes[ti]))))
+    kinks, bad = [], []
+    for ti, j in picks:
+        x = flat[ti].view(-1)
+        orig = float(x[j])
+
+        def fd(h):
+            x[j] = orig + h
+            lp = loss()
+            x[j] = orig - h
+            lm = loss()
+            x[j] = orig
+            return (lp - lm) / (2 * h)
+        g = got[ti][j]
+        tol = 1e-3 * np.abs(got[ti]).max() + 1e-8
+        d3 = fd(1e-3)
+        if abs(d3 - g) <= tol:
+            continue
+        d6 = fd(1e-6)
+        (kinks if abs(d6 - g) <= tol else bad).append((NAMES[ti], j, d3, d6, g))
+    print(f"h=1e-3: {256 - len(kinks) - len(bad)} of 256 within bar, {len(kinks)} across a "
+          f"ReLU kink (agree at h=1e-6), {len(bad)} failing", flush=True)
+    assert not bad, bad
+    assert len(kinks) <= 0.05 * 256, kinks
