@@ -194,16 +194,34 @@ def profile_kernels(cfg):
         del p
 
 
-def precision_form(cfg, steps, warmup):
+def precision_form(cfg, steps, warmup, n_err_rays=512, n_err_voxels=200000):
     """Config-3 block: the C2 scene rendered at 1024^2 (1.05 M rays, 48
-    samples) in FP32, BF16 and FP16; frames/s and PSNR against the FP32
-    render."""
+    samples).  Forms: FP32 plain; FP32 with Kahan-compensated TSDF and
+    compositing sums; BF16 and FP16 (feature maps quantised per channel with
+    power-of-two scales from the channel's max |x|, folded into the first
+    layer; MLP on MFMA with FP32 accumulation; compensated sums).  Per form:
+    frames/s, PSNR against the FP32 render, and max-abs / RMS error against
+    FP64 of the per-sample occupancy (sigma), RGB and depth on n_err_rays
+    seeded valid rays (float64 restatement oracle/nerf_ref.py) and of the
+    TSDF on n_err_voxels seeded observed voxels (float64 restatement of the
+    fusion)."""
+    import numpy as np
     import torch
+    from oracle import nerf_ref
+    from oracle import oracle as orc
     from sailor_amd.pipeline import FramePipeline
-    out = {"workload": f"{cfg.name}: C2 scene, {cfg.target}^2 render, {cfg.S} samples/ray"}
-    ref = None
-    for prec in ("fp32", "bf16", "fp16"):
-        p = FramePipeline.synthetic(cfg, prec=prec)
+    from sailor_amd.render import render_rays
+    out = {"workload": f"{cfg.name}: C2 scene, {cfg.target}^2 render, {cfg.S} samples/ray",
+           "error_sample": f"{n_err_rays} seeded valid rays (sigma, rgb, depth), "
+                           f"{n_err_voxels} seeded observed voxels (tsdf), against float64"}
+    forms = (("fp32", dict(prec="fp32")),
+             ("fp32_compensated", dict(prec="fp32", compensated=True)),
+             ("bf16", dict(prec="bf16", compensated=True)),
+             ("fp16", dict(prec="fp16", compensated=True)))
+    ref = mask = sub = ref64 = vox = t64 = None
+    errs = {}
+    for name, kw in forms:
+        p = FramePipeline.synthetic(cfg, **kw)
         p.run()
         torch.cuda.synchronize()
         sec = time_frames(p, steps, warmup)
@@ -212,11 +230,44 @@ def precision_form(cfg, steps, warmup):
             ref = rgb
             mask = (p.sorted[0, :, 0] >= 0).cpu().numpy()
             out["valid_rays"] = int(mask.sum())
-        out[prec] = {"frames_per_s": 1.0 / sec, "ms_per_step": sec * 1e3}
-        if prec != "fp32":
-            out[prec]["psnr_vs_fp32_db"] = psnr(ref, rgb, mask)
-            out[prec]["speedup_vs_fp32"] = out["fp32"]["ms_per_step"] / (sec * 1e3)
+            valid = np.nonzero(mask)[0]
+            sub = np.sort(np.random.default_rng(33).choice(valid, n_err_rays, replace=False))
+            ref64 = nerf_ref.render_rays(p.weights, p.sorted[:, sub].cpu().numpy(),
+                                         p.rays[:, sub].cpu().numpy(), p.K.cpu().numpy(),
+                                         p.RT.cpu().numpy(), p.fmaps_np[None],
+                                         p.bbox.cpu().numpy(), p.zdir.cpu().numpy(),
+                                         (cfg.img, cfg.img))
+            seen = np.nonzero(p.weight[0].cpu().numpy().reshape(-1) > 0)[0]
+            vox = np.sort(np.random.default_rng(34).choice(seen, min(n_err_voxels, seen.size),
+                                                           replace=False))
+            sc = p.scene
+            t64, _ = orc.integrate_f64(sc.K()[None], sc.RT()[None],
+                                       None, sc.depth[None], p.bbox.cpu().numpy(),
+                                       p.vs.cpu().numpy().reshape(-1), (cfg.dim,) * 3,
+                                       voxels=vox)
+        out[name] = {"frames_per_s": 1.0 / sec, "ms_per_step": sec * 1e3}
+        if name != "fp32":
+            out[name]["psnr_vs_fp32_db"] = psnr(ref, rgb, mask)
+            out[name]["speedup_vs_fp32"] = out["fp32"]["ms_per_step"] / (sec * 1e3)
+        idx = torch.from_numpy(sub).cuda()
+        maps = p.fmap_lo if p.half_maps else p.fmap
+        o = render_rays(p.sorted[:, idx].contiguous(), p.rays[:, idx].contiguous(), p.K, p.RT,
+                        (cfg.img, cfg.img), maps, p.bbox, p.zdir, p.model, want_occ=True,
+                        compensated=p.compensated,
+                        feature_scales=p.fscale if p.quantize else None)
+        torch.cuda.synchronize()
+        tsdf = p.tsdf[0].cpu().numpy().reshape(-1)[vox].astype(np.float64)
+        row = {}
+        for key, got, want in (("sigma", o["occ"][0], ref64[4][0]), ("rgb", o["rgb"][0], ref64[0][0]),
+                               ("depth", o["depth"][0], ref64[1][0]),
+                               ("tsdf", tsdf, t64[0])):
+            g = got.double().cpu().numpy() if torch.is_tensor(got) else got
+            d = np.abs(g - want.numpy() if torch.is_tensor(want) else g - want)
+            row[key] = {"max_abs": float(d.max()), "rms": float(np.sqrt((d ** 2).mean()))}
+        errs[name] = row
         del p
+        torch.cuda.empty_cache()
+    out["error_vs_fp64"] = errs
     return out
 
 

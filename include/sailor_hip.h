@@ -61,6 +61,22 @@ int sailor_integrate(int32_t* occ_vol, int32_t* num_occ, const float* K,
                      float* color, float* vol_origin, float* vol_bbox,
                      float* voxel_size, void* stream);
 
+/* sailor_integrate with the TSDF formed as a Kahan-compensated sum of
+ * obs_weight * dist over the observing views divided once by the weight sum
+ * (precision form; no counterpart in the reference, whose running average
+ * (tsdf w_old + obs_weight dist) / w_new has the same value in exact
+ * arithmetic).  Same arguments; the occupancy test is applied to the
+ * compensated value, so a voxel on the threshold may differ. */
+int sailor_integrate_compensated(int32_t* occ_vol, int32_t* num_occ, const float* K,
+                                 const float* RT, const float* R, const float* T,
+                                 const float* colors, const float* depths,
+                                 const int32_t* masks, int B, int V, int H, int W, int X,
+                                 int Y, int Z, float obs_weight, float th_bbox,
+                                 float trunc_weight, float tsdf_th_low, float tsdf_th_high,
+                                 const float* fixed_bbox, float* tsdf, float* weight,
+                                 float* color, float* vol_origin, float* vol_bbox,
+                                 float* voxel_size, void* stream);
+
 /* Replaces get_origin_bbox (fusion_kernel.cu:473-534).
  *  depths f32 [B,V,H,W], K/R f32 [B,V,3,3], T f32 [B,V,3] -> vol_origin [B,3,2] */
 int sailor_get_origin_bbox(const float* depths, const float* K, const float* R,
@@ -389,8 +405,11 @@ size_t sailor_render_workspace_bytes_views(int B, int R, int F, int V);
  *  zdir     f32 [B,3] third row of the target rotation (depth = t * zdir.dir)
  *  precision 0 = fp32 (MFMA f32), 1 = bf16, 2 = fp16 (MFMA, fp32 accumulate)
  *  w_dense  packed dense layers (sailor_amd/model.py), b_dense f32 [4,H]
- *  early_exit  nonzero: stop evaluating a ray once its transmittance is
- *              below t_thresh (outputs are unchanged)
+ *  early_exit  flags: bit 0 -- stop evaluating a ray once its transmittance
+ *              is below t_thresh (outputs are unchanged); bit 1 -- Kahan-
+ *              compensated compositing sums (depth, weight and alpha sums,
+ *              per-view tokens; precision form, no counterpart in the
+ *              reference)
  *  chunk_samples  samples evaluated together per workgroup step (8, 16, 24
  *              or 48; 0 = fp32: 24, 16-bit: 16, or the largest of 16 and 8
  *              that divides S; with more than 4 views: fp32 12 (8 when S
@@ -433,6 +452,43 @@ int sailor_render_rays_stages(int stages, const float* t, const float* rays, int
                               int chunk_samples, float* out_rgb, float* out_depth,
                               float* out_aux, float* out_occ, void* workspace,
                               size_t workspace_bytes, void* stream);
+
+/* Per-channel input quantisation of the precision form (no counterpart in
+ * the reference): x f32 [rows, C] channels-last -> out [rows, C] in bf16
+ * (precision 1) or fp16 (precision 2) holding x / s_c, scales f32 [C] with
+ * s_c the smallest power of two >= max |x[:, c]| (1 for an all-zero
+ * channel).  C must divide 256 or be a multiple of it; workspace: 4 C
+ * bytes.  No memsets (graph-capturable). */
+int sailor_quantize_channels(const float* x, long long rows, int C, int precision, void* out,
+                             float* scales, void* workspace, size_t workspace_bytes,
+                             void* stream);
+
+/* Folds per-channel scales into a packed layer: out[i] = w[i] * scales[k]
+ * if k = kidx[i] < F else w[i], converted to bf16 (precision 1) or fp16
+ * (precision 2).  w f32 [n] (the packed layer in fp32), kidx int16 [n] (the
+ * input index of every packed slot).  Used on the first dense layer, whose
+ * inputs k < F are the feature channels quantised by
+ * sailor_quantize_channels. */
+int sailor_scale_packed_layer(const float* w, const int16_t* kidx, long long n, int F,
+                              const float* scales, int precision, void* out, void* stream);
+
+/* sailor_render_rays_stages on maps quantised by sailor_quantize_channels
+ * (precision form, no counterpart in the reference): fmap holds x / s_c in
+ * the form's 16-bit type, feature_scales f32 [F] the scales s_c, and
+ * w_dense's first layer must have its feature inputs multiplied by s_c
+ * (sailor_scale_packed_layer); the per-view tokens are scaled back by s_c.
+ * flags as early_exit of sailor_render_rays (bit 0 early exit, bit 1
+ * compensated sums).  With power-of-two scales every product and sum equals
+ * the unquantised 16-bit form's for inputs within its range. */
+int sailor_render_rays_quantized(int stages, const float* t, const float* rays, int B, int R,
+                                 int S, const float* Kv, const float* RTv, int V, int img_h,
+                                 int img_w, const void* fmap, int Hf, int Wf, int F,
+                                 const float* feature_scales, const float* bbox,
+                                 const float* zdir, int H, int precision, const void* w_dense,
+                                 const float* b_dense, const float* w_small, float t_thresh,
+                                 int flags, int chunk_samples, float* out_rgb, float* out_depth,
+                                 float* out_aux, float* out_occ, void* workspace,
+                                 size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Differentiation of the fused render                                       */

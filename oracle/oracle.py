@@ -373,19 +373,27 @@ def freq_bwd(gy, y, D, deg):
 
 # --------------------------------------------------------------------------
 # float64 accuracy oracles
-def integrate_f64(K, RT, colors, depths, bbox, vs, dims, obs_weight=1.0, trunc_weight=3.0):
+def integrate_f64(K, RT, colors, depths, bbox, vs, dims, obs_weight=1.0, trunc_weight=3.0,
+                  voxels=None):
     """float64 restatement of integrate_cuda's TSDF/weight (colour omitted);
-    returns (tsdf, weight) in float64 for error bounds."""
+    returns (tsdf, weight) in float64 for error bounds.  With `voxels` (flat
+    raster indices of one batch's grid) only those voxels are evaluated and
+    the arrays are [B, len(voxels)]."""
     K = np.asarray(K, np.float64)
     RT = np.asarray(RT, np.float64)
     depths = np.asarray(depths, np.float64)
     B, V, H, W = depths.shape
     X, Y, Z = dims
-    tsdf = np.ones((B, X, Y, Z))
-    weight = np.zeros((B, X, Y, Z))
+    shape = (B, X, Y, Z) if voxels is None else (B, len(voxels))
+    tsdf = np.ones(shape)
+    weight = np.zeros(shape)
     for b in range(B):
         trunc = float(vs[b]) * trunc_weight
-        gx, gy, gz = np.meshgrid(np.arange(X), np.arange(Y), np.arange(Z), indexing="ij")
+        if voxels is None:
+            gx, gy, gz = np.meshgrid(np.arange(X), np.arange(Y), np.arange(Z), indexing="ij")
+        else:
+            v = np.asarray(voxels, np.int64)
+            gx, gy, gz = v // (Y * Z), (v // Z) % Y, v % Z
         pts = np.stack([bbox[b, 0, 0] + gx * float(vs[b]), bbox[b, 1, 0] + gy * float(vs[b]),
                         bbox[b, 2, 0] + gz * float(vs[b])], -1).reshape(-1, 3)
         ts = tsdf[b].reshape(-1)
@@ -410,8 +418,8 @@ def integrate_f64(K, RT, colors, depths, bbox, vs, dims, obs_weight=1.0, trunc_w
             ws = np.where(ok, ws + obs_weight, ws)
             with np.errstate(divide="ignore", invalid="ignore"):
                 ts = np.where(ok, (ts * w_old + obs_weight * dist) / ws, ts)
-        tsdf[b] = ts.reshape(X, Y, Z)
-        weight[b] = ws.reshape(X, Y, Z)
+        tsdf[b] = ts.reshape(shape[1:])
+        weight[b] = ws.reshape(shape[1:])
     return tsdf, weight
 
 

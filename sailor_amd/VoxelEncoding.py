@@ -38,13 +38,15 @@ def _dev(device):
 # TSDF fusion
 def integrate(occupied_vol, num_occ_voxel, cam_intr, cam_pose, cam_R, cam_T, color_ims,
               depth_ims, mask_ims, obs_weight, th_bbox, trunc_weight, tsdf_th_low,
-              tsdf_th_high, device, fixed_bbox=None):
+              tsdf_th_high, device, fixed_bbox=None, compensated=False):
     """fusion_cuda_integrate (fusion_kernel.cu:600-727).
 
     Returns (tsdf, weight, color, vol_origin[B,3,2], vol_bbox[B,3,2],
     voxel_size[B,1]); fills `occupied_vol` with v_id/-1 and adds the number of
     occupied voxels to `num_occ_voxel`.  `fixed_bbox` ([B,3,2], optional,
-    not in the reference) replaces the adaptive depth-derived box.
+    not in the reference) replaces the adaptive depth-derived box;
+    `compensated` (not in the reference) forms the TSDF as a
+    Kahan-compensated sum divided once by the weight.
     """
     dev = occupied_vol.device
     check_input(occupied_vol, "occupied_vol", _i32)
@@ -73,7 +75,8 @@ def integrate(occupied_vol, num_occ_voxel, cam_intr, cam_pose, cam_R, cam_T, col
     vol_bbox = torch.empty((B, 3, 2), **opts)
     voxel_size = torch.empty((B, 1), **opts)
     with torch.cuda.device(dev):
-        call("sailor_integrate", ptr(occupied_vol), ptr(num_occ_voxel), ptr(K), ptr(RT),
+        call("sailor_integrate_compensated" if compensated else "sailor_integrate",
+             ptr(occupied_vol), ptr(num_occ_voxel), ptr(K), ptr(RT),
              ptr(R), ptr(T), ptr(colors), ptr(depths), ptr(masks), B, V, H, W, X, Y, Z,
              float(obs_weight), float(th_bbox), float(trunc_weight), float(tsdf_th_low),
              float(tsdf_th_high), ptr(fb), ptr(tsdf), ptr(weight), ptr(color),

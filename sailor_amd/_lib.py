@@ -24,6 +24,7 @@ _SIGS = {
     "sailor_last_error": [],
     "sailor_abi_version": [],
     "sailor_integrate": [P] * 9 + [I] * 7 + [F] * 5 + [P] * 8,
+    "sailor_integrate_compensated": [P] * 9 + [I] * 7 + [F] * 5 + [P] * 8,
     "sailor_get_origin_bbox": [P] * 4 + [I] * 4 + [F, P, P],
     "sailor_get_center_xyz": [P, P, P, P, I, I, I, I, P, P, P, SZ, P],
     "sailor_build_multi_res_volumes": [P, P, I, I, I, I, P],
@@ -41,6 +42,8 @@ _SIGS = {
     "sailor_volume_rendering_training_backward": [P] * 12 + [I, I, I, I, F, I, P, P, P],
     "sailor_volume_render_torch_forward": [P, P, P, P, LL, I, I, I, I, P, P, P, P, P],
     "sailor_volume_render_torch_backward": [P] * 8 + [LL, I, I, I, P, P, P],
+    "sailor_quantize_channels": [P, LL, I, I, P, P, P, SZ, P],
+    "sailor_scale_packed_layer": [P, P, LL, I, P, I, P, P],
     "sailor_render_small_weights_floats": [I, I],
     "sailor_render_rays": [P, P, I, I, I, P, P, I, I, I, P, I, I, I, I, P, P, I, I, P, P, P, F,
                            I, I, P, P, P, P, P, SZ, P],
@@ -56,6 +59,8 @@ _SIGS = {
     "sailor_density_points": [P, I, I, P, P, I, I, I, P, I, I, I, I, P, I, I, P, P, P, I, P, P],
     "sailor_render_rays_stages": [I, P, P, I, I, I, P, P, I, I, I, P, I, I, I, I, P, P, I, I, P,
                                   P, P, F, I, I, P, P, P, P, P, SZ, P],
+    "sailor_render_rays_quantized": [I, P, P, I, I, I, P, P, I, I, I, P, I, I, I, P, P, P, I, I,
+                                     P, P, P, F, I, I, P, P, P, P, P, SZ, P],
     "sailor_depth_normalize": [P, I, I, I, I, F, F, F, I, I, P, P, P, P, P, SZ, P],
     "sailor_rays_selecting": [P, I, I, I, I, P, I, P, P],
     "sailor_rgbdv_selecting": [P, I, P, P, I, I, I, I, P, I, P, P, P, P, P],

@@ -173,7 +173,13 @@ __global__ void keys_to_float(float* buf, int n) {
   buf[i] = sailor::fkey_inv(reinterpret_cast<const int*>(buf)[i]);
 }
 
-// One lane per voxel; grid = (ceil(X*Y*Z / 256), B).
+// One lane per voxel; grid = (ceil(X*Y*Z / 256), B).  COMP (precision form):
+// the TSDF is formed as the Kahan-compensated sum of obs_weight * dist over
+// the observing views divided once by the weight sum, instead of the
+// reference's running average (tsdf * w_old + obs_weight * dist) / w_new --
+// the same value in exact arithmetic, with one rounding of the sum instead
+// of one per view.
+template <bool COMP>
 __global__ void __launch_bounds__(kThreads)
 integrate_kernel(float* __restrict__ tsdf_vol, float* __restrict__ weight_vol,
                  float* __restrict__ color_vol, int* __restrict__ occ_vol,
@@ -202,6 +208,7 @@ integrate_kernel(float* __restrict__ tsdf_vol, float* __restrict__ weight_vol,
     float pt_y = bb[2] + (float)vy * vs;
     float pt_z = bb[4] + (float)vz * vs;
     float tsdf = 1.0f, w = 0.0f, col = 0.0f;
+    float ksum = 0.0f, kcomp = 0.0f;  // COMP: sum of obs_weight * dist, lost low bits
     const size_t img = (size_t)H * W;
     for (int k = 0; k < V; ++k) {
       const float* rt = RT + ((size_t)b * V + k) * 12;
@@ -226,7 +233,14 @@ integrate_kernel(float* __restrict__ tsdf_vol, float* __restrict__ weight_vol,
       float w_old = w;
       float w_new = w_old + obs_weight;
       w = w_new;
-      tsdf = (tsdf * w_old + obs_weight * dist) / w_new;
+      if constexpr (COMP) {
+        const float y = obs_weight * dist - kcomp;
+        const float t = ksum + y;
+        kcomp = (t - ksum) - y;
+        ksum = t;
+      } else {
+        tsdf = (tsdf * w_old + obs_weight * dist) / w_new;
+      }
       // packed colour running average (fusion_kernel.cu:320-331)
       float old_b = floorf(col / 65536.0f);
       float old_g = floorf((col - old_b * 256.0f * 256.0f) / 256.0f);
@@ -239,6 +253,9 @@ integrate_kernel(float* __restrict__ tsdf_vol, float* __restrict__ weight_vol,
       new_g = fminf(roundf((old_g * w_old + obs_weight * new_g) / w_new), 255.0f);
       new_r = fminf(roundf((old_r * w_old + obs_weight * new_r) / w_new), 255.0f);
       col = new_b * 256.0f * 256.0f + new_g * 256.0f + new_r;
+    }
+    if constexpr (COMP) {
+      if (w > 0.0f) tsdf = (ksum - kcomp) / w;
     }
     occupied = tsdf < trunc * th_high && tsdf > -trunc * th_low;
     size_t o = (size_t)b * nvox + vid;
@@ -351,15 +368,15 @@ int launch_bbox(const float* depths, const float* K, const float* R,
 
 }  // namespace
 
-extern "C" int sailor_integrate(
-    int32_t* occ_vol, int32_t* num_occ, const float* K, const float* RT,
-    const float* R, const float* T, const float* colors, const float* depths,
-    const int32_t* masks, int B, int V, int H, int W, int X, int Y, int Z,
-    float obs_weight, float th_bbox, float trunc_weight, float tsdf_th_low,
-    float tsdf_th_high, const float* fixed_bbox, float* tsdf, float* weight,
-    float* color, float* vol_origin, float* vol_bbox, float* voxel_size,
-    void* stream) {
-  (void)masks;  // read by nothing in the reference kernel either (:316-317)
+namespace {
+
+int integrate_impl(bool comp, int32_t* occ_vol, int32_t* num_occ, const float* K,
+                   const float* RT, const float* R, const float* T, const float* colors,
+                   const float* depths, int B, int V, int H, int W, int X, int Y, int Z,
+                   float obs_weight, float th_bbox, float trunc_weight, float tsdf_th_low,
+                   float tsdf_th_high, const float* fixed_bbox, float* tsdf, float* weight,
+                   float* color, float* vol_origin, float* vol_bbox, float* voxel_size,
+                   void* stream) {
   SAILOR_REQUIRE(B > 0 && V > 0 && H > 0 && W > 0, "integrate: empty input");
   SAILOR_REQUIRE(V <= kMaxViews, "integrate: at most %d views", kMaxViews);
   SAILOR_REQUIRE(X > 1 && Y > 0 && Z > 0, "integrate: bad volume dims");
@@ -373,12 +390,45 @@ extern "C" int sailor_integrate(
   hipLaunchKernelGGL(bbox_finalize, dim3(sailor::ceil_div(B, 64)), dim3(64), 0,
                      st, vol_bbox, B, X, fixed_bbox, vol_origin, voxel_size);
   const int nvox = X * Y * Z;
-  hipLaunchKernelGGL(integrate_kernel, dim3(sailor::ceil_div(nvox, kThreads), B),
+  hipLaunchKernelGGL(comp ? integrate_kernel<true> : integrate_kernel<false>,
+                     dim3(sailor::ceil_div(nvox, kThreads), B),
                      dim3(kThreads), 0, st, tsdf, weight, color, occ_vol,
                      num_occ, vol_bbox, K, RT, colors, depths, voxel_size,
                      obs_weight, H, W, V, X, Y, Z, trunc_weight, tsdf_th_low,
                      tsdf_th_high);
   return sailor::check_launch("integrate");
+}
+
+}  // namespace
+
+extern "C" int sailor_integrate(
+    int32_t* occ_vol, int32_t* num_occ, const float* K, const float* RT,
+    const float* R, const float* T, const float* colors, const float* depths,
+    const int32_t* masks, int B, int V, int H, int W, int X, int Y, int Z,
+    float obs_weight, float th_bbox, float trunc_weight, float tsdf_th_low,
+    float tsdf_th_high, const float* fixed_bbox, float* tsdf, float* weight,
+    float* color, float* vol_origin, float* vol_bbox, float* voxel_size,
+    void* stream) {
+  (void)masks;  // read by nothing in the reference kernel either (:316-317)
+  return integrate_impl(false, occ_vol, num_occ, K, RT, R, T, colors, depths, B, V, H, W, X, Y,
+                        Z, obs_weight, th_bbox, trunc_weight, tsdf_th_low, tsdf_th_high,
+                        fixed_bbox, tsdf, weight, color, vol_origin, vol_bbox, voxel_size,
+                        stream);
+}
+
+extern "C" int sailor_integrate_compensated(
+    int32_t* occ_vol, int32_t* num_occ, const float* K, const float* RT,
+    const float* R, const float* T, const float* colors, const float* depths,
+    const int32_t* masks, int B, int V, int H, int W, int X, int Y, int Z,
+    float obs_weight, float th_bbox, float trunc_weight, float tsdf_th_low,
+    float tsdf_th_high, const float* fixed_bbox, float* tsdf, float* weight,
+    float* color, float* vol_origin, float* vol_bbox, float* voxel_size,
+    void* stream) {
+  (void)masks;
+  return integrate_impl(true, occ_vol, num_occ, K, RT, R, T, colors, depths, B, V, H, W, X, Y,
+                        Z, obs_weight, th_bbox, trunc_weight, tsdf_th_low, tsdf_th_high,
+                        fixed_bbox, tsdf, weight, color, vol_origin, vol_bbox, voxel_size,
+                        stream);
 }
 
 extern "C" int sailor_get_origin_bbox(const float* depths, const float* K,

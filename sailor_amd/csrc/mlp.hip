@@ -88,6 +88,8 @@ struct RenderArgs {
   int S, V, img_w, img_h, Hf, Wf;
   float t_thres;
   int early_exit;
+  int compensated;         // Kahan-compensated compositing sums (precision form)
+  const float* fscale;     // [F] per-channel scales of quantised maps (stored x / s) or null
   int stages;              // 1: compaction, 2: sample kernel, 4: Hydra/RGB kernel
   // point mode (density query): points [B, n_points, 3]; groups of NS points
   // take the place of rays and out_occ receives one density per point
@@ -114,6 +116,16 @@ template <> struct Tr<kF16> {
 };
 
 __device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+// One Kahan step: sum += x with the rounding error kept in comp (negated);
+// the compensated value is sum - comp.  (No fast-math here: the compiler
+// does not reassociate (t - sum) - y, and it contains no product to fuse.)
+__device__ __forceinline__ void kahan_add(float& sum, float& comp, float x) {
+  const float y = x - comp;
+  const float t = sum + y;
+  comp = (t - sum) - y;
+  sum = t;
+}
 
 // Element offset of 16-byte chunk `chunk` of column `col` in the swizzled
 // activation layout.
@@ -636,10 +648,11 @@ render_kernel(RenderArgs a) {
                 ihz = 2.0f / (bb[5] - bb[4]);
     // compositing state lives in thread 0; tokens: thread (v, c) = (tid / F, tid % F)
     float T = 1.0f, dsum = 0.0f, wsum = 0.0f, asum = 0.0f;
+    float dcmp = 0.0f, wcmp = 0.0f, acmp = 0.0f;  // compensated form: lost low bits
     bool done = false;
-    float tok[TE];
+    float tok[TE], tcmp[TE];
 #pragma unroll
-    for (int j = 0; j < TE; ++j) tok[j] = 0.0f;
+    for (int j = 0; j < TE; ++j) tok[j] = tcmp[j] = 0.0f;
     if (tid == 0) s_flag[0] = 0;
 
     for (int c0 = 0; c0 < S; c0 += NS) {
@@ -786,9 +799,15 @@ render_kernel(RenderArgs a) {
           if (!done) {
             const float al = s_alpha[sl];
             w = al * T;
-            dsum += w * s_t[sl];
-            wsum += w;
-            asum += w * al;
+            if (a.compensated) {
+              kahan_add(dsum, dcmp, w * s_t[sl]);
+              kahan_add(wsum, wcmp, w);
+              kahan_add(asum, acmp, w * al);
+            } else {
+              dsum += w * s_t[sl];
+              wsum += w;
+              asum += w * al;
+            }
             T *= 1.0f - al;
             if (T < a.t_thres) done = true;
           }
@@ -803,8 +822,13 @@ render_kernel(RenderArgs a) {
         const int e = tid + j * kThreads;
         if (e < VP * F) {
           const int v = e / F, c = e % F;
+          if (a.compensated) {
+            for (int sl = 0; sl < NS; ++sl)
+              kahan_add(tok[j], tcmp[j], s_w[sl] * keep[(sl * VP + v) * KF + c]);
+          } else {
 #pragma unroll 4
-          for (int sl = 0; sl < NS; ++sl) tok[j] += s_w[sl] * keep[(sl * VP + v) * KF + c];
+            for (int sl = 0; sl < NS; ++sl) tok[j] += s_w[sl] * keep[(sl * VP + v) * KF + c];
+          }
         }
       }
     }
@@ -814,14 +838,19 @@ render_kernel(RenderArgs a) {
 #pragma unroll
       for (int j = 0; j < TE; ++j) {
         const int e = tid + j * kThreads;
-        if (e < VP * F) a.out_tok[item * VP * F + e] = (e / F) < V ? tok[j] : 0.0f;
+        if (e < VP * F) {
+          // quantised maps hold x / s_c: a power-of-two scale restores the
+          // token exactly
+          const float sc = a.fscale ? a.fscale[e % F] : 1.0f;
+          a.out_tok[item * VP * F + e] = (e / F) < V ? (tok[j] - tcmp[j]) * sc : 0.0f;
+        }
       }
       if (tid == 0) {
         const float* zd = a.zdir + b * 3;
-        a.out_depth[ray] = dsum * (zd[0] * dx + zd[1] * dy + zd[2] * dz);
+        a.out_depth[ray] = (dsum - dcmp) * (zd[0] * dx + zd[1] * dy + zd[2] * dz);
         if (a.out_aux) {
-          a.out_aux[ray * 2 + 0] = wsum;
-          a.out_aux[ray * 2 + 1] = asum;
+          a.out_aux[ray * 2 + 0] = wsum - wcmp;
+          a.out_aux[ray * 2 + 1] = asum - acmp;
         }
       }
     }
@@ -2282,7 +2311,8 @@ int render_stages(int stages, const float* t, const float* rays, int B, int R, i
                   const void* w_dense,
                   const float* b_dense, const float* w_small, float t_thresh, int early_exit,
                   int chunk_samples, float* out_rgb, float* out_depth, float* out_aux,
-                  float* out_occ, void* workspace, size_t workspace_bytes, void* stream) {
+                  float* out_occ, void* workspace, size_t workspace_bytes, void* stream,
+                  const float* feature_scales = nullptr) {
   SAILOR_REQUIRE(B > 0 && R >= 0 && S > 0, "render_rays: bad sizes");
   SAILOR_REQUIRE((long long)B * R < (1ll << 31), "render_rays: at most 2^31-1 rays per call");
   SAILOR_REQUIRE(V >= 1 && V <= kMaxViews, "render_rays: 1 to %d views supported", kMaxViews);
@@ -2317,7 +2347,10 @@ int render_stages(int stages, const float* t, const float* rays, int B, int R, i
   a.list = list;
   a.n_valid = counter;
   a.S = S; a.V = V; a.img_w = img_w; a.img_h = img_h; a.Hf = Hf; a.Wf = Wf;
-  a.t_thres = t_thresh; a.early_exit = early_exit;
+  a.t_thres = t_thresh; a.early_exit = early_exit & 1; a.compensated = (early_exit >> 1) & 1;
+  a.fscale = feature_scales;
+  SAILOR_REQUIRE(feature_scales == nullptr || fmap_precision == precision,
+                 "render_rays: feature scales apply to quantised 16-bit maps only");
   a.stages = stages;
   hipStream_t st = sailor::as_stream(stream);
   SAILOR_REQUIRE(stages >= 1 && stages <= 7, "render_rays: stage mask %d out of range", stages);
@@ -2660,4 +2693,19 @@ extern "C" int sailor_render_rays_stages(int stages, const float* t, const float
                        fmap_precision, Hf, Wf, F, bbox, zdir, H, precision, w_dense, b_dense,
                        w_small, t_thresh, early_exit, chunk_samples, out_rgb, out_depth,
                        out_aux, out_occ, workspace, workspace_bytes, stream);
+}
+
+extern "C" int sailor_render_rays_quantized(
+    int stages, const float* t, const float* rays, int B, int R, int S, const float* Kv,
+    const float* RTv, int V, int img_h, int img_w, const void* fmap, int Hf, int Wf, int F,
+    const float* feature_scales, const float* bbox, const float* zdir, int H, int precision,
+    const void* w_dense, const float* b_dense, const float* w_small, float t_thresh, int flags,
+    int chunk_samples, float* out_rgb, float* out_depth, float* out_aux, float* out_occ,
+    void* workspace, size_t workspace_bytes, void* stream) {
+  SAILOR_REQUIRE(precision != kF32, "render_rays_quantized: a 16-bit form is required");
+  SAILOR_REQUIRE(feature_scales != nullptr, "render_rays_quantized: feature scales missing");
+  return render_stages(stages, t, rays, B, R, S, Kv, RTv, V, img_h, img_w, fmap, precision, Hf,
+                       Wf, F, bbox, zdir, H, precision, w_dense, b_dense, w_small, t_thresh,
+                       flags, chunk_samples, out_rgb, out_depth, out_aux, out_occ, workspace,
+                       workspace_bytes, stream, feature_scales);
 }

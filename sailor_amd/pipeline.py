@@ -58,7 +58,12 @@ CONFIGS = {
 
 class FramePipeline:
     def __init__(self, cfg: FrameConfig, prec="fp32", device="cuda", early_exit=False,
-                 weights: NerfWeights = None, chunk_samples=0, half_maps=None):
+                 weights: NerfWeights = None, chunk_samples=0, half_maps=None, quantize=None,
+                 compensated=False):
+        """prec: "fp32" / "bf16" / "fp16" (MLP arithmetic).  16-bit forms read
+        16-bit feature maps (half_maps), by default quantised per channel
+        with power-of-two scales folded into the first layer (quantize);
+        compensated: Kahan-compensated TSDF and compositing sums."""
         self.cfg = cfg
         self.chunk_samples = chunk_samples
         self.dev = torch.device(device)
@@ -89,6 +94,14 @@ class FramePipeline:
         if self.half_maps:
             dt = torch.bfloat16 if prec == "bf16" else torch.float16
             self.fmap_lo = torch.empty((1, V, mc.Hf, mc.Wf, mc.F), dtype=dt, **o)
+        self.quantize = self.half_maps if quantize is None else bool(quantize)
+        if self.quantize and not self.half_maps:
+            raise RuntimeError("quantised feature maps need a 16-bit form")
+        self.compensated = bool(compensated)
+        if self.quantize:
+            self.fscale = torch.ones(mc.F, dtype=_f32, **o)
+            self.q_ws = torch.empty(4 * mc.F, dtype=torch.uint8, **o)
+            self.w0_f32, self.w0_kidx = self.model.first_layer()
         self.occ0 = torch.empty((1, d, d, d), dtype=_i32, **o)
         self.occ1 = torch.empty((1, dc, dc, dc), dtype=_i32, **o)
         self.num = torch.zeros((2, 1), dtype=_i32, **o)
@@ -166,10 +179,19 @@ class FramePipeline:
         cfg, V, n = self.cfg, self.cfg.n_views, self.cfg.img
         d, dc = cfg.dim, self.dc
         st = stream_of(self.dev)
-        if self.fmap_lo is not None:
+        if self.quantize:
+            from .model import PREC_IDS
+            pid = PREC_IDS[self.prec]
+            mc = self.mc
+            call("sailor_quantize_channels", ptr(self.fmap), self.fmap.numel() // mc.F, mc.F,
+                 pid, ptr(self.fmap_lo), ptr(self.fscale), ptr(self.q_ws), self.q_ws.numel(), st)
+            call("sailor_scale_packed_layer", ptr(self.w0_f32), ptr(self.w0_kidx),
+                 self.w0_f32.numel(), mc.F, ptr(self.fscale), pid, ptr(self.model.w_dense), st)
+        elif self.fmap_lo is not None:
             self.fmap_lo.copy_(self.fmap)
         self.num.zero_()
-        call("sailor_integrate", ptr(self.occ0), ptr(self.num[0]), ptr(self.K), ptr(self.RT),
+        call("sailor_integrate_compensated" if self.compensated else "sailor_integrate",
+             ptr(self.occ0), ptr(self.num[0]), ptr(self.K), ptr(self.RT),
              ptr(self.R), ptr(self.T), ptr(self.colors), ptr(self.depths), ptr(self.masks), 1,
              V, n, n, d, d, d, 1.0, 0.025, cfg.trunc_weight, cfg.th_low, cfg.th_high,
              ptr(self.fixed_bbox), ptr(self.tsdf), ptr(self.weight), ptr(self.color),
@@ -198,12 +220,22 @@ class FramePipeline:
         `stages` selects launches of sailor_render_rays_stages (7 = all)."""
         cfg, mc, m = self.cfg, self.mc, self.model
         from .model import PREC_IDS
+        flags = int(self.early_exit) | (2 if self.compensated else 0)
+        if self.quantize:
+            call("sailor_render_rays_quantized", int(stages), ptr(self.sorted), ptr(self.rays), 1,
+                 cfg.target * cfg.target, cfg.S, ptr(self.K), ptr(self.RT), cfg.n_views, cfg.img,
+                 cfg.img, ptr(self.fmap_lo), mc.Hf, mc.Wf, mc.F, ptr(self.fscale), ptr(self.bbox),
+                 ptr(self.zdir), mc.H, PREC_IDS[m.prec], ptr(m.w_dense), ptr(m.b_dense),
+                 ptr(m.w_small), 1e-4, flags, int(self.chunk_samples), ptr(self.rgb),
+                 ptr(self.depth), ptr(self.aux), None, ptr(self.render_ws),
+                 self.render_ws.numel(), stream_of(self.dev))
+            return
         call("sailor_render_rays_stages", int(stages), ptr(self.sorted), ptr(self.rays), 1,
              cfg.target * cfg.target, cfg.S, ptr(self.K), ptr(self.RT), cfg.n_views, cfg.img,
              cfg.img, ptr(self.fmap_lo if self.half_maps else self.fmap),
              PREC_IDS[m.prec] if self.half_maps else 0, mc.Hf, mc.Wf, mc.F, ptr(self.bbox), ptr(self.zdir), mc.H,
              PREC_IDS[m.prec], ptr(m.w_dense), ptr(m.b_dense), ptr(m.w_small), 1e-4,
-             int(self.early_exit), int(self.chunk_samples), ptr(self.rgb), ptr(self.depth),
+             flags, int(self.chunk_samples), ptr(self.rgb), ptr(self.depth),
              ptr(self.aux), None, ptr(self.render_ws), self.render_ws.numel(),
              stream_of(self.dev))
 

@@ -77,18 +77,40 @@ class PackedModel:
 
     def __init__(self, weights, prec="fp32", device="cuda"):
         self.F, self.H, self.prec = weights.F, weights.H, prec
+        self.weights, self.device = weights, device
         self.w_dense, self.b_dense, self.w_small = weights.pack(prec, device)
         want = _lib.lib().sailor_render_small_weights_floats(self.F, self.H)
         if self.w_small.numel() != want:
             raise RuntimeError(f"small-weight buffer has {self.w_small.numel()} floats, "
                                f"kernel layout expects {want}")
+        self._first = None
+
+    def first_layer(self):
+        """(packed first layer in fp32 [H * (F + 64)], int16 input index of
+        every packed slot) for folding per-channel feature scales into it
+        (sailor_scale_packed_layer); the first layer is the head of w_dense."""
+        if self._first is None:
+            import numpy as np
+            from .model import kmap, natural_first_layer, pack_layer, unit_rows
+            K0 = self.F + 64
+            w0 = pack_layer(natural_first_layer(self.weights.dense_w[0], self.F), self.prec)
+            rows, cols = np.broadcast_arrays(unit_rows(self.prec, self.H),
+                                             kmap(self.prec, K0)[None, :, :, None, :])
+            self._first = (torch.from_numpy(np.ascontiguousarray(w0, np.float32).reshape(-1))
+                           .to(self.device),
+                           torch.from_numpy(cols.reshape(-1).astype(np.int16)).to(self.device))
+        return self._first
 
 
 def render_rays(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, model: PackedModel,
-                t_thres=1e-4, early_exit=False, want_occ=False, out=None, chunk_samples=0):
+                t_thres=1e-4, early_exit=False, want_occ=False, out=None, chunk_samples=0,
+                compensated=False, feature_scales=None):
     """t [B,R,S] sorted; rays [B,R,6]; Kv [B,V,3,3]; RTv [B,V,3,4];
     fmap_cl [B,V,Hf,Wf,F] channels-last, float32 or the model's 16-bit dtype;
-    bbox [B,3,2]; zdir [B,3].
+    bbox [B,3,2]; zdir [B,3].  compensated: Kahan-compensated compositing
+    sums (precision form).  feature_scales [F]: fmap_cl holds maps quantised
+    by sailor_quantize_channels with these scales, and the model's first
+    layer carries them (FramePipeline with quantize=True)."""
     Returns dict(rgb [B,R,3], depth [B,R], aux [B,R,2], occ [B,R,S] or None)."""
     dev = t.device
     check_input(t, "t", _f32)
@@ -129,13 +151,22 @@ def render_rays(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, model: PackedMode
     ws_bytes = _lib.lib().sailor_render_workspace_bytes_views(B, R, F, V)
     if out.get("workspace") is None or out["workspace"].numel() < ws_bytes:
         out["workspace"] = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    flags = int(bool(early_exit)) | (2 if compensated else 0)
+    if feature_scales is not None:
+        _check(feature_scales, "feature_scales", _f32, (F,), dev)
+        call("sailor_render_rays_quantized", 7, ptr(t), ptr(rays), B, R, S, ptr(Kv), ptr(RTv), V,
+             int(img_hw[0]), int(img_hw[1]), ptr(fmap_cl), Hf, Wf, F, ptr(feature_scales),
+             ptr(bbox), ptr(zdir), model.H, PREC_IDS[model.prec], ptr(model.w_dense),
+             ptr(model.b_dense), ptr(model.w_small), float(t_thres), flags, int(chunk_samples),
+             ptr(out["rgb"]), ptr(out["depth"]), ptr(out["aux"]), ptr(out["occ"]),
+             ptr(out["workspace"]), out["workspace"].numel(), stream_of(dev))
+        return out
     call("sailor_render_rays", ptr(t), ptr(rays), B, R, S, ptr(Kv), ptr(RTv), V, int(img_hw[0]),
          int(img_hw[1]), ptr(fmap_cl), map_prec, Hf, Wf, F, ptr(bbox), ptr(zdir), model.H,
          PREC_IDS[model.prec], ptr(model.w_dense), ptr(model.b_dense), ptr(model.w_small),
-         float(t_thres), int(bool(early_exit)), int(chunk_samples), ptr(out["rgb"]),
-         ptr(out["depth"]),
-         ptr(out["aux"]), ptr(out["occ"]), ptr(out["workspace"]),
-         out["workspace"].numel(), stream_of(dev))
+         float(t_thres), flags, int(chunk_samples), ptr(out["rgb"]), ptr(out["depth"]),
+         ptr(out["aux"]), ptr(out["occ"]), ptr(out["workspace"]), out["workspace"].numel(),
+         stream_of(dev))
     return out
 
 

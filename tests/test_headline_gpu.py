@@ -188,8 +188,9 @@ def test_c2_model_16bit_forms_match_float64(prec):
     dt = torch.bfloat16 if prec == "bf16" else torch.float16
     from sailor_amd.render import render_rays
     cfg = p.cfg
+    assert p.quantize     # per-channel quantised 16-bit maps (the precision form)
     out = render_rays(p.sorted, p.rays, p.K, p.RT, (cfg.img, cfg.img), p.fmap_lo, p.bbox,
-                      p.zdir, p.model, want_occ=True)
+                      p.zdir, p.model, want_occ=True, feature_scales=p.fscale)
     torch.cuda.synchronize()
     assert torch.equal(out["rgb"], p.rgb)
     rgb_e, dep_e, _, _, occ_e = _ref(p, sub, fm_dtype=dt, mm_dtype=dt)

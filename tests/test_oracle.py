@@ -689,3 +689,17 @@ def test_center_xyz_matches_float64_backprojection():
         cnt += z.size
     assert n[0] == cnt > 100
     np.testing.assert_allclose(cen[0], tot, rtol=1e-5)
+
+
+def test_integrate_f64_voxel_subset_matches_full_grid():
+    from oracle import oracle as O
+    from tests.helpers import scene_inputs
+    s = scene_inputs(0, 4, 64, 64, 75.0)
+    bbox = np.array([[[-1.1, 1.1], [-0.1, 2.1], [-1.1, 1.1]]], np.float32)
+    vs = np.array([2.2 / 23], np.float32)
+    full, wf = O.integrate_f64(s["K"], s["RT"], s["colors"], s["depths"], bbox, vs, (24, 24, 24))
+    idx = np.random.default_rng(0).choice(24 ** 3, 500, replace=False)
+    sub, ws = O.integrate_f64(s["K"], s["RT"], s["colors"], s["depths"], bbox, vs, (24, 24, 24),
+                              voxels=idx)
+    np.testing.assert_array_equal(sub[0], full.reshape(1, -1)[0, idx])
+    np.testing.assert_array_equal(ws[0], wf.reshape(1, -1)[0, idx])
