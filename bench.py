@@ -36,6 +36,11 @@ PEAKS = {"fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0}  # dense MFMA TFLOP/s (M
 HBM_GBS = 8000.0  # MI355X HBM3E peak, GB/s
 
 
+def progress(msg):
+    """One line per finished step on stderr (the JSON result goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def mlp_flops_per_sample_view(F, H):
     """Algorithmic FLOPs of the per-(sample, view) MLP: 2 * sum(in*out)."""
     return 2 * ((63 + F) * H + 3 * H * H)
@@ -246,6 +251,7 @@ def precision_form(cfg, steps, warmup, n_err_rays=512, n_err_voxels=200000):
                                        p.vs.cpu().numpy().reshape(-1), (cfg.dim,) * 3,
                                        voxels=vox)
         out[name] = {"frames_per_s": 1.0 / sec, "ms_per_step": sec * 1e3}
+        progress(f"c3 {name}: {sec * 1e3:.2f} ms per frame")
         if name != "fp32":
             out[name]["psnr_vs_fp32_db"] = psnr(ref, rgb, mask)
             out[name]["speedup_vs_fp32"] = out["fp32"]["ms_per_step"] / (sec * 1e3)
@@ -326,6 +332,7 @@ def training_form(steps, warmup, with_cpu=True):
         s_ms = _median_ms(step, steps)
         out[prec] = {"mrays_per_s": tb.n_rays / (s_ms / 1e3) / 1e6, "fwd_ms": f_ms,
                      "fwd_bwd_ms": s_ms, "fwd_bwd_over_fwd": s_ms / f_ms}
+        progress(f"c4 {prec}: fwd {f_ms:.2f} ms, fwd+bwd {s_ms:.2f} ms")
         del tr
     if with_cpu:
         from oracle import nerf_ref
@@ -637,20 +644,30 @@ def batching_form(steps, n_frames=16, precs=("fp32", "bf16")):
         t0 = time.perf_counter()
         bp = BatchPipeline(base, n_frames=n_frames, prec=prec)
         setup_s = time.perf_counter() - t0
-        chunks = bp.autotune()
+        progress(f"c5 {prec}: {n_frames} frames set up in {setup_s:.1f} s")
+        tuned = bp.autotune()
+        # sequential baseline: every frame alone, with its tuned chunk and the
+        # whole chip for its sample kernel
         lat = []
-        for p in bp.pipes:
+        for p, ns in zip(bp.pipes, tuned["chunk_samples"]):
+            keep = p.chunk_samples
+            p.chunk_samples = ns
             p.capture()
             lat.append(_median_ms(p.replay, steps))
             p.graph = None
+            p.chunk_samples = keep
+        progress(f"c5 {prec}: tuned {tuned['grid_share']}% of the chip per sample kernel, "
+                 f"{tuned['n_streams']} streams")
         batched = {}
-        for ns in (1, 4):
+        for ns in sorted({1, 4, tuned["n_streams"]}):
             bp.capture(n_streams=ns)
             bp.replay()
             torch.cuda.synchronize()
             ms = _median_ms(bp.replay, steps)
             batched[f"{ns}_streams"] = {"frames_per_s": n_frames / (ms / 1e3), "ms_per_batch": ms}
             bp.graph = None
+        chunks = tuned
+        progress(f"c5 {prec}: batched {batched}")
         info = [p.check() for p in bp.pipes]
         if "views" not in out:
             out["views"] = [c.n_views for c in bp.cfgs]
@@ -797,6 +814,7 @@ def main():
     info = p32.check()
     timing = {}
     sec = time_frames(p32, args.steps, args.warmup, timing)
+    progress(f"fp32 frame: median {timing['median_ms']:.2f} ms, p99 {timing['p99_ms']:.2f} ms")
     stages = stage_times(p32)
     # roofline of the dominant kernel, the fused sample kernel (stage 2 of
     # sailor_render_rays): algorithmic FLOPs count evaluated (valid) samples
@@ -846,6 +864,7 @@ def main():
             st_lo["sample_kernel"] = k_lo
             st_lo["hydra_kernel"] = kernel_ms(plo, 4)
             rgb_lo = plo.rgb[0].cpu().numpy()
+            progress(f"{prec} frame: median {t_lo['median_ms']:.2f} ms")
             result[prec] = {
                 "frames_per_s": 1.0 / s_lo, "ms_per_step": s_lo * 1e3,
                 "speedup_vs_fp32": sec / s_lo,
@@ -867,6 +886,7 @@ def main():
             "identical_image": bool(np.array_equal(pe.rgb[0].cpu().numpy(), rgb32)),
         }
         del pe
+    progress("early-exit frame done")
     if not args.no_c3 and args.workload == "c2":
         result["c3_precision_form"] = precision_form(CONFIGS["c3"], max(3, args.steps // 4),
                                                      args.warmup)
@@ -881,6 +901,7 @@ def main():
         result["gui_frame"] = gui_frame(max(3, args.steps // 4))
         result["standalone_kernels"] = standalone_kernels(max(3, args.steps // 4))
     if not args.no_cpu:
+        progress("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(p32)
     else:
         result["cpu_baseline"] = None

@@ -410,13 +410,17 @@ size_t sailor_render_workspace_bytes_views(int B, int R, int F, int V);
  *              compensated compositing sums (depth, weight and alpha sums,
  *              per-view tokens; precision form, no counterpart in the
  *              reference)
- *  chunk_samples  samples evaluated together per workgroup step (8, 16, 24
- *              or 48; 0 = fp32: 24, 16-bit: 16, or the largest of 16 and 8
- *              that divides S; with more than 4 views: fp32 12 (8 when S
- *              is not a multiple of 12), 16-bit 8, and the argument is
- *              ignored); S must be a multiple of it; a chunk
- *              whose activations do not fit the 160 KiB of LDS of a
- *              workgroup (fp32 with 48 samples) is rejected
+ *  chunk_samples  launch shape: bits 0-7 = samples evaluated together per
+ *              workgroup step (up to 4 views: 8, 16, 24 or 48; 5-8 views:
+ *              fp32 8, 12 or 16, 16-bit 4, 8 or 16; 0 = fp32: 24, 16-bit:
+ *              16, or the largest of 16 and 8 that divides S; with more
+ *              than 4 views fp32 12 (8 when S is not a multiple of 12),
+ *              16-bit 8); S must be a multiple of it; a chunk whose
+ *              activations do not fit the 160 KiB of LDS of a workgroup
+ *              (fp32 with 48 samples) is rejected.  Bits 8-15 = percent of
+ *              the device's resident workgroup slots the sample kernel
+ *              launches (0 = all), for frames that share the chip with
+ *              other streams' work
  *  out_rgb f32 [B,R,3], out_depth f32 [B,R]; out_aux f32 [B,R,2] (weight sum,
  *  alpha sum) and out_occ f32 [B,R,S] (per-sample occupancy) may be NULL.
  *  workspace  device buffer of at least sailor_render_workspace_bytes_views(B,

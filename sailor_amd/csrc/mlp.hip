@@ -90,6 +90,7 @@ struct RenderArgs {
   int early_exit;
   int compensated;         // Kahan-compensated compositing sums (precision form)
   const float* fscale;     // [F] per-channel scales of quantised maps (stored x / s) or null
+  int grid_share;          // percent of the resident workgroup slots the sample kernel takes (0: all)
   int stages;              // 1: compaction, 2: sample kernel, 4: Hydra/RGB kernel
   // point mode (density query): points [B, n_points, 3]; groups of NS points
   // take the place of rays and out_occ receives one density per point
@@ -1464,7 +1465,7 @@ render_backward_kernel(BackArgs ba) {
   float gsw[NU];
 #pragma unroll
   for (int u = 0; u < NU; ++u) gsw[u] = 0.0f;
-  float gsb = 0.0f;
+  double gsb = 0.0;   // occupancy-bias gradient: a sum of many cancelling terms
   [[maybe_unused]] DenseAcc<F, H, PREC> dacc;
   if constexpr (DW) dacc.zero();
 
@@ -1730,7 +1731,7 @@ render_backward_kernel(BackArgs ba) {
       if (c == 0) gp[so.sigma_w + unit_of32<H>(wave, k / 16, h, k % 16)] += v;
     }
   }
-  if (tid == 0) gp[so.sigma_b] += gsb;
+  if (tid == 0) gp[so.sigma_b] += (float)gsb;
   if constexpr (DW)
     dacc.store(ba.gdense + (size_t)blockIdx.x * dense_grad_total(F, H), wave, lane);
 }
@@ -2139,9 +2140,10 @@ __global__ void sum_partials(const float* __restrict__ part, int parts, int n,
                              float* __restrict__ out, int accumulate) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
-  float s = accumulate ? out[e] : 0.0f;
-  for (int p = 0; p < parts; ++p) s += part[(size_t)p * n + e];
-  out[e] = s;
+  // double accumulation: the partial gradients cancel heavily
+  double s = accumulate ? (double)out[e] : 0.0;
+  for (int p = 0; p < parts; ++p) s += (double)part[(size_t)p * n + e];
+  out[e] = (float)s;
 }
 
 // zero a float buffer (graph-capturable, no memset)
@@ -2237,7 +2239,12 @@ int launch_kernels(const RenderArgs& a, hipStream_t st) {
                                                                 L::total));
     grid = cus * std::max(per_cu, 1);
   }
-  const int blocks = (int)std::min<long long>(grid, a.n_rays);
+  // a frame batched with others on several streams may leave part of the
+  // chip to their kernels (grid_share percent of the resident slots)
+  const long long g = a.grid_share > 0 && a.grid_share < 100
+                          ? std::max<long long>(1, (long long)grid * a.grid_share / 100)
+                          : grid;
+  const int blocks = (int)std::min<long long>(g, a.n_rays);
   if (a.stages & 2) {
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(kThreads), L::total, st, a);
     if (int err = sailor::check_launch("render_rays")) return err;
@@ -2260,22 +2267,38 @@ int dispatch_prec(int prec, const RenderArgs& a, hipStream_t st, bool dry) {
   return sailor::set_error(SAILOR_ERR_ARG, "render_rays: precision %d unsupported", prec);
 }
 
-// 8 view slots: fp32 12-sample chunks (8 when S is not a multiple of 12),
-// 16-bit 8-sample chunks (the column counts of the 4-slot defaults).
+// 8 view slots: fp32 8, 12 or 16 samples per chunk (default 12, 8 when S is
+// not a multiple of 12), 16-bit 4, 8 or 16 (default 8): the 4-slot defaults'
+// column counts and their neighbours; chunks whose LDS does not fit are
+// rejected by launch().
 template <int F, int H>
-int dispatch_vp8(int prec, const RenderArgs& a, hipStream_t st, bool dry) {
-  switch (prec) {
-    case kF32: return a.S % 12 == 0 ? launch<F, H, kF32, 12, 8>(a, st, dry)
-                                    : launch<F, H, kF32, 8, 8>(a, st, dry);
-    case kBF16: return launch<F, H, kBF16, 8, 8>(a, st, dry);
-    case kF16: return launch<F, H, kF16, 8, 8>(a, st, dry);
+int dispatch_vp8(int ns, int prec, const RenderArgs& a, hipStream_t st, bool dry) {
+  if (prec == kF32) {
+    switch (ns) {
+      case 8: return launch<F, H, kF32, 8, 8>(a, st, dry);
+      case 12: return launch<F, H, kF32, 12, 8>(a, st, dry);
+      case 16: return launch<F, H, kF32, 16, 8>(a, st, dry);
+    }
+  } else if (prec == kBF16 || prec == kF16) {
+    switch (ns) {
+      case 4: return prec == kBF16 ? launch<F, H, kBF16, 4, 8>(a, st, dry)
+                                   : launch<F, H, kF16, 4, 8>(a, st, dry);
+      case 8: return prec == kBF16 ? launch<F, H, kBF16, 8, 8>(a, st, dry)
+                                   : launch<F, H, kF16, 8, 8>(a, st, dry);
+      case 16: return prec == kBF16 ? launch<F, H, kBF16, 16, 8>(a, st, dry)
+                                    : launch<F, H, kF16, 16, 8>(a, st, dry);
+    }
+  } else {
+    return sailor::set_error(SAILOR_ERR_ARG, "render_rays: precision %d unsupported", prec);
   }
-  return sailor::set_error(SAILOR_ERR_ARG, "render_rays: precision %d unsupported", prec);
+  return sailor::set_error(SAILOR_ERR_ARG, "render_rays: %d samples per chunk not "
+                           "instantiated for 8 view slots (fp32: 8, 12, 16; 16-bit: 4, 8, 16)",
+                           ns);
 }
 
 template <int F, int H>
 int dispatch_ns(int ns, int prec, const RenderArgs& a, hipStream_t st, bool dry) {
-  if (a.V > kVP) return dispatch_vp8<F, H>(prec, a, st, dry);
+  if (a.V > kVP) return dispatch_vp8<F, H>(ns, prec, a, st, dry);
   switch (ns) {
     case 8: return dispatch_prec<F, H, 8>(prec, a, st, dry);
     case 16: return dispatch_prec<F, H, 16>(prec, a, st, dry);
@@ -2321,13 +2344,20 @@ int render_stages(int stages, const float* t, const float* rays, int B, int R, i
                  "render_rays: workspace of %zu bytes is smaller than the %zu required",
                  workspace_bytes, sailor_render_workspace_bytes_views(B, R, F, V));
   SAILOR_REQUIRE(Hf > 1 && Wf > 1, "render_rays: feature map too small");
-  int ns = chunk_samples;
-  if (V > kVP) ns = (precision == kF32 && S % 12 == 0) ? 12 : 8;  // 8-slot instantiations
+  // launch shape: bits 0-7 samples per chunk, bits 8-15 grid share (percent)
+  int ns = chunk_samples & 0xff;
+  const int share = (chunk_samples >> 8) & 0xff;
+  SAILOR_REQUIRE(chunk_samples >= 0 && share <= 100,
+                 "render_rays: launch shape %#x: grid share %d%% above 100", chunk_samples, share);
   if (ns <= 0) {
-    // measured on MI355X (C2 frame): fp32 24-sample chunks, 16-bit 16-sample
-    // chunks (two workgroups per CU)
-    if (precision == kF32 && S % 24 == 0) ns = 24;
-    else ns = (S % 16 == 0) ? 16 : 8;
+    if (V > kVP) {
+      ns = (precision == kF32 && S % 12 == 0) ? 12 : 8;  // 8-slot instantiations
+    } else {
+      // measured on MI355X (C2 frame): fp32 24-sample chunks, 16-bit
+      // 16-sample chunks (two workgroups per CU)
+      if (precision == kF32 && S % 24 == 0) ns = 24;
+      else ns = (S % 16 == 0) ? 16 : 8;
+    }
   }
   SAILOR_REQUIRE(S % ns == 0, "render_rays: S=%d is not a multiple of the chunk (%d samples)",
                  S, ns);
@@ -2349,6 +2379,7 @@ int render_stages(int stages, const float* t, const float* rays, int B, int R, i
   a.S = S; a.V = V; a.img_w = img_w; a.img_h = img_h; a.Hf = Hf; a.Wf = Wf;
   a.t_thres = t_thresh; a.early_exit = early_exit & 1; a.compensated = (early_exit >> 1) & 1;
   a.fscale = feature_scales;
+  a.grid_share = share;
   SAILOR_REQUIRE(feature_scales == nullptr || fmap_precision == precision,
                  "render_rays: feature scales apply to quantised 16-bit maps only");
   a.stages = stages;

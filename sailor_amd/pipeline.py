@@ -344,19 +344,44 @@ class BatchPipeline:
         for p in self.pipes:
             p.run()
 
-    def autotune(self, candidates=(8, 16, 24, 48), reps=2):
-        """Per frame: the chunk size (samples per workgroup step) with the
-        shortest render, among the candidates that divide S and fit the LDS
-        (frames with more than 4 views use the fixed 8-slot instantiation)."""
+    def _chunk_candidates(self, p):
+        if p.cfg.n_views > 4:
+            return (8, 12, 16) if p.prec == "fp32" else (4, 8, 16)
+        return (8, 16, 24, 48)
+
+    @staticmethod
+    def _time_ms(fn, reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        fn()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def autotune(self, candidates=None, reps=2, shares=(100, 90, 75, 60), streams=(1, 2, 4, 8),
+                 batch_reps=3):
+        """Project autotuner of the batch's launch shapes, in two levels.
+
+        1. Per frame, the samples per workgroup step (the sample kernel's
+           tile: columns = samples x view slots) with the shortest render of
+           that frame alone, among the instantiated sizes that divide S and
+           fit the LDS (4 view slots: 8/16/24/48; 8 slots: fp32 8/12/16,
+           16-bit 4/8/16).
+        2. For the whole batch, the share of the chip each sample kernel
+           takes (percent of the resident workgroup slots, so other frames'
+           geometry kernels find room) and the number of streams the frames
+           are spread over, by timing the captured batch graph.
+
+        Returns dict(chunk_samples per frame, grid_share, n_streams, table of
+        the measured batch times in ms per (share, streams))."""
         chosen = []
         for p in self.pipes:
             p.run()
             torch.cuda.synchronize()
-            if p.cfg.n_views > 4:
-                chosen.append(0)
-                continue
             best = None
-            for ns in candidates:
+            for ns in candidates or self._chunk_candidates(p):
                 if p.cfg.S % ns:
                     continue
                 p.chunk_samples = ns
@@ -365,18 +390,28 @@ class BatchPipeline:
                 except _lib.SailorError:
                     continue
                 torch.cuda.synchronize()
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record()
-                for _ in range(reps):
-                    p.shade()
-                e1.record()
-                e1.synchronize()
-                ms = e0.elapsed_time(e1) / reps
+                ms = self._time_ms(p.shade, reps)
                 if best is None or ms < best[0]:
                     best = (ms, ns)
             p.chunk_samples = best[1] if best else 0
             chosen.append(p.chunk_samples)
-        return chosen
+        table, best = {}, None
+        for share in shares:
+            for p, ns in zip(self.pipes, chosen):
+                p.chunk_samples = ns | ((share % 100) << 8)
+            for ns_streams in streams:
+                self.capture(n_streams=ns_streams)
+                ms = self._time_ms(self.replay, batch_reps)
+                table[f"{share}%/{ns_streams}"] = round(ms, 3)
+                if best is None or ms < best[0]:
+                    best = (ms, share, ns_streams)
+                self.graph = None
+        _, share, n_streams = best
+        for p, ns in zip(self.pipes, chosen):
+            p.chunk_samples = ns | ((share % 100) << 8)
+        self.tuned = dict(chunk_samples=chosen, grid_share=share, n_streams=n_streams,
+                          batch_ms=table)
+        return self.tuned
 
     def capture(self, n_streams=4):
         """One graph for all frames; frame k runs on stream k % n_streams."""

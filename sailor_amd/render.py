@@ -110,7 +110,7 @@ def render_rays(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, model: PackedMode
     bbox [B,3,2]; zdir [B,3].  compensated: Kahan-compensated compositing
     sums (precision form).  feature_scales [F]: fmap_cl holds maps quantised
     by sailor_quantize_channels with these scales, and the model's first
-    layer carries them (FramePipeline with quantize=True)."""
+    layer carries them (FramePipeline with quantize=True).
     Returns dict(rgb [B,R,3], depth [B,R], aux [B,R,2], occ [B,R,S] or None)."""
     dev = t.device
     check_input(t, "t", _f32)

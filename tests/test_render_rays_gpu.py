@@ -226,3 +226,35 @@ def test_batch_graph_matches_eager_frames():
         torch.cuda.synchronize()
         for p, (r, d) in zip(bp.pipes, eager):
             assert torch.equal(p.rgb, r) and torch.equal(p.depth, d)
+
+
+@pytest.mark.parametrize("views,prec", [(4, "fp32"), (4, "bf16"), (6, "fp32"), (6, "bf16")])
+def test_every_launch_shape_renders_the_same_image(views, prec):
+    """Columns of the MLP are independent and the compositing and token sums
+    run in sample order, so every instantiated chunk size (4 view slots:
+    8/16/24/48; 8 slots: fp32 8/12/16, 16-bit 4/8/16) and a sample kernel
+    launched on part of the chip (grid share 40 %) give the default launch's
+    image bit for bit; chunks whose LDS does not fit are rejected."""
+    from sailor_amd import _lib
+    from sailor_amd.pipeline import FrameConfig
+    cfg = FrameConfig(f"shapes_v{views}", views, 512, 600.0, 128, 8, 40, 47.0, 48, "c1")
+    p = pipe(prec, cfg)
+    ref = (p.rgb.clone(), p.depth.clone(), p.aux.clone())
+    cands = (8, 16, 24, 48) if views <= 4 else ((8, 12, 16) if prec == "fp32" else (4, 8, 16))
+    ran = []
+    for ns in cands:
+        for share in (0, 40):
+            p.chunk_samples = ns | (share << 8)
+            p.rgb.zero_()
+            try:
+                p.shade()
+            except _lib.SailorError as e:
+                assert "LDS" in str(e), e
+                continue
+            torch.cuda.synchronize()
+            assert torch.equal(p.rgb, ref[0]) and torch.equal(p.depth, ref[1]), (ns, share)
+            assert torch.equal(p.aux, ref[2]), (ns, share)
+            ran.append((ns, share))
+    p.chunk_samples = 0
+    assert len(ran) >= 4, ran
+    print(views, prec, "launch shapes checked:", ran)
