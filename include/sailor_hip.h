@@ -409,7 +409,10 @@ size_t sailor_render_workspace_bytes_views(int B, int R, int F, int V);
  *              is below t_thresh (outputs are unchanged); bit 1 -- Kahan-
  *              compensated compositing sums (depth, weight and alpha sums,
  *              per-view tokens; precision form, no counterpart in the
- *              reference)
+ *              reference); bit 2 -- in the 16-bit forms, run the Hydra
+ *              encoder and RGB head on fp32 vector ALUs instead of 16-bit
+ *              MFMA (the differentiation form, whose backward recomputes
+ *              them in fp32)
  *  chunk_samples  launch shape: bits 0-7 = samples evaluated together per
  *              workgroup step (up to 4 views: 8, 16, 24 or 48; 5-8 views:
  *              fp32 8, 12 or 16, 16-bit 4, 8 or 16; 0 = fp32: 24, 16-bit:

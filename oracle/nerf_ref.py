@@ -54,34 +54,54 @@ def gather_features(x, Kv, RTv, fmaps, img_hw):
     return feats
 
 
-def hydra_encode(hydra, toks):
+def _mm(x, w, mm_dtype=None):
+    """x @ w.T; with mm_dtype both operands are rounded to it first and the
+    product accumulated in x's dtype (a 16-bit matrix unit).  float16 has a
+    narrow range (max 65504), so each row of x is divided by the power of two
+    2^e >= its max |x| (m = f 2^e, f in [0.5, 1)) before rounding and the
+    row's results multiplied back by it: exact scalings, the rounding of
+    every element that stays in fp16's normal range is unchanged."""
+    if mm_dtype is None:
+        return x @ w.T
+    rd = lambda z: z.to(mm_dtype).to(x.dtype)
+    if mm_dtype != torch.float16:
+        return rd(x) @ rd(w).T
+    m = x.detach().abs().amax(dim=-1, keepdim=True)
+    _, e = torch.frexp(m)
+    sig = torch.where(m > 0, torch.ldexp(torch.ones_like(m), e), torch.ones_like(m))
+    return (rd(x / sig) @ rd(w).T) * sig
+
+
+def hydra_encode(hydra, toks, mm_dtype=None):
     """Hydra encoder stacks over the view tokens toks [N, V, F]
     (aggregation.py:101-128 HydraAttention, :164-171 HydraForward,
     :299-307 HydraEncoder): q, k, v projections, q and k L2-normalised over
     features, kv = sum over views of k * v, out = fc(q * kv) + x, then
     x' = w2 relu(w1 out + b1) + b2 + out.  hydra: list of dicts of tensors
-    (q, k, v, fc, w1, b1, w2, b2).  Returns the tokens [N, V, F]."""
+    (q, k, v, fc, w1, b1, w2, b2).  mm_dtype: products on rounded operands.
+    Returns the tokens [N, V, F]."""
     x_tok = toks
     for st in hydra:
-        q = Fn.normalize(x_tok @ st["q"].T, dim=-1)
-        k = Fn.normalize(x_tok @ st["k"].T, dim=-1)
-        vv = x_tok @ st["v"].T
+        q = Fn.normalize(_mm(x_tok, st["q"], mm_dtype), dim=-1)
+        k = Fn.normalize(_mm(x_tok, st["k"], mm_dtype), dim=-1)
+        vv = _mm(x_tok, st["v"], mm_dtype)
         kv = (k * vv).sum(dim=-2, keepdim=True)
-        out = (q * kv) @ st["fc"].T + x_tok
-        x_tok = torch.relu(out @ st["w1"].T + st["b1"]) @ st["w2"].T + st["b2"] + out
+        out = _mm(q * kv, st["fc"], mm_dtype) + x_tok
+        hid = torch.relu(_mm(out, st["w1"], mm_dtype) + st["b1"])
+        x_tok = _mm(hid, st["w2"], mm_dtype) + st["b2"] + out
     return x_tok
 
 
-def hydra_rgb(hydra, rgbw, toks):
+def hydra_rgb(hydra, rgbw, toks, mm_dtype=None):
     """hydra_encode, then the RGB head on the first view's token
     (nerf.py:117-124).  rgbw: [(w1, b1), (w2, b2), (w3, b3)].  Returns rgb
     [N, 3]."""
-    x_tok = hydra_encode(hydra, toks)
+    x_tok = hydra_encode(hydra, toks, mm_dtype)
     h = x_tok[:, 0]
     (w1, b1), (w2, b2), (w3, b3) = rgbw
-    h = torch.relu(h @ w1.T + b1)
-    h = torch.relu(h @ w2.T + b2)
-    return torch.sigmoid(h @ w3.T + b3)
+    h = torch.relu(_mm(h, w1, mm_dtype) + b1)
+    h = torch.relu(_mm(h, w2, mm_dtype) + b2)
+    return torch.sigmoid(_mm(h, w3, mm_dtype) + b3)
 
 
 def density(weights, x, Kv, RTv, fmaps, bbox, img_hw, dtype=torch.float64, device="cpu"):
@@ -118,7 +138,8 @@ def params_of(weights, dtype=torch.float64, device="cpu", requires_grad=False):
 
 
 def render_rays(weights, t, rays, Kv, RTv, fmaps, bbox, zdir, img_hw, t_thres=1e-4,
-                dtype=torch.float64, device="cpu", params=None, trace=None, mm_dtype=None):
+                dtype=torch.float64, device="cpu", params=None, trace=None, mm_dtype=None,
+                hydra_mm_dtype=None):
     """weights: sailor_amd.model.NerfWeights; t [B,R,S] sorted; rays [B,R,6];
     Kv [B,V,3,3]; RTv [B,V,3,4]; fmaps [B,V,F,Hf,Wf] (channel-first);
     bbox [B,3,2]; zdir [B,3].  Returns rgb [B,R,3], depth [B,R], ws [B,R],
@@ -129,7 +150,8 @@ def render_rays(weights, t, rays, Kv, RTv, fmaps, bbox, zdir, img_hw, t_thres=1e
     the occupancies [N,S], for the valid rays in order.  With mm_dtype (e.g. bfloat16)
     the dense layers' inputs and weights are rounded to that type and the
     products accumulated in `dtype` (mixed-precision matrix units; gradients
-    are rounded where they pass the same casts)."""
+    are rounded where they pass the same casts).  hydra_mm_dtype does the
+    same for the Hydra encoder's and the RGB head's products."""
     T_ = lambda a: torch.as_tensor(a).to(device=device, dtype=dtype)
     t, rays, Kv, RTv, fmaps, bbox, zdir = map(T_, (t, rays, Kv, RTv, fmaps, bbox, zdir))
     B, R, S = t.shape
@@ -195,7 +217,7 @@ def render_rays(weights, t, rays, Kv, RTv, fmaps, bbox, zdir, img_hw, t_thres=1e
             alive = alive & ~(Tt < t_thres)
         if trace is not None:
             trace["toks"], trace["occ"] = toks, alpha
-        rgb = hydra_rgb(P["hydra"], P["rgb"], toks)
+        rgb = hydra_rgb(P["hydra"], P["rgb"], toks, hydra_mm_dtype)
         rgb_out[b, idx] = rgb
         dep_out[b, idx] = dsum * (d @ zdir[b])
         ws_out[b, idx] = wsum

@@ -62,6 +62,7 @@ enum Prec { kF32 = 0, kBF16 = 1, kF16 = 2 };
 constexpr int kThreads = kWaves * 64;
 constexpr int kVP = 4;       // view slots per sample (training form, point mode)
 constexpr int kMaxViews = 8;  // view slots of the 8-view rendering instantiations
+constexpr int kXcds = 8;      // accelerator dies of an MI355X (workgroups go round-robin)
 constexpr float kHalfPi = 1.5707963267948966f;
 
 struct RenderArgs {
@@ -91,6 +92,7 @@ struct RenderArgs {
   int compensated;         // Kahan-compensated compositing sums (precision form)
   const float* fscale;     // [F] per-channel scales of quantised maps (stored x / s) or null
   int grid_share;          // percent of the resident workgroup slots the sample kernel takes (0: all)
+  int fp32_hydra;          // 16-bit forms: Hydra + RGB head on fp32 vector ALUs instead of MFMA
   int stages;              // 1: compaction, 2: sample kernel, 4: Hydra/RGB kernel
   // point mode (density query): points [B, n_points, 3]; groups of NS points
   // take the place of rays and out_occ receives one density per point
@@ -632,7 +634,18 @@ render_kernel(RenderArgs a) {
   // compacted list balances the workgroups (dropped rays were finished by the
   // compaction kernel).  Point mode: item = a group of NS points.
   const long long n_items = POINTS ? a.n_rays : (long long)*a.n_valid;
-  for (long long item = blockIdx.x; item < n_items; item += gridDim.x) {
+  // Workgroups are dispatched to the 8 XCDs round-robin (blockIdx % 8), each
+  // XCD with its own L2.  Every XCD takes one contiguous band of the list
+  // (raster order), so the rays its workgroups hold at any time are image
+  // neighbours whose bilinear taps overlap in that L2.
+  const int bands = (int)gridDim.x < kXcds ? (int)gridDim.x : kXcds;
+  const int xcd = blockIdx.x % bands;
+  const int wg_in_xcd = blockIdx.x / bands;
+  const int wgs_of_xcd = ((int)gridDim.x - xcd + bands - 1) / bands;
+  const long long band = (n_items + bands - 1) / bands;
+  const long long item_lo = band * xcd;
+  const long long item_hi = item_lo + band < n_items ? item_lo + band : n_items;
+  for (long long item = item_lo + wg_in_xcd; item < item_hi; item += wgs_of_xcd) {
     __syncthreads();  // LDS of the previous ray is no longer read
     const long long ray = POINTS ? item : (long long)a.list[item];
     const int b = (int)(ray / a.rays_per_batch);
@@ -1041,6 +1054,261 @@ hydra_rgb_kernel(const float* __restrict__ tok, const int* __restrict__ list,
 }
 
 
+
+// ---- Hydra encoder + RGB head on 16-bit MFMA (the 16-bit forms) ----------
+// Same function as hydra_rgb_kernel with every dense product on
+// v_mfma_f32_16x16x32_{bf16,f16}: operands rounded to the form's 16-bit
+// type, fp32 accumulation; normalisations, the cross-view sum, residuals,
+// biases and activations stay in fp32.  Each wave owns 16 tokens (16 / VP
+// rays): the MFMA's N = 16 columns are the tokens, M = 16 output units per
+// tile, K = 32 inputs per step.  A fragments are read straight from the fp32
+// small-weight buffer (lane l: row 16 mt + l%16, inputs 32 s + 8 (l/16) ..
+// +8); B fragments from the wave's LDS rows [token][input]; the result
+// (lane l: units 16 mt + 4 (l/16) + i of token l%16) is post-processed in
+// registers and written back to LDS for the next product.  The cross-view
+// sum of k^ v is a shuffle over the lanes of one ray's tokens.
+template <int PREC> struct HydraOp;
+template <> struct HydraOp<kBF16> { typedef bf16x8 V; typedef __bf16 E; };
+template <> struct HydraOp<kF16> { typedef f16x8 V; typedef _Float16 E; };
+
+template <int PREC>
+__device__ __forceinline__ f32x4 hmfma(typename HydraOp<PREC>::V a, typename HydraOp<PREC>::V b,
+                                       f32x4 c) {
+  if constexpr (PREC == kBF16) return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  else return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// 8 consecutive floats -> one 16-bit operand (round to nearest even)
+template <int PREC>
+__device__ __forceinline__ typename HydraOp<PREC>::V hcvt8(const float* p) {
+  typedef typename HydraOp<PREC>::E E;
+  const f32x4 x0 = *reinterpret_cast<const f32x4*>(p);
+  const f32x4 x1 = *reinterpret_cast<const f32x4*>(p + 4);
+  typename HydraOp<PREC>::V v;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    v[j] = (E)x0[j];
+    v[4 + j] = (E)x1[j];
+  }
+  return v;
+}
+
+// acc[mt] = W[16 mt + l%16][0..K) . src[token l%16][0..K) for MT tiles (acc
+// zero on entry); W row-major [rows][K] fp32 (rows >= n_rows read as zero),
+// src fp32 LDS rows of pitch LD floats.  fp16 operands: each token's inputs
+// are divided by a power of two sigma >= their max |x| before rounding and
+// the token's results multiplied by sigma after (exact), so inputs beyond
+// fp16's range (65504) cannot overflow; bf16 has fp32's range and needs none.
+template <int PREC, int MT, int K, int LD>
+__device__ __forceinline__ void hydra_mm(const float* __restrict__ W, int n_rows,
+                                         const float* src, int lane, f32x4 (&acc)[MT]) {
+  typedef typename HydraOp<PREC>::V V;
+  const int n = lane & 15, g = lane >> 4;
+  float sigma = 1.0f, inv_sigma = 1.0f;
+  if constexpr (PREC == kF16) {
+    float m = 0.0f;
+#pragma unroll
+    for (int s = 0; s < K / 32; ++s)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) m = fmaxf(m, fabsf(src[n * LD + 32 * s + 8 * g + j]));
+    m = fmaxf(m, __shfl_xor(m, 16, 64));
+    m = fmaxf(m, __shfl_xor(m, 32, 64));
+    if (m > 0.0f && m <= 3.0e38f) {
+      int e = 0;
+      frexpf(m, &e);
+      sigma = ldexpf(1.0f, e);
+      inv_sigma = ldexpf(1.0f, -e);
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < K / 32; ++s) {
+    V b;
+    if constexpr (PREC == kF16) {
+      float x[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) x[j] = src[n * LD + 32 * s + 8 * g + j] * inv_sigma;
+      b = hcvt8<PREC>(x);
+    } else {
+      b = hcvt8<PREC>(src + n * LD + 32 * s + 8 * g);
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const int row = 16 * mt + n;
+      V a;
+      if (row < n_rows) {
+        a = hcvt8<PREC>(W + (size_t)row * K + 32 * s + 8 * g);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[j] = (typename HydraOp<PREC>::E)0.0f;
+      }
+      acc[mt] = hmfma<PREC>(a, b, acc[mt]);
+    }
+  }
+  if constexpr (PREC == kF16) {
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) acc[mt] *= sigma;
+  }
+}
+
+template <int MT>
+__device__ __forceinline__ void hzero(f32x4 (&acc)[MT]) {
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+}
+
+template <int F, int H, int VP, int PREC>
+__global__ void __launch_bounds__(kThreads)
+hydra_rgb_mfma_kernel(const float* __restrict__ tok, const int* __restrict__ list,
+                      const unsigned* __restrict__ n_valid_ptr, const float* __restrict__ ws,
+                      int V, float* __restrict__ out_rgb) {
+  constexpr int RPG = kTok / VP;          // rays per workgroup (64 tokens)
+  constexpr int HI = kHydraInner;         // 32
+  constexpr int LX = F + 4;               // fp32 row pitch of token rows (16-byte aligned)
+  constexpr int LH = 3 * HI + 4;          // q | k | v rows, then attention / hidden / RGB rows
+  constexpr SmallOffsets so = small_offsets(F, H);
+  constexpr int stack_stride = so.q[1] - so.q[0];
+  static_assert(F % 32 == 0 && kRgbH1 % 32 == 0 && kRgbH2 % 32 == 0, "whole K steps");
+  static_assert(kRgbH1 <= 3 * HI, "RGB layer 1 fits the q|k|v rows");
+  __shared__ __attribute__((aligned(16))) float sx[kWaves][16 * LX];   // tokens (fp32)
+  __shared__ __attribute__((aligned(16))) float sh[kWaves][16 * LH];   // products (fp32)
+
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int n = lane & 15, g = lane >> 4;
+  const long long n_valid = *n_valid_ptr;
+  const long long item0 = (long long)blockIdx.x * RPG;
+  if (item0 >= n_valid) return;
+  float* xs = sx[wave];
+  float* hs = sh[wave];
+  // this wave's 16 tokens: a contiguous block of 16 x F floats (zero past the list)
+  const long long t0 = item0 * VP + 16 * wave;
+  const long long n_tok = (n_valid - item0) * VP - 16 * wave;   // tokens of this wave that exist
+  for (int i = lane; i < 16 * F / 4; i += 64) {
+    const int tk = (4 * i) / F, c = (4 * i) % F;
+    f32x4 v = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    if (tk < n_tok) v = *reinterpret_cast<const f32x4*>(tok + (t0 + tk) * F + c);
+    *reinterpret_cast<f32x4*>(xs + tk * LX + c) = v;
+  }
+  __syncthreads();
+  const int view = n % VP;
+  const float vmask = view < V ? 1.0f : 0.0f;
+
+  for (int st = 0; st < kHydraStacks; ++st) {
+    const float* wqkv = ws + so.q[0] + st * stack_stride;   // q, k, v rows contiguous [96][F]
+    f32x4 a6[6];
+    hzero(a6);
+    hydra_mm<PREC, 6, F, LX>(wqkv, 3 * HI, xs, lane, a6);
+    // normalise q and k over their 32 units: this lane holds units 4g+i of
+    // tiles 0,1 (q), 2,3 (k), 4,5 (v); the other 24 units sit in the lanes
+    // with the same token and another g
+    float qq = 0.0f, kk = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      qq += a6[0][i] * a6[0][i] + a6[1][i] * a6[1][i];
+      kk += a6[2][i] * a6[2][i] + a6[3][i] * a6[3][i];
+    }
+    qq += __shfl_xor(qq, 16, 64);
+    qq += __shfl_xor(qq, 32, 64);
+    kk += __shfl_xor(kk, 16, 64);
+    kk += __shfl_xor(kk, 32, 64);
+    const float nq = 1.0f / fmaxf(sqrtf(qq), 1e-12f);
+    const float nk = vmask / fmaxf(sqrtf(kk), 1e-12f);
+    // attention: q^ * sum_views k^ v, written as rows [token][unit] for fc
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float kv = a6[2 + h][i] * nk * a6[4 + h][i];
+#pragma unroll
+        for (int off = 1; off < VP; off <<= 1) kv += __shfl_xor(kv, off, 64);
+        hs[n * LH + 16 * h + 4 * g + i] = a6[h][i] * nq * kv;
+      }
+    __syncthreads();
+    // out = fc(attn) + x  (kept in the token rows as the new residual base)
+    {
+      constexpr int MT = F / 16;
+      f32x4 ao[MT];
+      hzero(ao);
+      hydra_mm<PREC, MT, HI, LH>(ws + so.fc[0] + st * stack_stride, F, hs, lane, ao);
+      __syncthreads();
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) xs[n * LX + 16 * mt + 4 * g + i] += ao[mt][i];
+    }
+    __syncthreads();
+    // FFN hidden: relu(W1 out + b1)
+    {
+      f32x4 ah[2];
+      hzero(ah);
+      hydra_mm<PREC, 2, F, LX>(ws + so.w1[0] + st * stack_stride, HI, xs, lane, ah);
+      const float* b1 = ws + so.b1[0] + st * stack_stride;
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int u = 16 * mt + 4 * g + i;
+          hs[n * LH + u] = fmaxf(ah[mt][i] + b1[u], 0.0f);
+        }
+    }
+    __syncthreads();
+    // x = W2 h + b2 + out
+    {
+      constexpr int MT = F / 16;
+      f32x4 ax[MT];
+      hzero(ax);
+      hydra_mm<PREC, MT, HI, LH>(ws + so.w2[0] + st * stack_stride, F, hs, lane, ax);
+      const float* b2 = ws + so.b2[0] + st * stack_stride;
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int u = 16 * mt + 4 * g + i;
+          xs[n * LX + u] += ax[mt][i] + b2[u];
+        }
+    }
+    __syncthreads();
+  }
+  // ---- RGB head on the first view's token: F -> 64 -> 32 -> 3 --------------
+  {
+    f32x4 a1[kRgbH1 / 16];
+    hzero(a1);
+    hydra_mm<PREC, kRgbH1 / 16, F, LX>(ws + so.r1w, kRgbH1, xs, lane, a1);
+#pragma unroll
+    for (int mt = 0; mt < kRgbH1 / 16; ++mt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int u = 16 * mt + 4 * g + i;
+        hs[n * LH + u] = fmaxf(a1[mt][i] + ws[so.r1b + u], 0.0f);
+      }
+  }
+  __syncthreads();
+  {
+    f32x4 a2[kRgbH2 / 16];
+    hzero(a2);
+    hydra_mm<PREC, kRgbH2 / 16, kRgbH1, LH>(ws + so.r2w, kRgbH2, hs, lane, a2);
+    __syncthreads();
+#pragma unroll
+    for (int mt = 0; mt < kRgbH2 / 16; ++mt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int u = 16 * mt + 4 * g + i;
+        hs[n * LH + u] = fmaxf(a2[mt][i] + ws[so.r2b + u], 0.0f);
+      }
+  }
+  __syncthreads();
+  {
+    f32x4 a3[1];
+    hzero(a3);
+    hydra_mm<PREC, 1, kRgbH2, LH>(ws + so.r3w, 3, hs, lane, a3);
+    const long long item = item0 + (16 * wave + n) / VP;
+    if (g == 0 && view == 0 && item < n_valid) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        out_rgb[(long long)list[item] * 3 + j] = sigmoidf(a3[0][j] + ws[so.r3b + j]);
+    }
+  }
+}
 
 // ---- Backward of the sample kernel: compositing reverse scan + MLP ---------
 // Per ray (list order), with per-ray recompute instead of stored activations:
@@ -2160,34 +2428,95 @@ size_t workspace_token_offset(long long n_rays) {
   return 256 + ((size_t)n_rays * 4 + 255) / 256 * 256;
 }
 
-__global__ void queue_reset(unsigned* counter) {
-  if (threadIdx.x == 0) counter[0] = 0u;
+// Lists every ray with a full set of samples (t[ray, 0] >= 0) in ray order
+// and writes the outputs of dropped rays (zero colour, depth and sums).
+// Three launches (no atomics, so the list is the same on every run and
+// follows the image's raster order, which the sample kernel's XCD banding
+// relies on): per block of kCompactRays rays the number of listed rays; one
+// workgroup scans the block counts (and stores the total); each block writes
+// its rays at its offset.  The block counts live at the start of the token
+// area of the workspace, which the sample kernel overwrites afterwards.
+constexpr int kCompactRays = 1024;   // rays per compaction block (4 per thread)
+
+__device__ __forceinline__ int compact_valid(const float* t, int S, long long n_rays,
+                                             long long ray) {
+  return ray < n_rays && t[ray * S] >= 0.0f;
 }
 
-// Appends every ray with a full set of samples (t[ray, 0] >= 0) to the list
-// (one atomic per wave; list order is arbitrary, results do not depend on it)
-// and writes the outputs of dropped rays (zero colour, depth and sums).
+// inclusive scan of one int per thread over a 256-thread block; returns the
+// exclusive prefix and stores the block total in *total
+__device__ __forceinline__ int block_exclusive_scan(int v, int* total) {
+  __shared__ int s_w[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_w[wid] = x;
+  __syncthreads();
+  int before = 0, all = 0;
+  for (int w = 0; w < 4; ++w) {
+    if (w < wid) before += s_w[w];
+    all += s_w[w];
+  }
+  __syncthreads();
+  *total = all;
+  return before + x - v;
+}
+
 __global__ void __launch_bounds__(256)
-compact_rays(const float* __restrict__ t, int S, long long n_rays, unsigned* counter,
-             int* list, float* out_rgb, float* out_depth, float* out_aux) {
-  const long long ray = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool in = ray < n_rays;
-  const bool valid = in && t[ray * S] >= 0.0f;
-  const unsigned long long mask = __ballot(valid);
-  const int lane = threadIdx.x & 63;
-  unsigned base = 0;
-  if (lane == 0 && mask) base = atomicAdd(counter, (unsigned)__popcll(mask));
-  base = __shfl(base, 0, 64);
-  if (valid) {
-    list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int)ray;
-  } else if (in) {
-    out_rgb[ray * 3 + 0] = 0.0f;
-    out_rgb[ray * 3 + 1] = 0.0f;
-    out_rgb[ray * 3 + 2] = 0.0f;
-    out_depth[ray] = 0.0f;
-    if (out_aux) {
-      out_aux[ray * 2 + 0] = 0.0f;
-      out_aux[ray * 2 + 1] = 0.0f;
+compact_count(const float* __restrict__ t, int S, long long n_rays, int* __restrict__ bcnt) {
+  const long long base = (long long)blockIdx.x * kCompactRays + threadIdx.x * 4;
+  int c = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) c += compact_valid(t, S, n_rays, base + i);
+  int total;
+  block_exclusive_scan(c, &total);
+  if (threadIdx.x == 0) bcnt[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(256)
+compact_scan(int* __restrict__ bcnt, int nb, unsigned* __restrict__ counter) {
+  int carry = 0;
+  for (int b0 = 0; b0 < nb; b0 += 256) {
+    const int b = b0 + threadIdx.x;
+    const int v = b < nb ? bcnt[b] : 0;
+    int total;
+    const int ex = block_exclusive_scan(v, &total);
+    if (b < nb) bcnt[b] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) counter[0] = (unsigned)carry;
+}
+
+__global__ void __launch_bounds__(256)
+compact_write(const float* __restrict__ t, int S, long long n_rays, const int* __restrict__ bcnt,
+              int* __restrict__ list, float* __restrict__ out_rgb, float* __restrict__ out_depth,
+              float* __restrict__ out_aux) {
+  const long long base = (long long)blockIdx.x * kCompactRays + threadIdx.x * 4;
+  int v[4], c = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    v[i] = compact_valid(t, S, n_rays, base + i);
+    c += v[i];
+  }
+  int total;
+  int pos = bcnt[blockIdx.x] + block_exclusive_scan(c, &total);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const long long ray = base + i;
+    if (v[i]) {
+      list[pos++] = (int)ray;
+    } else if (ray < n_rays) {
+      out_rgb[ray * 3 + 0] = 0.0f;
+      out_rgb[ray * 3 + 1] = 0.0f;
+      out_rgb[ray * 3 + 2] = 0.0f;
+      out_depth[ray] = 0.0f;
+      if (out_aux) {
+        out_aux[ray * 2 + 0] = 0.0f;
+        out_aux[ray * 2 + 1] = 0.0f;
+      }
     }
   }
 }
@@ -2252,6 +2581,16 @@ int launch_kernels(const RenderArgs& a, hipStream_t st) {
   if (POINTS || !(a.stages & 4)) return SAILOR_OK;
   constexpr int rpg = kTok / VP;
   const long long groups = (a.n_rays + rpg - 1) / rpg;
+  if constexpr (PREC != kF32) {
+    // 16-bit forms: Hydra + RGB head on MFMA, unless the caller differentiates
+    // the render (its backward recomputes them in fp32)
+    if (!a.fp32_hydra) {
+      hipLaunchKernelGGL((hydra_rgb_mfma_kernel<F, H, VP, PREC>), dim3((unsigned)groups),
+                         dim3(kThreads), 0, st, a.out_tok, a.list, a.n_valid, a.wsmall, a.V,
+                         a.out_rgb);
+      return sailor::check_launch("render_rays (hydra, mfma)");
+    }
+  }
   hipLaunchKernelGGL((hydra_rgb_kernel<F, H, VP>), dim3((unsigned)groups), dim3(kThreads), 0,
                      st, a.out_tok, a.list, a.n_valid, a.wsmall, a.V, a.out_rgb);
   return sailor::check_launch("render_rays (hydra)");
@@ -2378,6 +2717,7 @@ int render_stages(int stages, const float* t, const float* rays, int B, int R, i
   a.n_valid = counter;
   a.S = S; a.V = V; a.img_w = img_w; a.img_h = img_h; a.Hf = Hf; a.Wf = Wf;
   a.t_thres = t_thresh; a.early_exit = early_exit & 1; a.compensated = (early_exit >> 1) & 1;
+  a.fp32_hydra = (early_exit >> 2) & 1;
   a.fscale = feature_scales;
   a.grid_share = share;
   SAILOR_REQUIRE(feature_scales == nullptr || fmap_precision == precision,
@@ -2403,10 +2743,13 @@ int render_stages(int stages, const float* t, const float* rays, int B, int R, i
   // the counter is cleared by a kernel, not a memset, so the call can be
   // captured into a graph
   if (stages & 1) {
-    hipLaunchKernelGGL(queue_reset, dim3(1), dim3(64), 0, st, counter);
-    if (int err = sailor::check_launch("render_rays (reset)")) return err;
-    hipLaunchKernelGGL(compact_rays, dim3((unsigned)((a.n_rays + 255) / 256)), dim3(256), 0, st,
-                       t, S, a.n_rays, counter, list, out_rgb, out_depth, out_aux);
+    const long long nb = (a.n_rays + kCompactRays - 1) / kCompactRays;
+    int* bcnt = reinterpret_cast<int*>(a.out_tok);   // free until the sample kernel runs
+    hipLaunchKernelGGL(compact_count, dim3((unsigned)nb), dim3(256), 0, st, t, S, a.n_rays,
+                       bcnt);
+    hipLaunchKernelGGL(compact_scan, dim3(1), dim3(256), 0, st, bcnt, (int)nb, counter);
+    hipLaunchKernelGGL(compact_write, dim3((unsigned)nb), dim3(256), 0, st, t, S, a.n_rays,
+                       bcnt, list, out_rgb, out_depth, out_aux);
     if (int err = sailor::check_launch("render_rays (compact)")) return err;
   }
   if (!(stages & 6)) return SAILOR_OK;

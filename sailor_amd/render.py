@@ -104,13 +104,16 @@ class PackedModel:
 
 def render_rays(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, model: PackedModel,
                 t_thres=1e-4, early_exit=False, want_occ=False, out=None, chunk_samples=0,
-                compensated=False, feature_scales=None):
+                compensated=False, feature_scales=None, fp32_hydra=False):
     """t [B,R,S] sorted; rays [B,R,6]; Kv [B,V,3,3]; RTv [B,V,3,4];
     fmap_cl [B,V,Hf,Wf,F] channels-last, float32 or the model's 16-bit dtype;
     bbox [B,3,2]; zdir [B,3].  compensated: Kahan-compensated compositing
     sums (precision form).  feature_scales [F]: fmap_cl holds maps quantised
     by sailor_quantize_channels with these scales, and the model's first
-    layer carries them (FramePipeline with quantize=True).
+    layer carries them (FramePipeline with quantize=True).  fp32_hydra: the
+    16-bit forms run the Hydra encoder and RGB head on fp32 vector ALUs
+    instead of 16-bit MFMA (the differentiation form, whose backward
+    recomputes them in fp32).
     Returns dict(rgb [B,R,3], depth [B,R], aux [B,R,2], occ [B,R,S] or None)."""
     dev = t.device
     check_input(t, "t", _f32)
@@ -151,7 +154,7 @@ def render_rays(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, model: PackedMode
     ws_bytes = _lib.lib().sailor_render_workspace_bytes_views(B, R, F, V)
     if out.get("workspace") is None or out["workspace"].numel() < ws_bytes:
         out["workspace"] = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
-    flags = int(bool(early_exit)) | (2 if compensated else 0)
+    flags = int(bool(early_exit)) | (2 if compensated else 0) | (4 if fp32_hydra else 0)
     if feature_scales is not None:
         _check(feature_scales, "feature_scales", _f32, (F,), dev)
         call("sailor_render_rays_quantized", 7, ptr(t), ptr(rays), B, R, S, ptr(Kv), ptr(RTv), V,

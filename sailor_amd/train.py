@@ -192,9 +192,11 @@ class RenderTrainer:
         o = self._out
         if o is not None and (tuple(o["occ"].shape) != (B, R, S) or o["occ"].device != t.device):
             self._out = None
+        # the backward recomputes the Hydra encoder and RGB head in fp32 from
+        # the tokens, so the forward evaluates them the same way
         self._out = render.render_rays(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir,
                                        self.model, t_thres, early_exit=False, want_occ=True,
-                                       out=self._out)
+                                       out=self._out, fp32_hydra=True)
         self._ctx = (t, rays, Kv, RTv, img_hw, fmap_cl, bbox, t_thres)
         return self._out
 

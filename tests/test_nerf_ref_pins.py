@@ -125,3 +125,27 @@ def test_render_rays_layers_match_nn_linear():
                              _linear(w3, b3), nn.Sigmoid())
         np.testing.assert_allclose(rgb[0].numpy(), head(tok0).numpy(), rtol=1e-12, atol=1e-12)
     assert float(occ.std()) > 1e-3                            # a non-degenerate case
+
+
+def test_sixteen_bit_product_emulation():
+    """_mm's 16-bit emulation: bf16 rounds both operands as they are; fp16
+    divides each row of x by the power of two >= its max |x|, which for
+    rows inside fp16's normal range changes no rounding (checked against the
+    plain rounded product) and keeps a row of magnitude 1e5 finite where
+    plain rounding overflows to inf."""
+    rng = np.random.default_rng(7)
+    x = torch.tensor(rng.uniform(0.5, 1.0, (6, 32)) * rng.choice([-1, 1], (6, 32)), dtype=D)
+    w = torch.tensor(rng.standard_normal((5, 32)) * 0.2, dtype=D)
+    rd = lambda z, dt: z.to(dt).to(D)
+    for dt in (torch.bfloat16, torch.float16):
+        want = rd(x, dt) @ rd(w, dt).T
+        np.testing.assert_allclose(nerf_ref._mm(x, w, dt).numpy(), want.numpy(), rtol=0, atol=0)
+    big = x.clone()
+    big[2] *= 1e5
+    plain = rd(big, torch.float16) @ rd(w, torch.float16).T
+    assert not bool(torch.isfinite(plain[2]).all())
+    got = nerf_ref._mm(big, w, torch.float16)
+    assert bool(torch.isfinite(got).all())
+    np.testing.assert_allclose(got[2].numpy(), (big[2] @ w.T).numpy(), rtol=5e-3, atol=1e-3 * 1e5)
+    np.testing.assert_allclose(got[[0, 1, 3, 4, 5]].numpy(), plain[[0, 1, 3, 4, 5]].numpy(),
+                               rtol=0, atol=0)

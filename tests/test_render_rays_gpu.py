@@ -258,3 +258,37 @@ def test_every_launch_shape_renders_the_same_image(views, prec):
     p.chunk_samples = 0
     assert len(ran) >= 4, ran
     print(views, prec, "launch shapes checked:", ran)
+
+
+@pytest.mark.parametrize("views,prec", [(4, "bf16"), (4, "fp16"), (6, "bf16")])
+def test_mfma_hydra_matches_rounded_float64_on_the_same_tokens(views, prec):
+    """The 16-bit forms' Hydra encoder + RGB head on MFMA, fed the tokens
+    the sample kernel left in the workspace, against the float64
+    restatement with every product's operands rounded to the same 16-bit
+    type (nerf_ref.hydra_rgb(mm_dtype)): the remaining difference is fp32
+    against float64 accumulation (|d rgb| <= 2e-4)."""
+    from sailor_amd.pipeline import FrameConfig
+    from sailor_amd.train import workspace_views
+    cfg = FrameConfig(f"hyd_v{views}", views, 512, 600.0, 128, 8, 48, 56.25, 48, "c1")
+    p = pipe(prec, cfg)
+    p.shade()
+    torch.cuda.synchronize()
+    n_rays = cfg.target ** 2
+    vp = 4 if views <= 4 else 8
+    n_valid = int(p.render_ws[:4].view(torch.int32)[0].item())
+    lst = p.render_ws[256:256 + 4 * n_rays].view(torch.int32)[:n_valid].long()
+    toff = 256 + (n_rays * 4 + 255) // 256 * 256
+    tok = p.render_ws[toff:toff + n_valid * vp * p.mc.F * 4].view(torch.float32)
+    tok = tok.view(n_valid, vp, p.mc.F)[:, :views].double().cpu()
+    assert n_valid > 50
+    w = p.weights
+    D = lambda a: torch.tensor(np.asarray(a), dtype=torch.float64)
+    hyd = [{k: D(v) for k, v in st.items()} for st in w.hydra]
+    rgbw = [(D(a), D(b)) for a, b in w.rgb]
+    dt = torch.bfloat16 if prec == "bf16" else torch.float16
+    ref = nerf_ref.hydra_rgb(hyd, rgbw, tok, mm_dtype=dt).numpy()
+    got = p.rgb[0][lst.cuda()].double().cpu().numpy()
+    err = np.abs(got - ref)
+    print(f"{views} views {prec}: MFMA Hydra vs rounded float64 max {err.max():.2e} "
+          f"(99.9% {np.quantile(err, 0.999):.2e})")
+    assert np.quantile(err, 0.999) <= 2e-4 and err.max() <= 2e-3, err.max()
