@@ -56,9 +56,6 @@ def test_occ_forward_bitexact(F, S, R):
 @pytest.mark.parametrize("occ_mode", [True, False])
 def test_training_forward_backward(occ_mode, F, S, R):
     from sailor_amd import VoxelEncoding as VE
-    if R > 1000 and not occ_mode:
-        pytest.skip("density form checked at small ray counts: its tolerance covers 1-ulp "
-                    "expf differences, which can flip the early-termination test of a ray")
     rng = np.random.default_rng(3 + F + S)
     B = 2
     t = rng.uniform(0.5, 4.0, (B, R, S)).astype(np.float32)
@@ -84,15 +81,33 @@ def test_training_forward_backward(occ_mode, F, S, R):
                                                 1e-4, occ_mode)
     ref = orc.render_train_fwd(t, order, sigma, feats, 1e-4, occ_mode)
     tol = 0 if occ_mode else 2e-5
+    # density form: the device expf and the C library's differ by up to 1 ulp,
+    # which can move a ray's transmittance across the early-termination
+    # threshold; such a ray then legitimately differs in every output.  At
+    # many rays a budget of 1e-4 of the rays (at least 2) may do so; all other
+    # rays meet the tolerance.
+    budget = 0 if (occ_mode or R <= 1000) else max(2, int(1e-4 * B * R))
+    bad = np.zeros(B * R, bool)
+
+    def check(got, want, tol_):
+        nonlocal bad
+        got = got.reshape(want.shape)
+        off = ~np.isclose(got, want, rtol=tol_, atol=tol_)
+        bad |= off.reshape(B * R, -1).any(1)
+
     for a, b in zip(outs, ref):
-        np.testing.assert_allclose(a.cpu().numpy().reshape(b.shape), b, rtol=tol, atol=tol)
+        check(a.cpu().numpy(), b, tol)
+    fwd_bad = int(bad.sum())
     gf, gs = VE.volume_rendering_training_backward(*[cu(x) for x in g], *outs, cu(t), cu(order),
                                                    cu(sigma), cu(feats), 0, 1e-4, occ_mode)
     rgf, rgs = orc.render_train_bwd(*g, *[o.cpu().numpy() for o in outs], t, order, sigma,
                                     feats, 1e-4, occ_mode)
-    np.testing.assert_allclose(gf.cpu().numpy(), rgf, rtol=tol, atol=tol)
-    np.testing.assert_allclose(gs.cpu().numpy()[..., 0], rgs, rtol=1e-4 if not occ_mode else 0,
-                               atol=1e-4 if not occ_mode else 0)
+    check(gf.cpu().numpy(), rgf, tol)
+    check(gs.cpu().numpy()[..., 0], rgs, 0 if occ_mode else 1e-4)
+    if budget:
+        print(f"density form, {B * R} rays: {fwd_bad} forward / {int(bad.sum())} total rays "
+              f"outside the tolerance (budget {budget})")
+    assert int(bad.sum()) <= budget, (int(bad.sum()), budget, np.nonzero(bad)[0][:10])
 
 
 def test_freq_encoding():

@@ -207,7 +207,8 @@ def test_render_more_views_matches_float64_reference(views):
 
 def test_batch_graph_matches_eager_frames():
     """Frames of uneven size (2-6 views, own scale) captured into one graph
-    over 4 streams: each replay writes exactly each frame's eager image."""
+    over 4 streams: each replay writes exactly each frame's eager image, and
+    the replayed images match the float64 restatement."""
     from sailor_amd.pipeline import BatchPipeline, FrameConfig
     base = FrameConfig("c5_small", 4, 256, 300.0, 128, 8, 40, 47.0, 48, "c1")
     bp = BatchPipeline(base, n_frames=5, prec="fp32", seed=77, workers=1)
@@ -226,6 +227,23 @@ def test_batch_graph_matches_eager_frames():
         torch.cuda.synchronize()
         for p, (r, d) in zip(bp.pipes, eager):
             assert torch.equal(p.rgb, r) and torch.equal(p.depth, d)
+    # the replayed images against the float64 restatement, on the frames
+    # with the fewest and the most views (4- and 8-view-slot kernels)
+    views = [c.n_views for c in bp.cfgs]
+    for i in sorted({int(np.argmin(views)), int(np.argmax(views))}):
+        p = bp.pipes[i]
+        valid = torch.nonzero(p.sorted[0, :, 0] >= 0).flatten().cpu().numpy()
+        sub = valid[np.linspace(0, len(valid) - 1, min(120, len(valid))).astype(int)]
+        rgb, dep = nerf_ref.render_rays(p.weights, p.sorted[:, sub].cpu().numpy(),
+                                        p.rays[:, sub].cpu().numpy(), p.K.cpu().numpy(),
+                                        p.RT.cpu().numpy(), p.fmaps_np[None],
+                                        p.bbox.cpu().numpy(), p.zdir.cpu().numpy(),
+                                        (p.cfg.img, p.cfg.img))[:2]
+        err_rgb = np.abs(p.rgb[0, sub].cpu().numpy() - rgb[0].numpy()).max()
+        err_dep = np.abs(p.depth[0, sub].cpu().numpy() - dep[0].numpy()).max()
+        print(f"batched frame {i} ({views[i]} views): max|rgb| err {err_rgb:.2e}, "
+              f"max|depth| err {err_dep:.2e}")
+        assert err_rgb < 1e-4 and err_dep < 5e-4
 
 
 @pytest.mark.parametrize("views,prec", [(4, "fp32"), (4, "bf16"), (6, "fp32"), (6, "bf16")])
