@@ -873,22 +873,14 @@ render_kernel(RenderArgs a) {
 
 // ---- Hydra encoder (2 stacks) + RGB head on the composited tokens ---------
 // (aggregation.py:101-128,164-171,299-307; nerf.py:117-124; infer_mlp.cu
-// hydra_attention_kernel :242-445).  Token t of a workgroup is (ray t/4 of
-// the group, view t%4); LDS holds feature-major rows [row][token] padded to
-// kLd floats so both the row reads and the transposing token load are free of
-// bank conflicts.
+// hydra_attention_kernel :242-445).  A group is 16 rays x 4 view slots = 64
+// tokens (token t: ray t/4, view t%4).  The VALU helpers below keep
+// feature-major rows [row][token] padded to kLd floats (row reads and the
+// transposing token load free of bank conflicts); they serve the backward
+// of the F = 64 model.
 constexpr int kRaysPerGroup = 16;          // rays per group with 4 view slots
 constexpr int kTok = kRaysPerGroup * kVP;  // 64 = one token per lane
 constexpr int kLd = kTok + 1;
-
-template <int F>
-struct HydraRows {
-  static constexpr int X = 0;                  // layer input / output [F]
-  static constexpr int QKV = F;                // q, k, v [96]; RGB layer 1 [64]
-  static constexpr int OUT = F + 3 * kHydraInner;  // attention output [F]
-  static constexpr int HID = OUT + F;          // FFN hidden [32]; RGB layer 2 [32]
-  static constexpr int ROWS = HID + kHydraInner;
-};
 
 // rows[o][lane] = act(bias[o] + sum_k W[o][k] in[k]) for this wave's NO
 // outputs starting at o0; W rows are wave-uniform so they are fetched once
@@ -914,159 +906,19 @@ __device__ __forceinline__ void load_rows(const float* rows, int lane, float (&d
   for (int k = 0; k < N; ++k) dst[k] = rows[k * LD + lane];
 }
 
-// VP tokens per ray (view slots): a group is 64 / VP rays, lane = ray * VP + slot.
-template <int F, int H, int VP>
-__global__ void __launch_bounds__(kThreads)
-hydra_rgb_kernel(const float* __restrict__ tok, const int* __restrict__ list,
-                 const unsigned* __restrict__ n_valid_ptr, const float* __restrict__ ws, int V,
-                 float* __restrict__ out_rgb) {
-  constexpr int RPG = kTok / VP;  // rays per group
-  typedef HydraRows<F> R;
-  constexpr SmallOffsets so = small_offsets(F, H);
-  constexpr int stack_stride = so.q[1] - so.q[0];
-  static_assert(F % kWaves == 0 && kRgbH1 % kWaves == 0 && kRgbH2 % kWaves == 0, "split");
-  __shared__ float sm[R::ROWS * kLd];
-
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  // the group is list entries [item0, item0 + 16); the grid covers every ray
-  // and groups past the device-side count leave at once
-  const long long n_valid = *n_valid_ptr;
-  const long long item0 = (long long)blockIdx.x * RPG;
-  if (item0 >= n_valid) return;
-  const int view = lane % VP;
-  const long long item = item0 + lane / VP;
-  const bool valid = item < n_valid;
-  // tokens of the group are one contiguous block of 64 x F floats
-  const long long n_floats = (n_valid - item0) * VP * F;
-  for (int i = tid; i < kTok * F; i += kThreads) {
-    const int tk = i / F, c = i % F;
-    sm[(R::X + c) * kLd + tk] = i < n_floats ? tok[item0 * VP * F + i] : 0.0f;
-  }
-  __syncthreads();
-  const float vmask = view < V ? 1.0f : 0.0f;
-
-  for (int st = 0; st < kHydraStacks; ++st) {
-    const float* wq = ws + so.q[0] + st * stack_stride;  // q, k, v rows are contiguous
-    {
-      float x[F];
-      load_rows(sm + R::X * kLd, lane, x);
-      constexpr int NO = 3 * kHydraInner / kWaves;
-      token_dense<F, NO, false>(wq, nullptr, x, wave * NO, sm + R::QKV * kLd, lane);
-    }
-    __syncthreads();
-    float attn[kHydraInner];
-    {
-      // F.normalize(q), F.normalize(k) (eps 1e-12); kv = sum_views khat * v
-      float qq = 0.0f, kk = 0.0f;
-#pragma unroll
-      for (int e = 0; e < kHydraInner; ++e) {
-        const float q = sm[(R::QKV + e) * kLd + lane];
-        const float k = sm[(R::QKV + kHydraInner + e) * kLd + lane];
-        qq += q * q;
-        kk += k * k;
-      }
-      const float nq = 1.0f / fmaxf(sqrtf(qq), 1e-12f);
-      const float nk = vmask / fmaxf(sqrtf(kk), 1e-12f);
-#pragma unroll
-      for (int e = 0; e < kHydraInner; ++e) {
-        const float q = sm[(R::QKV + e) * kLd + lane];
-        const float k = sm[(R::QKV + kHydraInner + e) * kLd + lane];
-        const float v = sm[(R::QKV + 2 * kHydraInner + e) * kLd + lane];
-        float kv = k * nk * v;
-#pragma unroll
-        for (int off = 1; off < VP; off <<= 1) kv += __shfl_xor(kv, off, 64);
-        attn[e] = q * nq * kv;
-      }
-    }
-    // out = fc(attn) + x
-    {
-      constexpr int NO = F / kWaves;
-      const float* wf = ws + so.fc[0] + st * stack_stride;
-#pragma unroll
-      for (int i = 0; i < NO; ++i) {
-        const int c = wave * NO + i;
-        const float* wr = wf + c * kHydraInner;
-        float acc = 0.0f;
-#pragma unroll
-        for (int e = 0; e < kHydraInner; ++e) acc += wr[e] * attn[e];
-        sm[(R::OUT + c) * kLd + lane] = acc + sm[(R::X + c) * kLd + lane];
-      }
-    }
-    __syncthreads();
-    // FFN hidden: relu(W1 out + b1)
-    {
-      float o[F];
-      load_rows(sm + R::OUT * kLd, lane, o);
-      constexpr int NO = kHydraInner / kWaves;
-      token_dense<F, NO, true>(ws + so.w1[0] + st * stack_stride,
-                               ws + so.b1[0] + st * stack_stride, o, wave * NO,
-                               sm + R::HID * kLd, lane);
-    }
-    __syncthreads();
-    // x = W2 h + b2 + out
-    {
-      float h[kHydraInner];
-      load_rows(sm + R::HID * kLd, lane, h);
-      constexpr int NO = F / kWaves;
-      const float* w2 = ws + so.w2[0] + st * stack_stride;
-      const float* b2 = ws + so.b2[0] + st * stack_stride;
-#pragma unroll
-      for (int i = 0; i < NO; ++i) {
-        const int c = wave * NO + i;
-        const float* wr = w2 + c * kHydraInner;
-        float acc = b2[c];
-#pragma unroll
-        for (int e = 0; e < kHydraInner; ++e) acc += wr[e] * h[e];
-        sm[(R::X + c) * kLd + lane] = acc + sm[(R::OUT + c) * kLd + lane];
-      }
-    }
-    __syncthreads();
-  }
-  // ---- RGB head on the first view's token: F -> 64 -> 32 -> 3 -------------
-  {
-    float x[F];
-    load_rows(sm + R::X * kLd, lane, x);
-    constexpr int NO = kRgbH1 / kWaves;
-    token_dense<F, NO, true>(ws + so.r1w, ws + so.r1b, x, wave * NO, sm + R::QKV * kLd, lane);
-  }
-  __syncthreads();
-  {
-    float r1[kRgbH1];
-    load_rows(sm + R::QKV * kLd, lane, r1);
-    constexpr int NO = kRgbH2 / kWaves;
-    token_dense<kRgbH1, NO, true>(ws + so.r2w, ws + so.r2b, r1, wave * NO, sm + R::HID * kLd,
-                                  lane);
-  }
-  __syncthreads();
-  if (wave == 0 && view == 0 && valid) {
-    float r2[kRgbH2];
-    load_rows(sm + R::HID * kLd, lane, r2);
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const float* wr = ws + so.r3w + j * kRgbH2;
-      float acc = ws[so.r3b + j];
-#pragma unroll
-      for (int k = 0; k < kRgbH2; ++k) acc += wr[k] * r2[k];
-      out_rgb[(long long)list[item] * 3 + j] = sigmoidf(acc);
-    }
-  }
-}
-
-
-
-// ---- Hydra encoder + RGB head on 16-bit MFMA (the 16-bit forms) ----------
-// Same function as hydra_rgb_kernel with every dense product on
-// v_mfma_f32_16x16x32_{bf16,f16}: operands rounded to the form's 16-bit
-// type, fp32 accumulation; normalisations, the cross-view sum, residuals,
-// biases and activations stay in fp32.  Each wave owns 16 tokens (16 / VP
-// rays): the MFMA's N = 16 columns are the tokens, M = 16 output units per
-// tile, K = 32 inputs per step.  A fragments are read straight from the fp32
-// small-weight buffer (lane l: row 16 mt + l%16, inputs 32 s + 8 (l/16) ..
-// +8); B fragments from the wave's LDS rows [token][input]; the result
-// (lane l: units 16 mt + 4 (l/16) + i of token l%16) is post-processed in
-// registers and written back to LDS for the next product.  The cross-view
-// sum of k^ v is a shuffle over the lanes of one ray's tokens.
+// ---- Hydra encoder + RGB head on MFMA (every form) ------------------------
+// Every dense product on MFMA with fp32 accumulation: the 16-bit forms on
+// v_mfma_f32_16x16x32_{bf16,f16} (operands rounded to the form's type), the
+// fp32 form -- and every render that is differentiated -- on
+// v_mfma_f32_16x16x4f32; normalisations, the cross-view sum, residuals,
+// biases and activations stay in fp32 VALU.  Each wave owns 16 tokens
+// (16 / VP rays): the MFMA's N = 16 columns are the tokens, M = 16 output
+// units per tile.  A fragments are read straight from the fp32 small-weight
+// buffer (lane l: row 16 mt + l%16), B fragments from the wave's LDS rows
+// [token][input]; the result (lane l: units 16 mt + 4 (l/16) + i of token
+// l%16) is post-processed in registers and written back to LDS for the next
+// product.  The cross-view sum of k^ v is a shuffle over the lanes of one
+// ray's tokens.
 template <int PREC> struct HydraOp;
 template <> struct HydraOp<kBF16> { typedef bf16x8 V; typedef __bf16 E; };
 template <> struct HydraOp<kF16> { typedef f16x8 V; typedef _Float16 E; };
@@ -1099,9 +951,47 @@ __device__ __forceinline__ typename HydraOp<PREC>::V hcvt8(const float* p) {
 // are divided by a power of two sigma >= their max |x| before rounding and
 // the token's results multiplied by sigma after (exact), so inputs beyond
 // fp16's range (65504) cannot overflow; bf16 has fp32's range and needs none.
+// fp32 form: lane group g takes inputs [g K/4, (g+1) K/4) (any assignment of
+// inputs to (step, group) pairs sums the same products), so a lane's A and
+// B values are K/4 contiguous floats: 16-byte loads from the weights and
+// from LDS.
+template <int MT, int K, int LD>
+__device__ __forceinline__ void hydra_mm32(const float* __restrict__ W, int n_rows,
+                                           const float* src, int lane, f32x4 (&acc)[MT]) {
+  constexpr int KQ = K / 4;
+  static_assert(KQ % 4 == 0 && LD % 4 == 0, "16-byte operand runs");
+  const int n = lane & 15, g = lane >> 4;
+  float b[KQ];
+#pragma unroll
+  for (int s = 0; s < KQ; s += 4) {
+    const f32x4 v = *reinterpret_cast<const f32x4*>(src + n * LD + g * KQ + s);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[s + j] = v[j];
+  }
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const int row = 16 * mt + n;
+    float a[KQ];
+#pragma unroll
+    for (int s = 0; s < KQ; s += 4) {
+      const f32x4 v = row < n_rows
+                          ? *reinterpret_cast<const f32x4*>(W + (size_t)row * K + g * KQ + s)
+                          : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) a[s + j] = v[j];
+    }
+#pragma unroll
+    for (int s = 0; s < KQ; ++s)
+      acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], b[s], acc[mt], 0, 0, 0);
+  }
+}
+
 template <int PREC, int MT, int K, int LD>
 __device__ __forceinline__ void hydra_mm(const float* __restrict__ W, int n_rows,
                                          const float* src, int lane, f32x4 (&acc)[MT]) {
+  if constexpr (PREC == kF32) {
+    hydra_mm32<MT, K, LD>(W, n_rows, src, lane, acc);
+  } else {
   typedef typename HydraOp<PREC>::V V;
   const int n = lane & 15, g = lane >> 4;
   float sigma = 1.0f, inv_sigma = 1.0f;
@@ -1147,6 +1037,7 @@ __device__ __forceinline__ void hydra_mm(const float* __restrict__ W, int n_rows
   if constexpr (PREC == kF16) {
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) acc[mt] *= sigma;
+  }
   }
 }
 
@@ -2403,6 +2294,543 @@ hydra_rgb_backward_kernel(const float* __restrict__ tok, const int* __restrict__
   }
 }
 
+// ---- Backward of the Hydra encoder + RGB head on fp32 MFMA (F = 32) --------
+// One wave per workgroup and 16 tokens (4 rays x 4 view slots) per step of a
+// persistent loop over the list.  The three kinds of product all run on
+// v_mfma_f32_16x16x4f32 from LDS rows [token][unit] of pitch LD:
+//   y = W x        (forward recompute)   hydra_mm32: lane holds units of a token
+//   dx = W^T dy    (input gradient)      bmm_wtd: lane holds tokens of a unit
+//   dW += dy x^T   (weight gradient)     bmm_dw: reduction over the 16 tokens
+// The weight gradients of every Hydra and RGB-head tensor stay in the wave's
+// accumulator registers for the whole loop (66 tiles of 16 x 16 at F = 32)
+// and are written once, to the workgroup's own partial slab; a second kernel
+// adds the slabs.  Stack internals are recomputed before each stack's
+// backward (only the stack inputs are kept), as in the forward's order, so
+// the recomputed values equal the fp32 forward's bit for bit.
+template <int F>
+struct HydraBwdSlots {   // floats of one token's LDS row
+  static constexpr int HI = kHydraInner;
+  static constexpr int X0 = 0, X1 = F, X2 = 2 * F;  // stack inputs, final tokens
+  static constexpr int QKV = 3 * F;     // q | k | v [96]; RGB r1 [64] | r2 [32]
+  static constexpr int ATT = QKV + 3 * HI;          // attention values [32]; dz3 [4]
+  static constexpr int OUT = ATT + HI;              // fc(att) + x [F]
+  static constexpr int HID = OUT + F;               // FFN hidden [32]
+  static constexpr int DX = HID + HI;               // dL/d stack output, then input [F]
+  static constexpr int DOUT = DX + F;               // [F]
+  static constexpr int DHID = DOUT + F;             // [32]
+  static constexpr int DATT = DHID + HI;            // [32]
+  static constexpr int DQKV = DATT + HI;            // [96]; RGB dr2 [32] | dr1 [64]
+  static constexpr int NORM = DQKV + 3 * HI;        // 1/|q|, vmask/|k| of the current stack
+  static constexpr int LD = NORM + 4;               // 516 at F = 32: rows 4 banks apart
+  static_assert(3 * HI == kRgbH1 + kRgbH2, "RGB rows share the q|k|v rows");
+};
+
+// acc[nt] = sum_o d[token][o] W[o][16 nt + n] over o < O (dx = W^T d); W is
+// [O][K] row-major in global memory, rows o >= n_o read as zero.  Lane (n =
+// l%16, g = l/16) takes o in [g OQ, (g+1) OQ): A = d[n][o] (contiguous in
+// LDS), B = W[o][16 nt + n].  The result lane holds tokens 4g + i of unit
+// 16 nt + n.
+template <int NT, int O, int K, int LD>
+__device__ __forceinline__ void bmm_wtd(const float* __restrict__ W, int n_o, const float* d,
+                                        int lane, f32x4 (&acc)[NT]) {
+  constexpr int OQ = (O + 3) / 4;
+  const int n = lane & 15, g = lane >> 4;
+  float a[OQ];
+  if constexpr (OQ % 4 == 0) {
+#pragma unroll
+    for (int s = 0; s < OQ; s += 4) {
+      const f32x4 v = *reinterpret_cast<const f32x4*>(d + n * LD + g * OQ + s);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) a[s + j] = v[j];
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < OQ; ++s) a[s] = g * OQ + s < O ? d[n * LD + g * OQ + s] : 0.0f;
+  }
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    acc[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < OQ; ++s) {
+      const int o = g * OQ + s;
+      const float b = o < n_o ? W[(size_t)o * K + 16 * nt + n] : 0.0f;
+      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], b, acc[nt], 0, 0, 0);
+    }
+  }
+}
+
+// acc[mt][nt] += sum over the 16 tokens of d[t][16 mt + o] x[t][16 nt + k]
+// (dW += d x^T; units o >= n_o of d read as zero) and, when db is given,
+// db[mt] += this lane's share of sum_t d[t][16 mt + n].  Step s, lane (n, g): A = d[4s + g][16 mt
+// + n], B = x[4s + g][16 nt + n]; the result lane holds dW[16 mt + 4g + i]
+// [16 nt + n].
+template <int MT, int NT, int LD>
+__device__ __forceinline__ void bmm_dw(const float* d, int n_o, const float* x, int lane,
+                                       f32x4 (&acc)[MT][NT], float* db) {
+  const int n = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int t = 4 * s + g;
+    float b[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) b[nt] = x[t * LD + 16 * nt + n];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const int o = 16 * mt + n;
+      const float av = o < n_o ? d[t * LD + o] : 0.0f;
+      if (db) db[mt] += av;
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b[nt], acc[mt][nt], 0, 0, 0);
+    }
+  }
+}
+
+// rows[4g + i][col0 + 16 nt + n] = v (masked by [mask row > 0] and offset by
+// add row when given) for the token-major layout of bmm_wtd's result.
+template <int NT, int LD>
+__device__ __forceinline__ void bput_t(float* rows, int col0, const f32x4 (&v)[NT],
+                                       const float* mask, int mcol0, const float* add,
+                                       int acol0, int lane) {
+  const int n = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int t = 4 * g + i, u = 16 * nt + n;
+      float x = v[nt][i];
+      if (add) x += add[t * LD + acol0 + u];
+      if (mask && !(mask[t * LD + mcol0 + u] > 0.0f)) x = 0.0f;
+      rows[t * LD + col0 + u] = x;
+    }
+}
+
+// Forward of one Hydra stack (fp32 MFMA) on the wave's 16 token rows: q|k|v,
+// the norms, attention values, out and hidden rows, and the stack output
+// into row block xout when xout >= 0.  Same arithmetic, in the same order,
+// as hydra_rgb_mfma_kernel<.., kF32>.
+template <int F, int H>
+__device__ __forceinline__ void bstack_fwd(const float* __restrict__ ws, int st, float* rw,
+                                           int xin, int xout, int lane, int V) {
+  typedef HydraBwdSlots<F> R;
+  constexpr int LD = R::LD, HI = kHydraInner;
+  constexpr SmallOffsets so = small_offsets(F, H);
+  constexpr int stride = so.q[1] - so.q[0];
+  const int n = lane & 15, g = lane >> 4;
+  const float vmask = (n % kVP) < V ? 1.0f : 0.0f;
+  f32x4 a6[6];
+  hzero(a6);
+  hydra_mm32<6, F, LD>(ws + so.q[0] + st * stride, 3 * HI, rw + xin, lane, a6);
+#pragma unroll
+  for (int mt = 0; mt < 6; ++mt)
+    *reinterpret_cast<f32x4*>(rw + n * LD + R::QKV + 16 * mt + 4 * g) = a6[mt];
+  float qq = 0.0f, kk = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    qq += a6[0][i] * a6[0][i] + a6[1][i] * a6[1][i];
+    kk += a6[2][i] * a6[2][i] + a6[3][i] * a6[3][i];
+  }
+  qq += __shfl_xor(qq, 16, 64);
+  qq += __shfl_xor(qq, 32, 64);
+  kk += __shfl_xor(kk, 16, 64);
+  kk += __shfl_xor(kk, 32, 64);
+  const float nq = 1.0f / fmaxf(sqrtf(qq), 1e-12f);
+  const float nk = vmask / fmaxf(sqrtf(kk), 1e-12f);
+  if (g == 0) {
+    rw[n * LD + R::NORM] = nq;
+    rw[n * LD + R::NORM + 1] = nk;
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      float kv = a6[2 + h][i] * nk * a6[4 + h][i];
+#pragma unroll
+      for (int off = 1; off < kVP; off <<= 1) kv += __shfl_xor(kv, off, 64);
+      rw[n * LD + R::ATT + 16 * h + 4 * g + i] = a6[h][i] * nq * kv;
+    }
+  __syncthreads();
+  {
+    f32x4 ao[F / 16];
+    hzero(ao);
+    hydra_mm32<F / 16, HI, LD>(ws + so.fc[0] + st * stride, F, rw + R::ATT, lane, ao);
+#pragma unroll
+    for (int mt = 0; mt < F / 16; ++mt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int u = 16 * mt + 4 * g + i;
+        rw[n * LD + R::OUT + u] = rw[n * LD + xin + u] + ao[mt][i];
+      }
+  }
+  __syncthreads();
+  {
+    f32x4 ah[2];
+    hzero(ah);
+    hydra_mm32<2, F, LD>(ws + so.w1[0] + st * stride, HI, rw + R::OUT, lane, ah);
+    const float* b1 = ws + so.b1[0] + st * stride;
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int u = 16 * mt + 4 * g + i;
+        rw[n * LD + R::HID + u] = fmaxf(ah[mt][i] + b1[u], 0.0f);
+      }
+  }
+  __syncthreads();
+  if (xout >= 0) {
+    f32x4 ax[F / 16];
+    hzero(ax);
+    hydra_mm32<F / 16, HI, LD>(ws + so.w2[0] + st * stride, F, rw + R::HID, lane, ax);
+    const float* b2 = ws + so.b2[0] + st * stride;
+#pragma unroll
+    for (int mt = 0; mt < F / 16; ++mt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int u = 16 * mt + 4 * g + i;
+        rw[n * LD + xout + u] = rw[n * LD + R::OUT + u] + (ax[mt][i] + b2[u]);
+      }
+    __syncthreads();
+  }
+}
+
+// Weight-gradient accumulators of one Hydra stack (rows x inputs in 16 x 16
+// tiles) and its bias-gradient lane partials.
+template <int F>
+struct StackGrad {
+  f32x4 qkv[6][F / 16];   // [96][F]
+  f32x4 fc[F / 16][2];    // [F][32]
+  f32x4 w1[2][F / 16];    // [32][F]
+  f32x4 w2[F / 16][2];    // [F][32]
+  float b1[2], b2[F / 16];
+};
+
+template <int MT, int NT>
+__device__ __forceinline__ void zero_tiles(f32x4 (&t)[MT][NT]) {
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) t[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+}
+
+// gp[o K + 16 nt + n] = tile entries (o = 16 mt + 4g + i < n_o)
+template <int MT, int NT>
+__device__ __forceinline__ void put_tiles(float* gp, const f32x4 (&t)[MT][NT], int n_o, int K,
+                                          int lane) {
+  const int n = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int o = 16 * mt + 4 * g + i;
+        if (o < n_o) gp[(size_t)o * K + 16 * nt + n] = t[mt][nt][i];
+      }
+}
+
+// gb[16 mt + n] = sum over the four lane groups of db[mt]
+template <int MT>
+__device__ __forceinline__ void put_bias(float* gb, const float (&db)[MT], int n_o, int lane) {
+  const int n = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    float v = db[mt];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    if (g == 0 && 16 * mt + n < n_o) gb[16 * mt + n] = v;
+  }
+}
+
+// Backward of Hydra stack st from dL/d output (DX rows) to dL/d input (DX
+// rows again), accumulating the stack's weight gradients.
+template <int F, int H>
+__device__ __forceinline__ void bstack_bwd(const float* __restrict__ ws, int st, float* rw,
+                                           int xin, int lane, int V, StackGrad<F>& gr) {
+  typedef HydraBwdSlots<F> R;
+  constexpr int LD = R::LD, HI = kHydraInner;
+  constexpr SmallOffsets so = small_offsets(F, H);
+  constexpr int stride = so.q[1] - so.q[0];
+  const int n = lane & 15, g = lane >> 4;
+  bstack_fwd<F, H>(ws, st, rw, xin, -1, lane, V);
+  const float* w1 = ws + so.w1[0] + st * stride;
+  const float* w2 = ws + so.w2[0] + st * stride;
+  const float* wf = ws + so.fc[0] + st * stride;
+  const float* wq = ws + so.q[0] + st * stride;
+  // y = W2 hid + b2 + out
+  bmm_dw<F / 16, 2, LD>(rw + R::DX, F, rw + R::HID, lane, gr.w2, gr.b2);
+  {
+    f32x4 t[2];
+    bmm_wtd<2, F, HI, LD>(w2, F, rw + R::DX, lane, t);
+    bput_t<2, LD>(rw, R::DHID, t, rw, R::HID, nullptr, 0, lane);
+  }
+  __syncthreads();
+  // hid = relu(W1 out + b1); out also feeds y directly
+  bmm_dw<2, F / 16, LD>(rw + R::DHID, HI, rw + R::OUT, lane, gr.w1, gr.b1);
+  {
+    f32x4 t[F / 16];
+    bmm_wtd<F / 16, HI, F, LD>(w1, HI, rw + R::DHID, lane, t);
+    bput_t<F / 16, LD>(rw, R::DOUT, t, nullptr, 0, rw, R::DX, lane);
+  }
+  __syncthreads();
+  // out = fc(att) + x
+  bmm_dw<F / 16, 2, LD>(rw + R::DOUT, F, rw + R::ATT, lane, gr.fc, nullptr);
+  {
+    f32x4 t[2];
+    bmm_wtd<2, F, HI, LD>(wf, F, rw + R::DOUT, lane, t);
+    bput_t<2, LD>(rw, R::DATT, t, nullptr, 0, nullptr, 0, lane);
+  }
+  __syncthreads();
+  // att = q^ * sum_views k^ v with q^ = q nq, k^ = k nk (F.normalize, eps
+  // 1e-12; nk = 0 on masked view slots); lane: token n, units 8g .. 8g + 7
+  {
+    const float nq = rw[n * LD + R::NORM], nk = rw[n * LD + R::NORM + 1];
+    const float vmask = (n % kVP) < V ? 1.0f : 0.0f;
+    float q[8], k[8], v[8], da[8];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const f32x4 q4 = *reinterpret_cast<const f32x4*>(rw + n * LD + R::QKV + 8 * g + 4 * h);
+      const f32x4 k4 = *reinterpret_cast<const f32x4*>(rw + n * LD + R::QKV + HI + 8 * g + 4 * h);
+      const f32x4 v4 =
+          *reinterpret_cast<const f32x4*>(rw + n * LD + R::QKV + 2 * HI + 8 * g + 4 * h);
+      const f32x4 d4 = *reinterpret_cast<const f32x4*>(rw + n * LD + R::DATT + 8 * g + 4 * h);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        q[4 * h + j] = q4[j];
+        k[4 * h + j] = k4[j];
+        v[4 * h + j] = v4[j];
+        da[4 * h + j] = d4[j];
+      }
+    }
+    float dqh[8], dkv[8], qd = 0.0f, kd = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      float kv = k[j] * nk * v[j];
+      float dk = da[j] * q[j] * nq;
+#pragma unroll
+      for (int off = 1; off < kVP; off <<= 1) {
+        kv += __shfl_xor(kv, off, 64);
+        dk += __shfl_xor(dk, off, 64);
+      }
+      dqh[j] = da[j] * kv;     // dL/d q^
+      dkv[j] = dk;             // dL/d (k^ v), summed over the ray's views
+      qd += q[j] * nq * dqh[j];
+      kd += k[j] * nk * (dkv[j] * v[j]);
+    }
+    qd += __shfl_xor(qd, 16, 64);
+    qd += __shfl_xor(qd, 32, 64);
+    kd += __shfl_xor(kd, 16, 64);
+    kd += __shfl_xor(kd, 32, 64);
+    // F.normalize backward: dx = n (dx^ - x^ (x^ . dx^)) when |x| > eps, n dx^
+    // otherwise (n = 1 / max(|x|, eps)); a masked view's k^ is 0 and gets no
+    // gradient
+    const float qn = 1.0f / nq, kn = vmask > 0.0f ? vmask / nk : 0.0f;
+    const bool q_big = qn > 1e-12f, k_big = kn > 1e-12f;
+    float dq[8], dkk[8], dv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const float dkh = dkv[j] * v[j];
+      dq[j] = q_big ? nq * (dqh[j] - q[j] * nq * qd) : nq * dqh[j];
+      dkk[j] = vmask > 0.0f ? (k_big ? nk * (dkh - k[j] * nk * kd) : nk * dkh) : 0.0f;
+      dv[j] = dkv[j] * k[j] * nk;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      float* base = rw + n * LD + R::DQKV + 8 * g + 4 * h;
+      *reinterpret_cast<f32x4*>(base) =
+          f32x4{dq[4 * h], dq[4 * h + 1], dq[4 * h + 2], dq[4 * h + 3]};
+      *reinterpret_cast<f32x4*>(base + HI) =
+          f32x4{dkk[4 * h], dkk[4 * h + 1], dkk[4 * h + 2], dkk[4 * h + 3]};
+      *reinterpret_cast<f32x4*>(base + 2 * HI) =
+          f32x4{dv[4 * h], dv[4 * h + 1], dv[4 * h + 2], dv[4 * h + 3]};
+    }
+  }
+  __syncthreads();
+  // q|k|v = Wqkv x; x also feeds out directly
+  bmm_dw<6, F / 16, LD>(rw + R::DQKV, 3 * HI, rw + xin, lane, gr.qkv, nullptr);
+  {
+    f32x4 t[F / 16];
+    bmm_wtd<F / 16, 3 * HI, F, LD>(wq, 3 * HI, rw + R::DQKV, lane, t);
+    bput_t<F / 16, LD>(rw, R::DX, t, nullptr, 0, rw, R::DOUT, lane);
+  }
+  __syncthreads();
+}
+
+// Part PART of the backward, over every 16-token step of the workgroup:
+//   0: forward of both stacks and the RGB head, dL/dz of the RGB output, the
+//      head's backward -> dL/d final tokens (in dtok);
+//   1: stack 0's forward, stack 1's backward from dtok -> dL/d stack-1 input
+//      (in dtok);
+//   2: stack 0's backward from dtok -> dL/d tokens (in dtok).
+// Three launches keep at most 24 accumulator tiles per kernel in registers
+// (all 66 at once do not fit beside the working set); dtok is read and
+// rewritten in place by the wave that owns the tokens.
+template <int F, int H, int PART>
+__global__ void __launch_bounds__(64)
+hydra_rgb_backward_mfma_kernel(const float* __restrict__ tok, const int* __restrict__ list,
+                               const unsigned* __restrict__ n_valid_ptr,
+                               const float* __restrict__ ws, int V,
+                               const float* __restrict__ grad_rgb, float* __restrict__ dtok,
+                               float* __restrict__ gpart) {
+  typedef HydraBwdSlots<F> R;
+  constexpr int LD = R::LD;
+  constexpr SmallOffsets so = small_offsets(F, H);
+  constexpr int stride = so.q[1] - so.q[0];
+  constexpr int RPW = 16 / kVP;   // rays per step
+  static_assert(F % 16 == 0 && kRgbH1 % 16 == 0 && kRgbH2 % 16 == 0, "whole tiles");
+  __shared__ __attribute__((aligned(16))) float rw[16 * LD];
+  const int lane = threadIdx.x;
+  const int n = lane & 15, g = lane >> 4;
+
+  [[maybe_unused]] StackGrad<F> gs;
+  [[maybe_unused]] f32x4 g_r1[kRgbH1 / 16][F / 16], g_r2[kRgbH2 / 16][kRgbH1 / 16],
+      g_r3[1][kRgbH2 / 16];
+  [[maybe_unused]] float b_r1[kRgbH1 / 16], b_r2[kRgbH2 / 16], b_r3[1];
+  if constexpr (PART == 0) {
+    zero_tiles(g_r1);
+    zero_tiles(g_r2);
+    zero_tiles(g_r3);
+#pragma unroll
+    for (int i = 0; i < kRgbH1 / 16; ++i) b_r1[i] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < kRgbH2 / 16; ++i) b_r2[i] = 0.0f;
+    b_r3[0] = 0.0f;
+  } else {
+    zero_tiles(gs.qkv);
+    zero_tiles(gs.fc);
+    zero_tiles(gs.w1);
+    zero_tiles(gs.w2);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) gs.b1[i] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < F / 16; ++i) gs.b2[i] = 0.0f;
+  }
+
+  const long long n_valid = *n_valid_ptr;
+  for (long long item0 = (long long)blockIdx.x * RPW; item0 < n_valid;
+       item0 += (long long)gridDim.x * RPW) {
+    const long long n_tok = (n_valid - item0) * kVP;   // tokens that exist from item0 on
+    // the weights are re-read from L1/L2 every step: an opaque copy of the
+    // pointer keeps the compiler from hoisting thousands of loop-invariant
+    // weight loads into registers
+    const float* wsl = ws;
+    asm volatile("" : "+s"(wsl));
+    __syncthreads();
+    for (int i = lane; i < 16 * F / 4; i += 64) {
+      const int tk = (4 * i) / F, c = (4 * i) % F;
+      f32x4 v = f32x4{0.0f, 0.0f, 0.0f, 0.0f}, d = v;
+      if (tk < n_tok) {
+        v = *reinterpret_cast<const f32x4*>(tok + (item0 * kVP + tk) * F + c);
+        if (PART > 0) d = *reinterpret_cast<const f32x4*>(dtok + (item0 * kVP + tk) * F + c);
+      }
+      *reinterpret_cast<f32x4*>(rw + tk * LD + R::X0 + c) = v;
+      if (PART > 0) *reinterpret_cast<f32x4*>(rw + tk * LD + R::DX + c) = d;
+    }
+    __syncthreads();
+    if constexpr (PART == 0) {
+      // ---- forward: both stacks, the RGB head, dL/dz of the RGB output ----
+      bstack_fwd<F, H>(wsl, 0, rw, R::X0, R::X1, lane, V);
+      bstack_fwd<F, H>(wsl, 1, rw, R::X1, R::X2, lane, V);
+      {
+        f32x4 a1[kRgbH1 / 16];
+        hzero(a1);
+        hydra_mm32<kRgbH1 / 16, F, LD>(wsl + so.r1w, kRgbH1, rw + R::X2, lane, a1);
+#pragma unroll
+        for (int mt = 0; mt < kRgbH1 / 16; ++mt)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int u = 16 * mt + 4 * g + i;
+            rw[n * LD + R::QKV + u] = fmaxf(a1[mt][i] + wsl[so.r1b + u], 0.0f);
+          }
+      }
+      __syncthreads();
+      {
+        f32x4 a2[kRgbH2 / 16];
+        hzero(a2);
+        hydra_mm32<kRgbH2 / 16, kRgbH1, LD>(wsl + so.r2w, kRgbH2, rw + R::QKV, lane, a2);
+#pragma unroll
+        for (int mt = 0; mt < kRgbH2 / 16; ++mt)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int u = 16 * mt + 4 * g + i;
+            rw[n * LD + R::QKV + kRgbH1 + u] = fmaxf(a2[mt][i] + wsl[so.r2b + u], 0.0f);
+          }
+      }
+      __syncthreads();
+      {
+        f32x4 a3[1];
+        hzero(a3);
+        hydra_mm32<1, kRgbH2, LD>(wsl + so.r3w, 3, rw + R::QKV + kRgbH1, lane, a3);
+        // lane (n, g = 0) holds the three outputs of token n
+        if (g == 0) {
+          const long long item = item0 + n / kVP;
+          const bool on = n % kVP == 0 && item < n_valid;
+          const long long ray = on ? (long long)list[item] : 0;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const float rgb = sigmoidf(a3[0][j] + wsl[so.r3b + j]);
+            const float gr = on ? grad_rgb[ray * 3 + j] : 0.0f;
+            rw[n * LD + R::ATT + j] = gr * rgb * (1.0f - rgb);
+          }
+          rw[n * LD + R::ATT + 3] = 0.0f;
+        }
+      }
+      __syncthreads();
+      // ---- RGB head backward -------------------------------------------------
+      bmm_dw<1, kRgbH2 / 16, LD>(rw + R::ATT, 3, rw + R::QKV + kRgbH1, lane, g_r3, b_r3);
+      {
+        f32x4 t[kRgbH2 / 16];
+        bmm_wtd<kRgbH2 / 16, 3, kRgbH2, LD>(wsl + so.r3w, 3, rw + R::ATT, lane, t);
+        bput_t<kRgbH2 / 16, LD>(rw, R::DQKV, t, rw, R::QKV + kRgbH1, nullptr, 0, lane);
+      }
+      __syncthreads();
+      bmm_dw<kRgbH2 / 16, kRgbH1 / 16, LD>(rw + R::DQKV, kRgbH2, rw + R::QKV, lane, g_r2, b_r2);
+      {
+        f32x4 t[kRgbH1 / 16];
+        bmm_wtd<kRgbH1 / 16, kRgbH2, kRgbH1, LD>(wsl + so.r2w, kRgbH2, rw + R::DQKV, lane, t);
+        bput_t<kRgbH1 / 16, LD>(rw, R::DQKV + kRgbH2, t, rw, R::QKV, nullptr, 0, lane);
+      }
+      __syncthreads();
+      bmm_dw<kRgbH1 / 16, F / 16, LD>(rw + R::DQKV + kRgbH2, kRgbH1, rw + R::X2, lane, g_r1,
+                                      b_r1);
+      {
+        f32x4 t[F / 16];
+        bmm_wtd<F / 16, kRgbH1, F, LD>(wsl + so.r1w, kRgbH1, rw + R::DQKV + kRgbH2, lane, t);
+        bput_t<F / 16, LD>(rw, R::DX, t, nullptr, 0, nullptr, 0, lane);
+      }
+      __syncthreads();
+    } else if constexpr (PART == 1) {
+      bstack_fwd<F, H>(wsl, 0, rw, R::X0, R::X1, lane, V);
+      bstack_bwd<F, H>(wsl, 1, rw, R::X1, lane, V, gs);
+    } else {
+      bstack_bwd<F, H>(wsl, 0, rw, R::X0, lane, V, gs);
+    }
+    for (int i = lane; i < 16 * F / 4; i += 64) {
+      const int tk = (4 * i) / F, c = (4 * i) % F;
+      if (tk < n_tok)
+        *reinterpret_cast<f32x4*>(dtok + (item0 * kVP + tk) * F + c) =
+            *reinterpret_cast<const f32x4*>(rw + tk * LD + R::DX + c);
+    }
+  }
+  // ---- this workgroup's partial gradients ---------------------------------
+  float* gp = gpart + (size_t)blockIdx.x * so.total;
+  if constexpr (PART == 0) {
+    put_tiles(gp + so.r1w, g_r1, kRgbH1, F, lane);
+    put_bias(gp + so.r1b, b_r1, kRgbH1, lane);
+    put_tiles(gp + so.r2w, g_r2, kRgbH2, kRgbH1, lane);
+    put_bias(gp + so.r2b, b_r2, kRgbH2, lane);
+    put_tiles(gp + so.r3w, g_r3, 3, kRgbH2, lane);
+    put_bias(gp + so.r3b, b_r3, 3, lane);
+  } else {
+    const int st = PART == 1 ? 1 : 0;
+    put_tiles(gp + so.q[0] + st * stride, gs.qkv, 3 * kHydraInner, F, lane);
+    put_tiles(gp + so.fc[0] + st * stride, gs.fc, F, kHydraInner, lane);
+    put_tiles(gp + so.w1[0] + st * stride, gs.w1, kHydraInner, F, lane);
+    put_bias(gp + so.b1[0] + st * stride, gs.b1, kHydraInner, lane);
+    put_tiles(gp + so.w2[0] + st * stride, gs.w2, F, kHydraInner, lane);
+    put_bias(gp + so.b2[0] + st * stride, gs.b2, F, lane);
+  }
+}
+
 // out[e] = sum over parts of part[p][e]
 __global__ void sum_partials(const float* __restrict__ part, int parts, int n,
                              float* __restrict__ out, int accumulate) {
@@ -2582,8 +3010,8 @@ int launch_kernels(const RenderArgs& a, hipStream_t st) {
   constexpr int rpg = kTok / VP;
   const long long groups = (a.n_rays + rpg - 1) / rpg;
   if constexpr (PREC != kF32) {
-    // 16-bit forms: Hydra + RGB head on MFMA, unless the caller differentiates
-    // the render (its backward recomputes them in fp32)
+    // 16-bit forms: Hydra + RGB head on 16-bit MFMA, unless the caller
+    // differentiates the render (its backward recomputes them in fp32)
     if (!a.fp32_hydra) {
       hipLaunchKernelGGL((hydra_rgb_mfma_kernel<F, H, VP, PREC>), dim3((unsigned)groups),
                          dim3(kThreads), 0, st, a.out_tok, a.list, a.n_valid, a.wsmall, a.V,
@@ -2591,9 +3019,10 @@ int launch_kernels(const RenderArgs& a, hipStream_t st) {
       return sailor::check_launch("render_rays (hydra, mfma)");
     }
   }
-  hipLaunchKernelGGL((hydra_rgb_kernel<F, H, VP>), dim3((unsigned)groups), dim3(kThreads), 0,
-                     st, a.out_tok, a.list, a.n_valid, a.wsmall, a.V, a.out_rgb);
-  return sailor::check_launch("render_rays (hydra)");
+  hipLaunchKernelGGL((hydra_rgb_mfma_kernel<F, H, VP, kF32>), dim3((unsigned)groups),
+                     dim3(kThreads), 0, st, a.out_tok, a.list, a.n_valid, a.wsmall, a.V,
+                     a.out_rgb);
+  return sailor::check_launch("render_rays (hydra, fp32 mfma)");
 }
 
 template <int F, int H, int NS>
@@ -2765,21 +3194,32 @@ int hydra_backward_launch(const float* grad_rgb, const float* tok, const int* li
                           float* grad_tokens, float* partials, int parts, float* grad_small,
                           hipStream_t st) {
   constexpr SmallOffsets so = small_offsets(F, H);
-  typedef HydraBwdRows<F> R;
-  auto kern = hydra_rgb_backward_kernel<F, H>;
-  static bool init = false;
-  if (!init) {
-    SAILOR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)R::bytes));
-    init = true;
-  }
   const long long nz = (long long)parts * so.total;
   hipLaunchKernelGGL(fill_zero, dim3((unsigned)std::min<long long>((nz + 255) / 256, 4096)),
                      dim3(256), 0, st, partials, nz);
   if (int err = sailor::check_launch("render_backward (zero)")) return err;
-  hipLaunchKernelGGL(kern, dim3(parts), dim3(kThreads), R::bytes, st, tok, list, counter,
-                     w_small, V, grad_rgb, grad_tokens, partials);
+  if constexpr (F == 32) {
+    // fp32 MFMA, one wave per workgroup, gradients in registers: RGB head,
+    // stack 1, stack 0, passing dL/dx on in grad_tokens
+    hipLaunchKernelGGL((hydra_rgb_backward_mfma_kernel<F, H, 0>), dim3(parts), dim3(64), 0, st,
+                       tok, list, counter, w_small, V, grad_rgb, grad_tokens, partials);
+    hipLaunchKernelGGL((hydra_rgb_backward_mfma_kernel<F, H, 1>), dim3(parts), dim3(64), 0, st,
+                       tok, list, counter, w_small, V, grad_rgb, grad_tokens, partials);
+    hipLaunchKernelGGL((hydra_rgb_backward_mfma_kernel<F, H, 2>), dim3(parts), dim3(64), 0, st,
+                       tok, list, counter, w_small, V, grad_rgb, grad_tokens, partials);
+  } else {
+    typedef HydraBwdRows<F> R;
+    auto kern = hydra_rgb_backward_kernel<F, H>;
+    static bool init = false;
+    if (!init) {
+      SAILOR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)R::bytes));
+      init = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(parts), dim3(kThreads), R::bytes, st, tok, list, counter,
+                       w_small, V, grad_rgb, grad_tokens, partials);
+  }
   if (int err = sailor::check_launch("render_backward (hydra)")) return err;
   hipLaunchKernelGGL(sum_partials, dim3((so.total + 255) / 256), dim3(256), 0, st, partials,
                      parts, so.total, grad_small, 0);
