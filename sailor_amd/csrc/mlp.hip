@@ -1303,7 +1303,8 @@ __device__ __forceinline__ void write_vals16(typename Tr<PREC>::E* dst,
 
 constexpr int kBackMaxS = 64;
 
-template <int F, int H, int PREC, int NS>
+// SMAX: samples per ray the buffers hold
+template <int F, int H, int PREC, int NS, int SMAX = kBackMaxS>
 struct SmemB {
   typedef typename Tr<PREC>::E E;
   static constexpr int NC = NS * kVP;
@@ -1312,13 +1313,11 @@ struct SmemB {
   static constexpr size_t h0 = x0 + buf, h1 = h0 + buf, h2 = h1 + buf, dl = h2 + buf;
   static constexpr int KF = F + 4;
   static constexpr size_t keep = dl + buf;                         // f32 [SMAX][4][KF]
-  static constexpr size_t alpha = keep + (size_t)kBackMaxS * kVP * KF * 4;
-  static constexpr size_t tr = alpha + kBackMaxS * 4;             // T before each sample
-  static constexpr size_t gw = tr + kBackMaxS * 4;                // dL/dw
-  static constexpr size_t dz = gw + kBackMaxS * 4;                // dL/d pre-sigmoid
-  static constexpr size_t pos = dz + kBackMaxS * 4;               // f32 [SMAX][4]
-  static constexpr size_t tval = pos + kBackMaxS * 16;
-  static constexpr size_t bias = tval + kBackMaxS * 4;            // f32 [5][H]
+  static constexpr size_t alpha = keep + (size_t)SMAX * kVP * KF * 4;
+  static constexpr size_t gw = alpha + SMAX * 4;                  // dL/dw
+  static constexpr size_t dz = gw + SMAX * 4;                     // dL/d pre-sigmoid
+  static constexpr size_t pos = dz + SMAX * 4;                    // f32 [SMAX][4]
+  static constexpr size_t bias = pos + SMAX * 16;                 // f32 [5][H]
   static constexpr size_t total = bias + (size_t)5 * H * 4;
 };
 
@@ -1582,11 +1581,11 @@ template <int F, int H> struct DenseAcc<F, H, kF16> : DenseAcc16<F, H, kF16> {};
 
 // DW: weight gradients accumulated in registers (H = 128 forms) instead of
 // column matrices written for separate GEMMs.
-template <int F, int H, int PREC, int NS, bool DW>
+template <int F, int H, int PREC, int NS, bool DW, int SMAX>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 1)))
 render_backward_kernel(BackArgs ba) {
   typedef typename Tr<PREC>::E E;
-  typedef SmemB<F, H, PREC, NS> L;
+  typedef SmemB<F, H, PREC, NS, SMAX> L;
   const RenderArgs& a = ba.r;
   constexpr int NC = L::NC, KF = L::KF, MTW = H / 64, CE = Tr<PREC>::CE;
   constexpr int CPL = F / 4, K0 = F + kPeSlots;
@@ -1600,7 +1599,6 @@ render_backward_kernel(BackArgs ba) {
   E* dl = reinterpret_cast<E*>(smem + L::dl);
   float* keep = reinterpret_cast<float*>(smem + L::keep);
   float* s_alpha = reinterpret_cast<float*>(smem + L::alpha);
-  float* s_T = reinterpret_cast<float*>(smem + L::tr);
   float* s_g = reinterpret_cast<float*>(smem + L::gw);
   float* s_dz = reinterpret_cast<float*>(smem + L::dz);
   float* s_pos = reinterpret_cast<float*>(smem + L::pos);
@@ -1679,27 +1677,38 @@ render_backward_kernel(BackArgs ba) {
       }
     }
     __syncthreads();
-    // 3. compositing replayed and walked back
-    if (tid == 0) {
-      float T = 1.0f;
-      int last = -1;
+    // 3. compositing replayed and walked back by wave 0 from registers,
+    // sample i in lane i: the recurrences run in sample order on values
+    // broadcast with readlane, the same operations in the same order as the
+    // forward's (so the early termination is the forward's)
+    if (wave == 0) {
+      const float al = lane < S ? s_alpha[lane] : 0.0f;
+      const float gl = lane < S ? s_g[lane] : 0.0f;
+      float T = 1.0f, myT = 0.0f;
+      int last = S - 1;
       for (int i = 0; i < S; ++i) {
-        s_T[i] = T;
-        last = i;
-        T *= 1.0f - s_alpha[i];
-        if (T < a.t_thres) break;
-      }
-      float U = 0.0f;
-      for (int i = S - 1; i >= 0; --i) {
-        float d = 0.0f;
-        if (i <= last) {
-          const float al = s_alpha[i];
-          d = s_T[i] * (s_g[i] - U) * al * (1.0f - al);
-          U = s_g[i] * al + (1.0f - al) * U;
+        const float ai = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(al), i));
+        if (lane == i) myT = T;
+        T *= 1.0f - ai;
+        if (__builtin_amdgcn_readfirstlane((int)(T < a.t_thres))) {
+          last = i;
+          break;
         }
-        s_dz[i] = d;
-        gsb += d;
       }
+      float U = 0.0f, myd = 0.0f;
+      for (int i = last; i >= 0; --i) {
+        const float ai = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(al), i));
+        const float gi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(gl), i));
+        const float ti = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myT), i));
+        const float d = ti * (gi - U) * ai * (1.0f - ai);
+        if (lane == i) myd = d;
+        U = gi * ai + (1.0f - ai) * U;
+      }
+      if (lane < S) s_dz[lane] = myd;
+      double dsum = myd;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) dsum += __shfl_xor(dsum, off, 64);
+      if (lane == 0) gsb += dsum;
     }
     __syncthreads();
     // 4./5. per chunk
@@ -3268,17 +3277,16 @@ extern "C" int sailor_render_hydra_backward(const float* grad_rgb, int B, int R,
 
 namespace {
 
-template <int F, int H, int PREC, bool DW>
-int sample_backward_launch(BackArgs& ba, int parts, float* grad_small, float* grad_dense,
-                           hipStream_t st) {
-  constexpr int NS = PREC == kF32 ? 4 : (DW ? 16 : 8);
+template <int F, int H, int PREC, bool DW, int NS, int SMAX>
+int sample_backward_launch_ns(BackArgs& ba, int parts, float* grad_small, float* grad_dense,
+                              hipStream_t st) {
   constexpr SmallOffsets so = small_offsets(F, H);
-  typedef SmemB<F, H, PREC, NS> L;
+  typedef SmemB<F, H, PREC, NS, SMAX> L;
   static_assert(L::total <= kLdsBytes, "backward LDS");
-  SAILOR_REQUIRE(ba.r.S % NS == 0 && ba.r.S <= kBackMaxS,
+  SAILOR_REQUIRE(ba.r.S % NS == 0 && ba.r.S <= SMAX,
                  "render_sample_backward: S=%d must be a multiple of %d and at most %d",
-                 ba.r.S, NS, kBackMaxS);
-  auto kern = render_backward_kernel<F, H, PREC, NS, DW>;
+                 ba.r.S, NS, SMAX);
+  auto kern = render_backward_kernel<F, H, PREC, NS, DW, SMAX>;
   static bool init = false;
   if (!init) {
     SAILOR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -3302,6 +3310,18 @@ int sample_backward_launch(BackArgs& ba, int parts, float* grad_small, float* gr
     return sailor::check_launch("render_sample_backward (dense sum)");
   }
   return SAILOR_OK;
+}
+
+// Samples per chunk: fp32 4; 16-bit with in-kernel weight gradients 16
+// (24 would cut the chunk phases by a third but spills ~120 registers per
+// lane beside the weight-gradient accumulators and runs slower); 16-bit
+// column form 8.
+template <int F, int H, int PREC, bool DW>
+int sample_backward_launch(BackArgs& ba, int parts, float* grad_small, float* grad_dense,
+                           hipStream_t st) {
+  constexpr int NS = PREC == kF32 ? 4 : (DW ? 16 : 8);
+  return sample_backward_launch_ns<F, H, PREC, DW, NS, kBackMaxS>(ba, parts, grad_small,
+                                                                  grad_dense, st);
 }
 
 // Shared argument checks and dispatch of the two backward entry points:
