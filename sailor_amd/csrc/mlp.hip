@@ -21,11 +21,13 @@
 //    weight fragment read from L2 is reused across all NC/16 column tiles;
 //  * layer inputs / activations of the chunk live in LDS as act[col][k],
 //    unpadded rows of H elements whose 16-byte chunks are XOR-swizzled by the
-//    column (chunk c of column col is stored at c ^ (col & 15)).  Measured on
-//    gfx950 with the SQ bank-conflict counter, this is the layout (among
-//    padded strides of 16/32 mod 256 bytes and partial swizzles) for which
-//    both the 16-byte B-fragment reads and the 16-byte stores of the layer
-//    outputs are conflict-free;
+//    column (chunk c of column col is stored at c ^ p(col), p a permutation
+//    of col & 15).  Measured on gfx950 with the SQ bank-conflict counter,
+//    the XOR swizzle is the layout (among padded strides of 16/32 mod 256
+//    bytes and partial swizzles) for which both the 16-byte B-fragment reads
+//    and the 16-byte stores of the layer outputs are conflict-free; the
+//    16-bit forms' permutation also keeps the backward's column-wise
+//    (transposing) reads conflict-free;
 //  * each lane's accumulators hold 4*MTW consecutive output units of one
 //    column (the packer orders the weight rows so), so a layer's outputs are
 //    written back with whole 16-byte stores;
@@ -131,10 +133,21 @@ __device__ __forceinline__ void kahan_add(float& sum, float& comp, float x) {
 }
 
 // Element offset of 16-byte chunk `chunk` of column `col` in the swizzled
-// activation layout.
+// activation layout: chunk ^ (col & 15) for fp32; chunk ^ q(col) for the
+// 16-bit forms, q(col) = (col & 3) << 1 | (col >> 2) & 1 | (col & 8).  The
+// LDS serves 128 bytes (8 chunk positions) per pass: 8 consecutive columns at
+// one chunk (the forward's B reads, the layer stores) take 8 different
+// positions under either map, and under q so do the 4 aligned consecutive
+// columns x 2 consecutive chunks of a pass of the backward's transposing
+// column reads (q's low bit is free for the chunk's).
 template <int H, int PREC>
 __device__ __forceinline__ int act_chunk(int col, int chunk) {
-  return col * H + ((chunk ^ (col & 15)) * Tr<PREC>::CE);
+  if constexpr (PREC == kF32) {
+    return col * H + ((chunk ^ (col & 15)) * Tr<PREC>::CE);
+  } else {
+    const int q = ((col & 3) << 1) | ((col >> 2) & 1) | (col & 8);
+    return col * H + ((chunk ^ q) * Tr<PREC>::CE);
+  }
 }
 
 // Output unit of accumulator element i of m-tile mt held by lane group g in
@@ -1716,7 +1729,7 @@ render_backward_kernel(BackArgs ba) {
       const long long col0 = (item - ba.item_begin) * S * kVP + (long long)c0 * kVP;
       // layer-0 input: features (from keep) and positional encoding
       for (int it = tid; it < NC * (F / CE); it += kThreads) {
-        const int col = it / (F / CE), ch = it % (F / CE);
+        const int col = it % NC, ch = it / NC;   // consecutive lanes: consecutive columns
         const float* kp = keep + ((c0 + (col >> 2)) * kVP + (col & 3)) * KF + ch * CE;
         E* d = x0 + act_chunk<H, PREC>(col, ch);
         if constexpr (F32) {
