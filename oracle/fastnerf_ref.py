@@ -26,8 +26,8 @@ restatement computes in `dtype` (float64 for the tests) from the same half
 parameters, so it is the exact value the reference approximates.  With
 act_dtype=torch.float16 the inputs of the density MLPs' matrix products
 (feature MLP layers, the two hidden occupancy layers, both geometry-feature
-layers) are rounded to half first: the numerics of a half-input,
-wide-accumulation evaluation.  Kept
+layers) and of every Hydra and RGB-head product are rounded to half first:
+the numerics of a half-input, wide-accumulation evaluation.  Kept
 quirks: q and k are divided by their norm with no epsilon; compositing skips
 samples with negative depth, returns zeros for a ray whose last sample depth
 is negative, and stops after the sample that takes T below 1e-4; the depth
@@ -194,26 +194,28 @@ def composite(t, alpha, feats, t_thres=1e-4):
     return out, dep
 
 
-def hydra_rgb(tokens, pc, dtype=torch.float64):
+def hydra_rgb(tokens, pc, dtype=torch.float64, act_dtype=None):
     """hydra_attention_kernel: tokens [M, V, 30] -> (colour features of
-    token 0 after both stacks [M, 30], rgb [M, 3])."""
+    token 0 after both stacks [M, 30], rgb [M, 3]); with act_dtype the input
+    of every matrix product is rounded to it first."""
     stacks, rgb = unpack_color(_params(pc, dtype))
+    r = lambda a: _round(a, act_dtype)
     x = tokens
     for st in stacks:
-        q = x @ st["q"].T
-        k = x @ st["k"].T
-        v = x @ st["v"].T
+        q = r(x) @ st["q"].T
+        k = r(x) @ st["k"].T
+        v = r(x) @ st["v"].T
         q = q / torch.sqrt((q * q).sum(-1, keepdim=True))
         k = k / torch.sqrt((k * k).sum(-1, keepdim=True))
         kv = (k * v).sum(1, keepdim=True)
-        out = (q * kv) @ st["fc"].T
+        out = r(q * kv) @ st["fc"].T
         res = x + out
-        h = _relu(res @ st["w1"].T + st["b1"])
-        x = h @ st["w2"].T + st["b2"] + res
+        h = _relu(r(res) @ st["w1"].T + st["b1"])
+        x = r(h) @ st["w2"].T + st["b2"] + res
     f = x[:, 0]
-    h = _relu(_lin(f, rgb[0]))
-    h = _relu(_lin(h, rgb[1]))
-    return f, torch.sigmoid(_lin(h, rgb[2]))
+    h = _relu(_lin(r(f), rgb[0]))
+    h = _relu(_lin(r(h), rgb[1]))
+    return f, torch.sigmoid(_lin(r(h), rgb[2]))
 
 
 def infer_nerf(sorted_dists, points_z, points_tpsdf, feats_geo, rgbs, feats_rgb, dirs, pd, pc,
@@ -237,7 +239,7 @@ def infer_nerf(sorted_dists, points_z, points_tpsdf, feats_geo, rgbs, feats_rgb,
         tk, dep = composite(t, alpha, fv)
         toks.append(tk)
     toks = torch.stack(toks, 1)                      # [B*R, Vt, 30]
-    feats, rgb = hydra_rgb(toks, pc, dtype)
+    feats, rgb = hydra_rgb(toks, pc, dtype, act_dtype)
     return dep.reshape(B, R, 1), rgb.reshape(B, R, 3), feats.reshape(B, R, 30)
 
 

@@ -505,6 +505,301 @@ color_kernel(const half_t* __restrict__ params, const float* __restrict__ t,
   }
 }
 
+// ---- Colour path of the fp16 form ------------------------------------------
+// color_composite_kernel: one lane per (ray, texture slot) composites the 30
+// features exactly as color_kernel does and writes the token, padded to 32
+// with zeros, to the workspace (slot 0's lane also writes the depth).
+// color_mfma_kernel: Hydra stacks + RGB head with every product on
+// v_mfma_f32_16x16x32_f16.  Each wave takes 16 tokens (4 rays x 4 slots): the
+// MFMA's N = 16 columns are the tokens, M = 16 output units per tile, K = 32
+// inputs per step.  The half parameters are exact in fp16; the layer inputs
+// are rounded to fp16 for the products, which accumulate in fp32; the norms
+// (no epsilon, as the reference), the cross-view sum, residuals, biases and
+// activations stay in fp32.  The parameters are staged once per persistent
+// workgroup as zero-padded fp16 rows of 32 inputs (64 for the RGB head's
+// second layer) whose 16-byte chunks are XOR-swizzled by the row, so that the
+// 8 lanes of an LDS pass (8 rows, one chunk) hit 8 different bank groups.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int kTokW = 32;                  // token width in the workspace
+constexpr int kCxLd = kTokW + 4;           // fp32 token rows in LDS
+constexpr int kChLd = 64 + 4;              // fp32 rows of attention / hidden units
+constexpr int kStackRows = 6 * 32;         // q | k | v | fc | w1 | w2 rows, 32 each
+
+__global__ void __launch_bounds__(kBlock)
+color_composite_kernel(const float* __restrict__ t, const float* __restrict__ occ,
+                       const float* __restrict__ feats_rgb, const float* __restrict__ rgbs,
+                       const float* __restrict__ geo8, const float* __restrict__ dirs, int B,
+                       int R, int S, int Vt, float t_thres, float* __restrict__ out_depth,
+                       float* __restrict__ tok) {
+  const long long lanes = (long long)B * R * kTexSlots;
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < lanes;
+       id += (long long)gridDim.x * blockDim.x) {
+    const long long ray = id / kTexSlots;
+    const int v = (int)(id % kTexSlots);
+    float x[kTokW];
+#pragma unroll
+    for (int c = 0; c < kTokW; ++c) x[c] = 0.0f;
+    float depth = 0.0f;
+    if (v < Vt) {
+      const long long b = ray / R, r = ray - b * R;
+      const float* tr = t + ray * S;
+      const float* oc = occ + ray * S;
+      if (!(tr[S - 1] < 0.0f)) {
+        const long long e0 = ((b * Vt + v) * R + r) * S;
+        float T = 1.0f;
+        for (int s = 0; s < S; ++s) {
+          const float ts = tr[s];
+          if (ts < 0.0f) continue;
+          const float a = oc[s];
+          const float wgt = a * T;
+          const long long e = e0 + s;
+          const float4* f4 = reinterpret_cast<const float4*>(feats_rgb + e * 16);
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const float4 f = f4[c];
+            x[4 * c + 0] += wgt * f.x;
+            x[4 * c + 1] += wgt * f.y;
+            x[4 * c + 2] += wgt * f.z;
+            x[4 * c + 3] += wgt * f.w;
+          }
+#pragma unroll
+          for (int c = 0; c < 3; ++c) x[16 + c] += wgt * rgbs[e * 3 + c];
+          const float4* g4 = reinterpret_cast<const float4*>(geo8 + e * 8);
+          const float4 ga = g4[0], gb = g4[1];
+          x[19] += wgt * ga.x; x[20] += wgt * ga.y; x[21] += wgt * ga.z; x[22] += wgt * ga.w;
+          x[23] += wgt * gb.x; x[24] += wgt * gb.y; x[25] += wgt * gb.z; x[26] += wgt * gb.w;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) x[27 + c] += wgt * dirs[e * 3 + c];
+          depth += wgt * ts;
+          T *= 1.0f - a;
+          if (T < t_thres) break;
+        }
+      }
+    }
+    float4* o4 = reinterpret_cast<float4*>(tok + id * kTokW);
+#pragma unroll
+    for (int c = 0; c < kTokW / 4; ++c)
+      o4[c] = make_float4(x[4 * c], x[4 * c + 1], x[4 * c + 2], x[4 * c + 3]);
+    if (v == 0) out_depth[ray] = depth;
+  }
+}
+
+struct ColorMfmaLds {
+  f16x8 wst[2][kStackRows][4];   // per stack, rows of 32 inputs (4 chunks)
+  f16x8 wr1[64][4];              // RGB layer 1 [64][30 + 2]
+  f16x8 wr2[32][8];              // RGB layer 2 [32][64]
+  f16x8 wr3[16][4];              // RGB layer 3 [3 + 13][32]
+  float b1[2][32], b2[2][32], rb1[64], rb2[32], rb3[4];
+  float sx[kBlock / 64][16 * kCxLd];   // per wave: token rows
+  float sh[kBlock / 64][16 * kChLd];   // per wave: attention / hidden rows
+};
+
+// position of chunk c of row r among a row's NCH chunks
+template <int NCH>
+__device__ __forceinline__ int cchunk(int r, int c) {
+  return NCH == 4 ? (c ^ ((r >> 1) & 3)) : (c ^ (r & 7));
+}
+
+// acc[mt] = sum_k W[16 mt + n][k] x[token n][k], K = 32 KS; W: LDS rows of
+// 4 KS swizzled f16x8 chunks; x: fp32 rows of pitch LD (rounded to fp16).
+template <int MT, int KS, int LD>
+__device__ __forceinline__ void cmm(const f16x8* W, const float* x, int lane, f32x4 (&acc)[MT]) {
+  const int n = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    const f32x4 x0 = *reinterpret_cast<const f32x4*>(x + n * LD + 32 * s + 8 * g);
+    const f32x4 x1 = *reinterpret_cast<const f32x4*>(x + n * LD + 32 * s + 8 * g + 4);
+    f16x8 b;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      b[j] = (half_t)x0[j];
+      b[4 + j] = (half_t)x1[j];
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const int r = 16 * mt + n;
+      const f16x8 a = W[r * 4 * KS + cchunk<4 * KS>(r, 4 * s + g)];
+      acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc[mt], 0, 0, 0);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+color_mfma_kernel(const half_t* __restrict__ params, const float* __restrict__ tok, int B, int R,
+                  int Vt, float* __restrict__ out_rgb, float* __restrict__ out_feats) {
+  __shared__ ColorMfmaLds L;
+  // ---- stage the parameters (zero-padded, swizzled) -----------------------
+  {
+    half_t* wst = reinterpret_cast<half_t*>(&L.wst[0][0][0]);
+    for (int i = threadIdx.x; i < 2 * kStackRows * 32; i += blockDim.x) {
+      const int st = i / (kStackRows * 32), r = (i / 32) % kStackRows, c = i % 32;
+      const half_t* ps = params + st * kStackParams;
+      const int blk = r / 32, o = r % 32;
+      float v = 0.0f;
+      if (blk < 3) v = c < 30 ? (float)ps[blk * 960 + o * 30 + c] : 0.0f;      // q, k, v
+      else if (blk == 3) v = o < 30 ? (float)ps[2880 + o * 32 + c] : 0.0f;    // fc
+      else if (blk == 4) v = c < 30 ? (float)ps[3840 + o * 30 + c] : 0.0f;    // w1
+      else v = o < 30 ? (float)ps[4832 + o * 32 + c] : 0.0f;                  // w2
+      wst[((size_t)st * kStackRows + r) * 32 + cchunk<4>(r, c / 8) * 8 + c % 8] = (half_t)v;
+    }
+    half_t* w1 = reinterpret_cast<half_t*>(&L.wr1[0][0]);
+    for (int i = threadIdx.x; i < 64 * 32; i += blockDim.x) {
+      const int r = i / 32, c = i % 32;
+      const float v = c < 30 ? (float)params[kRgbOffset + r * 30 + c] : 0.0f;
+      w1[r * 32 + cchunk<4>(r, c / 8) * 8 + c % 8] = (half_t)v;
+    }
+    half_t* w2 = reinterpret_cast<half_t*>(&L.wr2[0][0]);
+    for (int i = threadIdx.x; i < 32 * 64; i += blockDim.x) {
+      const int r = i / 64, c = i % 64;
+      w2[r * 64 + cchunk<8>(r, c / 8) * 8 + c % 8] = params[kRgbOffset + 1984 + r * 64 + c];
+    }
+    half_t* w3 = reinterpret_cast<half_t*>(&L.wr3[0][0]);
+    for (int i = threadIdx.x; i < 16 * 32; i += blockDim.x) {
+      const int r = i / 32, c = i % 32;
+      const float v = r < 3 ? (float)params[kRgbOffset + 4064 + r * 32 + c] : 0.0f;
+      w3[r * 32 + cchunk<4>(r, c / 8) * 8 + c % 8] = (half_t)v;
+    }
+    for (int i = threadIdx.x; i < 64; i += blockDim.x) {
+      const int st = i / 32, o = i % 32;
+      const half_t* ps = params + st * kStackParams;
+      L.b1[st][o] = (float)ps[4800 + o];
+      L.b2[st][o] = o < 30 ? (float)ps[5792 + o] : 0.0f;
+    }
+    for (int i = threadIdx.x; i < 64; i += blockDim.x) L.rb1[i] = (float)params[kRgbOffset + 1920 + i];
+    for (int i = threadIdx.x; i < 32; i += blockDim.x) L.rb2[i] = (float)params[kRgbOffset + 4032 + i];
+    for (int i = threadIdx.x; i < 4; i += blockDim.x)
+      L.rb3[i] = i < 3 ? (float)params[kRgbOffset + 4160 + i] : 0.0f;
+    __syncthreads();
+  }
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int n = lane & 15, g = lane >> 4;
+  float* xs = L.sx[wave];
+  float* hs = L.sh[wave];
+  const long long n_rays = (long long)B * R;
+  const long long n_tiles = (n_rays * kTexSlots + 15) / 16;
+  const int slot = n % kTexSlots;
+  // the workgroup's 4 waves take 4 consecutive tiles per step (a wave past
+  // the last tile runs on zero tokens and writes nothing), so every wave
+  // reaches every barrier
+  for (long long grp = blockIdx.x; grp * (kBlock / 64) < n_tiles; grp += gridDim.x) {
+    const long long t0 = (grp * (kBlock / 64) + wave) * 16;
+    const long long ray = (t0 + n) / kTexSlots;
+    const bool live = ray < n_rays && slot < Vt;
+    for (int i = lane; i < 16 * kTokW / 4; i += 64) {
+      const int tk = i / (kTokW / 4), c = 4 * (i % (kTokW / 4));
+      f32x4 v = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      if (t0 + tk < n_rays * kTexSlots) v = *reinterpret_cast<const f32x4*>(tok + (t0 + tk) * kTokW + c);
+      *reinterpret_cast<f32x4*>(xs + tk * kCxLd + c) = v;
+    }
+    __syncthreads();
+    for (int st = 0; st < 2; ++st) {
+      const f16x8* W = &L.wst[st][0][0];
+      f32x4 a6[6];
+      cmm<6, 1, kCxLd>(W, xs, lane, a6);
+      float qq = 0.0f, kk = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        qq += a6[0][i] * a6[0][i] + a6[1][i] * a6[1][i];
+        kk += a6[2][i] * a6[2][i] + a6[3][i] * a6[3][i];
+      }
+      qq += __shfl_xor(qq, 16, 64);
+      qq += __shfl_xor(qq, 32, 64);
+      kk += __shfl_xor(kk, 16, 64);
+      kk += __shfl_xor(kk, 32, 64);
+      // no epsilon, as the reference (a zero token gives 0/0)
+      const float rq = 1.0f / sqrtf(qq), rk = 1.0f / sqrtf(kk);
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          float kv = live ? a6[2 + h][i] * rk * a6[4 + h][i] : 0.0f;
+          kv += __shfl_xor(kv, 1, 64);
+          kv += __shfl_xor(kv, 2, 64);
+          hs[n * kChLd + 16 * h + 4 * g + i] = a6[h][i] * rq * kv;
+        }
+      __syncthreads();
+      {
+        f32x4 ao[2];   // res = x + fc(att) (units 30, 31 stay 0)
+        cmm<2, 1, kChLd>(W + 3 * 32 * 4, hs, lane, ao);
+        __syncthreads();
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) xs[n * kCxLd + 16 * mt + 4 * g + i] += ao[mt][i];
+      }
+      __syncthreads();
+      {
+        f32x4 ah[2];   // h = relu(W1 res + b1)
+        cmm<2, 1, kCxLd>(W + 4 * 32 * 4, xs, lane, ah);
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int u = 16 * mt + 4 * g + i;
+            hs[n * kChLd + u] = fmaxf(ah[mt][i] + L.b1[st][u], 0.0f);
+          }
+      }
+      __syncthreads();
+      {
+        f32x4 ax[2];   // x = W2 h + b2 + res
+        cmm<2, 1, kChLd>(W + 5 * 32 * 4, hs, lane, ax);
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int u = 16 * mt + 4 * g + i;
+            xs[n * kCxLd + u] = (ax[mt][i] + L.b2[st][u]) + xs[n * kCxLd + u];
+          }
+      }
+      __syncthreads();
+    }
+    // ---- RGB head: 30 -> 64 -> 32 -> 3 (used on slot 0's token) ------------
+    {
+      f32x4 a1[4];
+      cmm<4, 1, kCxLd>(&L.wr1[0][0], xs, lane, a1);
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int u = 16 * mt + 4 * g + i;
+          hs[n * kChLd + u] = fmaxf(a1[mt][i] + L.rb1[u], 0.0f);
+        }
+    }
+    __syncthreads();
+    {
+      f32x4 a2[2];
+      cmm<2, 2, kChLd>(&L.wr2[0][0], hs, lane, a2);
+      __syncthreads();
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int u = 16 * mt + 4 * g + i;
+          hs[n * kChLd + u] = fmaxf(a2[mt][i] + L.rb2[u], 0.0f);
+        }
+    }
+    __syncthreads();
+    {
+      f32x4 a3[1];
+      cmm<1, 1, kChLd>(&L.wr3[0][0], hs, lane, a3);
+      // lane (n, g = 0) holds the three outputs of token n
+      if (g == 0 && slot == 0 && ray < n_rays) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const float z = a3[0][j] + L.rb3[j];
+          out_rgb[ray * 3 + j] = 1.0f / (1.0f + expf(-z));
+        }
+#pragma unroll
+        for (int c = 0; c < 30; ++c) out_feats[ray * 30 + c] = xs[n * kCxLd + c];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // upsampling_kernel (infer_mlp.cu:449-506) for one point: [40] neighbour
 // features and [30] ray features (read from rf) -> 3 blend weights.
 __device__ __forceinline__ void blend_mlp(const half_t* __restrict__ w, const float (&x)[40],
@@ -973,7 +1268,10 @@ unsigned grid_for(long long work) {
 extern "C" size_t sailor_infer_nerf_workspace_bytes(int B, int R, int S, int tex_views) {
   if (B <= 0 || R < 0 || S <= 0 || tex_views <= 0) return 0;
   const size_t n = (size_t)B * R * S;
-  return (n * sizeof(float) + 255) / 256 * 256 + n * tex_views * 8 * sizeof(float);
+  // occupancies, geometry features of the texture views, colour tokens
+  return (n * sizeof(float) + 255) / 256 * 256 +
+         (n * tex_views * 8 * sizeof(float) + 255) / 256 * 256 +
+         (size_t)B * R * kTexSlots * kTokW * sizeof(float);
 }
 
 namespace {
@@ -1034,6 +1332,27 @@ int infer_nerf_impl(bool mfma, const float* sorted_dists, const float* points_z,
                                points_tpsdf, feats_geo, B, n_pts, geo_views, occ, geo8,
                                tex_views, st))
     return err;
+  if (mfma && geo_views <= 8) {
+    // fp16 form: compositing, then the Hydra stacks and RGB head on MFMA
+    float* tok = reinterpret_cast<float*>(
+        reinterpret_cast<char*>(geo8) +
+        ((size_t)B * n_pts * tex_views * 8 * sizeof(float) + 255) / 256 * 256);
+    hipLaunchKernelGGL(color_composite_kernel, dim3(grid_for((long long)B * R * kTexSlots)),
+                       dim3(kBlock), 0, st, sorted_dists, occ, feats_rgb, rgbs, geo8, dirs, B, R,
+                       S, tex_views, 1e-4f, out_depth, tok);
+    if (int err = sailor::check_launch("infer_nerf (compositing)")) return err;
+    const long long tiles = ((long long)B * R * kTexSlots + 15) / 16;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 256;
+    const unsigned grid = (unsigned)std::max<long long>(
+        1, std::min<long long>((tiles + kBlock / 64 - 1) / (kBlock / 64), 2LL * cus));
+    hipLaunchKernelGGL(color_mfma_kernel, dim3(grid), dim3(kBlock), 0, st,
+                       static_cast<const half_t*>(params_color), tok, B, R, tex_views, out_rgb,
+                       out_feats);
+    return sailor::check_launch("infer_nerf (colour, mfma)");
+  }
   hipLaunchKernelGGL(color_kernel, dim3(grid_for((long long)B * R * kTexSlots)), dim3(kBlock),
                      kColorLds, st, static_cast<const half_t*>(params_color), sorted_dists, occ,
                      feats_rgb, rgbs, geo8, dirs, B, R, S, tex_views, 1e-4f, out_depth, out_rgb,

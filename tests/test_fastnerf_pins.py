@@ -191,7 +191,7 @@ MUTANTS = {
                                    check_hydra_closed_forms),
     "feedforward_residual_dropped": ("hydra_rgb", "+ st[\"b2\"] + res", "+ st[\"b2\"]",
                                      check_hydra_closed_forms),
-    "rgb_relu_dropped": ("hydra_rgb", "h = _relu(_lin(h, rgb[1]))", "h = _lin(h, rgb[1])",
+    "rgb_relu_dropped": ("hydra_rgb", "h = _relu(_lin(r(h), rgb[1]))", "h = _lin(r(h), rgb[1])",
                          check_rgb_head),
 }
 

@@ -11,10 +11,11 @@ precision="fp16" (default; density MLPs on fp16 MFMA): against the
 restatement with the density layers' inputs rounded to half
 (act_dtype=float16), |d| <= 1e-3 + 1e-3 |ref|: the fp32 and float64 sums
 can round an activation to neighbouring halves (2^-11 relative), which moves
-the occupancies by ~1e-4.  Against the unrounded float64 values the error is
-that of half inputs (checked <= 2e-2, observed to be far above the
-kernel-vs-rounded difference, so the rounded restatement is the one
-matched)."""
+the occupancies by ~1e-4.  Against the unrounded float64 values the colour error is
+that of half inputs (checked <= 2e-2), and the colour features are nearer
+the rounded restatement than the float64 one (it is the one matched).  The
+colour path of this form (Hydra stacks and RGB head on fp16 MFMA) rounds
+the inputs of those products to half as well."""
 import numpy as np
 import pytest
 import torch
@@ -84,10 +85,13 @@ def test_infer_nerf_fp16_mfma(B, R, S, Vg, Vt):
         np.testing.assert_allclose(got.double().cpu().numpy(), ref.numpy(), rtol=tol, atol=tol,
                                    equal_nan=True)
     if act is not None:
-        _, rc64, _ = fr.infer_nerf(t, z, ps, geo, rgb, frgb, dirs, pd, pc, B, R, S, Vg, Vt)
+        _, rc64, rf64 = fr.infer_nerf(t, z, ps, geo, rgb, frgb, dirs, pd, pc, B, R, S, Vg, Vt)
         e64 = np.nanmax(np.abs(c.double().cpu().numpy() - rc64.numpy()))
-        e16 = np.nanmax(np.abs(c.double().cpu().numpy() - rc.numpy()))
-        assert e64 <= 2e-2 and e16 <= e64
+        f64 = np.nanmax(np.abs(f.double().cpu().numpy() - rf64.numpy()))
+        f16 = np.nanmax(np.abs(f.double().cpu().numpy() - rf.numpy()))
+        print(f"colour vs float64 {e64:.2e}; features vs float64 {f64:.2e}, vs the rounded "
+              f"restatement {f16:.2e}")
+        assert e64 <= 2e-2 and f16 <= f64
 
 
 def test_infer_density_fp16_mfma():
