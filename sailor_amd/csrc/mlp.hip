@@ -1672,63 +1672,10 @@ render_backward_kernel(BackArgs ba) {
         *reinterpret_cast<f32x4*>(kp + c) = f32x4{f[c], f[c + 1], f[c + 2], f[c + 3]};
     }
     __syncthreads();
-    // 2. g_i = dL/dtoken . f(x_i): lane holds dtoken entries lane + 64 j
-    {
-      float dt[kVP * F / 64];
-#pragma unroll
-      for (int j = 0; j < kVP * F / 64; ++j) dt[j] = ba.dtok[item * kVP * F + lane + 64 * j];
-      for (int i = wave; i < S; i += kWaves) {
-        float part = 0.0f;
-#pragma unroll
-        for (int j = 0; j < kVP * F / 64; ++j) {
-          const int e = lane + 64 * j, v = e / F, c = e % F;
-          part += dt[j] * keep[(i * kVP + v) * KF + c];
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
-        if (lane == 0) s_g[i] = part;
-      }
-    }
-    __syncthreads();
-    // 3. compositing replayed and walked back by wave 0 from registers,
-    // sample i in lane i: the recurrences run in sample order on values
-    // broadcast with readlane, the same operations in the same order as the
-    // forward's (so the early termination is the forward's)
-    if (wave == 0) {
-      const float al = lane < S ? s_alpha[lane] : 0.0f;
-      const float gl = lane < S ? s_g[lane] : 0.0f;
-      float T = 1.0f, myT = 0.0f;
-      int last = S - 1;
-      for (int i = 0; i < S; ++i) {
-        const float ai = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(al), i));
-        if (lane == i) myT = T;
-        T *= 1.0f - ai;
-        if (__builtin_amdgcn_readfirstlane((int)(T < a.t_thres))) {
-          last = i;
-          break;
-        }
-      }
-      float U = 0.0f, myd = 0.0f;
-      for (int i = last; i >= 0; --i) {
-        const float ai = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(al), i));
-        const float gi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(gl), i));
-        const float ti = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myT), i));
-        const float d = ti * (gi - U) * ai * (1.0f - ai);
-        if (lane == i) myd = d;
-        U = gi * ai + (1.0f - ai) * U;
-      }
-      if (lane < S) s_dz[lane] = myd;
-      double dsum = myd;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) dsum += __shfl_xor(dsum, off, 64);
-      if (lane == 0) gsb += dsum;
-    }
-    __syncthreads();
-    // 4./5. per chunk
-    for (int c0 = 0; c0 < S; c0 += NS) {
-      const long long col0 = (item - ba.item_begin) * S * kVP + (long long)c0 * kVP;
-      // layer-0 input: features (from keep) and positional encoding
-      for (int it = tid; it < NC * (F / CE); it += kThreads) {
+    // layer-0 input of the chunk at sample c0: features (from keep) and
+    // positional encoding, by threads t0, t0 + tstep, ...
+    auto build_input = [&](int c0, int t0, int tstep) {
+      for (int it = t0; it < NC * (F / CE); it += tstep) {
         const int col = it % NC, ch = it / NC;   // consecutive lanes: consecutive columns
         const float* kp = keep + ((c0 + (col >> 2)) * kVP + (col & 3)) * KF + ch * CE;
         E* d = x0 + act_chunk<H, PREC>(col, ch);
@@ -1741,7 +1688,7 @@ render_backward_kernel(BackArgs ba) {
           *reinterpret_cast<typename Tr<PREC>::V8*>(d) = h;
         }
       }
-      for (int it = tid; it < NS * kPeSlots / 2; it += kThreads) {
+      for (int it = t0; it < NS * kPeSlots / 2; it += tstep) {
         const int sl = it / (kPeSlots / 2), q0 = 2 * (it % (kPeSlots / 2));
         const float* ps = s_pos + 4 * (c0 + sl);
         float val[2];
@@ -1773,6 +1720,66 @@ render_backward_kernel(BackArgs ba) {
           }
         }
       }
+    };
+    // 2. g_i = dL/dtoken . f(x_i): thread t takes sample t / 4 and the F
+    // products of view t % 4; two shuffles add the 4 views
+    for (int i4 = tid; i4 < S * kVP; i4 += kThreads) {
+      const int i = i4 >> 2, v = i4 & 3;
+      const float* dtv = ba.dtok + item * kVP * F + v * F;
+      const float* kp = keep + (i * kVP + v) * KF;
+      float part = 0.0f;
+#pragma unroll
+      for (int c = 0; c < F; c += 4) {
+        const f32x4 d4 = *reinterpret_cast<const f32x4*>(dtv + c);
+        const f32x4 k4 = *reinterpret_cast<const f32x4*>(kp + c);
+        part += d4[0] * k4[0] + d4[1] * k4[1] + d4[2] * k4[2] + d4[3] * k4[3];
+      }
+      part += __shfl_xor(part, 1, 64);
+      part += __shfl_xor(part, 2, 64);
+      if (v == 0) s_g[i] = part;
+    }
+    __syncthreads();
+    // 3. compositing replayed and walked back by wave 0 from registers,
+    // sample i in lane i: the recurrences run in sample order on values
+    // broadcast with readlane, the same operations in the same order as the
+    // forward's (so the early termination is the forward's)
+    // while waves 1-3 build the first chunk's layer-0 input
+    if (wave == 0) {
+      const float al = lane < S ? s_alpha[lane] : 0.0f;
+      const float gl = lane < S ? s_g[lane] : 0.0f;
+      float T = 1.0f, myT = 0.0f;
+      int last = S - 1;
+      for (int i = 0; i < S; ++i) {
+        const float ai = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(al), i));
+        if (lane == i) myT = T;
+        T *= 1.0f - ai;
+        if (__builtin_amdgcn_readfirstlane((int)(T < a.t_thres))) {
+          last = i;
+          break;
+        }
+      }
+      float U = 0.0f, myd = 0.0f;
+      for (int i = last; i >= 0; --i) {
+        const float ai = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(al), i));
+        const float gi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(gl), i));
+        const float ti = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myT), i));
+        const float d = ti * (gi - U) * ai * (1.0f - ai);
+        if (lane == i) myd = d;
+        U = gi * ai + (1.0f - ai) * U;
+      }
+      if (lane < S) s_dz[lane] = myd;
+      double dsum = myd;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) dsum += __shfl_xor(dsum, off, 64);
+      if (lane == 0) gsb += dsum;
+    } else {
+      build_input(0, tid - 64, kThreads - 64);
+    }
+    __syncthreads();
+    // 4./5. per chunk
+    for (int c0 = 0; c0 < S; c0 += NS) {
+      const long long col0 = (item - ba.item_begin) * S * kVP + (long long)c0 * kVP;
+      if (c0 > 0) build_input(c0, tid, kThreads);
       __syncthreads();
       if constexpr (!DW) store_cols<H, PREC, NC, K0>(x0, ba.A0, col0, tid);
       if constexpr (F32) {
