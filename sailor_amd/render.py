@@ -111,9 +111,9 @@ def render_rays(t, rays, Kv, RTv, img_hw, fmap_cl, bbox, zdir, model: PackedMode
     sums (precision form).  feature_scales [F]: fmap_cl holds maps quantised
     by sailor_quantize_channels with these scales, and the model's first
     layer carries them (FramePipeline with quantize=True).  fp32_hydra: the
-    16-bit forms run the Hydra encoder and RGB head on fp32 vector ALUs
-    instead of 16-bit MFMA (the differentiation form, whose backward
-    recomputes them in fp32).
+    16-bit forms run the Hydra encoder and RGB head on fp32 MFMA instead of
+    16-bit MFMA (the differentiation form, whose backward recomputes them in
+    fp32).
     Returns dict(rgb [B,R,3], depth [B,R], aux [B,R,2], occ [B,R,S] or None)."""
     dev = t.device
     check_input(t, "t", _f32)
