@@ -3150,9 +3150,12 @@ int render_stages(int stages, const float* t, const float* rays, int B, int R, i
     if (V > kVP) {
       ns = (precision == kF32 && S % 12 == 0) ? 12 : 8;  // 8-slot instantiations
     } else {
-      // measured on MI355X (C2 frame): fp32 24-sample chunks, 16-bit
-      // 16-sample chunks (two workgroups per CU)
-      if (precision == kF32 && S % 24 == 0) ns = 24;
+      // measured on MI355X: fp32 24-sample chunks for the hidden-256 model
+      // (C2 frame; 48 do not fit the LDS), whole 48-sample rays for the
+      // hidden-128 model (C4 batch: 35.5 vs 38.2 ms); 16-bit 16-sample
+      // chunks (two workgroups per CU)
+      if (precision == kF32 && H <= 128 && S % 48 == 0) ns = 48;
+      else if (precision == kF32 && S % 24 == 0) ns = 24;
       else ns = (S % 16 == 0) ? 16 : 8;
     }
   }
