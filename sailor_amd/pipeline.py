@@ -59,11 +59,13 @@ CONFIGS = {
 class FramePipeline:
     def __init__(self, cfg: FrameConfig, prec="fp32", device="cuda", early_exit=False,
                  weights: NerfWeights = None, chunk_samples=0, half_maps=None, quantize=None,
-                 compensated=False):
+                 compensated=False, coarse_start=-1):
         """prec: "fp32" / "bf16" / "fp16" (MLP arithmetic).  16-bit forms read
         16-bit feature maps (half_maps), by default quantised per channel
         with power-of-two scales folded into the first layer (quantize);
-        compensated: Kahan-compensated TSDF and compositing sums."""
+        compensated: Kahan-compensated TSDF and compositing sums;
+        coarse_start: node index of the first coarse node (-1: right after
+        the fine nodes)."""
         self.cfg = cfg
         self.chunk_samples = chunk_samples
         self.dev = torch.device(device)
@@ -98,6 +100,7 @@ class FramePipeline:
         if self.quantize and not self.half_maps:
             raise RuntimeError("quantised feature maps need a 16-bit form")
         self.compensated = bool(compensated)
+        self.coarse_start = int(coarse_start)
         if self.quantize:
             self.fscale = torch.ones(mc.F, dtype=_f32, **o)
             self.q_ws = torch.empty(4 * mc.F, dtype=torch.uint8, **o)
@@ -200,7 +203,7 @@ class FramePipeline:
         cnts = _ptr_array([self.num[0], self.num[1]])
         call("sailor_build_multi_res_volumes", vols, cnts, 2, 1, d, cfg.rate, st)
         call("sailor_octree_build", ptr(self.occ0), ptr(self.occ1), ptr(self.num[0]),
-             ptr(self.num[1]), 1, d, dc, ptr(self.vs), ptr(self.bbox), -1, self.cap,
+             ptr(self.num[1]), 1, d, dc, ptr(self.vs), ptr(self.bbox), self.coarse_start, self.cap,
              ptr(self.node_pos), ptr(self.node_info), ptr(self.queue), ptr(self.tree_ws),
              self.tree_ws.numel() * 4, st)
 

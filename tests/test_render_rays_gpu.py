@@ -38,7 +38,23 @@ def pipe(prec="fp32", cfg=None):
 
 
 def test_fused_trace_matches_oracle_chain():
-    p = pipe()
+    _check_trace(pipe())
+
+
+def test_trace_with_coarse_nodes_after_a_gap():
+    """Tree built with the coarse nodes numbered from 70,000 on (the build's
+    coarse_start; the fine nodes stay at [0, n_fine)): the fused trace on
+    that layout equals the oracle chain on the same arrays."""
+    from sailor_amd.pipeline import FramePipeline
+    p = FramePipeline.synthetic(small_cfg(), prec="fp32", coarse_start=70000)
+    p.run()
+    torch.cuda.synchronize()
+    info = p.node_info.cpu().numpy()
+    assert (info[70000:, 0] >= 0).any() and not (info[:70000, 0] >= 0).all()
+    _check_trace(p)
+
+
+def _check_trace(p):
     cfg = p.cfg
     info = p.check()
     assert info["valid_rays"] > 50, info
