@@ -1673,7 +1673,9 @@ render_backward_kernel(BackArgs ba) {
     }
     __syncthreads();
     // layer-0 input of the chunk at sample c0: features (from keep) and
-    // positional encoding, by threads t0, t0 + tstep, ...
+    // positional encoding, by threads t0, t0 + tstep, ... (building the
+    // first chunk's on waves 1-3 during the replay measured no faster: the
+    // second copy of this code spills registers)
     auto build_input = [&](int c0, int t0, int tstep) {
       for (int it = t0; it < NC * (F / CE); it += tstep) {
         const int col = it % NC, ch = it / NC;   // consecutive lanes: consecutive columns
@@ -1743,7 +1745,6 @@ render_backward_kernel(BackArgs ba) {
     // sample i in lane i: the recurrences run in sample order on values
     // broadcast with readlane, the same operations in the same order as the
     // forward's (so the early termination is the forward's)
-    // while waves 1-3 build the first chunk's layer-0 input
     if (wave == 0) {
       const float al = lane < S ? s_alpha[lane] : 0.0f;
       const float gl = lane < S ? s_g[lane] : 0.0f;
@@ -1772,14 +1773,12 @@ render_backward_kernel(BackArgs ba) {
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) dsum += __shfl_xor(dsum, off, 64);
       if (lane == 0) gsb += dsum;
-    } else {
-      build_input(0, tid - 64, kThreads - 64);
     }
     __syncthreads();
     // 4./5. per chunk
     for (int c0 = 0; c0 < S; c0 += NS) {
       const long long col0 = (item - ba.item_begin) * S * kVP + (long long)c0 * kVP;
-      if (c0 > 0) build_input(c0, tid, kThreads);
+      build_input(c0, tid, kThreads);
       __syncthreads();
       if constexpr (!DW) store_cols<H, PREC, NC, K0>(x0, ba.A0, col0, tid);
       if constexpr (F32) {
