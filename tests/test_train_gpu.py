@@ -13,15 +13,15 @@ pytestmark = pytest.mark.gpu
 _pipes = {}
 
 
-def pipe(prec="fp32"):
+def pipe(prec="fp32", model="c1"):
     from sailor_amd.pipeline import FrameConfig, FramePipeline
-    cfg = FrameConfig("c1_small", 4, 512, 600.0, 128, 8, 48, 56.25, 64, "c1")
-    if prec not in _pipes:
+    cfg = FrameConfig(f"{model}_small", 4, 512, 600.0, 128, 8, 48, 56.25, 64, model)
+    if (prec, model) not in _pipes:
         p = FramePipeline.synthetic(cfg, prec=prec)
         p.run()
         torch.cuda.synchronize()
-        _pipes[prec] = p
-    return _pipes[prec]
+        _pipes[(prec, model)] = p
+    return _pipes[(prec, model)]
 
 
 def _small_layout(w, tensors):
@@ -35,9 +35,12 @@ def _small_layout(w, tensors):
     return torch.cat([p.reshape(-1) for p in parts])
 
 
-def test_hydra_backward_matches_autograd():
+@pytest.mark.parametrize("model", ["c1", "c2"])
+def test_hydra_backward_matches_autograd(model):
+    """F = 32 (c1): the fp32 MFMA backward kernels; F = 64 (c2): the vector
+    ALU kernel."""
     from sailor_amd.train import BackwardScratch, hydra_backward, workspace_views
-    p = pipe()
+    p = pipe(model=model)
     cfg, w = p.cfg, p.weights
     n_rays = cfg.target * cfg.target
     p.shade()
