@@ -524,8 +524,11 @@ struct Smem {
   static constexpr size_t tval = alpha + (size_t)NS * 4;               // f32 [NS]
   static constexpr size_t wts = tval + (size_t)NS * 4;                 // f32 [NS]
   static constexpr size_t flag = wts + (size_t)NS * 4;                 // int [4]
-  static constexpr size_t bias = flag + 16;                            // f32 [5][H]
-  static constexpr size_t pos = bias + (size_t)5 * H * 4;              // f32 [NS][4]
+  // f32 [4][H] layer biases, then (fp32 form) [H] the occupancy weights;
+  // the 16-bit forms read those from L1, which keeps their 24-sample chunks
+  // within two workgroups per CU
+  static constexpr size_t bias = flag + 16;
+  static constexpr size_t pos = bias + (size_t)(PREC == kF32 ? 5 : 4) * H * 4;  // f32 [NS][4]
   static constexpr size_t total = pos + (size_t)NS * 16;
 };
 
@@ -596,8 +599,9 @@ __device__ __forceinline__ void gather_quarter(const RenderArgs& a, int b, int v
 
 // Waves per SIMD the register allocation is fitted to: two workgroups per CU
 // let one workgroup's gather / compositing overlap the other's MFMA work.
-// (16-bit forms with at most 64 columns per chunk).
-constexpr int render_occupancy(int PREC, int NC) { return (PREC != kF32 && NC <= 64) ? 2 : 1; }
+// (16-bit forms with at most 96 columns per chunk: up to 24 samples x 4
+// view slots, 256 registers and 80 KB of LDS per workgroup).
+constexpr int render_occupancy(int PREC, int NC) { return (PREC != kF32 && NC <= 96) ? 2 : 1; }
 
 // VP view slots per sample: 4 (up to 4 views) or 8 (up to 8 views); the
 // columns of a chunk are (sample, view slot) pairs, col = sample * VP + slot.
@@ -640,7 +644,8 @@ render_kernel(RenderArgs a) {
   // layer biases and the occupancy-head weights stay in LDS for the whole
   // kernel (the first loop iteration's barrier publishes them)
   for (int i = tid; i < 4 * H; i += kThreads) s_bias[i] = a.bdense[i];
-  for (int i = tid; i < H; i += kThreads) s_bias[4 * H + i] = ws[so.sigma_w + i];
+  if constexpr (PREC == kF32)
+    for (int i = tid; i < H; i += kThreads) s_bias[4 * H + i] = ws[so.sigma_w + i];
   const float inv_v = 1.0f / (float)V;
 
   // every list entry costs about the same, so a static stride over the
@@ -778,7 +783,7 @@ render_kernel(RenderArgs a) {
             const int u0 = unit_of32<H>(wave, mt, h, 0);
 #pragma unroll
             for (int r = 0; r < 16; ++r)
-              part += s_bias[4 * H + u0 + r] * fmaxf(acc[mt][nt][r] + bias[u0 + r], 0.0f);
+              part += ws[so.sigma_w + u0 + r] * fmaxf(acc[mt][nt][r] + bias[u0 + r], 0.0f);
           }
           part += __shfl_xor(part, 32, 64);
           if (h == 0) sig[wave * NC + 32 * nt + c] = part;
@@ -3151,10 +3156,11 @@ int render_stages(int stages, const float* t, const float* rays, int B, int R, i
     } else {
       // measured on MI355X: fp32 24-sample chunks for the hidden-256 model
       // (C2 frame; 48 do not fit the LDS), whole 48-sample rays for the
-      // hidden-128 model (C4 batch: 35.5 vs 38.2 ms); 16-bit 16-sample
-      // chunks (two workgroups per CU)
+      // hidden-128 model (C4 batch: 35.5 vs 38.2 ms); 16-bit 24-sample
+      // chunks at two workgroups per CU (C2 shade 8.25 vs 8.67 ms for 16,
+      // C4 forward 8.54 vs 11.0 ms), 16 when S is not a multiple of 24
       if (precision == kF32 && H <= 128 && S % 48 == 0) ns = 48;
-      else if (precision == kF32 && S % 24 == 0) ns = 24;
+      else if (S % 24 == 0) ns = 24;
       else ns = (S % 16 == 0) ? 16 : 8;
     }
   }
