@@ -410,17 +410,19 @@ size_t sailor_render_workspace_bytes_views(int B, int R, int F, int V);
  *              compensated compositing sums (depth, weight and alpha sums,
  *              per-view tokens; precision form, no counterpart in the
  *              reference); bit 2 -- in the 16-bit forms, run the Hydra
- *              encoder and RGB head on fp32 vector ALUs instead of 16-bit
- *              MFMA (the differentiation form, whose backward recomputes
- *              them in fp32)
+ *              encoder and RGB head on fp32 MFMA instead of 16-bit MFMA
+ *              (the differentiation form, whose backward recomputes them
+ *              in fp32)
  *  chunk_samples  launch shape: bits 0-7 = samples evaluated together per
  *              workgroup step (up to 4 views: 8, 16, 24 or 48; 5-8 views:
- *              fp32 8, 12 or 16, 16-bit 4, 8 or 16; 0 = fp32: 24, 16-bit:
- *              16, or the largest of 16 and 8 that divides S; with more
- *              than 4 views fp32 12 (8 when S is not a multiple of 12),
- *              16-bit 8); S must be a multiple of it; a chunk whose
- *              activations do not fit the 160 KiB of LDS of a workgroup
- *              (fp32 with 48 samples) is rejected.  Bits 8-15 = percent of
+ *              fp32 8, 12 or 16, 16-bit 4, 8, 12 or 16; 0 = up to 4
+ *              views: fp32 48 for H <= 128 when S is a multiple of 48,
+ *              else 24 when S is a multiple of 24 (fp32 and 16-bit), else
+ *              the largest of 16 and 8 that divides S; more than 4 views:
+ *              12, or 8 when S is not a multiple of 12); S must be a
+ *              multiple of it; a chunk whose activations do not fit the
+ *              160 KiB of LDS of a workgroup (fp32 hidden-256 with 48
+ *              samples) is rejected.  Bits 8-15 = percent of
  *              the device's resident workgroup slots the sample kernel
  *              launches (0 = all), for frames that share the chip with
  *              other streams' work

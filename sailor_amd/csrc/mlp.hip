@@ -3068,8 +3068,8 @@ int dispatch_prec(int prec, const RenderArgs& a, hipStream_t st, bool dry) {
   return sailor::set_error(SAILOR_ERR_ARG, "render_rays: precision %d unsupported", prec);
 }
 
-// 8 view slots: fp32 8, 12 or 16 samples per chunk (default 12, 8 when S is
-// not a multiple of 12), 16-bit 4, 8 or 16 (default 8): the 4-slot defaults'
+// 8 view slots: fp32 8, 12 or 16 samples per chunk, 16-bit 4, 8, 12 or 16
+// (default 12, 8 when S is not a multiple of 12): the 4-slot defaults'
 // column counts and their neighbours; chunks whose LDS does not fit are
 // rejected by launch().
 template <int F, int H>
@@ -3086,6 +3086,8 @@ int dispatch_vp8(int ns, int prec, const RenderArgs& a, hipStream_t st, bool dry
                                    : launch<F, H, kF16, 4, 8>(a, st, dry);
       case 8: return prec == kBF16 ? launch<F, H, kBF16, 8, 8>(a, st, dry)
                                    : launch<F, H, kF16, 8, 8>(a, st, dry);
+      case 12: return prec == kBF16 ? launch<F, H, kBF16, 12, 8>(a, st, dry)
+                                    : launch<F, H, kF16, 12, 8>(a, st, dry);
       case 16: return prec == kBF16 ? launch<F, H, kBF16, 16, 8>(a, st, dry)
                                     : launch<F, H, kF16, 16, 8>(a, st, dry);
     }
@@ -3093,8 +3095,8 @@ int dispatch_vp8(int ns, int prec, const RenderArgs& a, hipStream_t st, bool dry
     return sailor::set_error(SAILOR_ERR_ARG, "render_rays: precision %d unsupported", prec);
   }
   return sailor::set_error(SAILOR_ERR_ARG, "render_rays: %d samples per chunk not "
-                           "instantiated for 8 view slots (fp32: 8, 12, 16; 16-bit: 4, 8, 16)",
-                           ns);
+                           "instantiated for 8 view slots (fp32: 8, 12, 16; 16-bit: 4, 8, 12, "
+                           "16)", ns);
 }
 
 template <int F, int H>
@@ -3152,7 +3154,7 @@ int render_stages(int stages, const float* t, const float* rays, int B, int R, i
                  "render_rays: launch shape %#x: grid share %d%% above 100", chunk_samples, share);
   if (ns <= 0) {
     if (V > kVP) {
-      ns = (precision == kF32 && S % 12 == 0) ? 12 : 8;  // 8-slot instantiations
+      ns = S % 12 == 0 ? 12 : 8;  // 8-slot instantiations (96 columns, as 24 x 4)
     } else {
       // measured on MI355X: fp32 24-sample chunks for the hidden-256 model
       // (C2 frame; 48 do not fit the LDS), whole 48-sample rays for the

@@ -349,7 +349,7 @@ class BatchPipeline:
 
     def _chunk_candidates(self, p):
         if p.cfg.n_views > 4:
-            return (8, 12, 16) if p.prec == "fp32" else (4, 8, 16)
+            return (8, 12, 16) if p.prec == "fp32" else (4, 8, 12, 16)
         return (8, 16, 24, 48)
 
     @staticmethod
@@ -371,7 +371,7 @@ class BatchPipeline:
            tile: columns = samples x view slots) with the shortest render of
            that frame alone, among the instantiated sizes that divide S and
            fit the LDS (4 view slots: 8/16/24/48; 8 slots: fp32 8/12/16,
-           16-bit 4/8/16).
+           16-bit 4/8/12/16).
         2. For the whole batch, the share of the chip each sample kernel
            takes (percent of the resident workgroup slots, so other frames'
            geometry kernels find room) and the number of streams the frames

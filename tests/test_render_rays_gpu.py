@@ -266,7 +266,7 @@ def test_batch_graph_matches_eager_frames():
 def test_every_launch_shape_renders_the_same_image(views, prec):
     """Columns of the MLP are independent and the compositing and token sums
     run in sample order, so every instantiated chunk size (4 view slots:
-    8/16/24/48; 8 slots: fp32 8/12/16, 16-bit 4/8/16) and a sample kernel
+    8/16/24/48; 8 slots: fp32 8/12/16, 16-bit 4/8/12/16) and a sample kernel
     launched on part of the chip (grid share 40 %) give the default launch's
     image bit for bit; chunks whose LDS does not fit are rejected."""
     from sailor_amd import _lib
@@ -274,7 +274,7 @@ def test_every_launch_shape_renders_the_same_image(views, prec):
     cfg = FrameConfig(f"shapes_v{views}", views, 512, 600.0, 128, 8, 40, 47.0, 48, "c1")
     p = pipe(prec, cfg)
     ref = (p.rgb.clone(), p.depth.clone(), p.aux.clone())
-    cands = (8, 16, 24, 48) if views <= 4 else ((8, 12, 16) if prec == "fp32" else (4, 8, 16))
+    cands = (8, 16, 24, 48) if views <= 4 else ((8, 12, 16) if prec == "fp32" else (4, 8, 12, 16))
     ran = []
     for ns in cands:
         for share in (0, 40):
