@@ -416,10 +416,9 @@ size_t sailor_render_workspace_bytes_views(int B, int R, int F, int V);
  *  chunk_samples  launch shape: bits 0-7 = samples evaluated together per
  *              workgroup step (up to 4 views: 8, 16, 24 or 48; 5-8 views:
  *              fp32 8, 12 or 16, 16-bit 4, 8, 12 or 16; 0 = up to 4
- *              views: fp32 48 for H <= 128 when S is a multiple of 48,
- *              else 24 when S is a multiple of 24 (fp32 and 16-bit), else
- *              the largest of 16 and 8 that divides S; more than 4 views:
- *              12, or 8 when S is not a multiple of 12); S must be a
+ *              views: 24 when S is a multiple of 24, else the largest of
+ *              16 and 8 that divides S; more than 4 views: 12, or 8 when S
+ *              is not a multiple of 12); S must be a
  *              multiple of it; a chunk whose activations do not fit the
  *              160 KiB of LDS of a workgroup (fp32 hidden-256 with 48
  *              samples) is rejected.  Bits 8-15 = percent of

@@ -599,16 +599,19 @@ __device__ __forceinline__ void gather_quarter(const RenderArgs& a, int b, int v
 
 // Waves per SIMD the register allocation is fitted to: two workgroups per CU
 // let one workgroup's gather / compositing overlap the other's MFMA work.
-// (16-bit forms with at most 96 columns per chunk: up to 24 samples x 4
-// view slots, 256 registers and 80 KB of LDS per workgroup).
-constexpr int render_occupancy(int PREC, int NC) { return (PREC != kF32 && NC <= 96) ? 2 : 1; }
+// (chunks of at most 96 columns -- up to 24 samples x 4 view slots -- of
+// the 16-bit forms and of the fp32 hidden-128 model: 256 registers and 80 KB
+// of LDS per workgroup; fp32 hidden-256 activations take 96 KB).
+constexpr int render_occupancy(int PREC, int NC, int H) {
+  return (NC <= 96 && (PREC != kF32 || H <= 128)) ? 2 : 1;
+}
 
 // VP view slots per sample: 4 (up to 4 views) or 8 (up to 8 views); the
 // columns of a chunk are (sample, view slot) pairs, col = sample * VP + slot.
 template <int F, int H, int PREC, int NS, bool HALF_MAP, bool POINTS, int VP>
 __global__ void __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(render_occupancy(PREC, NS * VP),
-                                   render_occupancy(PREC, NS * VP))))
+__attribute__((amdgpu_waves_per_eu(render_occupancy(PREC, NS * VP, H),
+                                   render_occupancy(PREC, NS * VP, H))))
 render_kernel(RenderArgs a) {
   typedef typename Tr<PREC>::E E;
   typedef Smem<F, H, PREC, NS, VP> L;
@@ -3156,13 +3159,12 @@ int render_stages(int stages, const float* t, const float* rays, int B, int R, i
     if (V > kVP) {
       ns = S % 12 == 0 ? 12 : 8;  // 8-slot instantiations (96 columns, as 24 x 4)
     } else {
-      // measured on MI355X: fp32 24-sample chunks for the hidden-256 model
-      // (C2 frame; 48 do not fit the LDS), whole 48-sample rays for the
-      // hidden-128 model (C4 batch: 35.5 vs 38.2 ms); 16-bit 24-sample
-      // chunks at two workgroups per CU (C2 shade 8.25 vs 8.67 ms for 16,
-      // C4 forward 8.54 vs 11.0 ms), 16 when S is not a multiple of 24
-      if (precision == kF32 && H <= 128 && S % 48 == 0) ns = 48;
-      else if (S % 24 == 0) ns = 24;
+      // measured on MI355X: 24-sample chunks -- fp32 hidden-256 at one
+      // workgroup per CU (C2 frame; 48 do not fit the LDS), fp32 hidden-128
+      // and the 16-bit forms at two (C2 shade 8.25 vs 8.67 ms for 16; C4
+      // forward bf16 8.54 vs 11.0 ms, fp32 32.0 vs 35.4 ms for 48); 16 or 8
+      // when S is not a multiple of 24
+      if (S % 24 == 0) ns = 24;
       else ns = (S % 16 == 0) ? 16 : 8;
     }
   }
