@@ -131,8 +131,11 @@ __device__ __forceinline__ int traverse(const RayCtx& c, int th_early_stop,
           int r = count;
           if (!training && info.z >= 0) {
             const int ch_end = min(info.z + info.w, c.n_nodes);
+            // the next child's record is loaded while this one is tested
+            float4 qn = info.z < ch_end ? c.np[info.z] : make_float4(0.f, 0.f, 0.f, 0.f);
             for (int ch = info.z; ch < ch_end; ++ch) {
-              const float4 q = c.np[ch];
+              const float4 q = qn;
+              if (ch + 1 < ch_end) qn = c.np[ch + 1];
               const float2 cd = slab_test(c.o, c.inv, make_float3(q.x, q.y, q.z), q.w);
               if (cd.x < 0 || cd.y < 0) continue;
               sink.put(r, ch, cd.x, cd.y);
