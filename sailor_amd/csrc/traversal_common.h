@@ -71,6 +71,56 @@ struct RayCtx {
   }
 };
 
+// Coarse-grid DDA of a ray (intersection_kernel.cu:128-439): the cell
+// (x, y, z) and the ray parameters of the next x / y / z boundaries.
+struct Dda {
+  int x, y, z, ix, iy, iz;
+  float tx, ty, tz, dtx, dty, dtz;
+  // entry cell at ray parameter t_in (the volume's slab-test entry)
+  __device__ __forceinline__ void start(const RayCtx& c, float t_in) {
+    const float spx = c.o.x + t_in * c.d.x;
+    const float spy = c.o.y + t_in * c.d.y;
+    const float spz = c.o.z + t_in * c.d.z;
+    const float sox = spx - c.bst.x, soy = spy - c.bst.y, soz = spz - c.bst.z;
+    x = sat_f2i(sox / c.vs_top);
+    y = sat_f2i(soy / c.vs_top);
+    z = sat_f2i(soz / c.vs_top);
+    x = x < 0 ? 0 : x;
+    y = y < 0 ? 0 : y;
+    z = z < 0 ? 0 : z;
+    x = x < c.dc ? x : c.dc - 1;
+    y = y < c.dc ? y : c.dc - 1;
+    z = z < c.dc ? z : c.dc - 1;
+    const float stx = (c.d.x >= 0) ? 1.f : -1.f;
+    const float sty = (c.d.y >= 0) ? 1.f : -1.f;
+    const float stz = (c.d.z >= 0) ? 1.f : -1.f;
+    const float nbx = (stx > 0) ? ((float)x + stx) * c.vs_top : (float)x * c.vs_top;
+    const float nby = (sty > 0) ? ((float)y + sty) * c.vs_top : (float)y * c.vs_top;
+    const float nbz = (stz > 0) ? ((float)z + stz) * c.vs_top : (float)z * c.vs_top;
+    tx = (c.d.x != 0) ? (nbx - sox) / c.d.x : kDdaFar;
+    ty = (c.d.y != 0) ? (nby - soy) / c.d.y : kDdaFar;
+    tz = (c.d.z != 0) ? (nbz - soz) / c.d.z : kDdaFar;
+    dtx = (c.d.x != 0) ? c.vs_top / c.d.x * stx : kDdaFar;
+    dty = (c.d.y != 0) ? c.vs_top / c.d.y * sty : kDdaFar;
+    dtz = (c.d.z != 0) ? c.vs_top / c.d.z * stz : kDdaFar;
+    ix = (int)stx;
+    iy = (int)sty;
+    iz = (int)stz;
+  }
+  __device__ __forceinline__ int cell(int dc) const { return (x * dc + y) * dc + z; }
+  // one step; false once the cell leaves the grid
+  __device__ __forceinline__ bool advance(int dc) {
+    if (tx < ty) {
+      if (tx < tz) { x += ix; tx += dtx; }
+      else { z += iz; tz += dtz; }
+    } else {
+      if (ty < tz) { y += iy; ty += dty; }
+      else { z += iz; tz += dtz; }
+    }
+    return !(x < 0 || x >= dc || y < 0 || y >= dc || z < 0 || z >= dc);
+  }
+};
+
 // Sink interface: put(pos, node, t0, t1) receives hit number `pos`;
 // stop() is polled after every coarse cell and may end the walk early (used
 // by the fused kernel once all samples are placed; never changes hits that
@@ -89,44 +139,29 @@ __device__ __forceinline__ int traverse(const RayCtx& c, int th_early_stop,
     *vl_out = vl;
     return 0;
   }
-  const float spx = c.o.x + dep.x * c.d.x;
-  const float spy = c.o.y + dep.x * c.d.y;
-  const float spz = c.o.z + dep.x * c.d.z;
-  const float sox = spx - c.bst.x, soy = spy - c.bst.y, soz = spz - c.bst.z;
-  int x = sat_f2i(sox / c.vs_top), y = sat_f2i(soy / c.vs_top),
-      z = sat_f2i(soz / c.vs_top);
-  x = x < 0 ? 0 : x;
-  y = y < 0 ? 0 : y;
-  z = z < 0 ? 0 : z;
-  x = x < c.dc ? x : c.dc - 1;
-  y = y < c.dc ? y : c.dc - 1;
-  z = z < c.dc ? z : c.dc - 1;
-  const float stx = (c.d.x >= 0) ? 1.f : -1.f;
-  const float sty = (c.d.y >= 0) ? 1.f : -1.f;
-  const float stz = (c.d.z >= 0) ? 1.f : -1.f;
-  const float nbx = (stx > 0) ? ((float)x + stx) * c.vs_top : (float)x * c.vs_top;
-  const float nby = (sty > 0) ? ((float)y + sty) * c.vs_top : (float)y * c.vs_top;
-  const float nbz = (stz > 0) ? ((float)z + stz) * c.vs_top : (float)z * c.vs_top;
-  float tx = (c.d.x != 0) ? (nbx - sox) / c.d.x : kDdaFar;
-  float ty = (c.d.y != 0) ? (nby - soy) / c.d.y : kDdaFar;
-  float tz = (c.d.z != 0) ? (nbz - soz) / c.d.z : kDdaFar;
-  const float dtx = (c.d.x != 0) ? c.vs_top / c.d.x * stx : kDdaFar;
-  const float dty = (c.d.y != 0) ? c.vs_top / c.d.y * sty : kDdaFar;
-  const float dtz = (c.d.z != 0) ? c.vs_top / c.d.z * stz : kDdaFar;
-  const int ix = (int)stx, iy = (int)sty, iz = (int)stz;
-
+  Dda g;
+  g.start(c, dep.x);
   const int J = 4 * c.dc;
   bool pre_empty = false, has_valid = false;
   int n_succ_empty = 0, n_inter = 0;
+  // The DDA's path depends on the ray alone, so the coarse-volume entry of
+  // the next cell is loaded while the current cell is visited.
+  int node_next = c.vc[g.cell(c.dc)];
   for (int j = 0; j < J; ++j) {
     // ---- visit coarse cell (x, y, z): RayVoxelIntersectionNodeListNew
     int rid = -1;
-    const int node = c.vc[(x * c.dc + y) * c.dc + z];
+    const int node = node_next;
+    Dda nx = g;   // the cell after this one (the step below)
+    const bool next_in = nx.advance(c.dc);
+    if (next_in && j + 1 < J) node_next = c.vc[nx.cell(c.dc)];
     if (node > -1 && node < c.n_nodes) {
+      // the node's record and its info are loaded together (the info is
+      // used only when the slab test hits, which it nearly always does for
+      // a cell the DDA visits)
       const float4 p = c.np[node];
+      const int4 info = c.ni[node];
       const float2 vd = slab_test(c.o, c.inv, make_float3(p.x, p.y, p.z), p.w);
       if (!(vd.x < 0 || vd.y < 0)) {
-        const int4 info = c.ni[node];
         if (info.x >= 0) {
           int r = count;
           if (!training && info.z >= 0) {
@@ -170,15 +205,9 @@ __device__ __forceinline__ int traverse(const RayCtx& c, int th_early_stop,
       if (n_inter >= opt_num) break;
     }
     if (sink.stop()) break;
-    // ---- DDA step
-    if (tx < ty) {
-      if (tx < tz) { x += ix; tx += dtx; }
-      else { z += iz; tz += dtz; }
-    } else {
-      if (ty < tz) { y += iy; ty += dty; }
-      else { z += iz; tz += dtz; }
-    }
-    if (x < 0 || x >= c.dc || y < 0 || y >= c.dc || z < 0 || z >= c.dc) break;
+    // ---- DDA step (computed above)
+    g = nx;
+    if (!next_in) break;
   }
   *vl_out = vl;
   return count;
