@@ -294,6 +294,38 @@ def _median_ms(fn, steps):
     return float(np.median(ts))
 
 
+def measured_peaks(steps=20):
+    """What this box reaches against the datasheet peaks: HBM by a 2 GiB
+    device-to-device copy (read + write bytes over the copy's time), MFMA by
+    square 8192 GEMMs in bf16, fp16 and fp32 (torch.matmul on contiguous 2-D
+    operands, i.e. the vendor GEMM library).  Reported beside the datasheet
+    fractions, never in their place."""
+    import torch
+    n = (2 << 30) // 4
+    src = torch.empty(n, dtype=torch.float32, device="cuda").uniform_()
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    ms = _median_ms(lambda: dst.copy_(src), steps)
+    hbm = 2.0 * n * 4 / (ms / 1e3) / 1e9
+    del src, dst
+    out = {"hbm_copy_GB_per_s": hbm, "hbm_datasheet_GB_per_s": HBM_GBS,
+           "hbm_copy_frac_of_datasheet": hbm / HBM_GBS}
+    m = 8192
+    for name, dt in (("bf16", torch.bfloat16), ("fp16", torch.float16), ("fp32", torch.float32)):
+        a = torch.randn(m, m, device="cuda").to(dt).contiguous()
+        b = torch.randn(m, m, device="cuda").to(dt).contiguous()
+        for _ in range(3):
+            torch.matmul(a, b)
+        ms = _median_ms(lambda: torch.matmul(a, b), max(5, steps // 2))
+        tf = 2.0 * m ** 3 / (ms / 1e3) / 1e12
+        out[f"{name}_gemm_TFLOP_per_s"] = tf
+        out[f"{name}_gemm_frac_of_datasheet"] = tf / PEAKS[name]
+        del a, b
+    torch.cuda.synchronize()
+    return out
+
+
 def training_form(steps, warmup, with_cpu=True):
     """Config-4 block (differentiation form): 8 target views x 256^2 rays of
     the config-1 scene and model (4 x 1024^2 source views, 64 samples/ray).
@@ -900,6 +932,15 @@ def main():
         result["fastnerf_upsampling"] = fastnerf_upsampling(max(3, args.steps // 4))
         result["gui_frame"] = gui_frame(max(3, args.steps // 4))
         result["standalone_kernels"] = standalone_kernels(max(3, args.steps // 4))
+    mp = measured_peaks()
+    result["measured_peaks"] = mp
+    result["roofline"]["frac_of_measured_peak"] = (
+        result["roofline"]["achieved"] / mp["fp32_gemm_TFLOP_per_s"])
+    for prec in ("bf16", "fp16"):
+        if prec in result:
+            result[prec]["sample_kernel_frac_of_measured_gemm"] = (
+                result[prec]["sample_kernel_tflops"] / mp[f"{prec}_gemm_TFLOP_per_s"])
+    progress("peak probes done")
     if not args.no_cpu:
         progress("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(p32)
