@@ -25,16 +25,23 @@ def small_cfg(name="prec_small"):
     return FrameConfig(name, 4, 512, 600.0, 128, 8, 48, 56.25, 64, "c1")
 
 
-@pytest.mark.parametrize("prec", ["bf16", "fp16"])
-def test_quantize_channels_bitexact(prec):
+@pytest.mark.parametrize("prec,C", [("bf16", 64), ("fp16", 64), ("bf16", 32), ("fp16", 2),
+                                     ("bf16", 512)])
+def test_quantize_channels_bitexact(prec, C):
+    """C a multiple of 4 up to 256 takes the 16-byte-load reduction, C = 2
+    and C = 512 the scalar one; 5000 rows leave a remainder for both, and a
+    channel's maximum sits in the last row."""
     from sailor_amd._lib import call, ptr, stream_of
     from sailor_amd.model import PREC_IDS
     rng = np.random.default_rng(1)
-    C, rows = 64, 5000
+    rows = 5000
     x = rng.standard_normal((rows, C)).astype(np.float32)
     x *= np.float32(10.0) ** rng.uniform(-6, 6, C).astype(np.float32)   # wide per-channel ranges
-    x[:, 5] = 0.0                                                        # all-zero channel
-    x[:, 7] = 4.0                                                        # exact power of two
+    x[:, C - 1] *= np.float32(1e-3)
+    x[rows - 1, C - 1] = 3.0e4 if prec == "bf16" else 3.0e3              # maximum in the tail
+    if C > 7:
+        x[:, 5] = 0.0                                                    # all-zero channel
+        x[:, 7] = 4.0                                                    # exact power of two
     xt = torch.from_numpy(x).cuda()
     dt = torch.bfloat16 if prec == "bf16" else torch.float16
     out = torch.empty((rows, C), dtype=dt, device="cuda")
@@ -46,7 +53,8 @@ def test_quantize_channels_bitexact(prec):
     _, e = np.frexp(m)
     want_s = np.where(m > 0, np.ldexp(1.0, e), 1.0).astype(np.float32)
     np.testing.assert_array_equal(sc.cpu().numpy(), want_s)
-    assert sc[7].item() == 8.0 and sc[5].item() == 1.0
+    if C > 7:
+        assert sc[7].item() == 8.0 and sc[5].item() == 1.0
     want_q = torch.from_numpy(x / want_s).to(dt)
     assert torch.equal(out.cpu(), want_q)
     assert float(out.float().abs().max()) <= 1.0
