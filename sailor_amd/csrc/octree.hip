@@ -59,6 +59,67 @@ merge_kernel(int* __restrict__ coarse, int* __restrict__ count,
   }
 }
 
+// The fine rows of a coarse cell spread over lanes: rate^2 lanes per cell
+// (a whole wave at rate 8), lane r reading fine row r (rate voxels along z).
+// One lane per cell walking all rate^3 voxels leaves the grid at 32^3 lanes
+// and each lane's 64 row reads in sequence.  Used when rate^2 divides 64.
+__host__ __device__ constexpr bool rows_per_lane_ok(int rate) {
+  return rate >= 2 && rate <= 8 && 64 % (rate * rate) == 0;
+}
+
+// fine row r of coarse cell (cx, cy, cz): any voxel occupied / how many
+__device__ __forceinline__ int row_occupied_count(const int* __restrict__ fb, int df, int rate,
+                                                  int cx, int cy, int cz, int r) {
+  const int k = cx * rate + r / rate, m = cy * rate + r % rate;
+  const int* row = fb + ((size_t)k * df + m) * df + cz * rate;
+  int n = 0;
+  if (rate % 4 == 0) {   // 16-byte aligned: rows start at multiples of rate ints
+    for (int q = 0; q < rate; q += 4) {
+      const int4 v = *reinterpret_cast<const int4*>(row + q);
+      n += (v.x > -1) + (v.y > -1) + (v.z > -1) + (v.w > -1);
+    }
+  } else {
+    for (int q = 0; q < rate; ++q) n += row[q] > -1;
+  }
+  return n;
+}
+
+// lanes of this lane's segment of rr lanes (rr divides 64, segments aligned)
+__device__ __forceinline__ unsigned long long segment_mask(int lane, int rr) {
+  return rr == 64 ? ~0ull : ((1ull << rr) - 1) << (lane - lane % rr);
+}
+
+// merge with rate^2 lanes per coarse cell.  grid = (ceil(dc^3 rate^2 / 256), B)
+__global__ void __launch_bounds__(kThreads)
+merge_rows_kernel(int* __restrict__ coarse, int* __restrict__ count,
+                  const int* __restrict__ fine, int dc, int rate) {
+  __shared__ int s_cnt[kThreads / 64];
+  const int b = blockIdx.y;
+  const int ncells = dc * dc * dc, rr = rate * rate;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = (int)(t / rr), r = (int)(t % rr);
+  const int lane = threadIdx.x & 63;
+  bool any = false;
+  if (c < ncells) {
+    const int df = dc * rate;
+    const int cx = c / (dc * dc);
+    const int cy = (c - cx * dc * dc) / dc;
+    const int cz = c - cx * dc * dc - cy * dc;
+    any = row_occupied_count(fine + (size_t)b * df * df * df, df, rate, cx, cy, cz, r) > 0;
+  }
+  const bool occ = (__ballot(any) & segment_mask(lane, rr)) != 0;
+  const bool leader = r == 0 && c < ncells;
+  if (leader) coarse[(size_t)b * ncells + c] = occ ? c : -1;
+  const unsigned long long msk = __ballot(leader && occ);
+  if (lane == 0) s_cnt[threadIdx.x >> 6] = __popcll(msk);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int i = 0; i < kThreads / 64; ++i) s += s_cnt[i];
+    if (s) atomicAdd(count + b, s);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // tree build, step 1: per coarse cell, is it allocated (occupied and not
 // isolated) and how many occupied fine children does it have.
@@ -95,6 +156,49 @@ tree_count(const int* __restrict__ vc, const int* __restrict__ vf, int B,
   }
   flag[gid] = f;
   nchild[gid] = n;
+}
+
+// tree_count with rate^2 lanes per coarse cell: the lanes of a cell share
+// its 26 neighbour tests and count one fine row each; the cell's first lane
+// writes.  grid = ceil(B dc^3 rate^2 / 256).
+__global__ void __launch_bounds__(kThreads)
+tree_count_rows(const int* __restrict__ vc, const int* __restrict__ vf, int B,
+                int dc, int rate, int* __restrict__ flag, int* __restrict__ nchild) {
+  const int ncells = dc * dc * dc, rr = rate * rate;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long gid = t / rr;
+  const int r = (int)(t % rr);
+  const int lane = threadIdx.x & 63;
+  const bool in = gid < (long long)B * ncells;
+  int b = 0, c = 0, cx = 0, cy = 0, cz = 0;
+  bool alloc = false, near = false;
+  if (in) {
+    b = (int)(gid / ncells);
+    c = (int)(gid - (long long)b * ncells);
+    alloc = vc[gid] > -1;
+    if (alloc) {
+      cx = c / (dc * dc);
+      cy = (c - cx * dc * dc) / dc;
+      cz = c - cx * dc * dc - cy * dc;
+      const int* base = vc + (size_t)b * ncells;
+      for (int j = r; j < 27; j += rr) {   // neighbour (j / 9, j / 3 % 3, j % 3) - 1
+        const int k = cx + j / 9 - 1, m = cy + j / 3 % 3 - 1, q = cz + j % 3 - 1;
+        if (j == 13 || k < 0 || k >= dc || m < 0 || m >= dc || q < 0 || q >= dc) continue;
+        if (base[(k * dc + m) * dc + q] > -1) near = true;
+      }
+    }
+  }
+  const bool f = (__ballot(near) & segment_mask(lane, rr)) != 0;
+  int n = 0;
+  if (f) {
+    const int df = dc * rate;
+    n = row_occupied_count(vf + (size_t)b * df * df * df, df, rate, cx, cy, cz, r);
+  }
+  for (int off = 1; off < rr; off <<= 1) n += __shfl_xor(n, off, 64);   // segment sum
+  if (in && r == 0) {
+    flag[gid] = f ? 1 : 0;
+    nchild[gid] = n;
+  }
 }
 
 // Block-level exclusive scan of (a, b) pairs held as kScanItems per thread.
@@ -373,8 +477,14 @@ extern "C" int sailor_build_multi_res_volumes(int32_t* const* vols,
                    "build_multi_res_volumes: dim %d of level %d not divisible by rate %d",
                    d, i, rate);
     int dc = d / rate;
-    hipLaunchKernelGGL(merge_kernel, dim3(sailor::ceil_div((long long)dc * dc * dc, kThreads), B),
-                       dim3(kThreads), 0, st, vols[i + 1], counts[i + 1], vols[i], dc, rate);
+    if (rows_per_lane_ok(rate))
+      hipLaunchKernelGGL(merge_rows_kernel,
+                         dim3(sailor::ceil_div((long long)dc * dc * dc * rate * rate, kThreads), B),
+                         dim3(kThreads), 0, st, vols[i + 1], counts[i + 1], vols[i], dc, rate);
+    else
+      hipLaunchKernelGGL(merge_kernel,
+                         dim3(sailor::ceil_div((long long)dc * dc * dc, kThreads), B),
+                         dim3(kThreads), 0, st, vols[i + 1], counts[i + 1], vols[i], dc, rate);
     int rc = sailor::check_launch("merge_non_empty_nodes");
     if (rc) return rc;
     d = dc;
@@ -411,8 +521,13 @@ int octree_build_launch(int32_t* vol_fine, int32_t* vol_coarse, const int32_t* f
   float4* pos = reinterpret_cast<float4*>(node_pos);
   int4* info = reinterpret_cast<int4*>(node_info);
 
-  hipLaunchKernelGGL(tree_count, dim3(sailor::ceil_div(n, kThreads)), dim3(kThreads), 0, st,
-                     vol_coarse, vol_fine, B, dim_coarse, rate, w.flag, w.nchild);
+  if (rows_per_lane_ok(rate))
+    hipLaunchKernelGGL(tree_count_rows, dim3(sailor::ceil_div(n * rate * rate, kThreads)),
+                       dim3(kThreads), 0, st, vol_coarse, vol_fine, B, dim_coarse, rate, w.flag,
+                       w.nchild);
+  else
+    hipLaunchKernelGGL(tree_count, dim3(sailor::ceil_div(n, kThreads)), dim3(kThreads), 0, st,
+                       vol_coarse, vol_fine, B, dim_coarse, rate, w.flag, w.nchild);
   hipLaunchKernelGGL(tree_block_sums, dim3(nblk), dim3(kThreads), 0, st, w.flag, w.nchild,
                      n, w.bsum_a, w.bsum_b);
   hipLaunchKernelGGL(tree_scan_sums, dim3(1), dim3(kThreads), 0, st, w.bsum_a, w.bsum_b,
