@@ -145,6 +145,18 @@ sort_rows(const float* __restrict__ t, int n_rows, int S, float* __restrict__ ou
   }
   __syncthreads();
   if (!active) return;
+  // A row already in non-decreasing order (every row without samples, all
+  // -1) is its own stable sort: rank of element i = i.  (A NaN fails the
+  // test and takes the rank loop, as before.)
+  bool out_of_order = false;
+  for (int i = lane; i + 1 < S; i += 64) out_of_order |= !(sv[i] <= sv[i + 1]);
+  if (__ballot(out_of_order) == 0) {
+    for (int i = lane; i < S; i += 64) {
+      out[row * S + i] = sv[i];
+      order[row * S + i] = i;
+    }
+    return;
+  }
   for (int i = lane; i < S; i += 64) {
     float v = sv[i];
     int r = 0;

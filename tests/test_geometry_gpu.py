@@ -185,6 +185,27 @@ def test_sort_bitexact(case):
     np.testing.assert_array_equal(g["order"], o["sort"][1])
 
 
+def test_sort_rows_edge_cases():
+    """sort_samplings on hand-made rows of S = 100 (two 64-lane passes) vs
+    numpy's stable argsort: already ascending with ties (the in-order fast
+    path), all -1, descending, ascending except one pair across lanes 63/64,
+    ascending except the last pair, random with many duplicates."""
+    from sailor_amd import VoxelEncoding as VE
+    S = 100
+    rng = np.random.default_rng(5)
+    asc = np.sort(rng.integers(0, 20, S)).astype(np.float32)        # ties
+    cross = np.arange(S, dtype=np.float32)
+    cross[[63, 64]] = cross[[64, 63]]
+    last = np.arange(S, dtype=np.float32)
+    last[[S - 2, S - 1]] = last[[S - 1, S - 2]]
+    rows = np.stack([asc, np.full(S, -1.0, np.float32), asc[::-1].copy(), cross, last,
+                     rng.integers(0, 5, S).astype(np.float32)])
+    srt, order = VE.sort_samplings(torch.from_numpy(rows).cuda())
+    want = np.argsort(rows, axis=1, kind="stable")
+    np.testing.assert_array_equal(order.cpu().numpy(), want)
+    np.testing.assert_array_equal(srt.cpu().numpy(), np.take_along_axis(rows, want, 1))
+
+
 def test_run_twice_identical():
     """Determinism: node order and hit lists do not depend on scheduling."""
     a = run_case("small_r4")[0]
