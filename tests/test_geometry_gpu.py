@@ -23,6 +23,8 @@ CASES = {
     "small_r4": (128, 150.0, 32, 4, 64, 75.0, False),
     "c1_r8": (512, 600.0, 128, 8, 256, 300.0, True),
     "c1_r4_adaptive": (512, 600.0, 128, 4, 256, 300.0, False),
+    # rate 2: 16 coarse cells per wave in the row-per-lane merge / tree count
+    "small_r2": (128, 150.0, 32, 2, 64, 75.0, False),
 }
 
 
@@ -140,6 +142,31 @@ def test_multi_res_merge(case):
     g, o = case
     np.testing.assert_array_equal(g["coarse"], o["coarse"])
     np.testing.assert_array_equal(g["coarse_cnt"], o["coarse_cnt"])
+
+
+@pytest.mark.parametrize("rate,dim", [(3, 36), (8, 64), (4, 32)])
+def test_merge_sparse_volumes(rate, dim):
+    """build_multi_res_volumes on seeded sparse occupancy (a few per cent of
+    voxels, one isolated cell, one full block) vs the oracle's merge; rate 3
+    (9 rows per cell do not divide a wave) takes the one-lane-per-cell
+    kernel, 4 and 8 the row-per-lane one."""
+    from sailor_amd import VoxelEncoding as VE
+    rng = np.random.default_rng(rate)
+    B = 2
+    hit = rng.random((B, dim, dim, dim)) < 0.03
+    idx = np.arange(dim ** 3, dtype=np.int32).reshape(dim, dim, dim)
+    fine = np.where(hit, idx[None], -1).astype(np.int32)
+    fine[1, :rate, :rate, :rate] = idx[:rate, :rate, :rate]          # one full block
+    fine[0, -1, -1, -1] = idx[-1, -1, -1]                             # a lone corner voxel
+    dc = dim // rate
+    occ0 = torch.from_numpy(fine).cuda()
+    occ1 = torch.full((B, dc, dc, dc), -7, dtype=torch.int32, device="cuda")
+    num0 = torch.from_numpy((fine > -1).reshape(B, -1).sum(1).astype(np.int32)).cuda()
+    num1 = torch.zeros(B, dtype=torch.int32, device="cuda")
+    VE.build_multi_res_volumes([occ0, occ1], [num0, num1], 2, rate, "cuda")
+    want, want_cnt = orc.merge(fine, rate)
+    np.testing.assert_array_equal(occ1.cpu().numpy(), want)
+    np.testing.assert_array_equal(num1.cpu().numpy(), want_cnt)
 
 
 def test_tree_nodes_bitexact(case):
