@@ -113,69 +113,83 @@ trace_kernel(const float* __restrict__ K, const float* __restrict__ Rm,
   }
   __syncthreads();
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= H * W) return;
+  // lanes past the image carry no ray but stay for the wave's tail fill
+  const bool live = p < H * W;
   const long long ray = (long long)b * H * W + p;
-  // ---- camera ray (rays_calculating_kernel, :443-484)
-  const int h = p / W, w = p - h * W;
-  const float* ki = s_m;
-  const float* ri = s_m + 9;
-  const float* t = Tv + (size_t)b * 3;
-  float o[3], cd[3], wd[3], d[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-    o[k] = -ri[3 * k + 0] * t[0] - ri[3 * k + 1] * t[1] - ri[3 * k + 2] * t[2];
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-    cd[k] = (float)w * ki[3 * k + 0] + (float)h * ki[3 * k + 1] + 1.0f * ki[3 * k + 2];
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-    wd[k] = cd[0] * ri[3 * k + 0] + cd[1] * ri[3 * k + 1] + cd[2] * ri[3 * k + 2];
-  const float nrm = sqrtf(wd[0] * wd[0] + wd[1] * wd[1] + wd[2] * wd[2]);
-  const double den = (double)nrm + 1e-10;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) d[k] = (float)((double)wd[k] / den);
-  if (rays_out) {
-    float* ro = rays_out + ray * 6;
-    ro[0] = o[0]; ro[1] = o[1]; ro[2] = o[2];
-    ro[3] = d[0]; ro[4] = d[1]; ro[5] = d[2];
-  }
-  // ---- walk 1: weighted valid length
-  sailor::RayCtx ctx;
-  ctx.init(vc, node_pos, node_info, n_nodes, vol_bbox, voxel_size, b, dim_fine, dim_coarse,
-           rate, o, d);
-  CacheSink cs{s_node + threadIdx.x};
-  float vl = 0.0f;
-  const int count = sailor::traverse(ctx, -1, false, -1, cs, &vl);
-  const bool cached = count <= kHitCache;
-  if (vl_out) vl_out[ray] = vl;
-  int* oi = sidx + ray * S;
-  float* ot = st + ray * S;
-  const float* nz = noise + ray * S;
-  int has = 0;
-  if (count > 0) {
-    // ---- walk 2: samples (sampling_kernel.cu:1125-1154)
-    SampleSink ss;
-    ss.ss.init(vl, voxel_size[b] * (float)start_scale, S);
-    ss.nz = nz; ss.oi = oi; ss.ot = ot; ss.np = node_pos; ss.has = 0;
-    float dummy;
-    if (cached)
-      replay(ctx, cs, count, ss);
-    else
-      sailor::traverse(ctx, -1, false, -1, ss, &dummy);
-    has = ss.has;
-    // ---- walk 3: leftover midpoints (:1156-1174)
-    if (has < S) {
-      LeftoverSink ls{nz, oi, ot, has, S - has, S};
+  int has = S;   // samples placed; rows past the image have no tail
+  if (live) {
+    // ---- camera ray (rays_calculating_kernel, :443-484)
+    const int h = p / W, w = p - h * W;
+    const float* ki = s_m;
+    const float* ri = s_m + 9;
+    const float* t = Tv + (size_t)b * 3;
+    float o[3], cd[3], wd[3], d[3];
+  #pragma unroll
+    for (int k = 0; k < 3; ++k)
+      o[k] = -ri[3 * k + 0] * t[0] - ri[3 * k + 1] * t[1] - ri[3 * k + 2] * t[2];
+  #pragma unroll
+    for (int k = 0; k < 3; ++k)
+      cd[k] = (float)w * ki[3 * k + 0] + (float)h * ki[3 * k + 1] + 1.0f * ki[3 * k + 2];
+  #pragma unroll
+    for (int k = 0; k < 3; ++k)
+      wd[k] = cd[0] * ri[3 * k + 0] + cd[1] * ri[3 * k + 1] + cd[2] * ri[3 * k + 2];
+    const float nrm = sqrtf(wd[0] * wd[0] + wd[1] * wd[1] + wd[2] * wd[2]);
+    const double den = (double)nrm + 1e-10;
+  #pragma unroll
+    for (int k = 0; k < 3; ++k) d[k] = (float)((double)wd[k] / den);
+    if (rays_out) {
+      float* ro = rays_out + ray * 6;
+      ro[0] = o[0]; ro[1] = o[1]; ro[2] = o[2];
+      ro[3] = d[0]; ro[4] = d[1]; ro[5] = d[2];
+    }
+    // ---- walk 1: weighted valid length
+    sailor::RayCtx ctx;
+    ctx.init(vc, node_pos, node_info, n_nodes, vol_bbox, voxel_size, b, dim_fine, dim_coarse,
+             rate, o, d);
+    CacheSink cs{s_node + threadIdx.x};
+    float vl = 0.0f;
+    const int count = sailor::traverse(ctx, -1, false, -1, cs, &vl);
+    const bool cached = count <= kHitCache;
+    if (vl_out) vl_out[ray] = vl;
+    int* oi = sidx + ray * S;
+    float* ot = st + ray * S;
+    const float* nz = noise + ray * S;
+    has = 0;
+    if (count > 0) {
+      // ---- walk 2: samples (sampling_kernel.cu:1125-1154)
+      SampleSink ss;
+      ss.ss.init(vl, voxel_size[b] * (float)start_scale, S);
+      ss.nz = nz; ss.oi = oi; ss.ot = ot; ss.np = node_pos; ss.has = 0;
+      float dummy;
       if (cached)
-        replay(ctx, cs, count, ls);
+        replay(ctx, cs, count, ss);
       else
-        sailor::traverse(ctx, -1, false, -1, ls, &dummy);
-      has = ls.has;
+        sailor::traverse(ctx, -1, false, -1, ss, &dummy);
+      has = ss.has;
+      // ---- walk 3: leftover midpoints (:1156-1174)
+      if (has < S) {
+        LeftoverSink ls{nz, oi, ot, has, S - has, S};
+        if (cached)
+          replay(ctx, cs, count, ls);
+        else
+          sailor::traverse(ctx, -1, false, -1, ls, &dummy);
+        has = ls.has;
+      }
     }
   }
-  for (int j = has; j < S; ++j) {
-    oi[j] = -1;
-    ot[j] = -1.0f;
+  // ---- the rest of every row is -1, written row by row by the whole wave
+  // (most rays have no sample: one lane per row would make each store
+  // instruction touch 64 rows)
+  const int lane = threadIdx.x & 63;
+  const long long ray0 = ray - lane;
+  for (int k = 0; k < 64; ++k) {
+    const int hk = __builtin_amdgcn_readlane(has, k);
+    if (hk >= S) continue;
+    const long long rk = ray0 + k;
+    for (int j = hk + lane; j < S; j += 64) {
+      sidx[rk * S + j] = -1;
+      st[rk * S + j] = -1.0f;
+    }
   }
 }
 
