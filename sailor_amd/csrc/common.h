@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <stdint.h>
 #include <stdio.h>
 #include <string.h>
@@ -24,6 +25,13 @@ int set_error(int code, const char* fmt, ...);
 int check_launch(const char* what);
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// One-time launch setup (dynamic LDS limits, resident grid sizes) is kept per
+// device in std::atomic slots: several host threads may reach it at once,
+// and the setup itself is idempotent.
+constexpr int kMaxDevices = 64;
+// The calling thread's current device as a slot index in [0, kMaxDevices).
+int device_slot(int* slot);
 
 inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
 

@@ -25,6 +25,16 @@ int check_launch(const char* what) {
   return SAILOR_OK;
 }
 
+int device_slot(int* slot) {
+  int dev = 0;
+  SAILOR_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return set_error(SAILOR_ERR_ARG, "device %d: launch setup is recorded for devices 0-%d",
+                     dev, kMaxDevices - 1);
+  *slot = dev;
+  return SAILOR_OK;
+}
+
 }  // namespace sailor
 
 extern "C" const char* sailor_last_error(void) { return g_last_error; }

@@ -1244,13 +1244,15 @@ constexpr size_t kColorLds = rows_bytes() + kColorParams * sizeof(half_t);
 constexpr size_t kBlendLds = rows_bytes() + kBlendParams * sizeof(half_t);
 
 int setup_all() {
-  static int done = -1;
-  if (done == SAILOR_OK) return SAILOR_OK;
+  static std::atomic<bool> done[sailor::kMaxDevices];   // LDS limits set, per device
+  int dev = 0;
+  if (int e = sailor::device_slot(&dev)) return e;
+  if (done[dev].load(std::memory_order_acquire)) return SAILOR_OK;
   if (int e = lds_setup(density_kernel, kDensityLds)) return e;
   if (int e = lds_setup(color_kernel, kColorLds)) return e;
   if (int e = lds_setup(blend_kernel, kBlendLds)) return e;
   if (int e = lds_setup(upsample_kernel, kBlendLds)) return e;
-  done = SAILOR_OK;
+  done[dev].store(true, std::memory_order_release);
   return SAILOR_OK;
 }
 

@@ -3020,17 +3020,20 @@ template <int F, int H, int PREC, int NS, bool HALF_MAP, bool POINTS, int VP>
 int launch_kernels(const RenderArgs& a, hipStream_t st) {
   typedef Smem<F, H, PREC, NS, VP> L;
   auto kern = render_kernel<F, H, PREC, NS, HALF_MAP, POINTS, VP>;
-  static int grid = 0;
+  static std::atomic<int> grid_of[sailor::kMaxDevices];   // resident grid per device
+  int dev = 0;
+  if (int e = sailor::device_slot(&dev)) return e;
+  int grid = grid_of[dev].load(std::memory_order_acquire);
   if (grid == 0) {
     SAILOR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)L::total));
-    int dev = 0, cus = 0, per_cu = 0;
-    SAILOR_HIP_TRY(hipGetDevice(&dev));
+    int cus = 0, per_cu = 0;
     SAILOR_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     SAILOR_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads,
                                                                 L::total));
     grid = cus * std::max(per_cu, 1);
+    grid_of[dev].store(grid, std::memory_order_release);
   }
   // a frame batched with others on several streams may leave part of the
   // chip to their kernels (grid_share percent of the resident slots)
@@ -3251,12 +3254,14 @@ int hydra_backward_launch(const float* grad_rgb, const float* tok, const int* li
   } else {
     typedef HydraBwdRows<F> R;
     auto kern = hydra_rgb_backward_kernel<F, H>;
-    static bool init = false;
-    if (!init) {
+    static std::atomic<bool> init[sailor::kMaxDevices];   // LDS limit set, per device
+    int dev = 0;
+    if (int e = sailor::device_slot(&dev)) return e;
+    if (!init[dev].load(std::memory_order_acquire)) {
       SAILOR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                          hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)R::bytes));
-      init = true;
+      init[dev].store(true, std::memory_order_release);
     }
     hipLaunchKernelGGL(kern, dim3(parts), dim3(kThreads), R::bytes, st, tok, list, counter,
                        w_small, V, grad_rgb, grad_tokens, partials);
@@ -3319,12 +3324,14 @@ int sample_backward_launch_ns(BackArgs& ba, int parts, float* grad_small, float*
                  "render_sample_backward: S=%d must be a multiple of %d and at most %d",
                  ba.r.S, NS, SMAX);
   auto kern = render_backward_kernel<F, H, PREC, NS, DW, SMAX>;
-  static bool init = false;
-  if (!init) {
+  static std::atomic<bool> init[sailor::kMaxDevices];   // LDS limit set, per device
+  int dev = 0;
+  if (int e = sailor::device_slot(&dev)) return e;
+  if (!init[dev].load(std::memory_order_acquire)) {
     SAILOR_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)L::total));
-    init = true;
+    init[dev].store(true, std::memory_order_release);
   }
   const long long nz = (long long)parts * so.total;
   hipLaunchKernelGGL(fill_zero, dim3((unsigned)std::min<long long>((nz + 255) / 256, 4096)),
