@@ -94,7 +94,8 @@ struct RenderArgs {
   int compensated;         // Kahan-compensated compositing sums (precision form)
   const float* fscale;     // [F] per-channel scales of quantised maps (stored x / s) or null
   int grid_share;          // percent of the resident workgroup slots the sample kernel takes (0: all)
-  int fp32_hydra;          // 16-bit forms: Hydra + RGB head on fp32 vector ALUs instead of MFMA
+  int fp32_hydra;          // 16-bit forms: Hydra + RGB head on fp32 MFMA instead of 16-bit
+  void* wsmall16;          // 16-bit copy of wsmall, element i at index i + 4 (workspace)
   int stages;              // 1: compaction, 2: sample kernel, 4: Hydra/RGB kernel
   // point mode (density query): points [B, n_points, 3]; groups of NS points
   // take the place of rays and out_occ receives one density per point
@@ -966,6 +967,16 @@ __device__ __forceinline__ typename HydraOp<PREC>::V hcvt8(const float* p) {
   return v;
 }
 
+// 8 consecutive weights as one 16-bit operand: fp32 rows converted here, or
+// rows of the 16-bit copy loaded as they are (the same rounding, done once)
+template <int PREC, typename WT>
+__device__ __forceinline__ typename HydraOp<PREC>::V hload8(const WT* p) {
+  if constexpr (std::is_same<WT, float>::value)
+    return hcvt8<PREC>(p);
+  else
+    return *reinterpret_cast<const typename HydraOp<PREC>::V*>(p);
+}
+
 // acc[mt] = W[16 mt + l%16][0..K) . src[token l%16][0..K) for MT tiles (acc
 // zero on entry); W row-major [rows][K] fp32 (rows >= n_rows read as zero),
 // src fp32 LDS rows of pitch LD floats.  fp16 operands: each token's inputs
@@ -1007,8 +1018,8 @@ __device__ __forceinline__ void hydra_mm32(const float* __restrict__ W, int n_ro
   }
 }
 
-template <int PREC, int MT, int K, int LD>
-__device__ __forceinline__ void hydra_mm(const float* __restrict__ W, int n_rows,
+template <int PREC, int MT, int K, int LD, typename WT>
+__device__ __forceinline__ void hydra_mm(const WT* __restrict__ W, int n_rows,
                                          const float* src, int lane, f32x4 (&acc)[MT]) {
   if constexpr (PREC == kF32) {
     hydra_mm32<MT, K, LD>(W, n_rows, src, lane, acc);
@@ -1047,7 +1058,7 @@ __device__ __forceinline__ void hydra_mm(const float* __restrict__ W, int n_rows
       const int row = 16 * mt + n;
       V a;
       if (row < n_rows) {
-        a = hcvt8<PREC>(W + (size_t)row * K + 32 * s + 8 * g);
+        a = hload8<PREC>(W + (size_t)row * K + 32 * s + 8 * g);
       } else {
 #pragma unroll
         for (int j = 0; j < 8; ++j) a[j] = (typename HydraOp<PREC>::E)0.0f;
@@ -1072,7 +1083,7 @@ template <int F, int H, int VP, int PREC>
 __global__ void __launch_bounds__(kThreads)
 hydra_rgb_mfma_kernel(const float* __restrict__ tok, const int* __restrict__ list,
                       const unsigned* __restrict__ n_valid_ptr, const float* __restrict__ ws,
-                      int V, float* __restrict__ out_rgb) {
+                      const void* __restrict__ ws16, int V, float* __restrict__ out_rgb) {
   constexpr int RPG = kTok / VP;          // rays per workgroup (64 tokens)
   constexpr int HI = kHydraInner;         // 32
   constexpr int LX = F + 4;               // fp32 row pitch of token rows (16-byte aligned)
@@ -1104,9 +1115,17 @@ hydra_rgb_mfma_kernel(const float* __restrict__ tok, const int* __restrict__ lis
   __syncthreads();
   const int view = n % VP;
   const float vmask = view < V ? 1.0f : 0.0f;
+  // weight matrices: fp32 (converted at each use) or the 16-bit copy, at the
+  // same element offsets; biases always from the fp32 buffer
+  const auto* wm = [&]() {
+    if constexpr (PREC == kF32)
+      return ws;
+    else
+      return static_cast<const typename HydraOp<PREC>::E*>(ws16) + 4;
+  }();
 
   for (int st = 0; st < kHydraStacks; ++st) {
-    const float* wqkv = ws + so.q[0] + st * stack_stride;   // q, k, v rows contiguous [96][F]
+    const auto* wqkv = wm + so.q[0] + st * stack_stride;   // q, k, v rows contiguous [96][F]
     f32x4 a6[6];
     hzero(a6);
     hydra_mm<PREC, 6, F, LX>(wqkv, 3 * HI, xs, lane, a6);
@@ -1141,7 +1160,7 @@ hydra_rgb_mfma_kernel(const float* __restrict__ tok, const int* __restrict__ lis
       constexpr int MT = F / 16;
       f32x4 ao[MT];
       hzero(ao);
-      hydra_mm<PREC, MT, HI, LH>(ws + so.fc[0] + st * stack_stride, F, hs, lane, ao);
+      hydra_mm<PREC, MT, HI, LH>(wm + so.fc[0] + st * stack_stride, F, hs, lane, ao);
       __syncthreads();
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt)
@@ -1153,7 +1172,7 @@ hydra_rgb_mfma_kernel(const float* __restrict__ tok, const int* __restrict__ lis
     {
       f32x4 ah[2];
       hzero(ah);
-      hydra_mm<PREC, 2, F, LX>(ws + so.w1[0] + st * stack_stride, HI, xs, lane, ah);
+      hydra_mm<PREC, 2, F, LX>(wm + so.w1[0] + st * stack_stride, HI, xs, lane, ah);
       const float* b1 = ws + so.b1[0] + st * stack_stride;
 #pragma unroll
       for (int mt = 0; mt < 2; ++mt)
@@ -1169,7 +1188,7 @@ hydra_rgb_mfma_kernel(const float* __restrict__ tok, const int* __restrict__ lis
       constexpr int MT = F / 16;
       f32x4 ax[MT];
       hzero(ax);
-      hydra_mm<PREC, MT, HI, LH>(ws + so.w2[0] + st * stack_stride, F, hs, lane, ax);
+      hydra_mm<PREC, MT, HI, LH>(wm + so.w2[0] + st * stack_stride, F, hs, lane, ax);
       const float* b2 = ws + so.b2[0] + st * stack_stride;
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt)
@@ -1185,7 +1204,7 @@ hydra_rgb_mfma_kernel(const float* __restrict__ tok, const int* __restrict__ lis
   {
     f32x4 a1[kRgbH1 / 16];
     hzero(a1);
-    hydra_mm<PREC, kRgbH1 / 16, F, LX>(ws + so.r1w, kRgbH1, xs, lane, a1);
+    hydra_mm<PREC, kRgbH1 / 16, F, LX>(wm + so.r1w, kRgbH1, xs, lane, a1);
 #pragma unroll
     for (int mt = 0; mt < kRgbH1 / 16; ++mt)
 #pragma unroll
@@ -1198,7 +1217,7 @@ hydra_rgb_mfma_kernel(const float* __restrict__ tok, const int* __restrict__ lis
   {
     f32x4 a2[kRgbH2 / 16];
     hzero(a2);
-    hydra_mm<PREC, kRgbH2 / 16, kRgbH1, LH>(ws + so.r2w, kRgbH2, hs, lane, a2);
+    hydra_mm<PREC, kRgbH2 / 16, kRgbH1, LH>(wm + so.r2w, kRgbH2, hs, lane, a2);
     __syncthreads();
 #pragma unroll
     for (int mt = 0; mt < kRgbH2 / 16; ++mt)
@@ -1212,7 +1231,7 @@ hydra_rgb_mfma_kernel(const float* __restrict__ tok, const int* __restrict__ lis
   {
     f32x4 a3[1];
     hzero(a3);
-    hydra_mm<PREC, 1, kRgbH2, LH>(ws + so.r3w, 3, hs, lane, a3);
+    hydra_mm<PREC, 1, kRgbH2, LH>(wm + so.r3w, 3, hs, lane, a3);
     const long long item = item0 + (16 * wave + n) / VP;
     if (g == 0 && view == 0 && item < n_valid) {
 #pragma unroll
@@ -2892,6 +2911,17 @@ size_t workspace_token_offset(long long n_rays) {
   return 256 + ((size_t)n_rays * 4 + 255) / 256 * 256;
 }
 
+// after the tokens: the 16-bit copy of the small weights (4 leading elements
+// put every weight matrix on a 16-byte boundary); sized for H = 4 F, the
+// instantiated models
+size_t workspace_small16_offset(long long n_rays, int vp, int F) {
+  return (workspace_token_offset(n_rays) + (size_t)n_rays * vp * F * sizeof(float) + 255) / 256 *
+         256;
+}
+size_t workspace_small16_bytes(int F) {
+  return ((size_t)small_offsets(F, 4 * F).total + 4) * 2;
+}
+
 // Lists every ray with a full set of samples (t[ray, 0] >= 0) in ray order
 // and writes the outputs of dropped rays (zero colour, depth and sums).
 // Three launches (no atomics, so the list is the same on every run and
@@ -3016,6 +3046,14 @@ int launch(const RenderArgs& a, hipStream_t st, bool dry) {
   }
 }
 
+// out[i] = in[i] rounded to the 16-bit type (round to nearest even, as
+// hcvt8 rounds)
+template <typename E>
+__global__ void small_to_16bit(const float* __restrict__ in, int n, E* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (E)in[i];
+}
+
 template <int F, int H, int PREC, int NS, bool HALF_MAP, bool POINTS, int VP>
 int launch_kernels(const RenderArgs& a, hipStream_t st) {
   typedef Smem<F, H, PREC, NS, VP> L;
@@ -3052,15 +3090,21 @@ int launch_kernels(const RenderArgs& a, hipStream_t st) {
     // 16-bit forms: Hydra + RGB head on 16-bit MFMA, unless the caller
     // differentiates the render (its backward recomputes them in fp32)
     if (!a.fp32_hydra) {
+      // the small weights in the 16-bit type, once per launch (every wave
+      // then reads half the bytes and converts nothing)
+      typedef typename HydraOp<PREC>::E E;
+      constexpr int n_small = small_offsets(F, H).total;
+      hipLaunchKernelGGL((small_to_16bit<E>), dim3((n_small + 255) / 256), dim3(256), 0, st,
+                         a.wsmall, n_small, static_cast<E*>(a.wsmall16) + 4);
       hipLaunchKernelGGL((hydra_rgb_mfma_kernel<F, H, VP, PREC>), dim3((unsigned)groups),
-                         dim3(kThreads), 0, st, a.out_tok, a.list, a.n_valid, a.wsmall, a.V,
-                         a.out_rgb);
+                         dim3(kThreads), 0, st, a.out_tok, a.list, a.n_valid, a.wsmall,
+                         a.wsmall16, a.V, a.out_rgb);
       return sailor::check_launch("render_rays (hydra, mfma)");
     }
   }
   hipLaunchKernelGGL((hydra_rgb_mfma_kernel<F, H, VP, kF32>), dim3((unsigned)groups),
-                     dim3(kThreads), 0, st, a.out_tok, a.list, a.n_valid, a.wsmall, a.V,
-                     a.out_rgb);
+                     dim3(kThreads), 0, st, a.out_tok, a.list, a.n_valid, a.wsmall, nullptr,
+                     a.V, a.out_rgb);
   return sailor::check_launch("render_rays (hydra, fp32 mfma)");
 }
 
@@ -3128,7 +3172,7 @@ extern "C" size_t sailor_render_workspace_bytes_views(int B, int R, int F, int V
   if (B <= 0 || R < 0 || F <= 0 || V < 1 || V > kMaxViews) return 0;
   const long long n = (long long)B * R;
   const int vp = V <= kVP ? kVP : kMaxViews;
-  return workspace_token_offset(n) + (size_t)n * vp * F * sizeof(float);
+  return workspace_small16_offset(n, vp, F) + workspace_small16_bytes(F);
 }
 
 extern "C" size_t sailor_render_workspace_bytes(int B, int R, int F) {
@@ -3186,6 +3230,7 @@ int render_stages(int stages, const float* t, const float* rays, int B, int R, i
   unsigned* counter = reinterpret_cast<unsigned*>(wsb);
   int* list = reinterpret_cast<int*>(wsb + workspace_list_offset());
   a.out_tok = reinterpret_cast<float*>(wsb + workspace_token_offset(a.n_rays));
+  a.wsmall16 = wsb + workspace_small16_offset(a.n_rays, V <= kVP ? kVP : kMaxViews, F);
   a.list = list;
   a.n_valid = counter;
   a.S = S; a.V = V; a.img_w = img_w; a.img_h = img_h; a.Hf = Hf; a.Wf = Wf;
