@@ -1686,7 +1686,11 @@ render_backward_kernel(BackArgs ba) {
       s_alpha[tid] = ba.occ[ray * S + tid];
     }
     __syncthreads();
-    // 1. features of all samples and views
+    // 1. features of all samples and views, and on the way
+    // 2. g_i = dL/dtoken . f(x_i): item `it` is (sample it / 16, view it / 4 %
+    // 4, channel quarter it % 4), so a sample's 16 items are 16 consecutive
+    // lanes; each dots its quarter of the channels with dL/dtoken, and four
+    // shuffles add the quarters and the views
     for (int it = tid; it < S * kVP * 4; it += kThreads) {
       const int col = it >> 2, quarter = it & 3;
       const int sl = col >> 2, v = col & 3;
@@ -1697,6 +1701,18 @@ render_backward_kernel(BackArgs ba) {
 #pragma unroll
       for (int c = 0; c < CPL; c += 4)
         *reinterpret_cast<f32x4*>(kp + c) = f32x4{f[c], f[c + 1], f[c + 2], f[c + 3]};
+      const float* dtv = ba.dtok + item * kVP * F + v * F + quarter * CPL;
+      float part = 0.0f;
+#pragma unroll
+      for (int c = 0; c < CPL; c += 4) {
+        const f32x4 d4 = *reinterpret_cast<const f32x4*>(dtv + c);
+        part += d4[0] * f[c] + d4[1] * f[c + 1] + d4[2] * f[c + 2] + d4[3] * f[c + 3];
+      }
+      part += __shfl_xor(part, 1, 64);
+      part += __shfl_xor(part, 2, 64);
+      part += __shfl_xor(part, 4, 64);
+      part += __shfl_xor(part, 8, 64);
+      if ((it & 15) == 0) s_g[sl] = part;
     }
     __syncthreads();
     // layer-0 input of the chunk at sample c0: features (from keep) and
@@ -1750,24 +1766,6 @@ render_backward_kernel(BackArgs ba) {
         }
       }
     };
-    // 2. g_i = dL/dtoken . f(x_i): thread t takes sample t / 4 and the F
-    // products of view t % 4; two shuffles add the 4 views
-    for (int i4 = tid; i4 < S * kVP; i4 += kThreads) {
-      const int i = i4 >> 2, v = i4 & 3;
-      const float* dtv = ba.dtok + item * kVP * F + v * F;
-      const float* kp = keep + (i * kVP + v) * KF;
-      float part = 0.0f;
-#pragma unroll
-      for (int c = 0; c < F; c += 4) {
-        const f32x4 d4 = *reinterpret_cast<const f32x4*>(dtv + c);
-        const f32x4 k4 = *reinterpret_cast<const f32x4*>(kp + c);
-        part += d4[0] * k4[0] + d4[1] * k4[1] + d4[2] * k4[2] + d4[3] * k4[3];
-      }
-      part += __shfl_xor(part, 1, 64);
-      part += __shfl_xor(part, 2, 64);
-      if (v == 0) s_g[i] = part;
-    }
-    __syncthreads();
     // 3. compositing replayed and walked back by wave 0 from registers,
     // sample i in lane i: the recurrences run in sample order on values
     // broadcast with readlane, the same operations in the same order as the
