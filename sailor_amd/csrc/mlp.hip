@@ -2885,14 +2885,35 @@ hydra_rgb_backward_mfma_kernel(const float* __restrict__ tok, const int* __restr
 }
 
 // out[e] = sum over parts of part[p][e]
-__global__ void sum_partials(const float* __restrict__ part, int parts, int n,
-                             float* __restrict__ out, int accumulate) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  // double accumulation: the partial gradients cancel heavily
-  double s = accumulate ? (double)out[e] : 0.0;
-  for (int p = 0; p < parts; ++p) s += (double)part[(size_t)p * n + e];
-  out[e] = (float)s;
+// out[e] (+)= sum_p part[p][e], accumulated in double (the partial gradients
+// cancel heavily).  A 256-thread block takes kSumEntries consecutive entries
+// with 8 threads each, thread g summing partials g, g + 8, ...; the 8 sums
+// are added in a fixed order, so the result does not depend on scheduling.
+// (One thread per entry walking all partials left most CUs idle: a few
+// hundred workgroups of dependent loads.)
+constexpr int kSumEntries = 32;
+__global__ void __launch_bounds__(256)
+sum_partials(const float* __restrict__ part, int parts, int n, float* __restrict__ out,
+             int accumulate) {
+  __shared__ double s_sum[8][kSumEntries];
+  const int o = threadIdx.x % kSumEntries, g = threadIdx.x / kSumEntries;
+  const int e = blockIdx.x * kSumEntries + o;
+  double s = 0.0;
+  if (e < n)
+    for (int p = g; p < parts; p += 8) s += (double)part[(size_t)p * n + e];
+  s_sum[g][o] = s;
+  __syncthreads();
+  if (g == 0 && e < n) {
+    double t = accumulate ? (double)out[e] : 0.0;
+    for (int k = 0; k < 8; ++k) t += s_sum[k][o];
+    out[e] = (float)t;
+  }
+}
+
+void launch_sum_partials(const float* part, int parts, int n, float* out, int accumulate,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(sum_partials, dim3((unsigned)((n + kSumEntries - 1) / kSumEntries)),
+                     dim3(256), 0, st, part, parts, n, out, accumulate);
 }
 
 // zero a float buffer (graph-capturable, no memset)
@@ -3310,8 +3331,7 @@ int hydra_backward_launch(const float* grad_rgb, const float* tok, const int* li
                        w_small, V, grad_rgb, grad_tokens, partials);
   }
   if (int err = sailor::check_launch("render_backward (hydra)")) return err;
-  hipLaunchKernelGGL(sum_partials, dim3((so.total + 255) / 256), dim3(256), 0, st, partials,
-                     parts, so.total, grad_small, 0);
+  launch_sum_partials(partials, parts, so.total, grad_small, 0, st);
   return sailor::check_launch("render_backward (sum)");
 }
 
@@ -3382,13 +3402,11 @@ int sample_backward_launch_ns(BackArgs& ba, int parts, float* grad_small, float*
   if (int err = sailor::check_launch("render_sample_backward (zero)")) return err;
   hipLaunchKernelGGL(kern, dim3(parts), dim3(kThreads), L::total, st, ba);
   if (int err = sailor::check_launch("render_sample_backward")) return err;
-  hipLaunchKernelGGL(sum_partials, dim3((so.total + 255) / 256), dim3(256), 0, st, ba.gpart,
-                     parts, so.total, grad_small, 1);
+  launch_sum_partials(ba.gpart, parts, so.total, grad_small, 1, st);
   if (int err = sailor::check_launch("render_sample_backward (sum)")) return err;
   if constexpr (DW) {
     constexpr int nd = dense_grad_total(F, H);
-    hipLaunchKernelGGL(sum_partials, dim3((nd + 255) / 256), dim3(256), 0, st, ba.gdense, parts,
-                       nd, grad_dense, 0);
+    launch_sum_partials(ba.gdense, parts, nd, grad_dense, 0, st);
     return sailor::check_launch("render_sample_backward (dense sum)");
   }
   return SAILOR_OK;
